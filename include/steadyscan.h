@@ -1,0 +1,205 @@
+/*
+ * steadyscan.h -- C ABI of the MI355X-native align_group_to_template path.
+ *
+ * Drop-in for the MATLAB mex `align_group_to_template`
+ * (mex/align_group_to_template.cpp:21 -> MexGateway<T>::execute,
+ * mex/mexgateway.cpp:373-445).  The 26 mex arguments map onto
+ * ss_group_in (arrays, args 1-13) and ss_settings (scalars, args 0, 14-25);
+ * the 6 outputs used by steady_scan.m:239-242 (plus the diagnostics of
+ * mexgateway.cpp:266-354) map onto ss_group_out.
+ *
+ * Conventions
+ *  - Every matrix is MATLAB column-major: element (r, c) of an R x C matrix
+ *    is at [c * R + r].  A mex wrapper passes mxGetData() pointers unchanged.
+ *  - All buffers are owned by the caller.  The handle owns device state
+ *    (workspaces) across calls; it never resets the device
+ *    (unlike task.cpp:190).
+ *  - Errors are returned as ss_status, never thrown or longjmp'ed
+ *    (the mex raised GPUMC:* errors through mexErrMsgIdAndTxt,
+ *    mex/reporter/reporter.cpp:25-27).  ss_last_error() has the message.
+ *  - A handle is used by one host thread at a time; different handles are
+ *    independent.
+ */
+#ifndef STEADYSCAN_H_
+#define STEADYSCAN_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SS_ABI_VERSION 1
+
+typedef struct ss_handle ss_handle;
+
+/* Status codes (GPUMC:* report ids of mex/reporter/reporter.cpp:3-37). */
+typedef enum {
+    SS_OK = 0,
+    SS_ERR_INVALID_ARGUMENT = 1, /* GPUMC:InvalidArgument, GPUMC:NotEnoughInputs */
+    SS_ERR_OUT_OF_MEMORY = 2,    /* "unable to allocate device memory" (task.cpp:116-122) */
+    SS_ERR_DEVICE = 3,           /* GPUMC:CudaError equivalents: a HIP call failed */
+    SS_ERR_NUMERICAL = 4,        /* a line search that cannot terminate (non-finite step) */
+    SS_ERR_NOT_AVAILABLE = 5     /* no usable gfx950 device */
+} ss_status;
+
+/* Stop criterion, same values as StopCriterion (interface/stopcriterion.h:4-10). */
+typedef enum {
+    SS_STOP_CORRELATION_INCREASE = 0,
+    SS_STOP_HALT_CORRELATION = 1,
+    SS_STOP_MOVEMENT = 2,
+    SS_STOP_ITERATION_LIMIT = 3,
+    SS_STOP_ERROR = 4
+} ss_stop_criterion;
+
+/* Element type of the frame data (arg 8, bmData). */
+typedef enum { SS_F64 = 0, SS_F32 = 1, SS_U16 = 2, SS_I16 = 3 } ss_dtype;
+
+/* Scalar arguments: args 14-25 of the mex (mexgateway.cpp:116-174) plus the
+ * precision letter of arg 0 (configuration.h:10-11). */
+typedef struct {
+    int32_t max_iterations;                  /* arg 14 max_iter                          */
+    double move_threshold;                   /* arg 15 move_thresh                       */
+    double halt_correlation;                 /* arg 16 halt_correlation                  */
+    double correlation_increase_threshold;   /* arg 17                                   */
+    double filter_strength;                  /* arg 18                                   */
+    int32_t max_filter_iterations;           /* arg 19                                   */
+    double filter_correlation_threshold;     /* arg 20 filter_deactivation_threshold     */
+    double lambda;                           /* arg 21: 0 = reference behaviour (unused there);
+                                                >0 adds lambda*(delta_{k+1}-delta_k)^2 per axis */
+    double linesearch_reduction_multiplier;  /* arg 22                                   */
+    int32_t solver_id;                       /* arg 23: 0 ILU(0) (x-then-y order), 1/2 exact */
+    int32_t filter_id;                       /* arg 24: 0 none, 1 smearing, 2 range      */
+    int32_t device_id;                       /* arg 25 gpu_device_id                     */
+    char precision;                          /* 'D' or 'S' (arg 0): mask epsilon         */
+    int32_t linesearch_candidates;           /* fused candidates per round (0 = default 4) */
+} ss_settings;
+
+/* Fills the defaults of default_alignment_settings (steady_scan.m:836-867). */
+void ss_settings_default(ss_settings *s);
+
+/* Array arguments 1-13 of the mex (mexgateway.cpp:45-114).  frame_h and
+ * n_blocks are the size of base_x (mexgateway.cpp:176-177). */
+typedef struct {
+    int32_t frame_h;             /* rows of every block matrix                     */
+    int32_t n_blocks;            /* knot intervals per frame (nknots - 1)           */
+    int32_t n_frames;            /* arg 12 nsetframes                               */
+    int32_t group_size;          /* arg 11 (2 * n_blocks from steady_scan.m:195)    */
+    int32_t template_h, template_w;
+    const double *template_image;/* arg 1: template_h x template_w                  */
+    const double *xgrad;         /* arg 2: [diff(template,1,2) NaN]                 */
+    const double *ygrad;         /* arg 3: [diff(template,1,1); NaN]                */
+    const double *frac;          /* arg 4: tfrac block matrix, frame_h x n_blocks   */
+    const double *base_x;        /* arg 5: basex - 1                                */
+    const double *base_y;        /* arg 6: basey - 1                                */
+    const uint8_t *base_mask;    /* arg 7                                           */
+    ss_dtype data_type;          /* element type of data                            */
+    const void *data;            /* arg 8: frame_h x (n_blocks * n_frames)          */
+    int32_t data_on_device;      /* nonzero: data is a device pointer (HBM resident)*/
+    const double *init_x;        /* arg 9: est_dx_knots, (n_blocks+1) x >= n_frames */
+    const double *init_y;        /* arg 10                                          */
+    const int32_t *min_points;   /* arg 13: n_blocks + 1 (truncated toward zero)    */
+    int32_t template_id;         /* > 0: use a resident template (args 1-3 ignored) */
+    int32_t geometry_id;         /* > 0: use resident geometry (args 4-7 ignored)   */
+} ss_group_in;
+
+/* Outputs 1-7 of the mex (mexgateway.cpp:320-354) and selected diagnostics
+ * (outputs 8 and 15).  NULL pointers skip an output. */
+typedef struct {
+    double *knots_x;             /* out 1: (n_blocks+1) x n_frames                  */
+    double *knots_y;             /* out 2                                           */
+    double *frame_corr;          /* out 3: n_frames                                 */
+    int32_t iterations;          /* out 4                                           */
+    uint8_t *mask;               /* out 5: frame_h x (n_blocks * n_frames)          */
+    double *residrms;            /* out 6: n_frames                                 */
+    double device_init_time;     /* out 7: seconds                                  */
+    double *iteration_corr;      /* out 8 ('C'): max_iterations                     */
+    double *all_params;          /* out 15: 2*(n_blocks*n_frames+1) x max_iterations */
+    int32_t stop_criterion;      /* ss_stop_criterion ('E' flag message)            */
+    int32_t half_iterations;     /* executed half-iterations                        */
+    int32_t linesearch_rounds;   /* executed fused line-search rounds               */
+    int32_t linesearch_tries;    /* sequential tries those rounds resolved          */
+} ss_group_out;
+
+/* Handle lifecycle.  Replaces CudaHelper<T>::initDevice (cudahelper.cu:8-27)
+ * and Task<T>::initialize / freePreAllocatedMemory (task.cpp:47-197). */
+ss_status ss_create(int32_t device_id, ss_handle **out);
+void ss_destroy(ss_handle *h);
+const char *ss_last_error(const ss_handle *h);
+
+/* Resident inputs.  A template (args 1-3) or a scan geometry (args 4-7)
+ * uploaded once is converted to the device layout and kept in HBM until
+ * released; ss_group_in refers to it by id.  No reference counterpart: the
+ * mex re-uploads every argument per call (task_allocate.cu:290-302). */
+ss_status ss_upload_template(ss_handle *h, int32_t template_h, int32_t template_w,
+                             const double *template_image, const double *xgrad,
+                             const double *ygrad, int32_t *id);
+ss_status ss_upload_geometry(ss_handle *h, int32_t frame_h, int32_t n_blocks, const double *frac,
+                             const double *base_x, const double *base_y,
+                             const uint8_t *base_mask, int32_t *id);
+ss_status ss_release(ss_handle *h, int32_t id);
+
+/* One align_group_to_template call (MexGateway<T>::execute,
+ * mexgateway.cpp:373-445): Task::alignFrames + getParameters. */
+ss_status ss_align_group(ss_handle *h, const ss_group_in *in, const ss_settings *s,
+                         ss_group_out *out);
+
+/* Several independent calls in one submission (groups may differ in frame
+ * count, size and knot spacing).  Results equal n separate ss_align_group
+ * calls.  No reference counterpart: steady_scan.m:216-287 loops over sets. */
+ss_status ss_align_batch(ss_handle *h, int32_t n, const ss_group_in *in,
+                         const ss_settings *s, ss_group_out *out);
+
+/* ---- host-side geometry: init_settings + image2blockmatrix ---- */
+
+/* Scan geometry (settings fields of steady_scan.m:748-828, 836-867). */
+typedef struct {
+    int32_t nlines, linewidth;      /* raw frame size (H x W)                     */
+    int32_t template_h, template_w;
+    int32_t nknots;                 /* settings.nknots                            */
+    int32_t cropping[4];            /* [left right bottom top]                    */
+    double subrec[4];               /* [left right bottom top], if has_subrec     */
+    int32_t has_subrec;
+    int32_t bidirectional;          /* settings.isbidirectional                   */
+    double min_blockpoints_ratio;   /* settings.min_blockpoints_ratio (0.3)       */
+} ss_geometry_desc;
+
+/* Sizes of the geometry: cropped frame hc x wc and block-matrix frame_h. */
+ss_status ss_geometry_size(const ss_geometry_desc *g, int32_t *hc, int32_t *wc,
+                           int32_t *frame_h);
+
+/* init_settings (steady_scan.m:787-827) and the block-matrix packing of
+ * estimate_displacements (steady_scan.m:166-175).  Image-shaped outputs are
+ * hc x wc; block outputs frame_h x (nknots-1); min_points has nknots entries.
+ * Any output pointer may be NULL. */
+ss_status ss_geometry_init(const ss_geometry_desc *g, int32_t *prevknotind, double *tfrac,
+                           double *basex, double *basey, double *frac_bm, double *base_x_bm,
+                           double *base_y_bm, uint8_t *base_mask_bm, int32_t *min_points);
+
+/* image2blockmatrix of a stack of frames (steady_scan.m:229-234, 601-613):
+ * frames are nlines x linewidth x n (raw, uncropped); out is
+ * frame_h x ((nknots-1) * n) of the same element type. */
+ss_status ss_pack_frames(const ss_geometry_desc *g, ss_dtype type, const void *frames,
+                         int32_t n, void *out);
+
+/* ---- corrected frames: resample_frame / pushgrid (steady_scan.m:562-599) ---- */
+typedef struct {
+    const ss_geometry_desc *geometry;
+    int32_t n_frames;
+    ss_dtype data_type;
+    const void *frames;          /* nlines x linewidth x n_frames (raw, uncropped) */
+    int32_t data_on_device;
+    const double *knots_x;       /* nknots x n_frames                              */
+    const double *knots_y;
+} ss_resample_in;
+
+/* Forward bilinear splat of every frame into template space, normalised by
+ * the splatted weights: out is template_h x template_w x n_frames float32,
+ * NaN where nothing landed.  Repeated targets keep the last contribution,
+ * as MATLAB's indexed assignment does. */
+ss_status ss_resample(ss_handle *h, const ss_resample_in *in, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

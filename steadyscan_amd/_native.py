@@ -1,0 +1,175 @@
+"""ctypes declarations of include/steadyscan.h and include/steadyscan_synth.h."""
+import ctypes
+import os
+from ctypes import POINTER, c_char, c_char_p, c_double, c_int32, c_uint8, c_uint64, c_void_p
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsteadyscan_amd.so")
+SYNTH_PATH = os.path.join(HERE, "libss_synth.so")
+
+SS_OK, SS_ERR_INVALID_ARGUMENT, SS_ERR_OUT_OF_MEMORY, SS_ERR_DEVICE, SS_ERR_NUMERICAL, SS_ERR_NOT_AVAILABLE = range(6)
+SS_F64, SS_F32, SS_U16, SS_I16 = range(4)
+
+# report ids of the mex (mex/reporter/reporter.cpp:3-37)
+STATUS_IDS = {
+    SS_ERR_INVALID_ARGUMENT: "GPUMC:InvalidArgument",
+    SS_ERR_OUT_OF_MEMORY: "GPUMC:CudaError",
+    SS_ERR_DEVICE: "GPUMC:CudaError",
+    SS_ERR_NUMERICAL: "GPUMC:Unknown",
+    SS_ERR_NOT_AVAILABLE: "GPUMC:CudaError",
+}
+
+STOP_NAMES = ["CorrelationIncreaseThreshold", "HaltCorrelationReached", "MovementThreshold",
+              "IterationLimit", "Error"]
+
+
+class SSSettings(ctypes.Structure):
+    _fields_ = [
+        ("max_iterations", c_int32),
+        ("move_threshold", c_double),
+        ("halt_correlation", c_double),
+        ("correlation_increase_threshold", c_double),
+        ("filter_strength", c_double),
+        ("max_filter_iterations", c_int32),
+        ("filter_correlation_threshold", c_double),
+        ("lambda_", c_double),
+        ("linesearch_reduction_multiplier", c_double),
+        ("solver_id", c_int32),
+        ("filter_id", c_int32),
+        ("device_id", c_int32),
+        ("precision", c_char),
+        ("linesearch_candidates", c_int32),
+    ]
+
+
+class SSGroupIn(ctypes.Structure):
+    _fields_ = [
+        ("frame_h", c_int32), ("n_blocks", c_int32), ("n_frames", c_int32), ("group_size", c_int32),
+        ("template_h", c_int32), ("template_w", c_int32),
+        ("template_image", POINTER(c_double)), ("xgrad", POINTER(c_double)), ("ygrad", POINTER(c_double)),
+        ("frac", POINTER(c_double)), ("base_x", POINTER(c_double)), ("base_y", POINTER(c_double)),
+        ("base_mask", POINTER(c_uint8)),
+        ("data_type", c_int32), ("data", c_void_p), ("data_on_device", c_int32),
+        ("init_x", POINTER(c_double)), ("init_y", POINTER(c_double)),
+        ("min_points", POINTER(c_int32)),
+        ("template_id", c_int32), ("geometry_id", c_int32),
+    ]
+
+
+class SSGroupOut(ctypes.Structure):
+    _fields_ = [
+        ("knots_x", POINTER(c_double)), ("knots_y", POINTER(c_double)),
+        ("frame_corr", POINTER(c_double)), ("iterations", c_int32),
+        ("mask", POINTER(c_uint8)), ("residrms", POINTER(c_double)),
+        ("device_init_time", c_double),
+        ("iteration_corr", POINTER(c_double)), ("all_params", POINTER(c_double)),
+        ("stop_criterion", c_int32), ("half_iterations", c_int32),
+        ("linesearch_rounds", c_int32), ("linesearch_tries", c_int32),
+    ]
+
+
+class SSGeometryDesc(ctypes.Structure):
+    _fields_ = [
+        ("nlines", c_int32), ("linewidth", c_int32),
+        ("template_h", c_int32), ("template_w", c_int32),
+        ("nknots", c_int32),
+        ("cropping", c_int32 * 4),
+        ("subrec", c_double * 4),
+        ("has_subrec", c_int32),
+        ("bidirectional", c_int32),
+        ("min_blockpoints_ratio", c_double),
+    ]
+
+
+class SSResampleIn(ctypes.Structure):
+    _fields_ = [
+        ("geometry", POINTER(SSGeometryDesc)),
+        ("n_frames", c_int32),
+        ("data_type", c_int32),
+        ("frames", c_void_p),
+        ("data_on_device", c_int32),
+        ("knots_x", POINTER(c_double)),
+        ("knots_y", POINTER(c_double)),
+    ]
+
+
+class SynthTemplateDesc(ctypes.Structure):
+    _fields_ = [
+        ("h", c_int32), ("w", c_int32), ("n_blobs", c_int32),
+        ("sigma_lo", c_double), ("sigma_hi", c_double),
+        ("amp_lo", c_double), ("amp_hi", c_double), ("offset", c_double),
+        ("n_background", c_int32), ("background_sigma", c_double),
+        ("background_amp_lo", c_double), ("background_amp_hi", c_double),
+        ("target_mean", c_double), ("seed", c_uint64),
+    ]
+
+
+class SynthMotionDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_frames", c_int32), ("nlines", c_int32), ("frame_rate", c_double),
+        ("walk_sigma", c_double), ("walk_revert", c_double),
+        ("sin_amp_x", c_double), ("sin_amp_y", c_double), ("sin_freq", c_double),
+        ("sin_phase_x", c_double), ("sin_phase_y", c_double),
+        ("jump_prob", c_double), ("jump_size", c_double), ("seed", c_uint64),
+    ]
+
+
+# exported by libsteadyscan_amd.so: (name, restype, argtypes)
+API = [
+    ("ss_settings_default", None, [POINTER(SSSettings)]),
+    ("ss_create", c_int32, [c_int32, POINTER(c_void_p)]),
+    ("ss_destroy", None, [c_void_p]),
+    ("ss_last_error", c_char_p, [c_void_p]),
+    ("ss_upload_template", c_int32, [c_void_p, c_int32, c_int32, POINTER(c_double), POINTER(c_double),
+                                     POINTER(c_double), POINTER(c_int32)]),
+    ("ss_upload_geometry", c_int32, [c_void_p, c_int32, c_int32, POINTER(c_double), POINTER(c_double),
+                                     POINTER(c_double), POINTER(c_uint8), POINTER(c_int32)]),
+    ("ss_release", c_int32, [c_void_p, c_int32]),
+    ("ss_align_group", c_int32, [c_void_p, POINTER(SSGroupIn), POINTER(SSSettings), POINTER(SSGroupOut)]),
+    ("ss_align_batch", c_int32, [c_void_p, c_int32, POINTER(SSGroupIn), POINTER(SSSettings), POINTER(SSGroupOut)]),
+    ("ss_geometry_size", c_int32, [POINTER(SSGeometryDesc), POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
+    ("ss_geometry_init", c_int32, [POINTER(SSGeometryDesc), POINTER(c_int32), POINTER(c_double), POINTER(c_double),
+                                   POINTER(c_double), POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                                   POINTER(c_uint8), POINTER(c_int32)]),
+    ("ss_pack_frames", c_int32, [POINTER(SSGeometryDesc), c_int32, c_void_p, c_int32, c_void_p]),
+    ("ss_resample", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(ctypes.c_float)]),
+]
+
+SYNTH_API = [
+    ("ss_synth_template", c_int32, [POINTER(SynthTemplateDesc), POINTER(c_double)]),
+    ("ss_synth_motion", c_int32, [POINTER(SynthMotionDesc), POINTER(c_double), POINTER(c_double)]),
+    ("ss_synth_frames", c_int32, [POINTER(c_double), c_int32, c_int32, c_int32, c_int32, c_int32,
+                                  POINTER(c_double), POINTER(c_double), c_int32, c_uint64, c_int32,
+                                  c_void_p, c_int32]),
+]
+
+_lib = None
+_synth = None
+
+
+def _bind(lib, table):
+    for name, res, args in table:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def lib():
+    """The HIP library.  Raises if it has not been built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                               "(python -m steadyscan_amd.build) first")
+        _lib = _bind(ctypes.CDLL(LIB_PATH), API)
+    return _lib
+
+
+def synth():
+    global _synth
+    if _synth is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise RuntimeError(f"{SYNTH_PATH} is missing: run python -m steadyscan_amd.build first")
+        _synth = _bind(ctypes.CDLL(SYNTH_PATH), SYNTH_API)
+    return _synth

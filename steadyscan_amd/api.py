@@ -1,0 +1,338 @@
+"""Host-side mirror of SteadyScan's `align_group_to_template` interface.
+
+`align_group_to_template` takes the 26 arguments of the mex call in the same
+order and meaning (steady_scan.m:239-242, mex/mexgateway.cpp:31-174) and
+returns its outputs (mexgateway.cpp:261-370), computed by the HIP library
+through the C ABI of include/steadyscan.h.  Matrices are numpy arrays in
+MATLAB orientation (rows x columns); they are passed column-major.
+
+`init_settings`, `image2blockmatrix` and `resample_frame` mirror the
+MATLAB helpers that build the call's geometry and the corrected frames
+(steady_scan.m:748-828, 601-613, 562-599).
+"""
+import ctypes
+from ctypes import POINTER, byref, c_double, c_float, c_int32, c_uint8, c_void_p
+
+import numpy as np
+
+from . import _native as N
+
+
+class SteadyScanError(RuntimeError):
+    """An error of the library; `identifier` is the mex's GPUMC:* id."""
+
+    def __init__(self, status, message):
+        self.status = int(status)
+        self.identifier = N.STATUS_IDS.get(self.status, "GPUMC:Unknown")
+        super().__init__(f"{self.identifier}: {message}")
+
+
+def _check(status, handle=None, what=""):
+    if status != N.SS_OK:
+        msg = ""
+        if handle is not None and handle.ptr:
+            m = N.lib().ss_last_error(handle.ptr)
+            msg = m.decode() if m else ""
+        raise SteadyScanError(status, msg or what or f"status {status}")
+
+
+def _f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(POINTER(ctype))
+
+
+class Handle:
+    """Owns a device context and its resident buffers (ss_create/ss_destroy)."""
+
+    def __init__(self, device=0):
+        self.ptr = c_void_p()
+        self.device = int(device)
+        _check(N.lib().ss_create(self.device, byref(self.ptr)), what="ss_create failed (no gfx950 device?)")
+
+    def close(self):
+        if self.ptr:
+            N.lib().ss_destroy(self.ptr)
+            self.ptr = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_template(self, template_image, xgrad, ygrad):
+        t, gx, gy = _f64(template_image), _f64(xgrad), _f64(ygrad)
+        tid = c_int32()
+        _check(N.lib().ss_upload_template(self.ptr, t.shape[0], t.shape[1], _ptr(t, c_double),
+                                          _ptr(gx, c_double), _ptr(gy, c_double), byref(tid)), self)
+        return tid.value
+
+    def upload_geometry(self, tfrac, basex, basey, base_mask):
+        f, bx, by = _f64(tfrac), _f64(basex), _f64(basey)
+        bm = np.asfortranarray(np.asarray(base_mask, dtype=np.uint8))
+        gid = c_int32()
+        _check(N.lib().ss_upload_geometry(self.ptr, f.shape[0], f.shape[1], _ptr(f, c_double),
+                                          _ptr(bx, c_double), _ptr(by, c_double), _ptr(bm, c_uint8),
+                                          byref(gid)), self)
+        return gid.value
+
+    def release(self, rid):
+        _check(N.lib().ss_release(self.ptr, int(rid)), self)
+
+
+_handles = {}
+
+
+def _default_handle(device):
+    h = _handles.get(device)
+    if h is None:
+        h = Handle(device)
+        _handles[device] = h
+    return h
+
+
+def default_settings():
+    s = N.SSSettings()
+    N.lib().ss_settings_default(byref(s))
+    return s
+
+
+def parse_flags(flags):
+    """Precision letter and diagnostics of the flags string (configuration.cpp:83-152)."""
+    precision = None
+    opts = set()
+    for ch in flags:
+        if ch in "SD":
+            precision = ch
+        elif ch in "MRrp":
+            raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, f"flag '{ch}' is not supported")
+        else:
+            opts.add(ch)
+    if precision is None:
+        raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT,
+                              "invalid precision type! use 'S' for single-precision and 'D' for double-precision.")
+    return precision, opts
+
+
+_DTYPES = {np.dtype(np.float64): N.SS_F64, np.dtype(np.float32): N.SS_F32,
+           np.dtype(np.uint16): N.SS_U16, np.dtype(np.int16): N.SS_I16}
+
+
+def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, basey, base_mask,
+                            bmData, est_dx_knots, est_dy_knots, group_size, nsetframes,
+                            min_points_per_block, max_iter, move_thresh, halt_correlation,
+                            correlation_increase_threshold, filter_strength, max_filter_iterations,
+                            filter_deactivation_threshold, lambda_, linesearch_reduction_multiplier,
+                            solver_id, filter_id, gpu_device_id, *, nargout=6, handle=None,
+                            linesearch_candidates=4, info=None):
+    """The mex call (MexGateway<T>::execute, mexgateway.cpp:373-445).
+
+    Returns the first `nargout` of
+    (block_dx_knots, block_dy_knots, frame_corr, block_iter_used, block_mask,
+     residrms, deviceInitTime, iterationCorrelations, ..., allParams).
+    `info`, if a dict, receives the stop criterion and work counters.
+    """
+    if nargout < 1 or nargout > 15:
+        raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, "invalid number of lhs arguments specified")
+    precision, opts = parse_flags(flags)
+    h = handle if handle is not None else _default_handle(int(gpu_device_id))
+    s = default_settings()
+    s.max_iterations = int(max_iter)
+    s.move_threshold = float(move_thresh)
+    s.halt_correlation = float(halt_correlation)
+    s.correlation_increase_threshold = float(correlation_increase_threshold)
+    s.filter_strength = float(filter_strength)
+    s.max_filter_iterations = int(max_filter_iterations)
+    s.filter_correlation_threshold = float(filter_deactivation_threshold)
+    s.lambda_ = float(lambda_)
+    s.linesearch_reduction_multiplier = float(linesearch_reduction_multiplier)
+    s.solver_id = int(solver_id)
+    s.filter_id = int(filter_id)
+    s.device_id = int(gpu_device_id)
+    s.precision = precision.encode()
+    s.linesearch_candidates = int(linesearch_candidates)
+
+    gi = N.SSGroupIn()
+    keep = []
+    tmpl_id = template_image if isinstance(template_image, int) else 0
+    geom_id = tfrac if isinstance(tfrac, int) else 0
+    if tmpl_id:
+        gi.template_id = tmpl_id
+    else:
+        t, gx, gy = _f64(template_image), _f64(xgrad), _f64(ygrad)
+        keep += [t, gx, gy]
+        gi.template_h, gi.template_w = t.shape
+        gi.template_image, gi.xgrad, gi.ygrad = _ptr(t, c_double), _ptr(gx, c_double), _ptr(gy, c_double)
+    if geom_id:
+        gi.geometry_id = geom_id
+        fh, nb = np.shape(basex) if not isinstance(basex, int) else (None, None)
+    else:
+        f, bx, by = _f64(tfrac), _f64(basex), _f64(basey)
+        bm = np.asfortranarray(np.asarray(base_mask, dtype=np.uint8))
+        keep += [f, bx, by, bm]
+        gi.frac, gi.base_x, gi.base_y, gi.base_mask = (_ptr(f, c_double), _ptr(bx, c_double),
+                                                       _ptr(by, c_double), _ptr(bm, c_uint8))
+        fh, nb = bx.shape
+    if isinstance(template_image, int):
+        gi.template_h, gi.template_w = np.shape(xgrad)
+    if fh is None:
+        fh, nb = np.shape(base_mask)
+    nF = int(nsetframes)
+    gi.frame_h, gi.n_blocks, gi.n_frames, gi.group_size = int(fh), int(nb), nF, int(group_size)
+    # frame data: host array (any of f64/f32/u16/i16) or (device_ptr, dtype) tuple
+    if isinstance(bmData, tuple):
+        ptr, dt = bmData
+        gi.data = c_void_p(int(ptr))
+        gi.data_type = _DTYPES[np.dtype(dt)]
+        gi.data_on_device = 1
+    else:
+        d = np.asarray(bmData)
+        if d.dtype not in _DTYPES:
+            d = d.astype(np.float64)
+        d = np.asfortranarray(d)
+        if d.shape != (fh, nb * nF):
+            raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, f"bmData must be {fh} x {nb * nF}")
+        keep.append(d)
+        gi.data = c_void_p(d.ctypes.data)
+        gi.data_type = _DTYPES[d.dtype]
+    ex, ey = _f64(est_dx_knots), _f64(est_dy_knots)
+    if ex.shape[0] != nb + 1 or ex.shape[1] < nF or ey.shape != ex.shape:
+        raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, "est knots must be (n_blocks+1) x >= nFrames")
+    mp = np.ascontiguousarray(np.trunc(np.asarray(min_points_per_block, dtype=np.float64)).astype(np.int32).ravel())
+    if mp.size < nb + 1:
+        raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, "min_points_per_block needs nknots entries")
+    keep += [ex, ey, mp]
+    gi.init_x, gi.init_y, gi.min_points = _ptr(ex, c_double), _ptr(ey, c_double), _ptr(mp, c_int32)
+
+    go = N.SSGroupOut()
+    kx = np.zeros((nb + 1) * nF)
+    ky = np.zeros((nb + 1) * nF)
+    fc = np.zeros(nF)
+    rr = np.zeros(nF)
+    go.knots_x, go.knots_y, go.frame_corr, go.residrms = (_ptr(kx, c_double), _ptr(ky, c_double),
+                                                          _ptr(fc, c_double), _ptr(rr, c_double))
+    mask = None
+    if nargout >= 5:
+        mask = np.zeros((fh, nb * nF), dtype=np.uint8, order="F")
+        go.mask = _ptr(mask, c_uint8)
+    icorr = np.zeros(int(max_iter)) if nargout >= 8 else None
+    if icorr is not None:
+        go.iteration_corr = _ptr(icorr, c_double)
+    allp = np.zeros((2 * (nb * nF + 1), int(max_iter)), order="F") if nargout >= 15 else None
+    if allp is not None:
+        go.all_params = _ptr(allp, c_double)
+    _check(N.lib().ss_align_group(h.ptr, byref(gi), byref(s), byref(go)), h)
+    del keep
+    if info is not None:
+        info.update(stop_criterion=N.STOP_NAMES[go.stop_criterion], iterations=go.iterations,
+                    half_iterations=go.half_iterations, linesearch_rounds=go.linesearch_rounds,
+                    linesearch_tries=go.linesearch_tries)
+    if "E" in opts:
+        print(f"Execution terminated:{N.STOP_NAMES[go.stop_criterion]}\niterations: {go.iterations}")
+    outs = [kx.reshape(-1, 1), ky.reshape(-1, 1), fc.reshape(-1, 1), np.int32(go.iterations),
+            None if mask is None else mask.astype(bool), rr.reshape(-1, 1), go.device_init_time,
+            icorr if (icorr is not None and "C" in opts) else np.zeros((0, 0))]
+    outs += [np.zeros((0, 0))] * 6
+    outs.append(allp if allp is not None else np.zeros((0, 0)))
+    return tuple(outs[:nargout])
+
+
+# ---------------------------------------------------------------------------
+# geometry and corrected frames
+# ---------------------------------------------------------------------------
+def geometry_desc(nlines, linewidth, template_shape, nknots=33, cropping=None, subrec=None,
+                  isbidirectional=False, min_blockpoints_ratio=0.3):
+    g = N.SSGeometryDesc()
+    g.nlines, g.linewidth = int(nlines), int(linewidth)
+    g.template_h, g.template_w = int(template_shape[0]), int(template_shape[1])
+    g.nknots = int(nknots)
+    for i, v in enumerate(cropping if cropping is not None else (0, 0, 0, 0)):
+        g.cropping[i] = int(v)
+    if subrec is not None:
+        g.has_subrec = 1
+        for i, v in enumerate(subrec):
+            g.subrec[i] = float(v)
+    g.bidirectional = 1 if isbidirectional else 0
+    g.min_blockpoints_ratio = float(min_blockpoints_ratio)
+    return g
+
+
+def init_settings(nlines, linewidth, template_shape, **kw):
+    """init_settings + the block-matrix packing of estimate_displacements.
+
+    Returns a dict with the image-shaped prevknotind/tfrac/basex/basey
+    (steady_scan.m:813-824) and the block matrices tfrac_bm, basex_bm
+    (= basex - 1), basey_bm, base_mask_bm and min_points_per_block
+    (steady_scan.m:170-175)."""
+    g = geometry_desc(nlines, linewidth, template_shape, **kw)
+    hc, wc, fh = c_int32(), c_int32(), c_int32()
+    _check(N.lib().ss_geometry_size(byref(g), byref(hc), byref(wc), byref(fh)), what="invalid geometry")
+    hc, wc, fh = hc.value, wc.value, fh.value
+    nb = g.nknots - 1
+    out = dict(
+        prevknotind=np.zeros((hc, wc), np.int32, order="F"), tfrac=np.zeros((hc, wc), order="F"),
+        basex=np.zeros((hc, wc), order="F"), basey=np.zeros((hc, wc), order="F"),
+        tfrac_bm=np.zeros((fh, nb), order="F"), basex_bm=np.zeros((fh, nb), order="F"),
+        basey_bm=np.zeros((fh, nb), order="F"), base_mask_bm=np.zeros((fh, nb), np.uint8, order="F"),
+        min_points_per_block=np.zeros(g.nknots, np.int32))
+    _check(N.lib().ss_geometry_init(
+        byref(g), _ptr(out["prevknotind"], c_int32), _ptr(out["tfrac"], c_double),
+        _ptr(out["basex"], c_double), _ptr(out["basey"], c_double), _ptr(out["tfrac_bm"], c_double),
+        _ptr(out["basex_bm"], c_double), _ptr(out["basey_bm"], c_double),
+        _ptr(out["base_mask_bm"], c_uint8), _ptr(out["min_points_per_block"], c_int32)),
+        what="invalid geometry")
+    out["desc"] = g
+    out["frame_h"] = fh
+    out["nknots"] = g.nknots
+    return out
+
+
+def image2blockmatrix(frames, geometry):
+    """Packs raw frames (H x W x n) into the block matrix frame_h x (nb*n)
+    (steady_scan.m:229-234, 601-613)."""
+    fr = np.asarray(frames)
+    if fr.ndim == 2:
+        fr = fr[:, :, None]
+    if fr.dtype not in _DTYPES:
+        fr = fr.astype(np.float64)
+    fr = np.asfortranarray(fr)
+    g = geometry["desc"]
+    n = fr.shape[2]
+    nb = g.nknots - 1
+    out = np.zeros((geometry["frame_h"], nb * n), dtype=fr.dtype, order="F")
+    _check(N.lib().ss_pack_frames(byref(g), _DTYPES[fr.dtype], c_void_p(fr.ctypes.data), n,
+                                  c_void_p(out.ctypes.data)), what="pack failed")
+    return out
+
+
+def resample_frames(frames, geometry, dx_knots, dy_knots, handle=None, device=0):
+    """resample_frame for every frame (steady_scan.m:562-572): forward splat
+    into template space, float32, NaN where no data landed."""
+    fr = np.asarray(frames)
+    if fr.ndim == 2:
+        fr = fr[:, :, None]
+    if fr.dtype not in _DTYPES:
+        fr = fr.astype(np.float64)
+    fr = np.asfortranarray(fr)
+    g = geometry["desc"]
+    n = fr.shape[2]
+    kx, ky = _f64(np.reshape(dx_knots, (g.nknots, n), order="F")), _f64(np.reshape(dy_knots, (g.nknots, n), order="F"))
+    h = handle if handle is not None else _default_handle(device)
+    ri = N.SSResampleIn()
+    ri.geometry = ctypes.pointer(g)
+    ri.n_frames = n
+    ri.data_type = _DTYPES[fr.dtype]
+    ri.frames = c_void_p(fr.ctypes.data)
+    ri.knots_x, ri.knots_y = _ptr(kx, c_double), _ptr(ky, c_double)
+    out = np.zeros((g.template_h, g.template_w, n), dtype=np.float32, order="F")
+    _check(N.lib().ss_resample(h.ptr, byref(ri), out.ctypes.data_as(POINTER(c_float))), h)
+    return out

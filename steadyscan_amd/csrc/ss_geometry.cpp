@@ -1,0 +1,237 @@
+// ss_geometry.cpp -- host-side scan geometry: the parts of steady_scan.m that
+// build align_group_to_template's geometry arguments (init_settings,
+// steady_scan.m:748-828; the block-matrix packing of estimate_displacements,
+// steady_scan.m:166-175, 229-234; image2blockmatrix, steady_scan.m:601-613)
+// and default_alignment_settings (steady_scan.m:836-867).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "steadyscan.h"
+
+namespace {
+
+// MATLAB linspace(a, b, n): a + (0:n-1).*(b-a)/(n-1) with both ends exact.
+double matlab_linspace(double a, double b, int n, int i) {
+    if (n == 1 || i == n - 1) return b;
+    if (i == 0) return a;
+    return a + ((double)i * (b - a)) / (double)(n - 1);
+}
+
+struct ScanTiming {
+    int H = 0, W = 0, Hc = 0, Wc = 0, nknots = 0;
+    int c_left = 0, c_right = 0, c_bottom = 0, c_top = 0;
+    bool bidirectional = false;
+    std::vector<double> knot_time;
+
+    // Normalised time of raw pixel (line i, column j), 0-based
+    // (pixelscantime, steady_scan.m:790-795).
+    double pixel_time(int i, int j) const {
+        const double npix = (double)H * (double)W;
+        const int jj = (bidirectional && (i % 2 == 1)) ? (W - 1 - j) : j;
+        return (1.0 / npix) / 2 + (double)((long long)i * W + jj) / npix;
+    }
+    // Time of cropped pixel (i, j).
+    double cropped_time(int i, int j) const { return pixel_time(i + c_bottom, j + c_left); }
+
+    bool build(const ss_geometry_desc &g) {
+        H = g.nlines;
+        W = g.linewidth;
+        c_left = g.cropping[0];
+        c_right = g.cropping[1];
+        c_bottom = g.cropping[2];
+        c_top = g.cropping[3];
+        Hc = H - c_bottom - c_top;
+        Wc = W - c_left - c_right;
+        nknots = g.nknots;
+        bidirectional = g.bidirectional != 0;
+        if (H < 1 || W < 1 || Hc < 1 || Wc < 1 || nknots < 2) return false;
+        if (c_left < 0 || c_right < 0 || c_bottom < 0 || c_top < 0) return false;
+        knot_time.assign(nknots, 0.0);
+        const double dwell = 1.0 / ((double)H * (double)W);
+        const double lines_per_knot = (double)Hc / (double)(nknots - 1);
+        if (lines_per_knot == std::floor(lines_per_knot)) {
+            // knots at the first pixel of every lines_per_knot-th raw line
+            // (steady_scan.m:801-804)
+            const int L = (int)lines_per_knot;
+            for (int k = 0; k + 1 < nknots; k++) {
+                double mn = pixel_time(k * L, 0);
+                for (int j = 1; j < W; j++) mn = std::fmin(mn, pixel_time(k * L, j));
+                knot_time[k] = mn - dwell / 2;
+            }
+            knot_time[nknots - 1] = 1;
+            knot_time[0] = 0;
+        } else {
+            // equal spacing over the cropped scan (steady_scan.m:805-811)
+            double lo = cropped_time(0, 0), hi = lo;
+            for (int i = 0; i < Hc; i++)
+                for (int j = 0; j < Wc; j++) {
+                    const double t = cropped_time(i, j);
+                    lo = std::fmin(lo, t);
+                    hi = std::fmax(hi, t);
+                }
+            for (int k = 0; k < nknots; k++)
+                knot_time[k] = matlab_linspace(lo - dwell / 2, hi + dwell / 2, nknots, k);
+            knot_time[0] = 0;
+            knot_time[nknots - 1] = 1;
+        }
+        return true;
+    }
+
+    // histc bin of t against the knot times (1-based; 0 outside).
+    int bin(double t) const {
+        const int n = nknots;
+        if (!(t >= knot_time[0]) || t > knot_time[n - 1]) return 0;
+        if (t == knot_time[n - 1]) return n;
+        int lo = 0, hi = n - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (knot_time[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        return lo + 1;
+    }
+};
+
+void subrec_of(const ss_geometry_desc &g, int Hc, int Wc, double sr[4]) {
+    if (g.has_subrec) {
+        for (int i = 0; i < 4; i++) sr[i] = g.subrec[i];
+    } else if (Hc == g.template_h && Wc == g.template_w) {  // steady_scan.m:777-778
+        sr[0] = 1 - g.cropping[0];
+        sr[1] = g.template_w + g.cropping[1];
+        sr[2] = 1 - g.cropping[2];
+        sr[3] = g.template_h + g.cropping[3];
+    } else {  // steady_scan.m:781
+        sr[0] = 1;
+        sr[1] = g.template_w;
+        sr[2] = 1;
+        sr[3] = g.template_h;
+    }
+}
+
+}  // namespace
+
+extern "C" void ss_settings_default(ss_settings *s) {
+    if (!s) return;
+    std::memset(s, 0, sizeof(*s));
+    s->max_iterations = 120;
+    s->move_threshold = 0.010;
+    s->halt_correlation = 0.995;
+    s->correlation_increase_threshold = -1;
+    s->filter_strength = 0.8;
+    s->max_filter_iterations = 10;
+    s->filter_correlation_threshold = 0.01;
+    s->lambda = 0;
+    s->linesearch_reduction_multiplier = 0.5;
+    s->solver_id = 2;
+    s->filter_id = 1;
+    s->device_id = 0;
+    s->precision = 'D';
+    s->linesearch_candidates = 4;
+}
+
+extern "C" ss_status ss_geometry_size(const ss_geometry_desc *g, int32_t *hc, int32_t *wc,
+                                      int32_t *frame_h) {
+    if (!g) return SS_ERR_INVALID_ARGUMENT;
+    ScanTiming st;
+    if (!st.build(*g)) return SS_ERR_INVALID_ARGUMENT;
+    // The default basey is built over the cropped line count and cropped
+    // again (steady_scan.m:799, 822-824): vertical cropping mis-sizes it.
+    if (st.c_bottom + st.c_top != 0) return SS_ERR_INVALID_ARGUMENT;
+    std::vector<int> count(st.nknots - 1, 0);
+    for (int j = 0; j < st.Wc; j++)
+        for (int i = 0; i < st.Hc; i++) {
+            const int k = st.bin(st.cropped_time(i, j));
+            if (k <= 0 || k >= st.nknots) return SS_ERR_INVALID_ARGUMENT;  // steady_scan.m:814
+            count[k - 1]++;
+        }
+    int fh = 0;
+    for (int c : count) fh = c > fh ? c : fh;
+    if (hc) *hc = st.Hc;
+    if (wc) *wc = st.Wc;
+    if (frame_h) *frame_h = fh;
+    return SS_OK;
+}
+
+extern "C" ss_status ss_geometry_init(const ss_geometry_desc *g, int32_t *prevknotind,
+                                      double *tfrac, double *basex, double *basey,
+                                      double *frac_bm, double *base_x_bm, double *base_y_bm,
+                                      uint8_t *base_mask_bm, int32_t *min_points) {
+    int32_t Hc, Wc, fh;
+    ss_status rc = ss_geometry_size(g, &Hc, &Wc, &fh);
+    if (rc != SS_OK) return rc;
+    ScanTiming st;
+    st.build(*g);
+    double sr[4];
+    subrec_of(*g, Hc, Wc, sr);
+    const int nb = st.nknots - 1;
+    std::vector<int> fill(nb, 0), per_knot(st.nknots, 0);
+    const size_t bm = (size_t)fh * nb;
+    if (frac_bm) std::memset(frac_bm, 0, sizeof(double) * bm);
+    if (base_x_bm) std::memset(base_x_bm, 0, sizeof(double) * bm);
+    if (base_y_bm) std::memset(base_y_bm, 0, sizeof(double) * bm);
+    if (base_mask_bm) std::memset(base_mask_bm, 0, bm);
+    // column-major walk of the cropped image: the order of x(blockind == b)
+    for (int j = 0; j < Wc; j++) {
+        // meshgrid(linspace(0,1,linewidth)*diff(subrec(1:2)) + subrec(1), ...)
+        // cropped by columns (steady_scan.m:820-823)
+        const double bx = matlab_linspace(0, 1, st.W, j + st.c_left) * (sr[1] - sr[0]) + sr[0];
+        for (int i = 0; i < Hc; i++) {
+            const double by = matlab_linspace(0, 1, Hc, i) * (sr[3] - sr[2]) + sr[2];
+            const double t = st.cropped_time(i, j);
+            const int k = st.bin(t);
+            const double tf = (t - st.knot_time[k - 1]) / (st.knot_time[k] - st.knot_time[k - 1]);
+            const size_t o = (size_t)j * Hc + i;
+            if (prevknotind) prevknotind[o] = k;
+            if (tfrac) tfrac[o] = tf;
+            if (basex) basex[o] = bx;
+            if (basey) basey[o] = by;
+            per_knot[k - 1]++;
+            const size_t q = (size_t)(k - 1) * fh + fill[k - 1]++;
+            if (frac_bm) frac_bm[q] = tf;
+            if (base_x_bm) base_x_bm[q] = bx - 1;   // basex - 1 (steady_scan.m:171)
+            if (base_y_bm) base_y_bm[q] = by - 1;
+            if (base_mask_bm) base_mask_bm[q] = 1;
+        }
+    }
+    if (min_points) {
+        // histc(prevknotind, 1:nknots) * ratio, truncated toward zero by the
+        // gateway's int conversion (steady_scan.m:174-175, mexconverter.cpp:40-59)
+        for (int k = 0; k < st.nknots; k++)
+            min_points[k] = (int32_t)((double)per_knot[k] * g->min_blockpoints_ratio);
+    }
+    return SS_OK;
+}
+
+extern "C" ss_status ss_pack_frames(const ss_geometry_desc *g, ss_dtype type, const void *frames,
+                                    int32_t n, void *out) {
+    int32_t Hc, Wc, fh;
+    ss_status rc = ss_geometry_size(g, &Hc, &Wc, &fh);
+    if (rc != SS_OK) return rc;
+    if (!frames || !out || n < 1) return SS_ERR_INVALID_ARGUMENT;
+    ScanTiming st;
+    st.build(*g);
+    const int nb = st.nknots - 1;
+    size_t esz = type == SS_F64 ? 8 : type == SS_F32 ? 4 : 2;
+    std::vector<int> bin_of((size_t)Hc * Wc);
+    for (int j = 0; j < Wc; j++)
+        for (int i = 0; i < Hc; i++) bin_of[(size_t)j * Hc + i] = st.bin(st.cropped_time(i, j)) - 1;
+    const size_t per_frame_out = (size_t)fh * nb;
+    const char *src = (const char *)frames;
+    char *dst = (char *)out;
+    std::memset(out, 0, per_frame_out * n * esz);
+    for (int f = 0; f < n; f++) {
+        std::vector<int> fill(nb, 0);
+        const char *fr = src + (size_t)f * g->nlines * g->linewidth * esz;
+        char *fo = dst + (size_t)f * per_frame_out * esz;
+        for (int j = 0; j < Wc; j++)
+            for (int i = 0; i < Hc; i++) {
+                const int b = bin_of[(size_t)j * Hc + i];
+                // cropimage (steady_scan.m:830-834) then column-major order
+                const size_t si = (size_t)(j + g->cropping[0]) * g->nlines + (size_t)(i + g->cropping[2]);
+                const size_t di = (size_t)b * fh + fill[b]++;
+                std::memcpy(fo + di * esz, fr + si * esz, esz);
+            }
+    }
+    return SS_OK;
+}
