@@ -150,6 +150,27 @@ ss_status ss_align_group(ss_handle *h, const ss_group_in *in, const ss_settings 
 ss_status ss_align_batch(ss_handle *h, int32_t n, const ss_group_in *in,
                          const ss_settings *s, ss_group_out *out);
 
+/* ---- kernel timing ---- */
+
+/* Kernel classes timed by ss_profile_* (HIP events on the handle's stream). */
+typedef enum {
+    SS_K_ASSEMBLE = 0,   /* fused warp / residual / J^T J / J^T d pass       */
+    SS_K_SOLVE = 1,      /* FP64 block-tridiagonal solve + filter + expand   */
+    SS_K_LS_ROUND = 2,   /* fused line-search candidates pass                */
+    SS_K_LS_DECIDE = 3,  /* sequential resolution of the candidates          */
+    SS_K_CORR = 4,       /* global correlation pass + reduction              */
+    SS_K_FINAL = 5,      /* final errval / frame correlation / mask pass     */
+    SS_K_COUNT = 6
+} ss_kernel_class;
+
+/* Enables (1) or disables (0) per-launch HIP event timing and clears the
+ * counters.  No reference counterpart (the 't' flag's host timers,
+ * timedmainiteration.cpp:46-145, synchronised the device per stage). */
+ss_status ss_profile_enable(ss_handle *h, int32_t on);
+/* Launch counts, summed kernel milliseconds and summed algorithmic frame
+ * bytes (frame pixels read x element size) per kernel class. */
+ss_status ss_profile_read(ss_handle *h, int64_t *launches, double *ms, double *frame_bytes);
+
 /* ---- host-side geometry: init_settings + image2blockmatrix ---- */
 
 /* Scan geometry (settings fields of steady_scan.m:748-828, 836-867). */

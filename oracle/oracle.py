@@ -72,6 +72,7 @@ def lib():
         L.orc_state_set_delta.argtypes = [c_void_p, PD]
         L.orc_half_iteration_assemble.argtypes = [c_void_p, c_int32, POINTER(c_uint8), PD, PD, PD]
         L.orc_half_iteration_assemble.restype = c_int32
+        L.orc_state_pixels.argtypes = [c_void_p, POINTER(c_uint8), PD, PD, PD]
         L.orc_half_iteration_linesearch.argtypes = [c_void_p, c_int32]
         L.orc_half_iteration_linesearch.restype = c_int32
         L.orc_global_correlation.argtypes = [c_void_p]
@@ -208,6 +209,15 @@ class State:
         n = lib().orc_half_iteration_assemble(self.ptr, shift, used.ctypes.data_as(POINTER(c_uint8)),
                                               _p(jtr), _p(h), _p(dn))
         return dict(n_used=n, used=used.astype(bool), jtr=jtr, h=h.reshape(self.N, 6), delta=dn)
+
+    def pixels(self):
+        fh, nc = self.inp.fh, self.inp.nb * self.inp.nF
+        m = np.zeros((fh, nc), np.uint8, order="F")
+        x = np.zeros((fh, nc), order="F")
+        y = np.zeros((fh, nc), order="F")
+        d = np.zeros((fh, nc), order="F")
+        lib().orc_state_pixels(self.ptr, m.ctypes.data_as(POINTER(c_uint8)), _p(x), _p(y), _p(d))
+        return m.astype(bool), x, y, d
 
     def linesearch(self, shift):
         return bool(lib().orc_half_iteration_linesearch(self.ptr, shift))

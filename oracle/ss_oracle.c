@@ -858,6 +858,16 @@ void orc_state_get(const orc_state *st, double *p, double *delta_p, double *errv
     if (delta_p) memcpy(delta_p, st->deltaP, sizeof(double) * (size_t)st->nParams);
     if (errval) memcpy(errval, st->errval, sizeof(double) * (size_t)st->nFrames * 2);
 }
+void orc_state_pixels(const orc_state *st, uint8_t *mask, double *x, double *y, double *d) {
+    for (int c = 0; c < st->fframeW; c++)
+        for (int r = 0; r < st->frameH; r++) {
+            size_t cm = (size_t)c * st->frameH + r, rm = IDX2R(r, c, st->fframeW);
+            if (mask) mask[cm] = st->mask[rm];
+            if (x) x[cm] = st->x[rm];
+            if (y) y[cm] = st->y[rm];
+            if (d) d[cm] = st->d[rm];
+        }
+}
 void orc_state_set_p(orc_state *st, const double *p) {
     memcpy(st->p, p, sizeof(double) * (size_t)st->nParams);
 }

@@ -97,6 +97,9 @@ void orc_state_set_delta(orc_state *st, const double *delta_p);
  *   delta_new[2*nParamPoints] expanded step. */
 int orc_half_iteration_assemble(orc_state *st, int shift, uint8_t *used, double *jtr_full,
                                 double *h_full, double *delta_new);
+/* Per-pixel state (frame_h x n_blocks*n_frames, column-major): mask, x, y,
+ * differenceImage; any pointer may be NULL. */
+void orc_state_pixels(const orc_state *st, uint8_t *mask, double *x, double *y, double *d);
 /* The line search of that half-iteration; returns the `improved` flag. */
 int orc_half_iteration_linesearch(orc_state *st, int shift);
 /* Global correlation statistic of the current state (task_correlation.cu:63-99). */

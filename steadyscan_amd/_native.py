@@ -133,7 +133,11 @@ API = [
                                    POINTER(c_uint8), POINTER(c_int32)]),
     ("ss_pack_frames", c_int32, [POINTER(SSGeometryDesc), c_int32, c_void_p, c_int32, c_void_p]),
     ("ss_resample", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(ctypes.c_float)]),
+    ("ss_profile_enable", c_int32, [c_void_p, c_int32]),
+    ("ss_profile_read", c_int32, [c_void_p, POINTER(ctypes.c_int64), POINTER(c_double), POINTER(c_double)]),
 ]
+
+KERNEL_CLASSES = ["assemble", "solve", "ls_round", "ls_decide", "corr", "final"]
 
 SYNTH_API = [
     ("ss_synth_template", c_int32, [POINTER(SynthTemplateDesc), POINTER(c_double)]),

@@ -41,7 +41,7 @@ struct Status {
     int32_t need_more;    // some frame still needs another line-search round
     int32_t improved;     // `improved` of the last completed line search
     int32_t tries;        // sequential tries resolved (accumulated)
-    int32_t pad;
+    int32_t active;       // frames still unresolved after the last decide
 };
 
 enum ImgType { IMG_F32 = 0, IMG_U16 = 1, IMG_I16 = 2 };

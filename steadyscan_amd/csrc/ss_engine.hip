@@ -51,8 +51,23 @@ constexpr int kMaxRounds = 512;  // 4096 candidates at K = 8
 
 }  // namespace
 
+struct Profile {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    struct Pending {
+        int cls;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Pending> pending;
+    int64_t launches[SS_K_COUNT] = {};
+    double ms[SS_K_COUNT] = {};
+    double bytes[SS_K_COUNT] = {};
+};
+
 struct ss_handle {
     int device = 0;
+    Profile prof;
     hipStream_t stream = nullptr;
     std::string err;
     std::map<std::string, DevBuf> bufs;
@@ -68,6 +83,50 @@ namespace {
 ss_status fail(ss_handle *h, ss_status st, const std::string &msg) {
     if (h) h->err = msg;
     return st;
+}
+
+hipEvent_t prof_event(ss_handle *h) {
+    if (!h->prof.pool.empty()) {
+        hipEvent_t e = h->prof.pool.back();
+        h->prof.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Folds completed event pairs into the counters (synchronises the stream).
+void prof_collect(ss_handle *h) {
+    if (h->prof.pending.empty()) return;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    (void)e;
+    for (auto &p : h->prof.pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            h->prof.ms[p.cls] += ms;
+            h->prof.launches[p.cls] += 1;
+            h->prof.bytes[p.cls] += p.bytes;
+        }
+        h->prof.pool.push_back(p.a);
+        h->prof.pool.push_back(p.b);
+    }
+    h->prof.pending.clear();
+}
+
+// Runs one launcher, bracketed by events when profiling is on.
+template <typename F>
+hipError_t timed(ss_handle *h, int cls, double bytes, F &&launch) {
+    if (!h->prof.on) return launch();
+    hipEvent_t a = prof_event(h), b = prof_event(h);
+    if (!a || !b) return launch();
+    hipError_t e = hipEventRecord(a, h->stream);
+    if (e != hipSuccess) return e;
+    e = launch();
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(b, h->stream);
+    h->prof.pending.push_back({cls, a, b, bytes});
+    return e;
 }
 
 #define HIPCHK(expr)                                                                        \
@@ -248,6 +307,7 @@ struct Loop {
     ss::GroupDev g;
     const ss_settings *s;
     int K;
+    double frameBytes = 0;  // algorithmic bytes of one frame (pixels x element size)
     // filter lifecycle (task.cpp:54-70, mainiteration.cpp:80-115, task.cpp:313-325)
     int solverFilter = 0, filterID = 0, nFilterIterations = 0;
     int halfIterations = 0, rounds = 0;
@@ -259,14 +319,16 @@ struct Loop {
     }
 
     ss_status linesearch(int shift, int *improved) {
+        int active = g.nF;
         for (int round = 0;; round++) {
             if (round >= kMaxRounds)
                 return fail(h, SS_ERR_NUMERICAL, "line search did not reach the movement threshold (non-finite step?)");
-            HIPCHK(ss::launch_ls_round(g, K, h->stream));
-            HIPCHK(ss::launch_ls_decide(g, shift, K, s->move_threshold, h->stream));
+            HIPCHK(timed(h, SS_K_LS_ROUND, (double)active * frameBytes, [&] { return ss::launch_ls_round(g, K, h->stream); }));
+            HIPCHK(timed(h, SS_K_LS_DECIDE, 0.0, [&] { return ss::launch_ls_decide(g, shift, K, s->move_threshold, h->stream); }));
             rounds++;
             ss_status st = read_status();
             if (st) return st;
+            active = h->hstatus->active;
             if (!h->hstatus->need_more) break;
         }
         *improved = h->hstatus->improved;
@@ -277,13 +339,13 @@ struct Loop {
     ss_status main_iteration(int iteration, int *improved_out) {
         int improved = 1;
         for (int shift = 0; shift < 2; shift++) {
-            HIPCHK(ss::launch_assemble(g, h->stream));
+            HIPCHK(timed(h, SS_K_ASSEMBLE, frameBytes * g.nF, [&] { return ss::launch_assemble(g, h->stream); }));
             if (nFilterIterations != 0 && iteration > nFilterIterations) {
                 solverFilter = 0;
                 filterID = 0;
                 nFilterIterations = 0;
             }
-            HIPCHK(ss::launch_solve(g, shift, solverFilter, s->filter_strength, s->lambda, h->stream));
+            HIPCHK(timed(h, SS_K_SOLVE, 0.0, [&] { return ss::launch_solve(g, shift, solverFilter, s->filter_strength, s->lambda, h->stream); }));
             int imp = 0;
             ss_status st = linesearch(shift, &imp);
             if (st) return st;
@@ -418,6 +480,7 @@ ss_status run_group(ss_handle *h, const ss_group_in *in, const ss_settings *s, s
     L.g = g;
     L.s = s;
     L.K = s->linesearch_candidates <= 0 ? 4 : s->linesearch_candidates;
+    L.frameBytes = (double)fh * nb * (imgType == ss::IMG_F32 ? 4.0 : 2.0);
     if (L.K != 1 && L.K != 2 && L.K != 4 && L.K != 8)
         return fail(h, SS_ERR_INVALID_ARGUMENT, "linesearch_candidates must be 1, 2, 4 or 8");
     L.filterID = s->filter_id;
@@ -437,7 +500,7 @@ ss_status run_group(ss_handle *h, const ss_group_in *in, const ss_settings *s, s
         if (out->all_params && iteration <= s->max_iterations)
             HIPCHK(hipMemcpyAsync(out->all_params + P * (iteration - 1), g.p, P * sizeof(double),
                                   hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(ss::launch_corr(g, h->stream));
+        HIPCHK(timed(h, SS_K_CORR, L.frameBytes * nF, [&] { return ss::launch_corr(g, h->stream); }));
         if ((st = L.read_status())) return st;
         corr = h->hstatus->corr;
         if (out->iteration_corr && iteration <= s->max_iterations) out->iteration_corr[iteration - 1] = corr;
@@ -471,7 +534,7 @@ ss_status run_group(ss_handle *h, const ss_group_in *in, const ss_settings *s, s
     }
     if (iteration == 1)
         HIPCHK(hipMemcpyAsync(g.pLast, g.p, P * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(ss::launch_final(g, h->stream));
+    HIPCHK(timed(h, SS_K_FINAL, L.frameBytes * nF, [&] { return ss::launch_final(g, h->stream); }));
     std::vector<double> pl(P);
     HIPCHK(hipMemcpyAsync(pl.data(), g.pLast, P * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (out->residrms)
@@ -486,6 +549,7 @@ ss_status run_group(ss_handle *h, const ss_group_in *in, const ss_settings *s, s
     out->half_iterations = L.halfIterations;
     out->linesearch_rounds = L.rounds;
     out->linesearch_tries = h->hstatus->tries;
+    if (h->prof.on) prof_collect(h);
     return SS_OK;
 }
 
@@ -527,6 +591,11 @@ void ss_destroy(ss_handle *h) {
         e = hipFree(kv.second.geo);
         if (kv.second.fnext) e = hipFree(kv.second.fnext);
     }
+    for (auto &p : h->prof.pending) {
+        e = hipEventDestroy(p.a);
+        e = hipEventDestroy(p.b);
+    }
+    for (auto ev : h->prof.pool) e = hipEventDestroy(ev);
     if (h->dstatus) e = hipFree(h->dstatus);
     if (h->hstatus) e = hipHostFree(h->hstatus);
     if (h->stream) e = hipStreamDestroy(h->stream);
@@ -590,6 +659,31 @@ ss_status ss_align_group(ss_handle *h, const ss_group_in *in, const ss_settings 
     HIPCHK(hipSetDevice(h->device));
     h->err.clear();
     return run_group(h, in, s, out);
+}
+
+ss_status ss_profile_enable(ss_handle *h, int32_t on) {
+    if (!h) return SS_ERR_INVALID_ARGUMENT;
+    HIPCHK(hipSetDevice(h->device));
+    prof_collect(h);
+    h->prof.on = on != 0;
+    for (int i = 0; i < SS_K_COUNT; i++) {
+        h->prof.launches[i] = 0;
+        h->prof.ms[i] = 0;
+        h->prof.bytes[i] = 0;
+    }
+    return SS_OK;
+}
+
+ss_status ss_profile_read(ss_handle *h, int64_t *launches, double *ms, double *frame_bytes) {
+    if (!h) return SS_ERR_INVALID_ARGUMENT;
+    HIPCHK(hipSetDevice(h->device));
+    prof_collect(h);
+    for (int i = 0; i < SS_K_COUNT; i++) {
+        if (launches) launches[i] = h->prof.launches[i];
+        if (ms) ms[i] = h->prof.ms[i];
+        if (frame_bytes) frame_bytes[i] = h->prof.bytes[i];
+    }
+    return SS_OK;
 }
 
 ss_status ss_align_batch(ss_handle *h, int32_t n, const ss_group_in *in, const ss_settings *s,

@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kSolveWG) void k_ls_decide(GroupDev g, int shift, i
                                                         double thr) {
     const int nF = g.nF, nb = g.nb, N = g.N;
     const double r = g.rmult;
-    int tries = 0;
+    int tries = 0, pendingCount = 0;
     bool pending = false;
     for (int f = threadIdx.x; f < nF; f += kSolveWG) {
         if (!g.lsActive[f]) continue;
@@ -563,17 +563,19 @@ __global__ __launch_bounds__(kSolveWG) void k_ls_decide(GroupDev g, int shift, i
         } else {
             g.lsNextK[f] = k0 + K;
             pending = true;
+            pendingCount++;
         }
     }
     // workgroup-wide: anything pending?  then the `improved` flag
     __shared__ int shp, sht;
     if (threadIdx.x == 0) { shp = 0; sht = 0; }
     __syncthreads();
-    if (pending) atomicOr(&shp, 1);
+    if (pending) atomicAdd(&shp, pendingCount);
     atomicAdd(&sht, tries);
     __syncthreads();
     if (threadIdx.x == 0) {
-        g.status->need_more = shp;
+        g.status->need_more = shp ? 1 : 0;
+        g.status->active = shp;
         g.status->tries += sht;
         if (!shp) {
             double e0 = 0.0, e1 = 0.0;

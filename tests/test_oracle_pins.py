@@ -1,0 +1,569 @@
+"""Pins of the FP64 oracle (CPU only).
+
+The reference ships no tests, fixtures or golden vectors, and its path does
+not build here except ParameterOrganizer (oracle/_ref).  Each oracle function
+is pinned by what the mathematics or the reference's own code fixes:
+  - knot packing: bit-exact against the reference's ParameterOrganizer
+    compiled from its sources, plus closed-form index tests;
+  - warp / residual / Jacobian / normal equations: a dense Jacobian built
+    independently in numpy on tiny inputs, with the bilinear derivative
+    checked by finite differences;
+  - solve: dense numpy solves of random SPD systems in the reference's
+    compacted COO layout; ILU(0) equals LU where the pattern has no fill;
+  - filters, expand rule, line-search structure: closed forms and invariants;
+  - the global correlation statistic: a hand-computed known-answer case;
+  - end to end: zero motion, global translation, linear motion;
+  - resampling: identity at zero displacement, MATLAB last-write-wins KAT.
+"""
+import ctypes
+from ctypes import POINTER, c_double, c_int32
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from steadyscan_amd import workloads as W
+import steadyscan_amd as ssa
+
+PD = POINTER(c_double)
+
+
+def _p(a):
+    return a.ctypes.data_as(PD)
+
+
+# ---------------------------------------------------------------------------
+# knot packing (A4, A22)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("nf,nb", [(1, 4), (2, 4), (5, 16), (50, 32), (3, 1), (7, 64)])
+def test_organize_matches_reference_build(nf, nb):
+    ref = O.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(nf * 100 + nb)
+    px = rng.standard_normal((nb + 1) * nf)
+    py = rng.standard_normal((nb + 1) * nf)
+    npp = nf * nb + 1
+    a, b = np.zeros(2 * npp), np.zeros(2 * npp)
+    ref.ref_organize_in(_p(px), _p(py), nf, nb, _p(a))
+    O.lib().orc_organize_in(_p(px), _p(py), nf, nb, _p(b))
+    assert np.array_equal(a, b)
+    p = rng.standard_normal(2 * npp)
+    o1 = [np.zeros((nb + 1) * nf) for _ in range(4)]
+    ref.ref_organize_out(_p(p), nf, nb, _p(o1[0]), _p(o1[1]))
+    O.lib().orc_organize_out(_p(p), nf, nb, _p(o1[2]), _p(o1[3]))
+    assert np.array_equal(o1[0], o1[2]) and np.array_equal(o1[1], o1[3])
+
+
+def test_organize_closed_form():
+    nf, nb = 3, 4
+    npp = nf * nb + 1
+    px = np.array([10.0 * f + i for f in range(nf) for i in range(nb + 1)])
+    py = -px
+    p = np.zeros(2 * npp)
+    O.lib().orc_organize_in(_p(px), _p(py), nf, nb, _p(p))
+    exp = np.zeros(npp)
+    for f in range(nf):
+        for i in range(nb):
+            exp[f * nb + i] = 10 * f + i
+        if f > 0:  # shared knot: mean of the previous frame's last and this frame's first
+            exp[f * nb] = ((10 * (f - 1) + nb) + (10 * f)) / 2
+    exp[npp - 1] = 10 * (nf - 1) + nb
+    assert np.array_equal(p[:npp], exp) and np.array_equal(p[npp:], -exp)
+    kx, ky = np.zeros(nf * (nb + 1)), np.zeros(nf * (nb + 1))
+    O.lib().orc_organize_out(_p(p), nf, nb, _p(kx), _p(ky))
+    for f in range(nf):
+        assert np.array_equal(kx[f * (nb + 1):f * (nb + 1) + nb], p[f * nb:f * nb + nb])
+        assert kx[f * (nb + 1) + nb] == p[f * nb + nb - 1]  # duplicate, not the shared knot
+
+
+# ---------------------------------------------------------------------------
+# geometry (A1, A2)
+# ---------------------------------------------------------------------------
+def oracle_geometry(H, W_, th, tw, nknots, crop=(0, 0, 0, 0), bidi=False, subrec=None):
+    g = O.OrcGeometryIn(nlines=H, linewidth=W_, template_h=th, template_w=tw, nknots=nknots,
+                        bidirectional=1 if bidi else 0, min_blockpoints_ratio=0.3)
+    for i, v in enumerate(crop):
+        g.cropping[i] = v
+    if subrec is not None:
+        g.has_subrec = 1
+        for i, v in enumerate(subrec):
+            g.subrec[i] = v
+    hc, wc = c_int32(), c_int32()
+    rc = O.lib().orc_geometry(ctypes.byref(g), None, None, None, None, ctypes.byref(hc), ctypes.byref(wc))
+    if rc:
+        return None
+    n = hc.value * wc.value
+    pk = np.zeros(n, np.int32)
+    tf, bx, by = np.zeros(n), np.zeros(n), np.zeros(n)
+    rc = O.lib().orc_geometry(ctypes.byref(g), pk.ctypes.data_as(POINTER(c_int32)), _p(tf), _p(bx),
+                              _p(by), ctypes.byref(hc), ctypes.byref(wc))
+    assert rc == 0
+    shp = (hc.value, wc.value)
+    return dict(prevknotind=pk.reshape(shp, order="F"), tfrac=tf.reshape(shp, order="F"),
+                basex=bx.reshape(shp, order="F"), basey=by.reshape(shp, order="F"))
+
+
+def test_geometry_closed_form_integer_spacing():
+    H, Wd, nk = 16, 8, 5
+    L = H // (nk - 1)
+    g = ssa.init_settings(H, Wd, (H, Wd), nknots=nk)
+    o = oracle_geometry(H, Wd, H, Wd, nk)
+    i, j = np.meshgrid(np.arange(H), np.arange(Wd), indexing="ij")
+    assert np.array_equal(g["prevknotind"], i // L + 1)
+    assert np.array_equal(o["prevknotind"], i // L + 1)
+    tfrac = ((i % L) * Wd + j + 0.5) / (L * Wd)
+    assert np.allclose(g["tfrac"], tfrac, atol=1e-13, rtol=0)
+    assert np.allclose(o["tfrac"], tfrac, atol=1e-13, rtol=0)
+    assert np.allclose(g["basex"], j + 1, atol=1e-12) and np.allclose(g["basey"], i + 1, atol=1e-12)
+    assert g["frame_h"] == L * Wd
+    # block matrix: column-major pixel order inside each block
+    for b in range(nk - 1):
+        r = np.arange(L * Wd)
+        jj, ii = r // L, r % L
+        assert np.allclose(g["tfrac_bm"][:, b], (ii * Wd + jj + 0.5) / (L * Wd), atol=1e-13)
+        assert np.allclose(g["basex_bm"][:, b], jj, atol=1e-12)
+        assert np.allclose(g["basey_bm"][:, b], b * L + ii, atol=1e-12)
+    assert g["base_mask_bm"].all()
+    assert list(g["min_points_per_block"]) == [int(0.3 * L * Wd)] * (nk - 1) + [0]
+
+
+@pytest.mark.parametrize("H,Wd,nk,crop,bidi,subrec", [
+    (16, 8, 5, (0, 0, 0, 0), False, None),
+    (15, 9, 5, (0, 0, 0, 0), False, None),      # non-integer lines per knot
+    (16, 8, 5, (0, 0, 0, 0), True, None),       # bidirectional
+    (17, 10, 4, (0, 0, 0, 0), True, None),
+    (16, 12, 5, (2, 1, 0, 0), False, None),     # horizontal cropping
+    (32, 32, 9, (0, 0, 0, 0), False, (-3.0, 40.0, 2.0, 30.0)),
+])
+def test_geometry_product_equals_oracle(H, Wd, nk, crop, bidi, subrec):
+    g = ssa.init_settings(H, Wd, (H, Wd), nknots=nk, cropping=crop, isbidirectional=bidi, subrec=subrec)
+    o = oracle_geometry(H, Wd, H, Wd, nk, crop, bidi, subrec)
+    for k in ("prevknotind", "tfrac", "basex", "basey"):
+        assert np.array_equal(g[k], o[k]), k
+    assert ((g["tfrac"] >= 0) & (g["tfrac"] < 1)).all()
+
+
+def test_geometry_vertical_crop_rejected():
+    with pytest.raises(ssa.SteadyScanError):
+        ssa.init_settings(16, 8, (16, 8), nknots=5, cropping=(0, 0, 1, 0))
+    assert oracle_geometry(16, 8, 16, 8, 5, (0, 0, 1, 0)) is None
+
+
+@pytest.mark.parametrize("H,Wd,nk,bidi", [(16, 8, 5, False), (15, 9, 5, True)])
+def test_pack_roundtrip_and_oracle(H, Wd, nk, bidi):
+    g = ssa.init_settings(H, Wd, (H, Wd), nknots=nk, isbidirectional=bidi)
+    rng = np.random.default_rng(3)
+    frames = np.asfortranarray(rng.standard_normal((H, Wd, 3)))
+    bm = ssa.image2blockmatrix(frames, g)
+    nb = nk - 1
+    pk = g["prevknotind"].ravel(order="F")
+    counts = np.bincount(pk - 1, minlength=nb)
+    assert bm.shape == (counts.max(), nb * 3)
+    for f in range(3):
+        img = frames[:, :, f].ravel(order="F")
+        back = np.zeros_like(img)
+        for b in range(nb):
+            sel = np.nonzero(pk == b + 1)[0]
+            back[sel] = bm[:len(sel), f * nb + b]
+            assert np.all(bm[len(sel):, f * nb + b] == 0)
+            y = np.zeros(counts.max())
+            m = np.zeros(counts.max(), np.uint8)
+        assert np.array_equal(back, img)
+        # oracle image2blockmatrix
+        fh = O.lib().orc_image2blockmatrix(None, pk.ctypes.data_as(POINTER(c_int32)), len(pk), nb, None, None)
+        y = np.zeros(fh * nb)
+        m = np.zeros(fh * nb, np.uint8)
+        O.lib().orc_image2blockmatrix(_p(np.ascontiguousarray(img)), pk.ctypes.data_as(POINTER(c_int32)),
+                                      len(pk), nb, _p(y), m.ctypes.data_as(POINTER(ctypes.c_uint8)))
+        assert np.array_equal(y.reshape(fh, nb, order="F"), bm[:, f * nb:(f + 1) * nb])
+        assert np.array_equal(m.reshape(fh, nb, order="F").astype(bool), g["base_mask_bm"].astype(bool))
+
+
+# ---------------------------------------------------------------------------
+# warp, residual, Jacobian and normal equations (A6-A13) against a dense J
+# ---------------------------------------------------------------------------
+def bilinear_np(T, x, y):
+    ix, iy = np.floor(x).astype(int), np.floor(y).astype(int)
+    fx, fy = x - ix, y - iy
+    return ((1 - fx) * ((1 - fy) * T[iy, ix] + fy * T[iy + 1, ix]) +
+            fx * ((1 - fy) * T[iy, ix + 1] + fy * T[iy + 1, ix + 1]))
+
+
+def bilinear_dx_np(T, x, y):
+    ix, iy = np.floor(x).astype(int), np.floor(y).astype(int)
+    fy = y - iy
+    return (1 - fy) * (T[iy, ix + 1] - T[iy, ix]) + fy * (T[iy + 1, ix + 1] - T[iy + 1, ix])
+
+
+def bilinear_dy_np(T, x, y):
+    ix, iy = np.floor(x).astype(int), np.floor(y).astype(int)
+    fx = x - ix
+    return (1 - fx) * (T[iy + 1, ix] - T[iy, ix]) + fx * (T[iy + 1, ix + 1] - T[iy, ix + 1])
+
+
+def tiny_problem(seed=5, size=12, nframes=3, nk=4, amp=1.2):
+    T = W.make_template(size, size, 6, (1.5, 3.0), offset=5.0, seed=seed)
+    dx, dy = W.make_motion(nframes, size, 0.05, 0.0, (0.5, 0.5), seed=seed + 1)
+    fr = W.make_frames(T, dx, dy, size, size, np.float64, True, seed=seed + 2)
+    geom = ssa.init_settings(size, size, T.shape, nknots=nk)
+    a = W.group_args(T, fr, geom, max_iter=5)
+    rng = np.random.default_rng(seed)
+    N = nframes * (nk - 1) + 1
+    p = rng.uniform(-amp, amp, 2 * N)
+    return T, a, p
+
+
+def test_bilinear_derivative_is_exact():
+    T = W.make_template(10, 10, 5, (1.5, 3.0), offset=5.0, seed=2)
+    rng = np.random.default_rng(0)
+    x = rng.uniform(0.05, 7.95, 200)
+    y = rng.uniform(0.05, 7.95, 200)
+    x = np.floor(x) + np.clip(x - np.floor(x), 0.05, 0.95)
+    y = np.floor(y) + np.clip(y - np.floor(y), 0.05, 0.95)
+    h = 1e-6
+    fdx = (bilinear_np(T, x + h, y) - bilinear_np(T, x - h, y)) / (2 * h)
+    fdy = (bilinear_np(T, x, y + h) - bilinear_np(T, x, y - h)) / (2 * h)
+    assert np.allclose(bilinear_dx_np(T, x, y), fdx, atol=1e-6)
+    assert np.allclose(bilinear_dy_np(T, x, y), fdy, atol=1e-6)
+    # the gradient arrays of steady_scan.m:178-179 give the same numbers
+    xg, yg = W.gradients(T)
+    ix, iy = np.floor(x).astype(int), np.floor(y).astype(int)
+    fx, fy = x - ix, y - iy
+    assert np.allclose((1 - fy) * xg[iy, ix] + fy * xg[iy + 1, ix], bilinear_dx_np(T, x, y), atol=1e-12)
+    assert np.allclose((1 - fx) * yg[iy, ix] + fx * yg[iy, ix + 1], bilinear_dy_np(T, x, y), atol=1e-12)
+
+
+@pytest.mark.parametrize("seed", [5, 6, 7])
+def test_normal_equations_against_dense_jacobian(seed):
+    T, a, p = tiny_problem(seed)
+    st = O.State(a)
+    st.set_p(p)
+    r = st.assemble(0)
+    mask, xs, ys, d = st.pixels()
+    fh, nb, nF = a["basex"].shape[0], a["basex"].shape[1], a["nsetframes"]
+    N = nb * nF + 1
+    th, tw = T.shape
+    # positions and mask (task_applyparameters.cuh:15-63), built independently
+    for c in range(nb * nF):
+        b = c % nb
+        f_ = a["tfrac"][:, b]
+        x = a["basex"][:, b] + p[c] * (1 - f_) + p[c + 1] * f_
+        y = a["basey"][:, b] + p[c + N] * (1 - f_) + p[c + N + 1] * f_
+        m = a["base_mask"][:, b].astype(bool) & (x >= 0) & (x < tw - 1) & (y >= 0) & (y < th - 1)
+        assert np.array_equal(m, mask[:, c])
+        assert np.allclose(xs[:, c], x, atol=1e-12) and np.allclose(ys[:, c], y, atol=1e-12)
+    # dense Jacobian over masked pixels
+    rows, resid = [], []
+    J = []
+    for c in range(nb * nF):
+        b = c % nb
+        f_ = a["tfrac"][:, b]
+        sel = mask[:, c]
+        x, y = xs[sel, c], ys[sel, c]
+        wI = bilinear_np(T, x, y)
+        dd = a["bmData"][sel, c] - wI
+        assert np.allclose(d[sel, c], dd, atol=1e-10)
+        assert np.all(d[~sel, c] == 0)
+        gx, gy = bilinear_dx_np(T, x, y), bilinear_dy_np(T, x, y)
+        jj = np.zeros((sel.sum(), 2 * N))
+        jj[:, c] = (1 - f_[sel]) * gx
+        jj[:, c + 1] = f_[sel] * gx
+        jj[:, c + N] = (1 - f_[sel]) * gy
+        jj[:, c + N + 1] = f_[sel] * gy
+        J.append(jj)
+        resid.append(dd)
+    J = np.vstack(J)
+    dvec = np.concatenate(resid)
+    H = J.T @ J
+    g = J.T @ dvec
+    # knot usage (task_createparametermask.cuh:12-20; the last knot stays unused)
+    cnt = mask.sum(axis=0)
+    minp = np.trunc(a["min_points_per_block"]).astype(int)
+    present = np.array([cnt[c] >= minp[c % nb] for c in range(nb * nF)])
+    used = np.zeros(N, bool)
+    used[0] = present[0]
+    used[1:nb * nF] = present[1:] | present[:-1]
+    assert np.array_equal(r["used"], used)
+    assert r["n_used"] == 2 * used.sum()
+    h = r["h"]
+    scale = np.abs(H).max()
+    for c in range(N - 1):
+        if not used[c]:
+            continue
+        assert abs(h[c, 0] - H[c, c]) <= 1e-9 * scale
+        assert abs(h[c, 1] - H[c + N, c + N]) <= 1e-9 * scale
+        assert abs(h[c, 2] - H[c, c + N]) <= 1e-9 * scale
+        if used[c + 1]:
+            assert abs(h[c, 3] - H[c, c + 1]) <= 1e-9 * scale
+            assert abs(h[c, 4] - H[c + N, c + N + 1]) <= 1e-9 * scale
+            assert abs(h[c, 5] - H[c, c + N + 1]) <= 1e-9 * scale
+            assert abs(h[c, 5] - H[c + 1, c + N]) <= 1e-9 * scale
+        assert abs(r["jtr"][c] - g[c]) <= 1e-9 * np.abs(g).max()
+        assert abs(r["jtr"][c + N] - g[c + N]) <= 1e-9 * np.abs(g).max()
+
+
+# ---------------------------------------------------------------------------
+# solve (A14)
+# ---------------------------------------------------------------------------
+def compacted_system(u, seed, coupled=True):
+    """Random SPD J^T J in the compacted [x...; y...] index space with the
+    block-tridiagonal pattern of d_fillDiagonals, as COO with INT_MAX gaps."""
+    rng = np.random.default_rng(seed)
+    n = 2 * u
+    J = []
+    for k in range(u):
+        for _ in range(6):
+            for axis in ([0, 1] if not coupled else [None]):
+                row = np.zeros(n)
+                if axis in (None, 0):
+                    row[k] = rng.standard_normal()
+                    if k + 1 < u:
+                        row[k + 1] = rng.standard_normal()
+                if axis in (None, 1):
+                    row[u + k] = rng.standard_normal()
+                    if k + 1 < u:
+                        row[u + k + 1] = rng.standard_normal()
+                J.append(row)
+    J = np.array(J)
+    H = J.T @ J
+    rr, cc = np.nonzero(np.abs(H) > 0)
+    rows, cols, vals = [], [], []
+    for i, j in zip(rr, cc):
+        rows.append(int(i)); cols.append(int(j)); vals.append(H[i, j])
+        rows.append(2**31 - 1); cols.append(0); vals.append(123.0)  # unused slot
+    perm = rng.permutation(len(rows))
+    return (H, np.array(rows, np.int32)[perm], np.array(cols, np.int32)[perm],
+            np.array(vals)[perm], rng.standard_normal(n))
+
+
+@pytest.mark.parametrize("u,seed", [(1, 0), (2, 1), (7, 2), (40, 3), (333, 4)])
+def test_exact_solve_matches_dense(u, seed):
+    H, r, c, v, b = compacted_system(u, seed)
+    x = np.zeros(2 * u)
+    O.lib().orc_solve_coo(2 * u, len(r), r.ctypes.data_as(POINTER(c_int32)),
+                          c.ctypes.data_as(POINTER(c_int32)), _p(v), _p(b), _p(x), 2)
+    xd = np.linalg.solve(H, b)
+    assert np.allclose(x, xd, rtol=1e-8, atol=1e-10 * np.abs(xd).max())
+
+
+@pytest.mark.parametrize("u,seed", [(5, 10), (60, 11)])
+def test_ilu0_is_exact_without_fill(u, seed):
+    # x and y decoupled: in x-then-y order the pattern is two tridiagonal
+    # blocks, where ILU(0) has no dropped fill and equals LU.
+    H, r, c, v, b = compacted_system(u, seed, coupled=False)
+    x = np.zeros(2 * u)
+    O.lib().orc_solve_coo(2 * u, len(r), r.ctypes.data_as(POINTER(c_int32)),
+                          c.ctypes.data_as(POINTER(c_int32)), _p(v), _p(b), _p(x), 0)
+    assert np.allclose(x, np.linalg.solve(H, b), rtol=1e-8, atol=1e-10)
+
+
+def test_ilu0_is_approximate_with_coupling():
+    H, r, c, v, b = compacted_system(30, 12, coupled=True)
+    x = np.zeros(60)
+    O.lib().orc_solve_coo(60, len(r), r.ctypes.data_as(POINTER(c_int32)),
+                          c.ctypes.data_as(POINTER(c_int32)), _p(v), _p(b), _p(x), 0)
+    xd = np.linalg.solve(H, b)
+    assert np.all(np.isfinite(x)) and not np.allclose(x, xd, rtol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# filters (A15), expand rule (A16)
+# ---------------------------------------------------------------------------
+def run_filter(fid, s, v):
+    v = np.array(v, dtype=np.float64)
+    O.lib().orc_filter_apply(fid, s, _p(v), len(v))
+    return v
+
+
+def test_smearing_closed_forms():
+    s = 0.8
+    out = run_filter(1, s, np.full(20, 3.0))
+    assert np.allclose(out, 3.0 * (1 - s ** np.arange(1, 21)), rtol=1e-14)
+    imp = np.zeros(15)
+    imp[0] = 1
+    assert np.allclose(run_filter(1, s, imp), (1 - s) * s ** np.arange(15), rtol=1e-14)
+
+
+def test_range_filter_uses_updated_predecessor():
+    assert np.array_equal(run_filter(2, 0.8, [0, 10, 1]), [0, 0.5, 1])
+    assert np.array_equal(run_filter(2, 0.8, [0, 0.6, 1]), [0, 0.6, 1])
+    # in-place: the second test sees the replaced 0.5, not 5
+    assert np.array_equal(run_filter(2, 0.0, [0, 5, 1, 2]), [0, 0.5, 1, 2])
+
+
+@pytest.mark.parametrize("shift", [0, 1])
+def test_expand_fixed_rule(shift):
+    T, a, p = tiny_problem(8, size=16, nframes=4, nk=5, amp=0.3)
+    st = O.State(a)
+    st.set_p(p)
+    N = st.N
+    G = int(a["group_size"])
+    prev = 1000.0 + np.arange(2 * N)
+    O.lib().orc_state_set_delta(st.ptr, _p(prev))
+    r = st.assemble(shift)
+    idx = np.arange(2 * N)
+    fixed = ((idx + 1 - shift * (G // 2)) % G == 0) & (idx + 1 - shift * (G // 2) >= 0)
+    used = np.concatenate([r["used"], r["used"]])
+    assert fixed.any()
+    assert np.array_equal(r["delta"][fixed], prev[fixed])
+    assert np.all(r["delta"][~fixed & ~used] == 0)
+    assert np.all(np.abs(r["delta"][~fixed & used]) < 100)
+    # x knots: fixed at c = G-1 mod G (shift 0) or G/2-1 mod G (shift 1)
+    cx = np.nonzero(fixed[:N])[0]
+    assert np.all((cx + 1 - shift * G // 2) % G == 0)
+
+
+# ---------------------------------------------------------------------------
+# line search structure (A17)
+# ---------------------------------------------------------------------------
+def test_linesearch_structure():
+    T, fr, geom, _ = W.config0(n_frames=6, size=64, seed=21)
+    a = W.group_args(T, fr, geom, max_iter=5)
+    st = O.State(a)
+    N = st.N
+    nb = a["basex"].shape[1]
+    nF = a["nsetframes"]
+    st.assemble(0)
+    p0, d0, e0 = st.get()
+    st.linesearch(0)
+    p1, d1, e1 = st.get()
+    r = float(a["linesearch_reduction_multiplier"])
+    assert p1[N - 1] == p0[N - 1] and p1[2 * N - 1] == p0[2 * N - 1]
+    accepted = 0
+    for f in range(nF):
+        own = np.r_[np.arange(f * nb, f * nb + nb), np.arange(f * nb, f * nb + nb) + N]
+        nz = d0[own] != 0
+        if nz.any():
+            ratio = d1[own][nz] / d0[own][nz]
+            k = np.round(np.log(ratio[0]) / np.log(r))
+            assert np.allclose(ratio, r ** k, rtol=1e-12)  # one power of r per frame
+        if not np.array_equal(p1[own], p0[own]):
+            accepted += 1
+            assert np.allclose(p1[own], p0[own] + d1[own], rtol=0, atol=1e-15)
+        else:
+            assert np.max(np.abs(d1[own])) <= float(a["move_thresh"]) * (1 / r) + 1e-15
+    assert accepted > 0
+    # only frame 0's errval slot can change (task_linesearch.cu:91)
+    assert np.array_equal(e1[1:nF], e0[1:nF])
+
+
+# ---------------------------------------------------------------------------
+# global correlation statistic (A18): hand-computed known answer
+# ---------------------------------------------------------------------------
+def test_global_correlation_known_answer():
+    # frame 4 lines x 2 pixels placed inside an 8x8 affine template by subrec;
+    # one knot interval, so the block matrix is one column of 8 rows in
+    # column-major pixel order: r = 4*j + i.
+    T = np.array([[1.0 + ix + 10 * iy for ix in range(8)] for iy in range(8)])
+    geom = ssa.init_settings(4, 2, T.shape, nknots=2, subrec=(2, 3, 2, 5))
+    a = W.group_args(T, np.zeros((4, 2, 1)), geom, max_iter=2)
+    I = np.array([10.0, 30.0, 1000.0, 40.0, 15.0, 20.0, 35.0, 41.0])
+    a["bmData"] = I.reshape(8, 1)
+    bm = np.ones((8, 1), np.uint8)
+    bm[2] = 0
+    a["base_mask"] = bm
+    a["min_points_per_block"] = np.array([0.0, 0.0])
+    st = O.State(a)
+    st.set_p(np.zeros(4))
+    st.assemble(0)
+    corr = st.global_correlation()
+    wI = np.array([12.0, 22.0, 32.0, 42.0, 13.0, 23.0, 33.0, 43.0])
+    masked = np.array([1, 1, 0, 1, 1, 1, 1, 1], bool)
+    meanA = I.sum() / 7          # image over ALL pixels / masked count
+    meanB = wI[masked].sum() / 7  # warped template over masked pixels
+    va = wI[:2] - meanA           # leading masked run: rows 0 and 1 only
+    vb = I[:2] - meanB
+    expected = (va * vb).sum() / np.sqrt((va * va).sum() * (vb * vb).sum())
+    assert abs(corr - expected) < 1e-12
+    assert abs(corr - 0.5921396185057259) < 1e-12  # exact rational arithmetic
+    # it is not the Pearson correlation of the masked pixels
+    assert abs(corr - np.corrcoef(wI[masked], I[masked])[0, 1]) > 0.1
+
+
+# ---------------------------------------------------------------------------
+# end to end
+# ---------------------------------------------------------------------------
+def test_zero_motion_noise_free():
+    T = W.make_template(48, 48, 15, (2.0, 4.0), offset=10.0, seed=31)
+    dx = np.zeros((4, 48))
+    fr = W.make_frames(T, dx, dx, 48, 48, np.float64, poisson=False)
+    geom = ssa.init_settings(48, 48, T.shape, nknots=7)
+    a = W.group_args(T, fr, geom, max_iter=10)
+    r = O.align_group_to_template(a)
+    assert np.all(r["knots_x"] == 0) and np.all(r["knots_y"] == 0)
+    assert np.allclose(r["residrms"], 0, atol=1e-12)
+    assert np.allclose(r["frame_corr"], 1, atol=1e-12)
+    assert r["stop_criterion"] == 2  # MovementThreshold
+
+
+def test_global_translation_recovered():
+    T = W.make_template(64, 64, 30, (2.5, 5.0), offset=10.0, seed=41)
+    n = 5
+    fr = W.make_frames(T, np.full((n, 64), 0.7), np.full((n, 64), -0.4), 64, 64, np.float64, poisson=False)
+    geom = ssa.init_settings(64, 64, T.shape, nknots=9)
+    # no early halt (haltcorr 0.995 stops such an easy case after 3 iterations)
+    a = W.group_args(T, fr, geom, max_iter=60, halt_correlation=2.0, move_thresh=1e-4)
+    r = O.align_group_to_template(a)
+    err = np.abs(np.concatenate([r["knots_x"].reshape(n, 9)[:-1].ravel() - 0.7,
+                                 r["knots_y"].reshape(n, 9)[:-1].ravel() + 0.4]))
+    # red/green fixed knots (task_expandresults.cu:20) may lag one update
+    assert np.median(err) < 1e-4 and np.mean(err < 1e-3) > 0.95
+
+
+def test_linear_motion_recovered():
+    T = W.make_template(64, 64, 30, (2.5, 5.0), offset=10.0, seed=43)
+    n = 4
+    t = np.arange(n * 64).reshape(n, 64) / 64.0
+    dx = 0.3 + 0.2 * t
+    dy = -0.2 + 0.1 * t
+    fr = W.make_frames(T, dx, dy, 64, 64, np.float64, poisson=False)
+    geom = ssa.init_settings(64, 64, T.shape, nknots=9)
+    a = W.group_args(T, fr, geom, max_iter=60, halt_correlation=2.0, move_thresh=1e-4)
+    r = O.align_group_to_template(a)
+    kx = r["knots_x"].reshape(n, 9)[:-1, :-1]
+    ky = r["knots_y"].reshape(n, 9)[:-1, :-1]
+    # knot k of frame f sits at the start of line 8k; the data hold the
+    # displacement constant along each line, a model error below 0.2/64/2 px
+    err = np.abs(np.concatenate([(kx - dx[:-1, ::8]).ravel(), (ky - dy[:-1, ::8]).ravel()]))
+    assert np.median(err) < 5e-3 and np.mean(err < 0.02) > 0.95
+
+
+# ---------------------------------------------------------------------------
+# resampling (A23)
+# ---------------------------------------------------------------------------
+def oracle_resample(data, dxk, dyk, geom_img, th, tw):
+    npix = data.size
+    out = np.zeros(th * tw, np.float32)
+    pk = np.ascontiguousarray(geom_img["prevknotind"].ravel(order="F"), np.int32)
+    O.lib().orc_resample_frame(_p(np.ascontiguousarray(data.ravel(order="F"), np.float64)), npix,
+                               _p(np.ascontiguousarray(dxk, np.float64)), _p(np.ascontiguousarray(dyk, np.float64)),
+                               _p(np.ascontiguousarray(geom_img["basex"].ravel(order="F"))),
+                               _p(np.ascontiguousarray(geom_img["basey"].ravel(order="F"))),
+                               pk.ctypes.data_as(POINTER(c_int32)),
+                               _p(np.ascontiguousarray(geom_img["tfrac"].ravel(order="F"))), th, tw,
+                               out.ctypes.data_as(POINTER(ctypes.c_float)))
+    return out.reshape(th, tw, order="F")
+
+
+def test_resample_identity_at_zero_displacement():
+    g = ssa.init_settings(16, 12, (16, 12), nknots=5)
+    data = np.random.default_rng(0).uniform(0, 100, (16, 12))
+    out = oracle_resample(data, np.zeros(5), np.zeros(5), g, 16, 12)
+    assert np.array_equal(out[:-1, :-1], data[:-1, :-1].astype(np.float32))
+    assert np.all(np.isnan(out[-1, :])) and np.all(np.isnan(out[:, -1]))
+
+
+def test_resample_last_write_wins():
+    # two data pixels in the same template cell: MATLAB's z(ind) = z(ind) + v
+    # keeps only the later pixel for each of the four corners
+    geo = dict(prevknotind=np.array([[1], [1]], np.int32), tfrac=np.zeros((2, 1)),
+               basex=np.array([[2.25], [2.75]]), basey=np.array([[3.5], [3.5]]))
+    data = np.array([[7.0], [11.0]])
+    out = oracle_resample(data, np.zeros(2), np.zeros(2), geo, 6, 6)
+    # 1-based cell (y=3, x=2) -> 0-based rows 2..3, cols 1..2
+    assert np.allclose(out[2:4, 1:3], 11.0)
+    out2 = oracle_resample(data[::-1].copy(), np.zeros(2), np.zeros(2),
+                           {k: v[::-1].copy() for k, v in geo.items()}, 6, 6)
+    assert np.allclose(out2[2:4, 1:3], 7.0)
+    assert np.isnan(out[0, 0])
