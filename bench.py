@@ -3,15 +3,17 @@
 Workload (the headline configuration): 1000 frames of 512x512 uint16 in
 groups of 50 frames, knots every 16 lines (33 knots), default alignment
 settings (steady_scan.m:836-867), synthetic data (seeded generator).  One
-step aligns all 20 groups through the C ABI (ss_align_group), with frames,
-template and geometry already resident in HBM.
+step aligns all 20 groups, submitted together through the C ABI
+(ss_align_batch), with frames, template and geometry resident in HBM.
 
-Printed JSON line: frames/s (whole job), p50/p99 group latency, knot RMS and
-corrected-frame error of one group against the FP64 oracle, the roofline of
-the dominant kernel (HIP events on the library's stream), and the oracle
-timed on the host as the CPU baseline.
+Printed JSON line: frames/s (whole job); group latency p50/p99 of
+single-group submissions (the plain path) and the batch latency; knot RMS
+and corrected-frame error of one group against the FP64 oracle; the
+roofline of the dominant kernel (HIP events on the library's stream); the
+oracle timed on the host as the CPU baseline.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -33,8 +35,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=1000)
     ap.add_argument("--group", type=int, default=50)
-    ap.add_argument("--candidates", type=int, default=4)
+    ap.add_argument("--candidates", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-latency", action="store_true", help="skip the single-group latency leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of known size (rocprof counter calibration)")
@@ -67,7 +70,6 @@ def main():
     import steadyscan_amd as ssa
     from steadyscan_amd import _native as N
     from steadyscan_amd import workloads as W
-    from steadyscan_amd.workloads import ARG_ORDER
 
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
@@ -90,66 +92,69 @@ def main():
         bm = ssa.image2blockmatrix(frames[:, :, g * G:(g + 1) * G], geom)
         dev.append(torch.from_numpy(bm.view(np.int16).ravel(order="F")).to("cuda"))
     torch.cuda.synchronize()
-    base = W.default_args()
     zeros = np.zeros((nk, G))
 
-    def args_for(g):
+    def call(g):
         a = dict(flags="D", template_image=tid, xgrad=xg, ygrad=yg, tfrac=gid,
                  basex=geom["basex_bm"], basey=geom["basey_bm"], base_mask=geom["base_mask_bm"],
                  bmData=(dev[g].data_ptr(), np.uint16), est_dx_knots=zeros, est_dy_knots=zeros,
                  group_size=2 * nb, nsetframes=G,
                  min_points_per_block=geom["min_points_per_block"].astype(np.float64))
-        a.update(base)
-        return [a[k] for k in ARG_ORDER]
+        a.update(W.default_args())
+        return a
 
-    results = {}
-    infos = {}
+    calls = [call(g) for g in range(ngroups)]
 
-    def step(record):
-        lat = []
-        for g in range(ngroups):
-            t = time.perf_counter()
-            info = {}
-            out = ssa.align_group_to_template(*args_for(g), nargout=6, handle=h, info=info,
-                                              linesearch_candidates=args.candidates)
-            lat.append(time.perf_counter() - t)
-            if record:
-                results[g] = out
-                infos[g] = info
-        return lat
+    def step():
+        return ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
 
+    results = None
     for _ in range(args.warmup):
-        step(True)
+        results = step()
     torch.cuda.synchronize()
     N.lib().ss_profile_enable(h.ptr, 1)
-    lat_all = []
+    batch_lat = []
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        lat_all += step(False)
+        t = time.perf_counter()
+        results = step()
+        batch_lat.append(time.perf_counter() - t)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    import ctypes
     launches = (ctypes.c_int64 * 6)()
     kms = (ctypes.c_double * 6)()
     kbytes = (ctypes.c_double * 6)()
     N.lib().ss_profile_read(h.ptr, launches, kms, kbytes)
     N.lib().ss_profile_enable(h.ptr, 0)
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
 
     frames_done = ngroups * G * args.steps
     fps = frames_done / elapsed
-    lat_ms = np.array(lat_all) * 1e3
-    per_kernel = {N.KERNEL_CLASSES[i]: {"launches": int(launches[i]), "ms": kms[i],
-                                        "avg_us": (kms[i] / launches[i] * 1e3) if launches[i] else 0.0}
-                  for i in range(6)}
+    per_kernel = {}
+    for i in range(6):
+        if launches[i]:
+            per_kernel[N.KERNEL_CLASSES[i]] = {
+                "launches": int(launches[i]), "ms": kms[i], "avg_us": kms[i] / launches[i] * 1e3,
+                "frame_GBps": (kbytes[i] / (kms[i] / 1e3) / 1e9) if kms[i] > 0 else 0.0}
     dom = max(range(6), key=lambda i: kms[i])
     avg_s = kms[dom] / launches[dom] / 1e3 if launches[dom] else float("nan")
     bytes_per_launch = kbytes[dom] / launches[dom] if launches[dom] else 0.0
     achieved = bytes_per_launch / avg_s / 1e9 if launches[dom] else 0.0
 
+    # latency of the plain path: one group per submission
+    single = []
+    if not args.no_latency:
+        for g in range(ngroups):
+            t = time.perf_counter()
+            ssa.align_groups([calls[g]], handle=h, linesearch_candidates=args.candidates, with_mask=False)
+            single.append(time.perf_counter() - t)
+    single_ms = np.array(single) * 1e3
+
     # parity of one group against the FP64 oracle, and the CPU baseline
     rms = None
     cpu = None
     corr_err = None
+    corr_rms = None
     if not args.no_cpu:
         from oracle import oracle as O
         g = 0
@@ -159,22 +164,26 @@ def main():
         ref = O.align_group_to_template(a, threads=threads, with_mask=False)
         tcpu = time.perf_counter() - t
         out = results[g]
-        dx = (out[0].ravel() - ref["knots_x"]).reshape(G, nk)[:-1]
-        dy = (out[1].ravel() - ref["knots_y"]).reshape(G, nk)[:-1]
+        dx = (out["knots_x"] - ref["knots_x"]).reshape(G, nk)[:-1]
+        dy = (out["knots_y"] - ref["knots_y"]).reshape(G, nk)[:-1]
         rms = float(np.sqrt(np.mean(np.concatenate([dx.ravel(), dy.ravel()]) ** 2)))
-        # corrected frames from the GPU knots vs the oracle knots (both through the GPU splat)
+        # corrected frames from the GPU knots vs the oracle knots (same splat)
         fr = frames[:, :, g * G:g * G + G - 1]
-        cg = ssa.resample_frames(fr, geom, out[0].ravel()[:nk * (G - 1)], out[1].ravel()[:nk * (G - 1)], handle=h)
-        co = ssa.resample_frames(fr, geom, ref["knots_x"][:nk * (G - 1)], ref["knots_y"][:nk * (G - 1)], handle=h)
+        m = nk * (G - 1)
+        cg = ssa.resample_frames(fr, geom, out["knots_x"][:m], out["knots_y"][:m], handle=h)
+        co = ssa.resample_frames(fr, geom, ref["knots_x"][:m], ref["knots_y"][:m], handle=h)
         both = np.isfinite(cg) & np.isfinite(co)
-        corr_err = float(np.max(np.abs(cg[both] - co[both]))) if both.any() else 0.0
+        diff = np.abs(cg[both] - co[both])
+        corr_err = float(diff.max()) if diff.size else 0.0
+        corr_rms = float(np.sqrt(np.mean(diff ** 2))) if diff.size else 0.0
         cpu = {"value": G / tcpu, "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"FP64 oracle (C restatement) on 1 group of {G} frames 512x512, "
-                         f"{ref['iterations']} GN iterations, {tcpu:.1f} s"}
-        print(f"[bench] oracle group: iterations {ref['iterations']} gpu {infos[g]['iterations']} "
+               "sample": f"FP64 oracle (C restatement, OpenMP) on 1 group of {G} frames 512x512 "
+                         f"(group 0 of the workload), {ref['iterations']} GN iterations, {tcpu:.1f} s"}
+        print(f"[bench] oracle group 0: iterations {ref['iterations']} gpu {out['iterations']} "
               f"rms {rms:.2e} px", file=sys.stderr, flush=True)
 
-    iters = [infos[g]["iterations"] for g in infos]
+    iters = [r["iterations"] for r in results]
+    lat_ms = np.array(batch_lat) * 1e3
     line = {
         "metric": METRIC,
         "value": fps,
@@ -190,15 +199,20 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{args.frames} frames 512x512 uint16 in groups of {G}, knots every 16 lines, "
-                        "default settings (max_iter 120, smearing filter, exact solve)",
+                        "default settings (max_iter 120, smearing filter, exact solve); "
+                        "all groups of a step in one batched submission",
             "frames_per_step": ngroups * G,
             "groups_per_step": ngroups,
-            "p50_group_ms": float(np.percentile(lat_ms, 50)),
-            "p99_group_ms": float(np.percentile(lat_ms, 99)),
+            "p50_group_ms": float(np.percentile(single_ms, 50)) if single else None,
+            "p99_group_ms": float(np.percentile(single_ms, 99)) if single else None,
+            "group_latency_mode": "single-group submissions, inputs resident",
+            "batch_ms_p50": float(np.percentile(lat_ms, 50)),
             "gn_iterations_mean": float(np.mean(iters)),
-            "linesearch_candidates": args.candidates,
+            "gn_iterations_max": int(np.max(iters)),
+            "linesearch_candidates_first_round": args.candidates,
             "knot_rms_vs_fp64_px": rms,
             "corrected_frame_max_abs_err": corr_err,
+            "corrected_frame_rms_err": corr_rms,
             "kernels": per_kernel,
             "storage": "u16 frames; f32 warp/residual/accumulation; f64 reductions and solve",
         },

@@ -1,5 +1,5 @@
-// ss_device.h -- device-side view of one alignment group (shared by the HIP
-// kernels and the host engine).
+// ss_device.h -- device-side view of a batch of alignment groups (shared by
+// the HIP kernels and the host engine).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,8 +10,9 @@ namespace ss {
 // Per-column partial sums written by the assembly pass (FP64).
 // Column c covers block b = c % nb of frame f = c / nb and holds:
 enum ColSum {
-    CS_D2 = 0,   // sum d^2 over masked pixels
-    CS_CNT,      // masked pixel count
+    CS_D2 = 0,                // sum d^2 over masked pixels
+    CS_SUMI,                  // sum of the image over all pixels
+    CS_CNT,                   // masked pixel count
     CS_AXX, CS_AYY, CS_AXY,   // sum w w' (1-f)^2   -> left-knot diagonal
     CS_BXX, CS_BYY, CS_BXY,   // sum w w' f^2       -> right-knot diagonal
     CS_CXX, CS_CYY, CS_CXY,   // sum w w' f (1-f)   -> left/right coupling
@@ -21,10 +22,9 @@ enum ColSum {
     CS_N
 };
 
-// Per-column sums of the correlation passes (FP64).
+// Per-column sums of the correlation pass (FP64).
 enum CorrSum {
-    CR_SUMI = 0,  // sum of image over all pixels
-    CR_SUMD,      // sum of the swapped difference image over all pixels
+    CR_SUMD = 0,  // sum of the swapped difference image over all pixels
     CR_NMASK,     // masked pixels
     CR_SA, CR_SAA, CR_SB, CR_SBB, CR_SAB, CR_NS,  // moments over the leading masked run
     CR_N
@@ -33,41 +33,83 @@ enum CorrSum {
 // Per-column sums of the final pass (FP64).
 enum FinalSum { FS_D2 = 0, FS_CNT, FS_SA, FS_SAA, FS_SB, FS_SBB, FS_SAB, FS_N };
 
-constexpr int kMaxCand = 8;   // fused line-search candidates per round
+constexpr int kMaxCand = 8;  // fused line-search candidates per round
 
-// Device status block read by the host after a stage.
-struct Status {
-    double corr;          // global correlation of the last correlation pass
-    int32_t need_more;    // some frame still needs another line-search round
-    int32_t improved;     // `improved` of the last completed line search
-    int32_t tries;        // sequential tries resolved (accumulated)
-    int32_t active;       // frames still unresolved after the last decide
+// Phases of the per-group state machine (Task<T>::alignFrames +
+// MainIteration<T>::execute).
+enum Phase {
+    PH_ASSEMBLE = 0,  // next: assembly pass of half-iteration `shift`
+    PH_LS = 1,        // line-search rounds of the current half-iteration
+    PH_CORR = 2,      // global correlation + stop tests
+    PH_FINAL = 3,     // final errval / frame correlation / mask
+    PH_DONE = 4,
+    PH_ERROR = 5
 };
 
 enum ImgType { IMG_F32 = 0, IMG_U16 = 1, IMG_I16 = 2 };
 
+// Static description of one group (written once per call by the host).
 struct GroupDev {
     // sizes (mexgateway.cpp:176-198)
     int frameH, nb, nF, fframeW, N, th, tw, groupSize;
-    int fracUniform;       // all frac columns equal: block c-1 diagonal term uses own frac
-    float xmax, ymax;      // template_w - 1 - eps, template_h - 1 - eps
-    double rmult;          // line-search reduction multiplier
-    // inputs
-    const void *img;       // frameH x fframeW, column-contiguous
-    int imgType;
-    const float4 *geo;     // frameH x nb: {base_x, base_y, frac, base_mask}
-    const float *fnext;    // frameH x nb: frac of block (b+1) % nb (only if !fracUniform)
-    const float *tmpl, *xg, *yg;  // template and gradients, row-major
-    const int *minPts;     // nb
+    int itemStart;        // first work item (= column) of this group in the batch
+    int seg;              // pixels per thread segment (ceil(frameH / 256))
+    // settings
+    float xmax, ymax;     // template_w - 1 - eps, template_h - 1 - eps
+    double rmult, thr, halt, cit, fct, strength, lambda;
+    int maxIter, K1, Kn, maxRounds;
+    // geometry: affine fast path or per-pixel table
+    int affine, L;
+    float gx0, gxj, gxi, gy0, gyb, gyj, gyi, f0, fj, fi;
+    int fracUniform;
+    const float4 *geo;    // frameH x nb: {base_x, base_y, frac, base_mask}
+    const float *fnext;   // frameH x nb: frac of block (b+1) % nb (only if !fracUniform)
+    // template: pairs (T[t], T[t+1]) row-major; gradient pairs unless derived
+    const float2 *tp, *gxv, *gyh;
+    int gradFD;           // gradients are forward differences of the template
+    // frame data, frameH x fframeW column-contiguous
+    const void *img;
+    int imgType, vec;
+    const int *minPts;
     // state
-    double *p, *pLast, *p0, *d0, *deltaP;
-    double *colsum, *lssum, *errval, *corrsum, *finalsum;
-    double *ws;            // solver workspace
-    uint8_t *used;         // nParamPoints
-    int *lsActive, *lsNextK, *lsMk, *lsDk;
-    uint8_t *maskOut;      // optional final mask, frameH x fframeW column-major
-    double *resid, *fcorr; // final per-frame outputs
-    Status *status;
+    double *p, *pLast, *p0, *d0, *deltaP, *errval;
+    double *colsum, *lssum, *corrsum, *finalsum, *ws;
+    uint8_t *used;
+    int *lsActive, *lsNextK, *lsMk, *lsDk, *frameArrive;
+    uint8_t *maskOut;
+    double *resid, *fcorr;
+    double *allParams, *iterCorr;  // optional diagnostics (or null)
 };
+
+// Dynamic control state of one group (read by the host after each tick).
+struct GroupCtl {
+    int phase, shift, iteration, improved, iterImproved;
+    int solverFilter, filterID, nFilterIterations;
+    int retv, error;
+    int halfIterations, rounds, tries, roundsThisLS;
+    int activeFrames, roundArrive, roundFrames, K;
+    int lsFramePasses;    // frames evaluated over all line-search rounds
+    int pad;
+    double correlation, corr;
+};
+
+struct Batch {
+    int nGroups, nItems;
+    GroupDev *groups;
+    GroupCtl *ctl;
+    int *itemGroup;       // work item -> group
+};
+
+struct LaunchBatch {
+    Batch b;
+    int hasType[3];
+    int itemGrid;
+};
+
+// Stages of one host submission, in order; see launch_stage.
+enum Stage { ST_ASSEMBLE = 0, ST_SOLVE, ST_LS, ST_CORR, ST_CORR_FINISH, ST_FINAL, ST_FINAL_FINISH };
+
+hipError_t launch_stage(const LaunchBatch &L, int stage, hipStream_t s);
+size_t solve_workspace_bytes(int n);
 
 }  // namespace ss

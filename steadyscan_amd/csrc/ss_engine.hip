@@ -1,12 +1,13 @@
 // ss_engine.hip -- the C ABI of include/steadyscan.h: handle, resident
-// inputs, and the host loop of one alignment group (Task<T>::alignFrames,
-// task/task.cpp:259-412, with MainIteration<T>::execute,
-// task/mainiteration/mainiteration.cpp:61-119).
+// inputs, batch set-up and the submission loop.
 //
-// Per half-iteration the device runs: assembly (1 kernel), solve + filter +
-// expand (1 kernel), then fused line-search rounds (round + decide kernels)
-// until every frame has resolved its sequential backtracking.  The host only
-// reads a 24-byte status block per line-search round and per iteration.
+// A call (one group, or a batch of groups) is laid out in one device arena.
+// Each group carries its own control state machine on the device (the
+// reference's Task<T>::alignFrames / MainIteration<T>::execute, task.cpp:
+// 259-412, mainiteration.cpp:61-119), so the host only submits "ticks" --
+// a fixed sequence of stage kernels -- and reads the groups' phases once per
+// tick, until every group is done.
+#include <algorithm>
 #include <chrono>
 #include <cfloat>
 #include <cmath>
@@ -18,38 +19,21 @@
 #include "ss_device.h"
 #include "steadyscan.h"
 
-namespace ss {
-hipError_t launch_assemble(const GroupDev &g, hipStream_t s);
-hipError_t launch_solve(const GroupDev &g, int shift, int filterKind, double strength,
-                        double lambda, hipStream_t s);
-hipError_t launch_ls_round(const GroupDev &g, int K, hipStream_t s);
-hipError_t launch_ls_decide(const GroupDev &g, int shift, int K, double thr, hipStream_t s);
-hipError_t launch_corr(const GroupDev &g, hipStream_t s);
-hipError_t launch_final(const GroupDev &g, hipStream_t s);
-size_t solve_workspace_bytes(int n);
-}  // namespace ss
-
 namespace {
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
+struct DevTemplate {
+    int th = 0, tw = 0, gradFD = 1;
+    float2 *tp = nullptr, *gxv = nullptr, *gyh = nullptr;
+    void *mem = nullptr;  // owning allocation (resident only)
 };
 
-struct ResidentTemplate {
-    int th = 0, tw = 0;
-    float *tmpl = nullptr, *xg = nullptr, *yg = nullptr;
-};
-
-struct ResidentGeometry {
-    int frameH = 0, nb = 0, fracUniform = 1;
+struct DevGeometry {
+    int frameH = 0, nb = 0, fracUniform = 1, affine = 0, L = 0;
+    float gx0 = 0, gxj = 0, gxi = 0, gy0 = 0, gyb = 0, gyj = 0, gyi = 0, f0 = 0, fj = 0, fi = 0;
     float4 *geo = nullptr;
     float *fnext = nullptr;
+    void *mem = nullptr;
 };
-
-constexpr int kMaxRounds = 512;  // 4096 candidates at K = 8
-
-}  // namespace
 
 struct Profile {
     bool on = false;
@@ -57,7 +41,6 @@ struct Profile {
     struct Pending {
         int cls;
         hipEvent_t a, b;
-        double bytes;
     };
     std::vector<Pending> pending;
     int64_t launches[SS_K_COUNT] = {};
@@ -65,17 +48,23 @@ struct Profile {
     double bytes[SS_K_COUNT] = {};
 };
 
+constexpr int kTickCap = 200000;
+
+}  // namespace
+
 struct ss_handle {
     int device = 0;
-    Profile prof;
+    int numCU = 256;
     hipStream_t stream = nullptr;
     std::string err;
-    std::map<std::string, DevBuf> bufs;
-    std::map<int, ResidentTemplate> templates;
-    std::map<int, ResidentGeometry> geometries;
+    std::map<int, DevTemplate> templates;
+    std::map<int, DevGeometry> geometries;
     int next_id = 1;
-    ss::Status *hstatus = nullptr;  // pinned
-    ss::Status *dstatus = nullptr;
+    void *arena = nullptr;
+    size_t arenaBytes = 0;
+    void *harena = nullptr;  // pinned host staging
+    size_t harenaBytes = 0;
+    Profile prof;
 };
 
 namespace {
@@ -85,6 +74,17 @@ ss_status fail(ss_handle *h, ss_status st, const std::string &msg) {
     return st;
 }
 
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(h, e_ == hipErrorOutOfMemory ? SS_ERR_OUT_OF_MEMORY : SS_ERR_DEVICE, \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// profiling: HIP events around stage launches on the handle's stream
+// ---------------------------------------------------------------------------
 hipEvent_t prof_event(ss_handle *h) {
     if (!h->prof.pool.empty()) {
         hipEvent_t e = h->prof.pool.back();
@@ -96,7 +96,6 @@ hipEvent_t prof_event(ss_handle *h) {
     return e;
 }
 
-// Folds completed event pairs into the counters (synchronises the stream).
 void prof_collect(ss_handle *h) {
     if (h->prof.pending.empty()) return;
     hipError_t e = hipStreamSynchronize(h->stream);
@@ -106,7 +105,6 @@ void prof_collect(ss_handle *h) {
         if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             h->prof.ms[p.cls] += ms;
             h->prof.launches[p.cls] += 1;
-            h->prof.bytes[p.cls] += p.bytes;
         }
         h->prof.pool.push_back(p.a);
         h->prof.pool.push_back(p.b);
@@ -114,131 +112,177 @@ void prof_collect(ss_handle *h) {
     h->prof.pending.clear();
 }
 
-// Runs one launcher, bracketed by events when profiling is on.
-template <typename F>
-hipError_t timed(ss_handle *h, int cls, double bytes, F &&launch) {
-    if (!h->prof.on) return launch();
+int stage_class(int stage) {
+    switch (stage) {
+    case ss::ST_ASSEMBLE: return SS_K_ASSEMBLE;
+    case ss::ST_SOLVE: return SS_K_SOLVE;
+    case ss::ST_LS: return SS_K_LS_ROUND;
+    case ss::ST_CORR:
+    case ss::ST_CORR_FINISH: return SS_K_CORR;
+    default: return SS_K_FINAL;
+    }
+}
+
+hipError_t run_stage(ss_handle *h, const ss::LaunchBatch &L, int stage) {
+    if (!h->prof.on) return ss::launch_stage(L, stage, h->stream);
     hipEvent_t a = prof_event(h), b = prof_event(h);
-    if (!a || !b) return launch();
+    if (!a || !b) return ss::launch_stage(L, stage, h->stream);
     hipError_t e = hipEventRecord(a, h->stream);
     if (e != hipSuccess) return e;
-    e = launch();
+    e = ss::launch_stage(L, stage, h->stream);
     if (e != hipSuccess) return e;
     e = hipEventRecord(b, h->stream);
-    h->prof.pending.push_back({cls, a, b, bytes});
+    h->prof.pending.push_back({stage_class(stage), a, b});
     return e;
 }
 
-#define HIPCHK(expr)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(h, e_ == hipErrorOutOfMemory ? SS_ERR_OUT_OF_MEMORY : SS_ERR_DEVICE, \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                 \
-    } while (0)
-
-// Grow-only named device buffer owned by the handle.
-ss_status buffer(ss_handle *h, const char *name, size_t bytes, void **out) {
-    DevBuf &b = h->bufs[name];
-    if (b.n < bytes) {
-        if (b.p) {
-            hipError_t e = hipFree(b.p);
-            (void)e;
-            b.p = nullptr;
-            b.n = 0;
-        }
-        void *p = nullptr;
-        hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
-        if (e != hipSuccess)
-            return fail(h, SS_ERR_OUT_OF_MEMORY, std::string("hipMalloc(") + name + ") failed: " + hipGetErrorString(e));
-        b.p = p;
-        b.n = bytes;
+// ---------------------------------------------------------------------------
+// device arena
+// ---------------------------------------------------------------------------
+struct Arena {
+    size_t off = 0;
+    template <typename T>
+    size_t take(size_t count) {
+        const size_t o = off;
+        off += (count * sizeof(T) + 255) & ~(size_t)255;
+        return o;
     }
-    *out = b.p;
+};
+
+ss_status ensure_arena(ss_handle *h, size_t bytes, size_t hostBytes) {
+    if (h->arenaBytes < bytes) {
+        if (h->arena) {
+            HIPCHK(hipStreamSynchronize(h->stream));
+            HIPCHK(hipFree(h->arena));
+            h->arena = nullptr;
+            h->arenaBytes = 0;
+        }
+        hipError_t e = hipMalloc(&h->arena, bytes);
+        if (e != hipSuccess) return fail(h, SS_ERR_OUT_OF_MEMORY, "unable to allocate device memory");
+        h->arenaBytes = bytes;
+    }
+    if (h->harenaBytes < hostBytes) {
+        if (h->harena) {
+            HIPCHK(hipStreamSynchronize(h->stream));
+            HIPCHK(hipHostFree(h->harena));
+            h->harena = nullptr;
+            h->harenaBytes = 0;
+        }
+        hipError_t e = hipHostMalloc(&h->harena, hostBytes, hipHostMallocDefault);
+        if (e != hipSuccess) return fail(h, SS_ERR_OUT_OF_MEMORY, "unable to allocate pinned host memory");
+        h->harenaBytes = hostBytes;
+    }
     return SS_OK;
 }
 
-template <typename T>
-ss_status buffer_t(ss_handle *h, const char *name, size_t count, T **out) {
-    void *p = nullptr;
-    ss_status st = buffer(h, name, count * sizeof(T), &p);
-    *out = (T *)p;
-    return st;
-}
-
-// Template and gradients to row-major float (mexconverter.cpp:223-285 makes
-// the same transposition for RowMajor targets).
-ss_status upload_template(ss_handle *h, int th, int tw, const double *t, const double *xg,
-                          const double *yg, ResidentTemplate &rt, bool resident) {
-    if (th < 2 || tw < 2 || !t || !xg || !yg) return fail(h, SS_ERR_INVALID_ARGUMENT, "template: bad size or NULL array");
+// ---------------------------------------------------------------------------
+// host-side conversions
+// ---------------------------------------------------------------------------
+// Template (column-major doubles) -> row-major float pairs (T[t], T[t+1]);
+// gradient pairs only when the gradients are not the template's forward
+// differences (steady_scan.m:178-179 makes them so).
+void convert_template(int th, int tw, const double *t, const double *xg, const double *yg,
+                      std::vector<float2> &tp, std::vector<float2> &gxv, std::vector<float2> &gyh,
+                      int &gradFD) {
     const size_t n = (size_t)th * tw;
-    std::vector<float> a(n), bx(n), by(n);
+    auto T = [&](int iy, int ix) { return t[(size_t)ix * th + iy]; };
+    auto GX = [&](int iy, int ix) { return xg[(size_t)ix * th + iy]; };
+    auto GY = [&](int iy, int ix) { return yg[(size_t)ix * th + iy]; };
+    tp.assign(n, make_float2(0.f, 0.f));
+    double tmax = 0;
     for (int iy = 0; iy < th; iy++)
         for (int ix = 0; ix < tw; ix++) {
-            const size_t cm = (size_t)ix * th + iy, rm = (size_t)iy * tw + ix;
-            a[rm] = (float)t[cm];
-            bx[rm] = (float)xg[cm];
-            by[rm] = (float)yg[cm];
+            tp[(size_t)iy * tw + ix] = make_float2((float)T(iy, ix), ix + 1 < tw ? (float)T(iy, ix + 1) : 0.f);
+            tmax = std::max(tmax, std::fabs(T(iy, ix)));
         }
-    rt.th = th;
-    rt.tw = tw;
-    if (resident) {
-        HIPCHK(hipMalloc(&rt.tmpl, n * sizeof(float)));
-        HIPCHK(hipMalloc(&rt.xg, n * sizeof(float)));
-        HIPCHK(hipMalloc(&rt.yg, n * sizeof(float)));
-    } else {
-        ss_status st;
-        if ((st = buffer_t(h, "tmpl", n, &rt.tmpl)) || (st = buffer_t(h, "xg", n, &rt.xg)) ||
-            (st = buffer_t(h, "yg", n, &rt.yg)))
-            return st;
+    // every gradient value the warp can read: xgrad(:, 1:end-1), ygrad(1:end-1, :)
+    bool fd = true;
+    const double tol = 1e-12 * std::max(1.0, tmax);
+    for (int iy = 0; iy < th && fd; iy++)
+        for (int ix = 0; ix + 1 < tw; ix++)
+            if (!(std::fabs(GX(iy, ix) - (T(iy, ix + 1) - T(iy, ix))) <= tol)) { fd = false; break; }
+    for (int iy = 0; iy + 1 < th && fd; iy++)
+        for (int ix = 0; ix < tw; ix++)
+            if (!(std::fabs(GY(iy, ix) - (T(iy + 1, ix) - T(iy, ix))) <= tol)) { fd = false; break; }
+    gradFD = fd ? 1 : 0;
+    gxv.clear();
+    gyh.clear();
+    if (!fd) {
+        gxv.assign(n, make_float2(0.f, 0.f));
+        gyh.assign(n, make_float2(0.f, 0.f));
+        for (int iy = 0; iy < th; iy++)
+            for (int ix = 0; ix < tw; ix++) {
+                const size_t o = (size_t)iy * tw + ix;
+                gxv[o] = make_float2((float)GX(iy, ix), iy + 1 < th ? (float)GX(iy + 1, ix) : 0.f);
+                gyh[o] = make_float2((float)GY(iy, ix), ix + 1 < tw ? (float)GY(iy, ix + 1) : 0.f);
+            }
     }
-    HIPCHK(hipMemcpyAsync(rt.tmpl, a.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(rt.xg, bx.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(rt.yg, by.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    return SS_OK;
 }
 
-// Geometry block matrices to {base_x, base_y, frac, base_mask} float4 per
-// pixel, column-contiguous; frac of the next block column only where the
-// frac columns differ (the diagonal term of task_filldiagonals.cu:66-103
-// reads column idx % frameW).
-ss_status upload_geometry(ss_handle *h, int fh, int nb, const double *frac, const double *bx,
-                          const double *by, const uint8_t *bm, ResidentGeometry &rg,
-                          bool resident) {
-    if (fh < 1 || nb < 1 || !frac || !bx || !by || !bm)
-        return fail(h, SS_ERR_INVALID_ARGUMENT, "geometry: bad size or NULL array");
+// Geometry block matrices -> affine description (when the base coordinates
+// and time fractions are affine in the block's (column, line) coordinates,
+// the case of every unidirectional integer-spacing scan) or the per-pixel
+// {base_x, base_y, frac, base_mask} table.
+void convert_geometry(int fh, int nb, const double *frac, const double *bx, const double *by,
+                      const uint8_t *bm, DevGeometry &g, std::vector<float4> &tab,
+                      std::vector<float> &fn) {
     const size_t n = (size_t)fh * nb;
-    std::vector<float4> g(n);
-    bool uniform = true;
-    for (int b = 0; b < nb; b++)
-        for (int r = 0; r < fh; r++) {
-            const size_t i = (size_t)b * fh + r;
-            g[i] = make_float4((float)bx[i], (float)by[i], (float)frac[i], bm[i] ? 1.f : 0.f);
-            if (frac[i] != frac[r]) uniform = false;
+    g.frameH = fh;
+    g.nb = nb;
+    bool uniform = true, allMask = true;
+    for (int b = 0; b < nb && uniform; b++)
+        for (int r = 0; r < fh; r++)
+            if (frac[(size_t)b * fh + r] != frac[r]) { uniform = false; break; }
+    for (size_t i = 0; i < n; i++)
+        if (!bm[i]) { allMask = false; break; }
+    g.fracUniform = uniform ? 1 : 0;
+    // affine detection: L = first row where base_x changes within block 0
+    g.affine = 0;
+    if (uniform && allMask && fh >= 1) {
+        int L = fh;
+        for (int r = 1; r < fh; r++)
+            if (bx[r] != bx[0]) { L = r; break; }
+        if (fh % L == 0) {
+            const int J = fh / L;
+            auto at = [&](const double *a, int b, int r) { return a[(size_t)b * fh + r]; };
+            const double gx0 = at(bx, 0, 0), gxi = L > 1 ? at(bx, 0, 1) - gx0 : 0.0,
+                         gxj = J > 1 ? at(bx, 0, L) - gx0 : 0.0;
+            const double gy0 = at(by, 0, 0), gyi = L > 1 ? at(by, 0, 1) - gy0 : 0.0,
+                         gyj = J > 1 ? at(by, 0, L) - gy0 : 0.0, gyb = nb > 1 ? at(by, 1, 0) - gy0 : 0.0;
+            const double f0 = frac[0], fi = L > 1 ? frac[1] - f0 : 0.0, fj = J > 1 ? frac[L] - f0 : 0.0;
+            bool ok = true;
+            const double tol = 1e-9;
+            for (int b = 0; b < nb && ok; b++)
+                for (int r = 0; r < fh; r++) {
+                    const int j = r / L, i = r % L;
+                    if (std::fabs(at(bx, b, r) - (gx0 + gxj * j + gxi * i)) > tol * std::max(1.0, std::fabs(gx0) + std::fabs(gxj) * J) ||
+                        std::fabs(at(by, b, r) - (gy0 + gyb * b + gyj * j + gyi * i)) > tol * std::max(1.0, std::fabs(gy0) + std::fabs(gyb) * nb) ||
+                        std::fabs(at(frac, b, r) - (f0 + fj * j + fi * i)) > tol) {
+                        ok = false;
+                        break;
+                    }
+                }
+            if (ok) {
+                g.affine = 1;
+                g.L = L;
+                g.gx0 = (float)gx0; g.gxj = (float)gxj; g.gxi = (float)gxi;
+                g.gy0 = (float)gy0; g.gyb = (float)gyb; g.gyj = (float)gyj; g.gyi = (float)gyi;
+                g.f0 = (float)f0; g.fj = (float)fj; g.fi = (float)fi;
+            }
         }
-    rg.frameH = fh;
-    rg.nb = nb;
-    rg.fracUniform = uniform ? 1 : 0;
-    std::vector<float> fn;
+    }
+    tab.clear();
+    fn.clear();
+    if (!g.affine) {
+        tab.resize(n);
+        for (size_t i = 0; i < n; i++)
+            tab[i] = make_float4((float)bx[i], (float)by[i], (float)frac[i], bm[i] ? 1.f : 0.f);
+    }
     if (!uniform) {
         fn.resize(n);
         for (int b = 0; b < nb; b++)
             for (int r = 0; r < fh; r++) fn[(size_t)b * fh + r] = (float)frac[(size_t)((b + 1) % nb) * fh + r];
     }
-    if (resident) {
-        HIPCHK(hipMalloc(&rg.geo, n * sizeof(float4)));
-        if (!uniform) HIPCHK(hipMalloc(&rg.fnext, n * sizeof(float)));
-    } else {
-        ss_status st;
-        if ((st = buffer_t(h, "geo", n, &rg.geo))) return st;
-        if (!uniform && (st = buffer_t(h, "fnext", n, &rg.fnext))) return st;
-    }
-    HIPCHK(hipMemcpyAsync(rg.geo, g.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
-    if (!uniform)
-        HIPCHK(hipMemcpyAsync(rg.fnext, fn.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    return SS_OK;
 }
 
 // ParameterOrganizer::organize2D1 (parameterorganizer.cpp:30-55): shared
@@ -289,267 +333,362 @@ ss_status validate(ss_handle *h, const ss_group_in *in, const ss_settings *s) {
         return fail(h, SS_ERR_INVALID_ARGUMENT, "invalid data_type");
     if (in->data_on_device && in->data_type == SS_F64)
         return fail(h, SS_ERR_INVALID_ARGUMENT, "device-resident data must be f32, u16 or i16");
+    if (in->template_id <= 0 && (!in->template_image || !in->xgrad || !in->ygrad || in->template_h < 2 || in->template_w < 2))
+        return fail(h, SS_ERR_INVALID_ARGUMENT, "template: bad size or NULL array");
+    if (in->geometry_id <= 0 && (!in->frac || !in->base_x || !in->base_y || !in->base_mask))
+        return fail(h, SS_ERR_INVALID_ARGUMENT, "geometry: NULL array");
     if (s->max_iterations < 1) return fail(h, SS_ERR_INVALID_ARGUMENT, "max_iterations must be >= 1");
     if (s->solver_id < 0 || s->solver_id > 2)  // task.cpp:74-88
         return fail(h, SS_ERR_INVALID_ARGUMENT, "invalid solver id\n0:CuSparse solver\n1:CuSolver solver\n2:Eigen solver");
+    if (s->solver_id == 0)
+        return fail(h, SS_ERR_NOT_AVAILABLE, "solver_id 0 (ILU(0)) is not available in this build");
     const int fsel = s->max_filter_iterations != 0 ? s->filter_id : 0;  // task.cpp:54
     if (fsel < 0 || fsel > 2)
         return fail(h, SS_ERR_INVALID_ARGUMENT, "invalid filter ID\n0: disable filter\n1: smearingfiter\n2:rangefilter");
     if (s->precision != 'S' && s->precision != 'D')
         return fail(h, SS_ERR_INVALID_ARGUMENT, "invalid precision type! use 'S' or 'D'");
+    const int K = s->linesearch_candidates <= 0 ? 2 : s->linesearch_candidates;
+    if (K != 1 && K != 2 && K != 4 && K != 8)
+        return fail(h, SS_ERR_INVALID_ARGUMENT, "linesearch_candidates must be 1, 2, 4 or 8");
     const long long npix = (long long)in->frame_h * in->n_blocks * in->n_frames;
-    if (npix > (1ll << 31)) return fail(h, SS_ERR_INVALID_ARGUMENT, "group too large");
+    if (npix >= (1ll << 31)) return fail(h, SS_ERR_INVALID_ARGUMENT, "group too large");
     return SS_OK;
 }
 
-struct Loop {
-    ss_handle *h;
-    ss::GroupDev g;
+// Per-group host-side plan of the arena.
+struct Plan {
+    const ss_group_in *in;
     const ss_settings *s;
-    int K;
-    double frameBytes = 0;  // algorithmic bytes of one frame (pixels x element size)
-    // filter lifecycle (task.cpp:54-70, mainiteration.cpp:80-115, task.cpp:313-325)
-    int solverFilter = 0, filterID = 0, nFilterIterations = 0;
-    int halfIterations = 0, rounds = 0;
-
-    ss_status read_status() {
-        HIPCHK(hipMemcpyAsync(h->hstatus, h->dstatus, sizeof(ss::Status), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        return SS_OK;
-    }
-
-    ss_status linesearch(int shift, int *improved) {
-        int active = g.nF;
-        for (int round = 0;; round++) {
-            if (round >= kMaxRounds)
-                return fail(h, SS_ERR_NUMERICAL, "line search did not reach the movement threshold (non-finite step?)");
-            HIPCHK(timed(h, SS_K_LS_ROUND, (double)active * frameBytes, [&] { return ss::launch_ls_round(g, K, h->stream); }));
-            HIPCHK(timed(h, SS_K_LS_DECIDE, 0.0, [&] { return ss::launch_ls_decide(g, shift, K, s->move_threshold, h->stream); }));
-            rounds++;
-            ss_status st = read_status();
-            if (st) return st;
-            active = h->hstatus->active;
-            if (!h->hstatus->need_more) break;
-        }
-        *improved = h->hstatus->improved;
-        return SS_OK;
-    }
-
-    // MainIteration::execute (mainiteration.cpp:61-119)
-    ss_status main_iteration(int iteration, int *improved_out) {
-        int improved = 1;
-        for (int shift = 0; shift < 2; shift++) {
-            HIPCHK(timed(h, SS_K_ASSEMBLE, frameBytes * g.nF, [&] { return ss::launch_assemble(g, h->stream); }));
-            if (nFilterIterations != 0 && iteration > nFilterIterations) {
-                solverFilter = 0;
-                filterID = 0;
-                nFilterIterations = 0;
-            }
-            HIPCHK(timed(h, SS_K_SOLVE, 0.0, [&] { return ss::launch_solve(g, shift, solverFilter, s->filter_strength, s->lambda, h->stream); }));
-            int imp = 0;
-            ss_status st = linesearch(shift, &imp);
-            if (st) return st;
-            improved &= imp;
-            halfIterations++;
-        }
-        if (!improved) {
-            if (nFilterIterations != 0) {
-                solverFilter = 0;
-                nFilterIterations = 0;
-                return main_iteration(iteration, improved_out);
-            }
-            *improved_out = 0;
-            return SS_OK;
-        }
-        *improved_out = 1;
-        return SS_OK;
-    }
+    ss_group_out *out;
+    DevTemplate tmpl;
+    DevGeometry geom;
+    std::vector<float2> tp, gxv, gyh;
+    std::vector<float4> tab;
+    std::vector<float> fn;
+    std::vector<double> p0;
+    size_t oTp = 0, oGxv = 0, oGyh = 0, oGeo = 0, oFn = 0, oImg = 0, oMin = 0;
+    size_t oP = 0, oPL = 0, oP0 = 0, oD0 = 0, oDP = 0, oErr = 0, oCol = 0, oLs = 0, oCorr = 0,
+           oFin = 0, oWs = 0, oUsed = 0, oAct = 0, oNext = 0, oMk = 0, oDk = 0, oArr = 0, oMask = 0,
+           oRes = 0, oFc = 0, oAll = 0, oIc = 0;
+    size_t hImg = 0, hP = 0;  // host staging offsets
+    int imgType = 0;
+    size_t imgBytes = 0;
 };
 
-ss_status run_group(ss_handle *h, const ss_group_in *in, const ss_settings *s, ss_group_out *out) {
-    ss_status st = validate(h, in, s);
-    if (st) return st;
-    if (s->solver_id == 0)
-        return fail(h, SS_ERR_NOT_AVAILABLE, "solver_id 0 (ILU(0)) is not available in this build");
+ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settings *ss_,
+                    ss_group_out *outs) {
     const auto t0 = std::chrono::steady_clock::now();
-    const int fh = in->frame_h, nb = in->n_blocks, nF = in->n_frames;
-    const int fframeW = nb * nF, N = fframeW + 1;
-    // inputs
-    ResidentTemplate rt;
-    if (in->template_id > 0) {
-        auto it = h->templates.find(in->template_id);
-        if (it == h->templates.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown template_id");
-        rt = it->second;
-    } else {
-        if ((st = upload_template(h, in->template_h, in->template_w, in->template_image, in->xgrad,
-                                  in->ygrad, rt, false)))
-            return st;
-    }
-    ResidentGeometry rg;
-    if (in->geometry_id > 0) {
-        auto it = h->geometries.find(in->geometry_id);
-        if (it == h->geometries.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown geometry_id");
-        rg = it->second;
-        if (rg.frameH != fh || rg.nb != nb)
-            return fail(h, SS_ERR_INVALID_ARGUMENT, "resident geometry does not match frame_h x n_blocks");
-    } else {
-        if ((st = upload_geometry(h, fh, nb, in->frac, in->base_x, in->base_y, in->base_mask, rg, false)))
-            return st;
-    }
-    const size_t npix = (size_t)fh * fframeW;
-    const void *dimg = nullptr;
-    int imgType = ss::IMG_F32;
-    if (in->data_type == SS_U16) imgType = ss::IMG_U16;
-    else if (in->data_type == SS_I16) imgType = ss::IMG_I16;
-    if (in->data_on_device) {
-        dimg = in->data;
-    } else {
-        void *dp = nullptr;
-        const size_t esz = (imgType == ss::IMG_F32) ? 4 : 2;
-        if ((st = buffer(h, "img", npix * esz, &dp))) return st;
-        if (in->data_type == SS_F64) {
-            std::vector<float> tmp(npix);
-            const double *src = (const double *)in->data;
-            for (size_t i = 0; i < npix; i++) tmp[i] = (float)src[i];
-            HIPCHK(hipMemcpyAsync(dp, tmp.data(), npix * 4, hipMemcpyHostToDevice, h->stream));
-            HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<Plan> plans(n);
+    Arena A, H;
+    const size_t oGroups = A.take<ss::GroupDev>(n);
+    const size_t oCtl = A.take<ss::GroupCtl>(n);
+    int nItems = 0;
+    for (int gi = 0; gi < n; gi++) {
+        Plan &P = plans[gi];
+        P.in = ins + gi;
+        P.s = ss_ + gi;
+        P.out = outs + gi;
+        ss_status st = validate(h, P.in, P.s);
+        if (st) return st;
+        const ss_group_in &in = *P.in;
+        const int fh = in.frame_h, nb = in.n_blocks, nF = in.n_frames, fw = nb * nF, N = fw + 1;
+        if (in.template_id > 0) {
+            auto it = h->templates.find(in.template_id);
+            if (it == h->templates.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown template_id");
+            P.tmpl = it->second;
         } else {
-            HIPCHK(hipMemcpyAsync(dp, in->data, npix * esz, hipMemcpyHostToDevice, h->stream));
-        }
-        dimg = dp;
-    }
-    int *minPts = nullptr;
-    std::vector<int> mp(in->min_points, in->min_points + nb);
-    if ((st = buffer_t(h, "minpts", (size_t)nb, &minPts))) return st;
-    HIPCHK(hipMemcpyAsync(minPts, mp.data(), nb * sizeof(int), hipMemcpyHostToDevice, h->stream));
-    // state (task_allocate.cu:124-184)
-    ss::GroupDev g;
-    std::memset(&g, 0, sizeof(g));
-    g.frameH = fh;
-    g.nb = nb;
-    g.nF = nF;
-    g.fframeW = fframeW;
-    g.N = N;
-    g.th = rt.th;
-    g.tw = rt.tw;
-    g.groupSize = in->group_size;
-    g.fracUniform = rg.fracUniform;
-    const double eps = s->precision == 'S' ? (double)FLT_EPSILON : DBL_EPSILON;
-    g.xmax = (float)((double)(rt.tw - 1) - eps);
-    g.ymax = (float)((double)(rt.th - 1) - eps);
-    g.rmult = s->linesearch_reduction_multiplier;
-    g.img = dimg;
-    g.imgType = imgType;
-    g.geo = rg.geo;
-    g.fnext = rg.fnext;
-    g.tmpl = rt.tmpl;
-    g.xg = rt.xg;
-    g.yg = rt.yg;
-    g.minPts = minPts;
-    const size_t P = 2 * (size_t)N;
-    if ((st = buffer_t(h, "p", P, &g.p)) || (st = buffer_t(h, "pLast", P, &g.pLast)) ||
-        (st = buffer_t(h, "p0", P, &g.p0)) || (st = buffer_t(h, "d0", P, &g.d0)) ||
-        (st = buffer_t(h, "deltaP", P, &g.deltaP)) ||
-        (st = buffer_t(h, "colsum", (size_t)fframeW * ss::CS_N, &g.colsum)) ||
-        (st = buffer_t(h, "lssum", (size_t)fframeW * 2 * ss::kMaxCand, &g.lssum)) ||
-        (st = buffer_t(h, "errval", 2 * (size_t)nF, &g.errval)) ||
-        (st = buffer_t(h, "corrsum", (size_t)fframeW * ss::CR_N, &g.corrsum)) ||
-        (st = buffer_t(h, "finalsum", (size_t)fframeW * ss::FS_N, &g.finalsum)) ||
-        (st = buffer(h, "ws", ss::solve_workspace_bytes(fframeW), (void **)&g.ws)) ||
-        (st = buffer_t(h, "used", (size_t)N, &g.used)) ||
-        (st = buffer_t(h, "lsActive", (size_t)nF, &g.lsActive)) ||
-        (st = buffer_t(h, "lsNextK", (size_t)nF, &g.lsNextK)) ||
-        (st = buffer_t(h, "lsMk", (size_t)nF, &g.lsMk)) ||
-        (st = buffer_t(h, "lsDk", (size_t)nF, &g.lsDk)) ||
-        (st = buffer_t(h, "resid", (size_t)nF, &g.resid)) ||
-        (st = buffer_t(h, "fcorr", (size_t)nF, &g.fcorr)))
-        return st;
-    if (out->mask && (st = buffer_t(h, "maskOut", npix, &g.maskOut))) return st;
-    g.status = h->dstatus;
-    std::vector<double> hp(P, 0.0), ev(2 * (size_t)nF, 1e6);
-    knots_in(in->init_x, in->init_y, nb + 1, nF, nb, hp.data());
-    HIPCHK(hipMemcpyAsync(g.p, hp.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemsetAsync(g.deltaP, 0, P * sizeof(double), h->stream));
-    HIPCHK(hipMemcpyAsync(g.errval, ev.data(), ev.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemsetAsync(h->dstatus, 0, sizeof(ss::Status), h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    out->device_init_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-
-    Loop L;
-    L.h = h;
-    L.g = g;
-    L.s = s;
-    L.K = s->linesearch_candidates <= 0 ? 4 : s->linesearch_candidates;
-    L.frameBytes = (double)fh * nb * (imgType == ss::IMG_F32 ? 4.0 : 2.0);
-    if (L.K != 1 && L.K != 2 && L.K != 4 && L.K != 8)
-        return fail(h, SS_ERR_INVALID_ARGUMENT, "linesearch_candidates must be 1, 2, 4 or 8");
-    L.filterID = s->filter_id;
-    L.nFilterIterations = s->max_filter_iterations;
-    switch (s->max_filter_iterations != 0 ? s->filter_id : 0) {
-    case 0: L.solverFilter = 0; L.nFilterIterations = 0; break;
-    case 1: L.solverFilter = 1; break;
-    default: L.solverFilter = 2; break;
-    }
-    // Task::alignFrames (task.cpp:259-412)
-    double correlation = 0, corr = 0;
-    int iteration = 0, retv = SS_STOP_ERROR;
-    for (;;) {
-        iteration++;
-        int improved = 0;
-        if ((st = L.main_iteration(iteration, &improved))) return st;
-        if (out->all_params && iteration <= s->max_iterations)
-            HIPCHK(hipMemcpyAsync(out->all_params + P * (iteration - 1), g.p, P * sizeof(double),
-                                  hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(timed(h, SS_K_CORR, L.frameBytes * nF, [&] { return ss::launch_corr(g, h->stream); }));
-        if ((st = L.read_status())) return st;
-        corr = h->hstatus->corr;
-        if (out->iteration_corr && iteration <= s->max_iterations) out->iteration_corr[iteration - 1] = corr;
-        if (corr - correlation < s->correlation_increase_threshold) {
-            retv = SS_STOP_CORRELATION_INCREASE;
-            break;
-        } else if (corr - correlation < s->filter_correlation_threshold) {
-            if (L.filterID != 0) {
-                L.solverFilter = 0;
-                L.nFilterIterations = 0;
-                L.filterID = 0;
-                if (iteration >= s->max_iterations) {  // never run past max_iterations
-                    retv = SS_STOP_ITERATION_LIMIT;
-                    break;
-                }
-                continue;
+            P.tmpl.th = in.template_h;
+            P.tmpl.tw = in.template_w;
+            convert_template(in.template_h, in.template_w, in.template_image, in.xgrad, in.ygrad, P.tp,
+                             P.gxv, P.gyh, P.tmpl.gradFD);
+            const size_t tn = (size_t)in.template_h * in.template_w;
+            P.oTp = A.take<float2>(tn);
+            if (!P.tmpl.gradFD) {
+                P.oGxv = A.take<float2>(tn);
+                P.oGyh = A.take<float2>(tn);
             }
         }
-        if (!improved) {
-            retv = SS_STOP_MOVEMENT;
-            break;
-        } else if (s->max_iterations == iteration) {
-            retv = SS_STOP_ITERATION_LIMIT;
-            break;
-        } else if (correlation >= s->halt_correlation) {
-            retv = SS_STOP_HALT_CORRELATION;
-            break;
+        if (in.geometry_id > 0) {
+            auto it = h->geometries.find(in.geometry_id);
+            if (it == h->geometries.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown geometry_id");
+            P.geom = it->second;
+            if (P.geom.frameH != fh || P.geom.nb != nb)
+                return fail(h, SS_ERR_INVALID_ARGUMENT, "resident geometry does not match frame_h x n_blocks");
+        } else {
+            convert_geometry(fh, nb, in.frac, in.base_x, in.base_y, in.base_mask, P.geom, P.tab, P.fn);
+            if (!P.tab.empty()) P.oGeo = A.take<float4>(P.tab.size());
+            if (!P.fn.empty()) P.oFn = A.take<float>(P.fn.size());
         }
-        correlation = corr;
-        HIPCHK(hipMemcpyAsync(g.pLast, g.p, P * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        P.imgType = in.data_type == SS_U16 ? ss::IMG_U16 : in.data_type == SS_I16 ? ss::IMG_I16 : ss::IMG_F32;
+        P.imgBytes = (size_t)fh * fw * (P.imgType == ss::IMG_F32 ? 4 : 2);
+        if (!in.data_on_device) {
+            P.oImg = A.take<char>(P.imgBytes);
+            P.hImg = H.take<char>(P.imgBytes);
+        }
+        P.oMin = A.take<int>(nb);
+        const size_t Pn = 2 * (size_t)N;
+        P.oP = A.take<double>(Pn);
+        P.oPL = A.take<double>(Pn);
+        P.oP0 = A.take<double>(Pn);
+        P.oD0 = A.take<double>(Pn);
+        P.oDP = A.take<double>(Pn);
+        P.oErr = A.take<double>(2 * (size_t)nF);
+        P.oCol = A.take<double>((size_t)fw * ss::CS_N);
+        P.oLs = A.take<double>((size_t)fw * 2 * ss::kMaxCand);
+        P.oCorr = A.take<double>((size_t)fw * ss::CR_N);
+        P.oFin = A.take<double>((size_t)fw * ss::FS_N);
+        P.oWs = A.take<char>(ss::solve_workspace_bytes(fw));
+        P.oUsed = A.take<uint8_t>(N);
+        P.oAct = A.take<int>(nF);
+        P.oNext = A.take<int>(nF);
+        P.oMk = A.take<int>(nF);
+        P.oDk = A.take<int>(nF);
+        P.oArr = A.take<int>(nF);
+        if (P.out->mask) P.oMask = A.take<uint8_t>((size_t)fh * fw);
+        P.oRes = A.take<double>(nF);
+        P.oFc = A.take<double>(nF);
+        if (P.out->all_params) P.oAll = A.take<double>(Pn * P.s->max_iterations);
+        if (P.out->iteration_corr) P.oIc = A.take<double>(P.s->max_iterations);
+        P.hP = H.take<double>(Pn + 2 * (size_t)nF);
+        nItems += fw;
     }
-    if (iteration == 1)
-        HIPCHK(hipMemcpyAsync(g.pLast, g.p, P * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(timed(h, SS_K_FINAL, L.frameBytes * nF, [&] { return ss::launch_final(g, h->stream); }));
-    std::vector<double> pl(P);
-    HIPCHK(hipMemcpyAsync(pl.data(), g.pLast, P * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    if (out->residrms)
-        HIPCHK(hipMemcpyAsync(out->residrms, g.resid, nF * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    if (out->frame_corr)
-        HIPCHK(hipMemcpyAsync(out->frame_corr, g.fcorr, nF * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    if (out->mask) HIPCHK(hipMemcpyAsync(out->mask, g.maskOut, npix, hipMemcpyDeviceToHost, h->stream));
-    if ((st = L.read_status())) return st;
-    if (out->knots_x && out->knots_y) knots_out(pl.data(), nF, nb, out->knots_x, out->knots_y);
-    out->iterations = iteration;
-    out->stop_criterion = retv;
-    out->half_iterations = L.halfIterations;
-    out->linesearch_rounds = L.rounds;
-    out->linesearch_tries = h->hstatus->tries;
+    const size_t oItems = A.take<int>(nItems);
+    const size_t hGroups = H.take<ss::GroupDev>(n);
+    const size_t hCtl = H.take<ss::GroupCtl>(n);
+    const size_t hItems = H.take<int>(nItems);
+    ss_status st = ensure_arena(h, A.off, H.off);
+    if (st) return st;
+    char *D = (char *)h->arena;
+    char *HS = (char *)h->harena;
+    auto dptr = [&](size_t o) { return (void *)(D + o); };
+    // fill host staging and enqueue uploads
+    std::vector<ss::GroupDev> gd(n);
+    ss::GroupDev *hg = (ss::GroupDev *)(HS + hGroups);
+    ss::GroupCtl *hc = (ss::GroupCtl *)(HS + hCtl);
+    int *hitems = (int *)(HS + hItems);
+    int itemStart = 0;
+    int hasType[3] = {0, 0, 0};
+    for (int gi = 0; gi < n; gi++) {
+        Plan &P = plans[gi];
+        const ss_group_in &in = *P.in;
+        const ss_settings &s = *P.s;
+        const int fh = in.frame_h, nb = in.n_blocks, nF = in.n_frames, fw = nb * nF, N = fw + 1;
+        const size_t Pn = 2 * (size_t)N;
+        ss::GroupDev g;
+        std::memset(&g, 0, sizeof(g));
+        g.frameH = fh;
+        g.nb = nb;
+        g.nF = nF;
+        g.fframeW = fw;
+        g.N = N;
+        g.th = P.tmpl.th;
+        g.tw = P.tmpl.tw;
+        g.groupSize = in.group_size;
+        g.itemStart = itemStart;
+        g.seg = (fh + 255) / 256;
+        const double eps = s.precision == 'S' ? (double)FLT_EPSILON : DBL_EPSILON;
+        g.xmax = (float)((double)(g.tw - 1) - eps);
+        g.ymax = (float)((double)(g.th - 1) - eps);
+        g.rmult = s.linesearch_reduction_multiplier;
+        g.thr = s.move_threshold;
+        g.halt = s.halt_correlation;
+        g.cit = s.correlation_increase_threshold;
+        g.fct = s.filter_correlation_threshold;
+        g.strength = s.filter_strength;
+        g.lambda = s.lambda;
+        g.maxIter = s.max_iterations;
+        g.K1 = s.linesearch_candidates <= 0 ? 2 : s.linesearch_candidates;
+        g.Kn = std::max(g.K1, 4);
+        g.maxRounds = 4096 / g.Kn + 1;
+        // template
+        if (in.template_id > 0) {
+            g.tp = P.tmpl.tp;
+            g.gxv = P.tmpl.gxv;
+            g.gyh = P.tmpl.gyh;
+        } else {
+            g.tp = (const float2 *)dptr(P.oTp);
+            HIPCHK(hipMemcpyAsync(dptr(P.oTp), P.tp.data(), P.tp.size() * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+            if (!P.tmpl.gradFD) {
+                g.gxv = (const float2 *)dptr(P.oGxv);
+                g.gyh = (const float2 *)dptr(P.oGyh);
+                HIPCHK(hipMemcpyAsync(dptr(P.oGxv), P.gxv.data(), P.gxv.size() * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+                HIPCHK(hipMemcpyAsync(dptr(P.oGyh), P.gyh.data(), P.gyh.size() * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+            }
+        }
+        g.gradFD = P.tmpl.gradFD;
+        // geometry
+        const DevGeometry &G = P.geom;
+        g.affine = G.affine;
+        g.L = G.L;
+        g.gx0 = G.gx0; g.gxj = G.gxj; g.gxi = G.gxi;
+        g.gy0 = G.gy0; g.gyb = G.gyb; g.gyj = G.gyj; g.gyi = G.gyi;
+        g.f0 = G.f0; g.fj = G.fj; g.fi = G.fi;
+        g.fracUniform = G.fracUniform;
+        if (in.geometry_id > 0) {
+            g.geo = G.geo;
+            g.fnext = G.fnext;
+        } else {
+            if (!P.tab.empty()) {
+                g.geo = (const float4 *)dptr(P.oGeo);
+                HIPCHK(hipMemcpyAsync(dptr(P.oGeo), P.tab.data(), P.tab.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+            }
+            if (!P.fn.empty()) {
+                g.fnext = (const float *)dptr(P.oFn);
+                HIPCHK(hipMemcpyAsync(dptr(P.oFn), P.fn.data(), P.fn.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            }
+        }
+        // frame data
+        g.imgType = P.imgType;
+        hasType[P.imgType] = 1;
+        if (in.data_on_device) {
+            g.img = in.data;
+        } else {
+            void *stage = HS + P.hImg;
+            const size_t npix = (size_t)fh * fw;
+            if (in.data_type == SS_F64) {
+                const double *src = (const double *)in.data;
+                float *dst = (float *)stage;
+                for (size_t i = 0; i < npix; i++) dst[i] = (float)src[i];
+            } else {
+                std::memcpy(stage, in.data, P.imgBytes);
+            }
+            HIPCHK(hipMemcpyAsync(dptr(P.oImg), stage, P.imgBytes, hipMemcpyHostToDevice, h->stream));
+            g.img = dptr(P.oImg);
+        }
+        const size_t esz = P.imgType == ss::IMG_F32 ? 4 : 2;
+        g.vec = ((g.seg % 8) == 0 && (fh % 8) == 0 && ((uintptr_t)g.img % 16) == 0 && (esz == 2 || (fh % 4) == 0)) ? 1 : 0;
+        // min points (first n_blocks entries are read: minPointsPerBlock[c % frameW])
+        HIPCHK(hipMemcpyAsync(dptr(P.oMin), in.min_points, nb * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        g.minPts = (const int *)dptr(P.oMin);
+        g.p = (double *)dptr(P.oP);
+        g.pLast = (double *)dptr(P.oPL);
+        g.p0 = (double *)dptr(P.oP0);
+        g.d0 = (double *)dptr(P.oD0);
+        g.deltaP = (double *)dptr(P.oDP);
+        g.errval = (double *)dptr(P.oErr);
+        g.colsum = (double *)dptr(P.oCol);
+        g.lssum = (double *)dptr(P.oLs);
+        g.corrsum = (double *)dptr(P.oCorr);
+        g.finalsum = (double *)dptr(P.oFin);
+        g.ws = (double *)dptr(P.oWs);
+        g.used = (uint8_t *)dptr(P.oUsed);
+        g.lsActive = (int *)dptr(P.oAct);
+        g.lsNextK = (int *)dptr(P.oNext);
+        g.lsMk = (int *)dptr(P.oMk);
+        g.lsDk = (int *)dptr(P.oDk);
+        g.frameArrive = (int *)dptr(P.oArr);
+        g.maskOut = P.out->mask ? (uint8_t *)dptr(P.oMask) : nullptr;
+        g.resid = (double *)dptr(P.oRes);
+        g.fcorr = (double *)dptr(P.oFc);
+        g.allParams = P.out->all_params ? (double *)dptr(P.oAll) : nullptr;
+        g.iterCorr = P.out->iteration_corr ? (double *)dptr(P.oIc) : nullptr;
+        // initial knots (task_allocate.cu:130-134), errval = 1e6 (task_allocate.cu:327)
+        double *hp = (double *)(HS + P.hP);
+        std::memset(hp, 0, sizeof(double) * Pn);
+        knots_in(in.init_x, in.init_y, nb + 1, nF, nb, hp);
+        for (int i = 0; i < 2 * nF; i++) hp[Pn + i] = 1e6;
+        HIPCHK(hipMemcpyAsync(g.p, hp, Pn * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(g.errval, hp + Pn, 2 * nF * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemsetAsync(g.deltaP, 0, Pn * sizeof(double), h->stream));
+        HIPCHK(hipMemsetAsync(g.frameArrive, 0, nF * sizeof(int), h->stream));
+        hg[gi] = g;
+        // control state (task.cpp:54-70 filter selection)
+        ss::GroupCtl c;
+        std::memset(&c, 0, sizeof(c));
+        c.phase = ss::PH_ASSEMBLE;
+        c.shift = 0;
+        c.iteration = 1;
+        c.improved = 1;
+        c.filterID = s.filter_id;
+        c.nFilterIterations = s.max_filter_iterations;
+        switch (s.max_filter_iterations != 0 ? s.filter_id : 0) {
+        case 0: c.solverFilter = 0; c.nFilterIterations = 0; break;
+        case 1: c.solverFilter = 1; break;
+        default: c.solverFilter = 2; break;
+        }
+        c.retv = SS_STOP_ERROR;
+        hc[gi] = c;
+        for (int i = 0; i < fw; i++) hitems[itemStart + i] = gi;
+        itemStart += fw;
+    }
+    HIPCHK(hipMemcpyAsync(dptr(oGroups), hg, n * sizeof(ss::GroupDev), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(dptr(oCtl), hc, n * sizeof(ss::GroupCtl), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(dptr(oItems), hitems, nItems * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const double initTime = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+
+    ss::LaunchBatch L;
+    L.b.nGroups = n;
+    L.b.nItems = nItems;
+    L.b.groups = (ss::GroupDev *)dptr(oGroups);
+    L.b.ctl = (ss::GroupCtl *)dptr(oCtl);
+    L.b.itemGroup = (int *)dptr(oItems);
+    for (int i = 0; i < 3; i++) L.hasType[i] = hasType[i];
+    L.itemGrid = std::max(1, std::min(nItems, 8 * h->numCU));
+    static const int tick[] = {ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS,
+                               ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS,
+                               ss::ST_CORR, ss::ST_CORR_FINISH, ss::ST_FINAL, ss::ST_FINAL_FINISH};
+    bool done = false;
+    for (int t = 0; t < kTickCap && !done; t++) {
+        for (int stg : tick) HIPCHK(run_stage(h, L, stg));
+        HIPCHK(hipMemcpyAsync(hc, L.b.ctl, n * sizeof(ss::GroupCtl), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        done = true;
+        for (int gi = 0; gi < n; gi++)
+            if (hc[gi].phase != ss::PH_DONE && hc[gi].phase != ss::PH_ERROR) done = false;
+    }
+    if (!done) return fail(h, SS_ERR_NUMERICAL, "alignment did not terminate");
+    // outputs
+    ss_status result = SS_OK;
+    for (int gi = 0; gi < n; gi++) {
+        Plan &P = plans[gi];
+        const ss_group_in &in = *P.in;
+        ss_group_out &o = *P.out;
+        const ss::GroupCtl &c = hc[gi];
+        const ss::GroupDev &g = hg[gi];
+        const int nb = in.n_blocks, nF = in.n_frames, fw = nb * nF;
+        const size_t Pn = 2 * (size_t)(fw + 1);
+        if (c.phase == ss::PH_ERROR) {
+            if (result == SS_OK) {
+                result = (ss_status)c.error;
+                h->err = "line search did not reach the movement threshold (non-finite step?) in group " + std::to_string(gi);
+            }
+            continue;
+        }
+        std::vector<double> pl(Pn);
+        HIPCHK(hipMemcpyAsync(pl.data(), g.pLast, Pn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        if (o.residrms) HIPCHK(hipMemcpyAsync(o.residrms, g.resid, nF * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        if (o.frame_corr) HIPCHK(hipMemcpyAsync(o.frame_corr, g.fcorr, nF * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        if (o.mask) HIPCHK(hipMemcpyAsync(o.mask, g.maskOut, (size_t)in.frame_h * fw, hipMemcpyDeviceToHost, h->stream));
+        const int nrec = std::min(c.iteration, P.s->max_iterations);
+        if (o.all_params && nrec > 0)
+            HIPCHK(hipMemcpyAsync(o.all_params, g.allParams, Pn * nrec * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        if (o.iteration_corr && nrec > 0)
+            HIPCHK(hipMemcpyAsync(o.iteration_corr, g.iterCorr, nrec * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (o.knots_x && o.knots_y) knots_out(pl.data(), nF, nb, o.knots_x, o.knots_y);
+        o.iterations = c.iteration;
+        o.stop_criterion = c.retv;
+        o.half_iterations = c.halfIterations;
+        o.linesearch_rounds = c.rounds;
+        o.linesearch_tries = c.tries;
+        o.device_init_time = initTime;
+        // algorithmic frame bytes per stage class for the roofline
+        if (h->prof.on) {
+            const double fb = (double)in.frame_h * nb * (P.imgType == ss::IMG_F32 ? 4.0 : 2.0);
+            h->prof.bytes[SS_K_ASSEMBLE] += fb * nF * c.halfIterations;
+            h->prof.bytes[SS_K_LS_ROUND] += fb * c.lsFramePasses;
+            h->prof.bytes[SS_K_CORR] += fb * nF * c.iteration;
+            h->prof.bytes[SS_K_FINAL] += fb * nF;
+        }
+    }
     if (h->prof.on) prof_collect(h);
+    return result;
+}
+
+ss_status upload_owned(ss_handle *h, const void *src, size_t bytes, void *base, size_t off) {
+    if (!bytes) return SS_OK;
+    HIPCHK(hipMemcpy((char *)base + off, src, bytes, hipMemcpyHostToDevice));
     return SS_OK;
 }
 
@@ -566,9 +705,10 @@ ss_status ss_create(int32_t device_id, ss_handle **out) {
     if (hipSetDevice(device_id) != hipSuccess) return SS_ERR_DEVICE;
     ss_handle *h = new ss_handle();
     h->device = device_id;
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void **)&h->hstatus, sizeof(ss::Status), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void **)&h->dstatus, sizeof(ss::Status)) != hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device_id) == hipSuccess && prop.multiProcessorCount > 0)
+        h->numCU = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         ss_destroy(h);
         return SS_ERR_DEVICE;
     }
@@ -580,24 +720,17 @@ void ss_destroy(ss_handle *h) {
     if (!h) return;
     hipError_t e = hipSetDevice(h->device);
     if (h->stream) e = hipStreamSynchronize(h->stream);
-    for (auto &kv : h->bufs)
-        if (kv.second.p) e = hipFree(kv.second.p);
-    for (auto &kv : h->templates) {
-        e = hipFree(kv.second.tmpl);
-        e = hipFree(kv.second.xg);
-        e = hipFree(kv.second.yg);
-    }
-    for (auto &kv : h->geometries) {
-        e = hipFree(kv.second.geo);
-        if (kv.second.fnext) e = hipFree(kv.second.fnext);
-    }
+    for (auto &kv : h->templates)
+        if (kv.second.mem) e = hipFree(kv.second.mem);
+    for (auto &kv : h->geometries)
+        if (kv.second.mem) e = hipFree(kv.second.mem);
     for (auto &p : h->prof.pending) {
         e = hipEventDestroy(p.a);
         e = hipEventDestroy(p.b);
     }
     for (auto ev : h->prof.pool) e = hipEventDestroy(ev);
-    if (h->dstatus) e = hipFree(h->dstatus);
-    if (h->hstatus) e = hipHostFree(h->hstatus);
+    if (h->arena) e = hipFree(h->arena);
+    if (h->harena) e = hipHostFree(h->harena);
     if (h->stream) e = hipStreamDestroy(h->stream);
     (void)e;
     delete h;
@@ -608,24 +741,58 @@ const char *ss_last_error(const ss_handle *h) { return h ? h->err.c_str() : "no 
 ss_status ss_upload_template(ss_handle *h, int32_t th, int32_t tw, const double *t,
                              const double *xg, const double *yg, int32_t *id) {
     if (!h || !id) return SS_ERR_INVALID_ARGUMENT;
+    if (th < 2 || tw < 2 || !t || !xg || !yg) return fail(h, SS_ERR_INVALID_ARGUMENT, "template: bad size or NULL array");
     HIPCHK(hipSetDevice(h->device));
-    ResidentTemplate rt;
-    ss_status st = upload_template(h, th, tw, t, xg, yg, rt, true);
-    if (st) return st;
+    DevTemplate dt;
+    std::vector<float2> tp, gxv, gyh;
+    dt.th = th;
+    dt.tw = tw;
+    convert_template(th, tw, t, xg, yg, tp, gxv, gyh, dt.gradFD);
+    const size_t tn = (size_t)th * tw, pb = (tn * sizeof(float2) + 255) & ~(size_t)255;
+    HIPCHK(hipMalloc(&dt.mem, dt.gradFD ? pb : 3 * pb));
+    dt.tp = (float2 *)dt.mem;
+    ss_status st = upload_owned(h, tp.data(), tn * sizeof(float2), dt.mem, 0);
+    if (!st && !dt.gradFD) {
+        dt.gxv = (float2 *)((char *)dt.mem + pb);
+        dt.gyh = (float2 *)((char *)dt.mem + 2 * pb);
+        st = upload_owned(h, gxv.data(), tn * sizeof(float2), dt.mem, pb);
+        if (!st) st = upload_owned(h, gyh.data(), tn * sizeof(float2), dt.mem, 2 * pb);
+    }
+    if (st) {
+        (void)hipFree(dt.mem);
+        return st;
+    }
     *id = h->next_id++;
-    h->templates[*id] = rt;
+    h->templates[*id] = dt;
     return SS_OK;
 }
 
 ss_status ss_upload_geometry(ss_handle *h, int32_t fh, int32_t nb, const double *frac,
                              const double *bx, const double *by, const uint8_t *bm, int32_t *id) {
     if (!h || !id) return SS_ERR_INVALID_ARGUMENT;
+    if (fh < 1 || nb < 1 || !frac || !bx || !by || !bm) return fail(h, SS_ERR_INVALID_ARGUMENT, "geometry: bad size or NULL array");
     HIPCHK(hipSetDevice(h->device));
-    ResidentGeometry rg;
-    ss_status st = upload_geometry(h, fh, nb, frac, bx, by, bm, rg, true);
-    if (st) return st;
+    DevGeometry g;
+    std::vector<float4> tab;
+    std::vector<float> fn;
+    convert_geometry(fh, nb, frac, bx, by, bm, g, tab, fn);
+    const size_t tb = (tab.size() * sizeof(float4) + 255) & ~(size_t)255, fb = fn.size() * sizeof(float);
+    HIPCHK(hipMalloc(&g.mem, tb + fb + 256));
+    ss_status st = SS_OK;
+    if (!tab.empty()) {
+        g.geo = (float4 *)g.mem;
+        st = upload_owned(h, tab.data(), tab.size() * sizeof(float4), g.mem, 0);
+    }
+    if (!st && !fn.empty()) {
+        g.fnext = (float *)((char *)g.mem + tb);
+        st = upload_owned(h, fn.data(), fb, g.mem, tb);
+    }
+    if (st) {
+        (void)hipFree(g.mem);
+        return st;
+    }
     *id = h->next_id++;
-    h->geometries[*id] = rg;
+    h->geometries[*id] = g;
     return SS_OK;
 }
 
@@ -635,16 +802,13 @@ ss_status ss_release(ss_handle *h, int32_t id) {
     HIPCHK(hipStreamSynchronize(h->stream));
     auto t = h->templates.find(id);
     if (t != h->templates.end()) {
-        HIPCHK(hipFree(t->second.tmpl));
-        HIPCHK(hipFree(t->second.xg));
-        HIPCHK(hipFree(t->second.yg));
+        HIPCHK(hipFree(t->second.mem));
         h->templates.erase(t);
         return SS_OK;
     }
     auto g = h->geometries.find(id);
     if (g != h->geometries.end()) {
-        HIPCHK(hipFree(g->second.geo));
-        if (g->second.fnext) HIPCHK(hipFree(g->second.fnext));
+        HIPCHK(hipFree(g->second.mem));
         h->geometries.erase(g);
         return SS_OK;
     }
@@ -653,12 +817,19 @@ ss_status ss_release(ss_handle *h, int32_t id) {
 
 ss_status ss_align_group(ss_handle *h, const ss_group_in *in, const ss_settings *s,
                          ss_group_out *out) {
-    if (!h || !out) return SS_ERR_INVALID_ARGUMENT;
-    if (s && s->device_id != h->device)
-        return fail(h, SS_ERR_INVALID_ARGUMENT, "settings.device_id differs from the handle's device");
+    return ss_align_batch(h, 1, in, s, out);
+}
+
+ss_status ss_align_batch(ss_handle *h, int32_t n, const ss_group_in *in, const ss_settings *s,
+                         ss_group_out *out) {
+    if (!h || n < 0 || (n > 0 && (!in || !s || !out))) return SS_ERR_INVALID_ARGUMENT;
+    if (n == 0) return SS_OK;
+    for (int i = 0; i < n; i++)
+        if (s[i].device_id != h->device)
+            return fail(h, SS_ERR_INVALID_ARGUMENT, "settings.device_id differs from the handle's device");
     HIPCHK(hipSetDevice(h->device));
     h->err.clear();
-    return run_group(h, in, s, out);
+    return run_batch(h, n, in, s, out);
 }
 
 ss_status ss_profile_enable(ss_handle *h, int32_t on) {
@@ -682,16 +853,6 @@ ss_status ss_profile_read(ss_handle *h, int64_t *launches, double *ms, double *f
         if (launches) launches[i] = h->prof.launches[i];
         if (ms) ms[i] = h->prof.ms[i];
         if (frame_bytes) frame_bytes[i] = h->prof.bytes[i];
-    }
-    return SS_OK;
-}
-
-ss_status ss_align_batch(ss_handle *h, int32_t n, const ss_group_in *in, const ss_settings *s,
-                         ss_group_out *out) {
-    if (!h || n < 0 || (n > 0 && (!in || !s || !out))) return SS_ERR_INVALID_ARGUMENT;
-    for (int i = 0; i < n; i++) {
-        ss_status st = ss_align_group(h, in + i, s + i, out + i);
-        if (st) return st;
     }
     return SS_OK;
 }

@@ -127,7 +127,7 @@ extern "C" void ss_settings_default(ss_settings *s) {
     s->filter_id = 1;
     s->device_id = 0;
     s->precision = 'D';
-    s->linesearch_candidates = 4;
+    s->linesearch_candidates = 2;
 }
 
 extern "C" ss_status ss_geometry_size(const ss_geometry_desc *g, int32_t *hc, int32_t *wc,

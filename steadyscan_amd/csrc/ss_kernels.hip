@@ -1,30 +1,77 @@
 // ss_kernels.hip -- gfx950 kernels of the align_group_to_template path.
 //
-// Work decomposition: one workgroup (4 wave64) per line-block column of a
-// frame (frame f, block b = column c = f * nb + b of the reference's block
-// matrix).  A column is contiguous in HBM (frame_h pixels), so a wave reads
-// 64 consecutive pixels per load; per-column partial sums are reduced in
-// registers, then across the 4 waves through LDS, and written once as FP64.
-// Nothing is accumulated with atomics, so every result is run-to-run
-// deterministic.
+// A batch holds any number of alignment groups.  Every group runs its own
+// copy of the reference's control flow (Task<T>::alignFrames and
+// MainIteration<T>::execute) as a small state machine in device memory
+// (GroupCtl), so one host submission ("tick") advances every group by one
+// stage without host round trips:
+//   k_assemble -> k_solve -> k_ls_round ... -> k_corr -> k_corr_finish -> k_final ...
+// Each kernel skips the groups that are not in its phase.
+//
+// Pixel work: one workgroup (4 wave64) per line-block column (frame f,
+// block b = column c = f * nb + b of the block matrix).  Each thread owns a
+// contiguous segment of the column, read with 16-byte loads; neighbouring
+// lanes walk along the scan line, so template gathers of a wave touch a few
+// cache lines.  Per-column partial sums are reduced in registers, then
+// across waves through LDS in FP64.  No floating-point atomics: every result
+// is run-to-run deterministic.
 #include "ss_device.h"
 
-#include <float.h>
+#include <type_traits>
 
 namespace ss {
 
-constexpr int kWG = 256;
-constexpr int kSolveWG = 256;
+constexpr int kWG = 256;       // pixel kernels
+constexpr int kSolveWG = 256;  // per-group kernels
+constexpr int kCH = 8;         // pixels per unrolled chunk
 
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-template <typename TI>
-__device__ __forceinline__ float load_px(const TI *p, int r) { return (float)p[r]; }
+__device__ __forceinline__ void load8(const float *p, bool vec, int n, float (&v)[kCH]) {
+    if (vec && n >= kCH) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        const float4 b = *reinterpret_cast<const float4 *>(p + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < kCH; i++) v[i] = i < n ? p[i] : 0.f;
+    }
+}
+__device__ __forceinline__ void load8(const uint16_t *p, bool vec, int n, float (&v)[kCH]) {
+    if (vec && n >= kCH) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(p);
+        const unsigned w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[2 * i] = (float)(w[i] & 0xffffu);
+            v[2 * i + 1] = (float)(w[i] >> 16);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kCH; i++) v[i] = i < n ? (float)p[i] : 0.f;
+    }
+}
+__device__ __forceinline__ void load8(const int16_t *p, bool vec, int n, float (&v)[kCH]) {
+    if (vec && n >= kCH) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(p);
+        const unsigned w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[2 * i] = (float)(int16_t)(w[i] & 0xffffu);
+            v[2 * i + 1] = (float)((int)w[i] >> 16);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kCH; i++) v[i] = i < n ? (float)p[i] : 0.f;
+    }
+}
 
-// Reduces NV per-thread doubles over the workgroup; thread i < NV writes out[i].
+// Sum over the workgroup of NV per-thread doubles; thread i < NV gets sum i
+// in out (LDS-backed, returns after a barrier).
 template <int NV>
-__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out) {
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *out) {
     __shared__ double sh[kWG / 64][NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) {
@@ -45,9 +92,10 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out)
         for (int w = 0; w < kWG / 64; w++) s += sh[w][threadIdx.x];
         out[threadIdx.x] = s;
     }
+    __syncthreads();
 }
 
-__device__ __forceinline__ int block_min_int(int v) {
+__device__ __forceinline__ int block_min(int v) {
     __shared__ int sh[kWG / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_down(v, off, 64));
@@ -61,132 +109,291 @@ __device__ __forceinline__ int block_min_int(int v) {
     return m;
 }
 
-// Bilinear sample of the row-major template at (x, y), x/y inside the mask
-// bounds (task_differenceimage.cuh:13-32).  Also returns the cell.
-__device__ __forceinline__ float bilinear(const float *__restrict__ T, int tw, float x, float y,
-                                          int &t, float &fx, float &fy) {
-    const int ix = (int)x, iy = (int)y;
-    fx = x - (float)ix;
-    fy = y - (float)iy;
-    t = iy * tw + ix;
-    const float t00 = T[t], t01 = T[t + 1], t10 = T[t + tw], t11 = T[t + tw + 1];
-    return (1.f - fx) * ((1.f - fy) * t00 + fy * t10) + fx * ((1.f - fy) * t01 + fy * t11);
-}
-
-__device__ __forceinline__ bool in_mask(const GroupDev &g, float m, float x, float y) {
-    return m != 0.f && x >= 0.f && x < g.xmax && y >= 0.f && y < g.ymax;
-}
-
 __device__ __forceinline__ double reduce_pow(double v, double r, int k) {
     for (int i = 0; i < k; i++) v *= r;
     return v;
 }
 
+// Geometry of the pixels of one thread segment: affine in the block's
+// (column j, line i) coordinates -- the block matrix lists a block's pixels
+// column-major, row r = j * L + i -- or read from the per-pixel table.
+struct GeoIter {
+    float fj, fi, Lf, byb;
+    __device__ __forceinline__ void start(const GroupDev &g, int b, int r0) {
+        if (g.affine) {
+            const int j = r0 / g.L;
+            fj = (float)j;
+            fi = (float)(r0 - j * g.L);
+            Lf = (float)g.L;
+            byb = g.gy0 + g.gyb * (float)b;
+        }
+    }
+    __device__ __forceinline__ void at(const GroupDev &g, const float4 *geo, int r, float &bx,
+                                       float &by, float &f, bool &m) const {
+        if (g.affine) {
+            f = fmaf(g.fi, fi, fmaf(g.fj, fj, g.f0));
+            bx = fmaf(g.gxi, fi, fmaf(g.gxj, fj, g.gx0));
+            by = fmaf(g.gyi, fi, fmaf(g.gyj, fj, byb));
+            m = true;
+        } else {
+            const float4 q = geo[r];
+            bx = q.x;
+            by = q.y;
+            f = q.z;
+            m = q.w != 0.f;
+        }
+    }
+    __device__ __forceinline__ void next() {
+        fi += 1.f;
+        if (fi == Lf) {
+            fi = 0.f;
+            fj += 1.f;
+        }
+    }
+};
+
+__device__ __forceinline__ bool in_bounds(const GroupDev &g, float x, float y) {
+    return x >= 0.f && x < g.xmax && y >= 0.f && y < g.ymax;
+}
+
+// Bilinear template sample at (x, y) inside the mask bounds
+// (task_differenceimage.cuh:13-32) from the paired layout.
+struct Cell {
+    float fx, fy;
+    float2 top, bot;
+    int t;
+};
+__device__ __forceinline__ float sample(const GroupDev &g, float x, float y, Cell &cl) {
+    const int ix = (int)x, iy = (int)y;
+    cl.fx = x - (float)ix;
+    cl.fy = y - (float)iy;
+    cl.t = iy * g.tw + ix;
+    cl.top = g.tp[cl.t];
+    cl.bot = g.tp[cl.t + g.tw];
+    const float a = fmaf(cl.fx, cl.top.y - cl.top.x, cl.top.x);
+    const float b = fmaf(cl.fx, cl.bot.y - cl.bot.x, cl.bot.x);
+    return fmaf(cl.fy, b - a, a);
+}
+
+// Knots of a column for candidate k of the line search (or the half-
+// iteration start state for k < 0), as x = base_x + a + d * frac.
+struct ColKnots {
+    float ax, dx, ay, dy;
+};
+__device__ __forceinline__ ColKnots state_knots(const GroupDev &g, int c, int k) {
+    const int N = g.N;
+    double lx, ly, rx, ry;
+    if (k < 0) {  // half-iteration start: p0 with the real right knot
+        lx = g.p0[c];
+        ly = g.p0[c + N];
+        rx = g.p0[c + 1];
+        ry = g.p0[c + 1 + N];
+    } else {
+        const double r = g.rmult;
+        lx = g.p0[c] + reduce_pow(g.d0[c], r, k);
+        ly = g.p0[c + N] + reduce_pow(g.d0[c + N], r, k);
+        if (c + 1 == g.fframeW) {  // pnew of the group's last knot is never written
+            rx = 1e6;
+            ry = 1e6;
+        } else {
+            rx = g.p0[c + 1] + reduce_pow(g.d0[c + 1], r, k);
+            ry = g.p0[c + 1 + N] + reduce_pow(g.d0[c + 1 + N], r, k);
+        }
+    }
+    ColKnots kn;
+    kn.ax = (float)lx;
+    kn.dx = (float)(rx - lx);
+    kn.ay = (float)ly;
+    kn.dy = (float)(ry - ly);
+    return kn;
+}
+
+// Knots of column c from the current parameter vector p (applyParameters on p).
+__device__ __forceinline__ ColKnots current_knots(const GroupDev &g, int c) {
+    const int N = g.N;
+    const double lx = g.p[c], ly = g.p[c + N], rx = g.p[c + 1], ry = g.p[c + 1 + N];
+    return ColKnots{(float)lx, (float)(rx - lx), (float)ly, (float)(ry - ly)};
+}
+
+template <typename TI>
+__device__ __forceinline__ const TI *column_ptr(const GroupDev &g, int c) {
+    return reinterpret_cast<const TI *>(g.img) + (size_t)c * g.frameH;
+}
+
 // ---------------------------------------------------------------------------
-// assembly: warp + residual + errval terms + J^T J / J^T d partials
+// assembly pass: warp + residual + errval terms + J^T J / J^T d partials
 // (d_applyParameters, d_createDifferenceImage, d_calcBaseErrval,
-//  d_updateGradients, d_fillDiagonals, d_fillJTr fused into one pass)
+//  d_updateGradients, d_fillDiagonals, d_fillJTr in one pass)
 // ---------------------------------------------------------------------------
 template <typename TI>
-__global__ __launch_bounds__(kWG) void k_assemble(GroupDev g) {
-    const int c = blockIdx.x;
+__device__ void assemble_column(const GroupDev &g, int c) {
     const int b = c % g.nb;
-    const int N = g.N;
-    const float lx = (float)g.p[c], rx = (float)g.p[c + 1];
-    const float ly = (float)g.p[c + N], ry = (float)g.p[c + 1 + N];
-    const TI *img = (const TI *)g.img + (size_t)c * g.frameH;
-    const float4 *geo = g.geo + (size_t)b * g.frameH;
-    const float *fn = g.fracUniform ? nullptr : g.fnext + (size_t)b * g.frameH;
-    const int tw = g.tw;
-    double d2 = 0.0;
-    int cnt = 0;
-    float h[CS_N - CS_AXX];
+    const ColKnots kn = current_knots(g, c);
+    const TI *img = column_ptr<TI>(g, c);
+    const float4 *geo = g.geo ? g.geo + (size_t)b * g.frameH : nullptr;
+    const float *fnx = g.fracUniform ? nullptr : g.fnext + (size_t)b * g.frameH;
+    const int r0 = threadIdx.x * g.seg;
+    const int n = min(g.seg, g.frameH - r0);
+    GeoIter it;
+    it.start(g, b, r0);
+    // h[0] = sum d^2, h[1] = sum I, h[i - 1] = column sum i for i >= CS_AXX
+    float h[CS_N - 1];
 #pragma unroll
-    for (int i = 0; i < CS_N - CS_AXX; i++) h[i] = 0.f;
-    for (int r = threadIdx.x; r < g.frameH; r += kWG) {
-        const float4 q = geo[r];
-        const float f = q.z, cf = 1.f - f;
-        const float x = q.x + lx * cf + rx * f;
-        const float y = q.y + ly * cf + ry * f;
-        if (in_mask(g, q.w, x, y)) {
-            int t;
-            float fx, fy;
-            const float wI = bilinear(g.tmpl, tw, x, y, t, fx, fy);
-            const float d = load_px(img, r) - wI;
-            const float wx = (1.f - fy) * g.xg[t] + fy * g.xg[t + tw];
-            const float wy = (1.f - fx) * g.yg[t] + fx * g.yg[t + 1];
-            d2 += (double)d * (double)d;
-            cnt++;
-            const float xx = wx * wx, yy = wy * wy, xy = wx * wy;
-            const float c2 = cf * cf, f2 = f * f, fc = f - f2;
-            h[CS_AXX - CS_AXX] += xx * c2;
-            h[CS_AYY - CS_AXX] += yy * c2;
-            h[CS_AXY - CS_AXX] += xy * c2;
-            h[CS_BXX - CS_AXX] += xx * f2;
-            h[CS_BYY - CS_AXX] += yy * f2;
-            h[CS_BXY - CS_AXX] += xy * f2;
-            h[CS_CXX - CS_AXX] += xx * fc;
-            h[CS_CYY - CS_AXX] += yy * fc;
-            h[CS_CXY - CS_AXX] += xy * fc;
-            const float dc = d * cf, df = d * f;
-            h[CS_JXA - CS_AXX] += dc * wx;
-            h[CS_JYA - CS_AXX] += dc * wy;
-            h[CS_JXB - CS_AXX] += df * wx;
-            h[CS_JYB - CS_AXX] += df * wy;
-            if (fn) {
-                const float n2 = fn[r] * fn[r];
-                h[CS_QXX - CS_AXX] += xx * n2;
-                h[CS_QYY - CS_AXX] += yy * n2;
-                h[CS_QXY - CS_AXX] += xy * n2;
+    for (int i = 0; i < CS_N - 1; i++) h[i] = 0.f;
+    int cnt = 0;
+    for (int ch = 0; ch < n; ch += kCH) {
+        float I[kCH];
+        load8(img + r0 + ch, g.vec, n - ch, I);
+#pragma unroll
+        for (int k = 0; k < kCH; k++) {
+            if (ch + k < n) {
+                float bx, by, f;
+                bool m;
+                it.at(g, geo, r0 + ch + k, bx, by, f, m);
+                it.next();
+                h[CS_SUMI] += I[k];
+                const float x = fmaf(kn.dx, f, bx + kn.ax);
+                const float y = fmaf(kn.dy, f, by + kn.ay);
+                if (m && in_bounds(g, x, y)) {
+                    Cell cl;
+                    const float d = I[k] - sample(g, x, y, cl);
+                    float wx, wy;
+                    if (g.gradFD) {
+                        wx = fmaf(cl.fy, (cl.bot.y - cl.bot.x) - (cl.top.y - cl.top.x), cl.top.y - cl.top.x);
+                        wy = fmaf(cl.fx, (cl.bot.y - cl.top.y) - (cl.bot.x - cl.top.x), cl.bot.x - cl.top.x);
+                    } else {
+                        const float2 gx = g.gxv[cl.t], gy = g.gyh[cl.t];
+                        wx = fmaf(cl.fy, gx.y - gx.x, gx.x);
+                        wy = fmaf(cl.fx, gy.y - gy.x, gy.x);
+                    }
+                    const float cf = 1.f - f;
+                    const float xx = wx * wx, yy = wy * wy, xy = wx * wy;
+                    const float c2 = cf * cf, f2 = f * f, fc = f - f2;
+                    cnt++;
+                    h[CS_D2] = fmaf(d, d, h[CS_D2]);
+                    h[CS_AXX - 1] = fmaf(xx, c2, h[CS_AXX - 1]);
+                    h[CS_AYY - 1] = fmaf(yy, c2, h[CS_AYY - 1]);
+                    h[CS_AXY - 1] = fmaf(xy, c2, h[CS_AXY - 1]);
+                    h[CS_BXX - 1] = fmaf(xx, f2, h[CS_BXX - 1]);
+                    h[CS_BYY - 1] = fmaf(yy, f2, h[CS_BYY - 1]);
+                    h[CS_BXY - 1] = fmaf(xy, f2, h[CS_BXY - 1]);
+                    h[CS_CXX - 1] = fmaf(xx, fc, h[CS_CXX - 1]);
+                    h[CS_CYY - 1] = fmaf(yy, fc, h[CS_CYY - 1]);
+                    h[CS_CXY - 1] = fmaf(xy, fc, h[CS_CXY - 1]);
+                    const float dc = d * cf, df = d * f;
+                    h[CS_JXA - 1] = fmaf(dc, wx, h[CS_JXA - 1]);
+                    h[CS_JYA - 1] = fmaf(dc, wy, h[CS_JYA - 1]);
+                    h[CS_JXB - 1] = fmaf(df, wx, h[CS_JXB - 1]);
+                    h[CS_JYB - 1] = fmaf(df, wy, h[CS_JYB - 1]);
+                    if (fnx) {
+                        const float fn = fnx[r0 + ch + k];
+                        const float n2 = fn * fn;
+                        h[CS_QXX - 1] = fmaf(xx, n2, h[CS_QXX - 1]);
+                        h[CS_QYY - 1] = fmaf(yy, n2, h[CS_QYY - 1]);
+                        h[CS_QXY - 1] = fmaf(xy, n2, h[CS_QXY - 1]);
+                    }
+                }
             }
         }
     }
     double v[CS_N];
-    v[CS_D2] = d2;
+    v[CS_D2] = (double)h[CS_D2];
+    v[CS_SUMI] = (double)h[CS_SUMI];
     v[CS_CNT] = (double)cnt;
 #pragma unroll
-    for (int i = CS_AXX; i < CS_N; i++) v[i] = (double)h[i - CS_AXX];
-    block_reduce_store<CS_N>(v, g.colsum + (size_t)c * CS_N);
+    for (int i = CS_AXX; i < CS_N; i++) v[i] = (double)h[i - 1];
+    __shared__ double out[CS_N];
+    block_sum<CS_N>(v, out);
+    if (threadIdx.x < CS_N) g.colsum[(size_t)c * CS_N + threadIdx.x] = out[threadIdx.x];
+}
+
+template <typename TI>
+__global__ __launch_bounds__(kWG) void k_assemble(Batch B) {
+    for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
+        const int gi = B.itemGroup[item];
+        if (B.ctl[gi].phase != PH_ASSEMBLE) continue;
+        const GroupDev &g = B.groups[gi];
+        if (g.imgType != (sizeof(TI) == 4 ? IMG_F32 : (std::is_signed<TI>::value ? IMG_I16 : IMG_U16))) continue;
+        assemble_column<TI>(g, item - g.itemStart);
+    }
 }
 
 // ---------------------------------------------------------------------------
 // solve: errvals, knot usage, block-tridiagonal normal equations, exact FP64
-// solve by block cyclic reduction, step filter, red/green expansion.
-// One workgroup per group.
+// solve (partitioned block Thomas + cyclic reduction of the interface
+// system), step filter, red/green expansion and line-search set-up.  One
+// workgroup per group.
 // ---------------------------------------------------------------------------
-struct Eq {  // A x_{i-1} + B x_i + C x_{i+1} = d (2x2 row-major blocks)
-    double A[4], B[4], C[4], d[2], x[2];
+// level-0 equation of knot i:  O_{i-1} x_{i-1} + D_i x_i + O_i x_{i+1} = g_i,
+// with D_i, O_i symmetric 2x2 ([xx xy; xy yy]).
+struct Eq0 {
+    double b0, b1, b2;  // D
+    double c0, c1, c2;  // O (coupling to knot i+1)
+    double d0, d1;      // rhs
 };
 
-__device__ __forceinline__ void inv2(const double *M, double *R) {
-    const double det = M[0] * M[3] - M[1] * M[2];
-    const double id = 1.0 / det;
-    R[0] = M[3] * id;
-    R[1] = -M[1] * id;
-    R[2] = -M[2] * id;
-    R[3] = M[0] * id;
+struct M2 {  // general 2x2, row-major
+    double a, b, c, d;
+};
+__device__ __forceinline__ M2 sym(double x0, double x1, double x2) { return M2{x0, x1, x1, x2}; }
+__device__ __forceinline__ M2 inv(const M2 &m) {
+    const double id = 1.0 / (m.a * m.d - m.b * m.c);
+    return M2{m.d * id, -m.b * id, -m.c * id, m.a * id};
 }
-__device__ __forceinline__ void mul2(const double *A, const double *B, double *R) {
-    R[0] = A[0] * B[0] + A[1] * B[2];
-    R[1] = A[0] * B[1] + A[1] * B[3];
-    R[2] = A[2] * B[0] + A[3] * B[2];
-    R[3] = A[2] * B[1] + A[3] * B[3];
+__device__ __forceinline__ M2 mul(const M2 &x, const M2 &y) {
+    return M2{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c,
+              x.c * y.b + x.d * y.d};
 }
-__device__ __forceinline__ void mulv2(const double *A, const double *v, double *r) {
-    r[0] = A[0] * v[0] + A[1] * v[1];
-    r[1] = A[2] * v[0] + A[3] * v[1];
-}
-
-__device__ __forceinline__ int level_size(int n, int lvl) {
-    int m = n;
-    for (int i = 0; i < lvl; i++) m >>= 1;
-    return m;
+__device__ __forceinline__ M2 add(const M2 &x, const M2 &y) { return M2{x.a + y.a, x.b + y.b, x.c + y.c, x.d + y.d}; }
+__device__ __forceinline__ M2 sub(const M2 &x, const M2 &y) { return M2{x.a - y.a, x.b - y.b, x.c - y.c, x.d - y.d}; }
+__device__ __forceinline__ M2 neg(const M2 &x) { return M2{-x.a, -x.b, -x.c, -x.d}; }
+__device__ __forceinline__ M2 zero2() { return M2{0, 0, 0, 0}; }
+__device__ __forceinline__ M2 eye2() { return M2{1, 0, 0, 1}; }
+__device__ __forceinline__ double2 mulv(const M2 &m, double2 v) {
+    return make_double2(m.a * v.x + m.b * v.y, m.c * v.x + m.d * v.y);
 }
 
-__global__ __launch_bounds__(kSolveWG) void k_solve(GroupDev g, int shift, int filterKind,
-                                                    double strength, double lambda) {
+// x as an affine function of the neighbouring interface unknowns:
+// x = y + V z_prev + W z_own
+struct Aff {
+    double2 y;
+    M2 V, W;
+};
+__device__ __forceinline__ Aff aff_apply(const M2 &m, const Aff &a) {
+    return Aff{mulv(m, a.y), mul(m, a.V), mul(m, a.W)};
+}
+__device__ __forceinline__ Aff aff_sub(const Aff &a, const Aff &b) {
+    return Aff{make_double2(a.y.x - b.y.x, a.y.y - b.y.y), sub(a.V, b.V), sub(a.W, b.W)};
+}
+
+struct RedEq {  // reduced (interface) equation, general 2x2 blocks
+    M2 A, B, C;
+    double2 d, x;
+};
+
+struct InteriorRow {  // forward-eliminated interior row: Bh^{-1} and rhs
+    M2 Bi;
+    Aff R;
+};
+
+__global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
+    const int gi = blockIdx.x;
+    GroupCtl &ctl = B.ctl[gi];
+    if (ctl.phase != PH_ASSEMBLE) return;
+    const GroupDev &g = B.groups[gi];
     const int tid = threadIdx.x;
     const int nF = g.nF, nb = g.nb, N = g.N, n = g.fframeW;
+    const int shift = ctl.shift;
+    // filter deactivation after max_filter_iterations (mainiteration.cpp:80-90)
+    __shared__ int s_filter;
+    if (tid == 0) {
+        if (ctl.nFilterIterations != 0 && ctl.iteration > ctl.nFilterIterations) {
+            ctl.solverFilter = 0;
+            ctl.filterID = 0;
+            ctl.nFilterIterations = 0;
+        }
+        s_filter = ctl.solverFilter;
+    }
     const double *S = g.colsum;
     // 1. per-frame errval of this shift (d_calcAllGroupErrval)
     for (int f = tid; f < nF; f += kSolveWG) {
@@ -207,159 +414,213 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(GroupDev g, int shift, int f
         }
         g.used[k] = u;
     }
+    __threadfence_block();
     __syncthreads();
-    // 3. level-0 equations (d_fillDiagonals / d_fillJTr entries)
-    Eq *L0 = reinterpret_cast<Eq *>(g.ws);
+    // 3. level-0 equations (d_fillDiagonals / d_fillJTr entries, lambda extension)
+    Eq0 *E = reinterpret_cast<Eq0 *>(g.ws);
+    const double lambda = g.lambda;
     for (int k = tid; k < n; k += kSolveWG) {
-        Eq e;
+        Eq0 e;
         const double *s = S + (size_t)k * CS_N;
         const bool u = g.used[k];
         const bool up = k > 0 && g.used[k - 1];
         const bool un = (k + 1 < n) && g.used[k + 1];
         if (!u) {
-            for (int i = 0; i < 4; i++) { e.A[i] = 0; e.C[i] = 0; e.B[i] = 0; }
-            e.B[0] = 1;
-            e.B[3] = 1;
-            e.d[0] = e.d[1] = 0;
+            e = Eq0{1, 0, 1, 0, 0, 0, 0, 0};
         } else {
             double dxx = s[CS_AXX], dyy = s[CS_AYY], dxy = s[CS_AXY];
             double jx = s[CS_JXA], jy = s[CS_JYA];
             if (k > 0) {
                 const double *sp = s - CS_N;
-                const int qx = g.fracUniform ? CS_BXX : CS_QXX;
-                dxx += sp[qx];
-                dyy += sp[qx + 1];
-                dxy += sp[qx + 2];
+                const int q = g.fracUniform ? CS_BXX : CS_QXX;
+                dxx += sp[q];
+                dyy += sp[q + 1];
+                dxy += sp[q + 2];
                 jx += sp[CS_JXB];
                 jy += sp[CS_JYB];
             }
             const double nl = (double)(up ? 1 : 0) + (double)(un ? 1 : 0);
-            e.B[0] = dxx + lambda * nl;
-            e.B[1] = dxy;
-            e.B[2] = dxy;
-            e.B[3] = dyy + lambda * nl;
+            e.b0 = dxx + lambda * nl;
+            e.b1 = dxy;
+            e.b2 = dyy + lambda * nl;
             if (un) {
-                e.C[0] = s[CS_CXX] - lambda;
-                e.C[1] = s[CS_CXY];
-                e.C[2] = s[CS_CXY];
-                e.C[3] = s[CS_CYY] - lambda;
+                e.c0 = s[CS_CXX] - lambda;
+                e.c1 = s[CS_CXY];
+                e.c2 = s[CS_CYY] - lambda;
             } else {
-                e.C[0] = e.C[1] = e.C[2] = e.C[3] = 0;
+                e.c0 = e.c1 = e.c2 = 0;
             }
-            if (up) {
-                const double *sp = s - CS_N;
-                e.A[0] = sp[CS_CXX] - lambda;
-                e.A[1] = sp[CS_CXY];
-                e.A[2] = sp[CS_CXY];
-                e.A[3] = sp[CS_CYY] - lambda;
-            } else {
-                e.A[0] = e.A[1] = e.A[2] = e.A[3] = 0;
-            }
-            e.d[0] = jx;
-            e.d[1] = jy;
+            e.d0 = jx;
+            e.d1 = jy;
         }
-        e.x[0] = e.x[1] = 0;
-        L0[k] = e;
+        E[k] = e;
     }
     __threadfence_block();
     __syncthreads();
-    // 4. block cyclic reduction, forward
+    // 4. partitioned solve: thread t owns knots [s, e); interior rows s..e-2
+    //    are eliminated in terms of the interfaces z_{t-1} = x_{s-1} and
+    //    z_t = x_{e-1}; the interface system is solved by cyclic reduction.
+    const int m = (n + kSolveWG - 1) / kSolveWG;
+    const int P = (n + m - 1) / m;
+    const int s0 = tid * m, e0 = min(n, s0 + m);
+    InteriorRow *IR = reinterpret_cast<InteriorRow *>(g.ws + (size_t)n * 8);
+    double2 *X = reinterpret_cast<double2 *>(IR + n);
+    Aff *firstAff = reinterpret_cast<Aff *>(X + n);
+    __shared__ RedEq red[kSolveWG];
+    __shared__ int hasInt[kSolveWG];
+    Aff lastA;  // last interior row as a function of (z_prev, z_own)
+    if (tid < P) {
+        const bool interior = e0 - s0 >= 2;
+        hasInt[tid] = interior ? 1 : 0;
+        if (interior) {
+            // forward elimination
+            M2 BiPrev = zero2();
+            Aff Rprev;
+            for (int i = s0; i <= e0 - 2; i++) {
+                const Eq0 q = E[i];
+                M2 Bh = sym(q.b0, q.b1, q.b2);
+                Aff R{make_double2(q.d0, q.d1), zero2(), zero2()};
+                if (i == s0) {
+                    if (tid > 0) {
+                        const Eq0 qp = E[i - 1];
+                        R.V = neg(sym(qp.c0, qp.c1, qp.c2));
+                    }
+                } else {
+                    const Eq0 qp = E[i - 1];
+                    const M2 Ap = sym(qp.c0, qp.c1, qp.c2);  // A_i = O_{i-1}
+                    const M2 Lm = mul(Ap, BiPrev);
+                    Bh = sub(Bh, mul(Lm, Ap));
+                    R = aff_sub(R, aff_apply(Lm, Rprev));
+                }
+                if (i == e0 - 2) R.W = neg(sym(q.c0, q.c1, q.c2));
+                const M2 Bi = inv(Bh);
+                IR[i] = InteriorRow{Bi, R};
+                BiPrev = Bi;
+                Rprev = R;
+            }
+            // back substitution of the affine forms
+            Aff Xa = aff_apply(BiPrev, Rprev);
+            lastA = Xa;
+            for (int i = e0 - 3; i >= s0; i--) {
+                const InteriorRow ir = IR[i];
+                const Eq0 q = E[i];
+                const Aff t = aff_sub(ir.R, aff_apply(sym(q.c0, q.c1, q.c2), Xa));
+                Xa = aff_apply(ir.Bi, t);
+            }
+            firstAff[tid] = Xa;
+        } else {
+            lastA = Aff{make_double2(0, 0), eye2(), zero2()};       // x_{e-2} := z_prev
+            firstAff[tid] = Aff{make_double2(0, 0), zero2(), eye2()};  // unused
+        }
+    }
+    __syncthreads();
+    if (tid < P) {
+        const int j = e0 - 1;
+        const Eq0 q = E[j];
+        M2 A = zero2();
+        if (j > 0) {
+            const Eq0 qp = E[j - 1];
+            A = sym(qp.c0, qp.c1, qp.c2);
+        }
+        const M2 C = sym(q.c0, q.c1, q.c2);
+        // x_{j+1}: first interior of partition t+1, or z_{t+1}
+        Aff F{make_double2(0, 0), zero2(), eye2()};
+        if (tid + 1 < P && hasInt[tid + 1]) F = firstAff[tid + 1];
+        RedEq r;
+        r.A = mul(A, lastA.V);
+        r.B = add(add(sym(q.b0, q.b1, q.b2), mul(A, lastA.W)), mul(C, F.V));
+        r.C = mul(C, F.W);
+        const double2 ay = mulv(A, lastA.y), cy = mulv(C, F.y);
+        r.d = make_double2(q.d0 - ay.x - cy.x, q.d1 - ay.y - cy.y);
+        r.x = make_double2(0, 0);
+        if (tid == 0) r.A = zero2();
+        red[tid] = r;
+    }
+    __syncthreads();
+    // cyclic reduction, in place: level-l equation k lives at k*2^l + 2^l - 1
     int nlev = 0;
-    {
-        int m = n;
-        while (m > 1) { m >>= 1; nlev++; }
-    }
-    int off = 0;
-    for (int lvl = 0; lvl < nlev; lvl++) {
-        const int m = level_size(n, lvl), mn = m >> 1;
-        Eq *Lc = reinterpret_cast<Eq *>(g.ws) + off;
-        Eq *Ln = Lc + m;
-        for (int j = tid; j < mn; j += kSolveWG) {
-            const int i = 2 * j + 1;
-            const Eq &e = Lc[i], &em = Lc[i - 1];
-            double iv[4], al[4], t[4];
-            Eq o;
-            inv2(em.B, iv);
-            mul2(e.A, iv, al);
-            for (int q = 0; q < 4; q++) al[q] = -al[q];
-            mul2(al, em.A, o.A);
-            mul2(al, em.C, t);
-            for (int q = 0; q < 4; q++) o.B[q] = e.B[q] + t[q];
-            double tv[2];
-            mulv2(al, em.d, tv);
-            o.d[0] = e.d[0] + tv[0];
-            o.d[1] = e.d[1] + tv[1];
-            if (i + 1 < m) {
-                const Eq &ep = Lc[i + 1];
-                double ga[4];
-                inv2(ep.B, iv);
-                mul2(e.C, iv, ga);
-                for (int q = 0; q < 4; q++) ga[q] = -ga[q];
-                mul2(ga, ep.A, t);
-                for (int q = 0; q < 4; q++) o.B[q] += t[q];
-                mul2(ga, ep.C, o.C);
-                mulv2(ga, ep.d, tv);
-                o.d[0] += tv[0];
-                o.d[1] += tv[1];
+    for (int mm = P; mm > 1; mm >>= 1) nlev++;
+    for (int l = 0; l < nlev; l++) {
+        const int st = 1 << l, mm = P >> l;
+        for (int jj = tid; jj < (mm >> 1); jj += kSolveWG) {
+            const int i = (2 * jj + 1) * st + st - 1;
+            const int im = i - st, ip = i + st;
+            RedEq e = red[i];
+            const RedEq em = red[im];
+            const M2 al = neg(mul(e.A, inv(em.B)));
+            RedEq o;
+            o.A = mul(al, em.A);
+            o.B = add(e.B, mul(al, em.C));
+            double2 t = mulv(al, em.d);
+            o.d = make_double2(e.d.x + t.x, e.d.y + t.y);
+            if (2 * jj + 2 < mm) {
+                const RedEq ep = red[ip];
+                const M2 ga = neg(mul(e.C, inv(ep.B)));
+                o.B = add(o.B, mul(ga, ep.A));
+                o.C = mul(ga, ep.C);
+                t = mulv(ga, ep.d);
+                o.d = make_double2(o.d.x + t.x, o.d.y + t.y);
             } else {
-                o.C[0] = o.C[1] = o.C[2] = o.C[3] = 0;
+                o.C = zero2();
             }
-            o.x[0] = o.x[1] = 0;
-            Ln[j] = o;
+            o.x = make_double2(0, 0);
+            red[i] = o;
         }
-        off += m;
-        __threadfence_block();
         __syncthreads();
     }
-    // top level has one equation
-    if (tid == 0 && n > 0) {
-        Eq *Lt = reinterpret_cast<Eq *>(g.ws) + off;
-        double iv[4];
-        inv2(Lt[0].B, iv);
-        mulv2(iv, Lt[0].d, Lt[0].x);
+    if (tid == 0 && P > 0) {
+        const int i = (1 << nlev) - 1;
+        red[i].x = mulv(inv(red[i].B), red[i].d);
+    }
+    __syncthreads();
+    for (int l = nlev - 1; l >= 0; l--) {
+        const int st = 1 << l, mm = P >> l;
+        for (int kk = tid; kk < (mm + 1) / 2; kk += kSolveWG) {
+            const int k = 2 * kk;  // even equations of level l
+            const int i = k * st + st - 1;
+            const RedEq e = red[i];
+            double2 r = e.d;
+            if (k > 0) {
+                const double2 t = mulv(e.A, red[i - st].x);
+                r.x -= t.x;
+                r.y -= t.y;
+            }
+            if (k + 1 < mm) {
+                const double2 t = mulv(e.C, red[i + st].x);
+                r.x -= t.x;
+                r.y -= t.y;
+            }
+            red[i].x = mulv(inv(e.B), r);
+        }
+        __syncthreads();
+    }
+    // interior reconstruction: x_i = Bi_i (R_i(z) - O_i x_{i+1})
+    if (tid < P) {
+        const double2 zo = red[tid].x;
+        const double2 zp = tid > 0 ? red[tid - 1].x : make_double2(0, 0);
+        X[e0 - 1] = zo;
+        double2 xn = zo;
+        for (int i = e0 - 2; i >= s0; i--) {
+            const InteriorRow ir = IR[i];
+            const double2 v = mulv(ir.R.V, zp), w = mulv(ir.R.W, zo);
+            double2 r = make_double2(ir.R.y.x + v.x + w.x, ir.R.y.y + v.y + w.y);
+            if (i < e0 - 2) {
+                const Eq0 q = E[i];
+                const double2 t = mulv(sym(q.c0, q.c1, q.c2), xn);
+                r.x -= t.x;
+                r.y -= t.y;
+            }
+            xn = mulv(ir.Bi, r);
+            X[i] = xn;
+        }
     }
     __threadfence_block();
     __syncthreads();
-    // 5. back substitution
-    for (int lvl = nlev - 1; lvl >= 0; lvl--) {
-        const int m = level_size(n, lvl);
-        off -= m;
-        Eq *Lc = reinterpret_cast<Eq *>(g.ws) + off;
-        Eq *Ln = Lc + m;
-        for (int i = tid; i < m; i += kSolveWG) {
-            if (i & 1) {
-                Lc[i].x[0] = Ln[i >> 1].x[0];
-                Lc[i].x[1] = Ln[i >> 1].x[1];
-            }
-        }
-        __threadfence_block();
-        __syncthreads();
-        for (int i = tid; i < m; i += kSolveWG) {
-            if (!(i & 1)) {
-                const Eq &e = Lc[i];
-                double r[2] = {e.d[0], e.d[1]}, tv[2];
-                if (i > 0) {
-                    mulv2(e.A, Lc[i - 1].x, tv);
-                    r[0] -= tv[0];
-                    r[1] -= tv[1];
-                }
-                if (i + 1 < m) {
-                    mulv2(e.C, Lc[i + 1].x, tv);
-                    r[0] -= tv[0];
-                    r[1] -= tv[1];
-                }
-                double iv[4];
-                inv2(e.B, iv);
-                mulv2(iv, r, Lc[i].x);
-            }
-        }
-        __threadfence_block();
-        __syncthreads();
-    }
-    // 6. step filter over the used knots, x sequence then y sequence
+    // 5. step filter over the used knots, x sequence then y sequence
     //    (SmearingFilter::apply / RangeFilter::apply on the compacted halves)
+    const int filterKind = s_filter;
+    const double strength = g.strength;
     if (filterKind == 1) {
-        // y_i = s * y_{i-1} + (1 - s) * v_i over used knots: affine scan
         __shared__ double sa[kSolveWG][4];
         const int chunk = (n + kSolveWG - 1) / kSolveWG;
         const int c0 = min(n, tid * chunk), c1 = min(n, c0 + chunk);
@@ -367,20 +628,26 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(GroupDev g, int shift, int f
         for (int k = c0; k < c1; k++) {
             if (!g.used[k]) continue;
             ax *= strength;
-            bx = strength * bx + (1 - strength) * L0[k].x[0];
+            bx = strength * bx + (1 - strength) * X[k].x;
             ay *= strength;
-            by = strength * by + (1 - strength) * L0[k].x[1];
+            by = strength * by + (1 - strength) * X[k].y;
         }
-        sa[tid][0] = ax; sa[tid][1] = bx; sa[tid][2] = ay; sa[tid][3] = by;
+        sa[tid][0] = ax;
+        sa[tid][1] = bx;
+        sa[tid][2] = ay;
+        sa[tid][3] = by;
         __syncthreads();
-        // inclusive Hillis-Steele scan of affine maps
         for (int st = 1; st < kSolveWG; st <<= 1) {
             double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
             const bool has = tid >= st;
-            if (has) { v0 = sa[tid - st][0]; v1 = sa[tid - st][1]; v2 = sa[tid - st][2]; v3 = sa[tid - st][3]; }
+            if (has) {
+                v0 = sa[tid - st][0];
+                v1 = sa[tid - st][1];
+                v2 = sa[tid - st][2];
+                v3 = sa[tid - st][3];
+            }
             __syncthreads();
             if (has) {
-                // compose: earlier (v) then current (sa[tid])
                 const double a0 = sa[tid][0], b0 = sa[tid][1], a1 = sa[tid][2], b1 = sa[tid][3];
                 sa[tid][0] = a0 * v0;
                 sa[tid][1] = a0 * v1 + b0;
@@ -393,10 +660,9 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(GroupDev g, int shift, int f
         double yy = tid > 0 ? sa[tid - 1][3] : 0.0;
         for (int k = c0; k < c1; k++) {
             if (!g.used[k]) continue;
-            yx = yx * strength + L0[k].x[0] * (1 - strength);
-            yy = yy * strength + L0[k].x[1] * (1 - strength);
-            L0[k].x[0] = yx;
-            L0[k].x[1] = yy;
+            yx = yx * strength + X[k].x * (1 - strength);
+            yy = yy * strength + X[k].y * (1 - strength);
+            X[k] = make_double2(yx, yy);
         }
         __threadfence_block();
         __syncthreads();
@@ -407,13 +673,18 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(GroupDev g, int shift, int f
                 for (int k = 0; k < n; k++) {
                     if (!g.used[k]) continue;
                     if (pv >= 0 && cu >= 0) {
-                        const double vm = L0[pv].x[axis], vi = L0[cu].x[axis], vp = L0[k].x[axis];
+                        const double vm = axis ? X[pv].y : X[pv].x;
+                        const double vi = axis ? X[cu].y : X[cu].x;
+                        const double vp = axis ? X[k].y : X[k].x;
                         const double tc = (vm + vp) / 2;
                         const double diff = (tc - vm) * strength;
                         bool rep;
                         if (vm < vp) rep = vi > vp + diff || vi < vm - diff;
                         else rep = vi < vp + diff || vi > vm - diff;
-                        if (rep) L0[cu].x[axis] = tc;
+                        if (rep) {
+                            if (axis) X[cu].y = tc;
+                            else X[cu].x = tc;
+                        }
                     }
                     pv = cu;
                     cu = k;
@@ -423,14 +694,14 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(GroupDev g, int shift, int f
         __threadfence_block();
         __syncthreads();
     }
-    // 7. d_expandResults (task_expandresults.cu:10-41) and line-search set-up
+    // 6. d_expandResults (task_expandresults.cu:10-41) and line-search set-up
     const int G = g.groupSize;
     for (int idx = tid; idx < 2 * N; idx += kSolveWG) {
         const int j = idx >= N ? idx - N : idx;
         const bool fixed = ((((idx + 1) - (shift * (G >> 1))) % G) == 0);
         double dv = g.deltaP[idx];
         if (!fixed) {
-            if (j < n && g.used[j]) dv = L0[j].x[idx >= N ? 1 : 0];
+            if (j < n && g.used[j]) dv = idx >= N ? X[j].y : X[j].x;
             else dv = 0.0;
         }
         g.deltaP[idx] = dv;
@@ -442,72 +713,120 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(GroupDev g, int shift, int f
         g.lsNextK[f] = 0;
         g.lsMk[f] = -1;
         g.lsDk[f] = -1;
+        g.frameArrive[f] = 0;
+    }
+    if (tid == 0) {
+        ctl.activeFrames = nF;
+        ctl.roundFrames = nF;
+        ctl.roundArrive = 0;
+        ctl.roundsThisLS = 0;
+        ctl.K = g.K1;
+        ctl.phase = PH_LS;
     }
 }
 
 // ---------------------------------------------------------------------------
-// line search: K fused candidates p0 + r^k * delta for every active frame
-// (d_linesearch's tries, task_linesearch.cu:35-106, evaluated in one pass)
+// line search: K fused candidates p0 + r^k * delta of every active frame
+// (the tries of d_linesearch, task_linesearch.cu:35-106, in one pass), then
+// the frame's last column to finish resolves the tries sequentially.
 // ---------------------------------------------------------------------------
 template <typename TI, int K>
-__global__ __launch_bounds__(kWG) void k_ls_round(GroupDev g) {
-    const int c = blockIdx.x, f = c / g.nb, b = c % g.nb;
-    if (!g.lsActive[f]) return;
-    const int k0 = g.lsNextK[f];
-    const int N = g.N;
-    const double r = g.rmult;
-    double dlx = reduce_pow(g.d0[c], r, k0), dly = reduce_pow(g.d0[c + N], r, k0);
-    double drx = reduce_pow(g.d0[c + 1], r, k0), dry = reduce_pow(g.d0[c + 1 + N], r, k0);
-    const bool lastKnot = (c + 1 == g.fframeW);
-    const double plx = g.p0[c], ply = g.p0[c + N], prx = g.p0[c + 1], pry = g.p0[c + 1 + N];
-    float klx[K], kly[K], krx[K], kry[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        klx[k] = (float)(plx + dlx);
-        kly[k] = (float)(ply + dly);
-        krx[k] = lastKnot ? 1e6f : (float)(prx + drx);
-        kry[k] = lastKnot ? 1e6f : (float)(pry + dry);
-        dlx *= r; dly *= r; drx *= r; dry *= r;
-    }
-    const TI *img = (const TI *)g.img + (size_t)c * g.frameH;
-    const float4 *geo = g.geo + (size_t)b * g.frameH;
-    double e2[K];
-    int cnt[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) { e2[k] = 0.0; cnt[k] = 0; }
-    for (int rr = threadIdx.x; rr < g.frameH; rr += kWG) {
-        const float4 q = geo[rr];
-        const float f1 = q.z, cf = 1.f - f1;
-        const float I = load_px(img, rr);
+__device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
+    ColKnots kn[K];
+    {
+        const int N = g.N;
+        const double r = g.rmult;
+        double dlx = reduce_pow(g.d0[c], r, k0), dly = reduce_pow(g.d0[c + N], r, k0);
+        double drx = reduce_pow(g.d0[c + 1], r, k0), dry = reduce_pow(g.d0[c + 1 + N], r, k0);
+        const bool lastKnot = (c + 1 == g.fframeW);
+        const double plx = g.p0[c], ply = g.p0[c + N], prx = g.p0[c + 1], pry = g.p0[c + 1 + N];
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const float x = q.x + klx[k] * cf + krx[k] * f1;
-            const float y = q.y + kly[k] * cf + kry[k] * f1;
-            if (in_mask(g, q.w, x, y)) {
-                int t;
-                float fx, fy;
-                const float d = I - bilinear(g.tmpl, g.tw, x, y, t, fx, fy);
-                e2[k] += (double)d * (double)d;
-                cnt[k]++;
+            const double lx = plx + dlx, ly = ply + dly;
+            const double rx = lastKnot ? 1e6 : prx + drx, ry = lastKnot ? 1e6 : pry + dry;
+            kn[k] = ColKnots{(float)lx, (float)(rx - lx), (float)ly, (float)(ry - ly)};
+            dlx *= r;
+            dly *= r;
+            drx *= r;
+            dry *= r;
+        }
+    }
+    const TI *img = column_ptr<TI>(g, c);
+    const float4 *geo = g.geo ? g.geo + (size_t)b * g.frameH : nullptr;
+    const int r0 = threadIdx.x * g.seg;
+    const int n = min(g.seg, g.frameH - r0);
+    GeoIter it;
+    it.start(g, b, r0);
+    float e2[K];
+    int cnt[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        e2[k] = 0.f;
+        cnt[k] = 0;
+    }
+    for (int ch = 0; ch < n; ch += kCH) {
+        float I[kCH];
+        load8(img + r0 + ch, g.vec, n - ch, I);
+#pragma unroll
+        for (int q = 0; q < kCH; q++) {
+            if (ch + q < n) {
+                float bx, by, f;
+                bool m;
+                it.at(g, geo, r0 + ch + q, bx, by, f, m);
+                it.next();
+                if (m) {
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const float x = fmaf(kn[k].dx, f, bx + kn[k].ax);
+                        const float y = fmaf(kn[k].dy, f, by + kn[k].ay);
+                        if (in_bounds(g, x, y)) {
+                            Cell cl;
+                            const float d = I[q] - sample(g, x, y, cl);
+                            e2[k] = fmaf(d, d, e2[k]);
+                            cnt[k]++;
+                        }
+                    }
+                }
             }
         }
     }
     double v[2 * K];
 #pragma unroll
-    for (int k = 0; k < K; k++) { v[2 * k] = e2[k]; v[2 * k + 1] = (double)cnt[k]; }
-    block_reduce_store<2 * K>(v, g.lssum + (size_t)c * 2 * kMaxCand);
+    for (int k = 0; k < K; k++) {
+        v[2 * k] = (double)e2[k];
+        v[2 * k + 1] = (double)cnt[k];
+    }
+    __shared__ double out[2 * kMaxCand];
+    block_sum<2 * K>(v, out);
+    if (threadIdx.x < 2 * K) g.lssum[(size_t)c * 2 * kMaxCand + threadIdx.x] = out[threadIdx.x];
 }
 
-// Sequential resolution of the fused candidates per frame: reproduces the
-// reference's try / reduce / movement-threshold order exactly.
-__global__ __launch_bounds__(kSolveWG) void k_ls_decide(GroupDev g, int shift, int K,
-                                                        double thr) {
-    const int nF = g.nF, nb = g.nb, N = g.N;
-    const double r = g.rmult;
-    int tries = 0, pendingCount = 0;
-    bool pending = false;
-    for (int f = threadIdx.x; f < nF; f += kSolveWG) {
-        if (!g.lsActive[f]) continue;
+// Sequential resolution of the candidates of frame f (reproduces the
+// reference's try / reduce / movement-threshold order), commit, and the
+// group's end-of-round bookkeeping.  Runs in the frame's last workgroup.
+__device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f, int K) {
+    const int nb = g.nb, N = g.N, nF = g.nF;
+    const int shift = ctl.shift;
+    __shared__ double se[kMaxCand], sn[kMaxCand];
+    __shared__ int sp[kMaxCand];
+    __shared__ int s_fin, s_acc, s_last;
+    if (threadIdx.x < K) {
+        const int kk = threadIdx.x;
+        double s = 0.0, c = 0.0;
+        int pres = 0;
+        for (int bb = 0; bb < nb; bb++) {
+            const double *ls = g.lssum + (size_t)(f * nb + bb) * 2 * kMaxCand + 2 * kk;
+            if (ls[1] >= (double)g.minPts[bb]) pres = 1;
+            s += ls[0];
+            c += ls[1];
+        }
+        se[kk] = s;
+        sn[kk] = c;
+        sp[kk] = pres;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double r = g.rmult, thr = g.thr;
         const int k0 = g.lsNextK[f];
         double dmax = 0.0;
         for (int t = 0; t < nb; t++) {
@@ -516,28 +835,18 @@ __global__ __launch_bounds__(kSolveWG) void k_ls_decide(GroupDev g, int shift, i
             if (bq > dmax) dmax = bq;
         }
         double dm = reduce_pow(dmax, r, k0);
-        int fin = -1;      // exponent of the final delta
-        int acc = -1;      // accepted candidate
-        int mk = -1;
+        int fin = -1, acc = -1, mk = -1, tries = 0;
         for (int kk = 0; kk < K && fin < 0; kk++) {
             const int k = k0 + kk;
             tries++;
             mk = k;
-            bool present = false;
-            double s = 0.0, cn = 0.0;
-            for (int bb = 0; bb < nb; bb++) {
-                const double *ls = g.lssum + (size_t)(f * nb + bb) * 2 * kMaxCand + 2 * kk;
-                if (ls[1] >= (double)g.minPts[bb]) present = true;
-                s += ls[0];
-                cn += ls[1];
-            }
-            if (!present) {
+            if (!sp[kk]) {
                 const double dm1 = dm * r;
                 if (!(dm1 > thr)) fin = k + 1;
                 dm = dm1;
             } else {
                 g.lsDk[f] = k;
-                const double e = s / cn;
+                const double e = se[kk] / sn[kk];
                 if (e < g.errval[shift * nF + f]) {
                     acc = k;
                     fin = k;
@@ -550,141 +859,259 @@ __global__ __launch_bounds__(kSolveWG) void k_ls_decide(GroupDev g, int shift, i
             }
         }
         g.lsMk[f] = mk;
-        if (fin >= 0) {
-            for (int t = 0; t < nb; t++) {
-                for (int ax = 0; ax < 2; ax++) {
-                    const int i = f * nb + t + ax * N;
-                    const double d0 = g.d0[i];
-                    g.deltaP[i] = reduce_pow(d0, r, fin);
-                    if (acc >= 0) g.p[i] = g.p0[i] + reduce_pow(d0, r, acc);
-                }
-            }
-            g.lsActive[f] = 0;
-        } else {
-            g.lsNextK[f] = k0 + K;
-            pending = true;
-            pendingCount++;
+        atomicAdd(&ctl.tries, tries);
+        if (fin < 0) g.lsNextK[f] = k0 + K;
+        s_fin = fin;
+        s_acc = acc;
+    }
+    __syncthreads();
+    const int fin = s_fin, acc = s_acc;
+    if (fin >= 0) {
+        const double r = g.rmult;
+        for (int t = threadIdx.x; t < 2 * nb; t += kWG) {
+            const int i = f * nb + (t % nb) + (t >= nb ? N : 0);
+            const double d0 = g.d0[i];
+            g.deltaP[i] = reduce_pow(d0, r, fin);
+            if (acc >= 0) g.p[i] = g.p0[i] + reduce_pow(d0, r, acc);
         }
     }
-    // workgroup-wide: anything pending?  then the `improved` flag
-    __shared__ int shp, sht;
-    if (threadIdx.x == 0) { shp = 0; sht = 0; }
-    __syncthreads();
-    if (pending) atomicAdd(&shp, pendingCount);
-    atomicAdd(&sht, tries);
+    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-        g.status->need_more = shp ? 1 : 0;
-        g.status->active = shp;
-        g.status->tries += sht;
-        if (!shp) {
-            double e0 = 0.0, e1 = 0.0;
-            for (int f = 0; f < nF; f++) {
-                e0 += g.errval[f];
-                e1 += g.errval[f + nF];
-            }
-            g.status->improved = (shift && e0 > e1) || (!shift && e0 < e1);
+        if (fin >= 0) {
+            g.lsActive[f] = 0;
+            atomicSub(&ctl.activeFrames, 1);
         }
+        __threadfence();
+        s_last = (atomicAdd(&ctl.roundArrive, 1) == ctl.roundFrames - 1);
     }
-}
-
-// ---------------------------------------------------------------------------
-// global correlation statistic (hd_globalCorrelation, task_correlation.cu:63-99)
-// of the last-tried state of every frame.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void state_knots(const GroupDev &g, int c, int k, float &lx, float &rx,
-                                            float &ly, float &ry) {
-    const int N = g.N;
-    if (k < 0) {  // state of the half-iteration start: p0 with the real right knot
-        lx = (float)g.p0[c];
-        ly = (float)g.p0[c + N];
-        rx = (float)g.p0[c + 1];
-        ry = (float)g.p0[c + 1 + N];
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    // end of the round for this group
+    __threadfence();
+    ctl.rounds++;
+    ctl.roundsThisLS++;
+    ctl.roundArrive = 0;
+    ctl.lsFramePasses += ctl.roundFrames;
+    const int active = atomicAdd(&ctl.activeFrames, 0);
+    if (active > 0) {
+        ctl.roundFrames = active;
+        ctl.K = g.Kn;
+        if (ctl.roundsThisLS >= g.maxRounds) {
+            ctl.error = 4;  // SS_ERR_NUMERICAL
+            ctl.phase = PH_ERROR;
+        }
         return;
     }
-    const double r = g.rmult;
-    lx = (float)(g.p0[c] + reduce_pow(g.d0[c], r, k));
-    ly = (float)(g.p0[c + N] + reduce_pow(g.d0[c + N], r, k));
-    if (c + 1 == g.fframeW) {
-        rx = 1e6f;
-        ry = 1e6f;
-    } else {
-        rx = (float)(g.p0[c + 1] + reduce_pow(g.d0[c + 1], r, k));
-        ry = (float)(g.p0[c + 1 + N] + reduce_pow(g.d0[c + 1 + N], r, k));
+    // half-iteration complete: `improved` (task_linesearch.cu:117-125)
+    double e0 = 0.0, e1 = 0.0;
+    for (int ff = 0; ff < nF; ff++) {
+        e0 += g.errval[ff];
+        e1 += g.errval[ff + nF];
     }
+    const int imp = (shift && e0 > e1) || (!shift && e0 < e1);
+    ctl.improved &= imp;
+    ctl.halfIterations++;
+    if (shift == 0) {
+        ctl.shift = 1;
+        ctl.phase = PH_ASSEMBLE;
+        return;
+    }
+    // end of MainIteration::execute (mainiteration.cpp:103-118)
+    if (!ctl.improved && ctl.nFilterIterations != 0) {
+        ctl.solverFilter = 0;
+        ctl.nFilterIterations = 0;
+        ctl.improved = 1;  // run this iteration again without the filter
+        ctl.shift = 0;
+        ctl.phase = PH_ASSEMBLE;
+        return;
+    }
+    ctl.iterImproved = ctl.improved;
+    ctl.phase = PH_CORR;
 }
 
 template <typename TI>
-__global__ __launch_bounds__(kWG) void k_corr(GroupDev g) {
-    const int c = blockIdx.x, f = c / g.nb, b = c % g.nb;
+__global__ __launch_bounds__(kWG) void k_ls_round(Batch B) {
+    __shared__ int s_last;
+    for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
+        const int gi = B.itemGroup[item];
+        GroupCtl &ctl = B.ctl[gi];
+        if (ctl.phase != PH_LS) continue;
+        const GroupDev &g = B.groups[gi];
+        if (g.imgType != (sizeof(TI) == 4 ? IMG_F32 : (std::is_signed<TI>::value ? IMG_I16 : IMG_U16))) continue;
+        const int c = item - g.itemStart, f = c / g.nb, b = c % g.nb;
+        if (!g.lsActive[f]) continue;
+        const int K = ctl.K, k0 = g.lsNextK[f];
+        switch (K) {
+        case 1: ls_column<TI, 1>(g, c, b, k0); break;
+        case 2: ls_column<TI, 2>(g, c, b, k0); break;
+        case 4: ls_column<TI, 4>(g, c, b, k0); break;
+        default: ls_column<TI, 8>(g, c, b, k0); break;
+        }
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) s_last = (atomicAdd(&g.frameArrive[f], 1) == g.nb - 1);
+        __syncthreads();
+        if (s_last) {
+            __threadfence();
+            if (threadIdx.x == 0) g.frameArrive[f] = 0;
+            ls_decide_frame(g, ctl, f, K);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// global correlation statistic (hd_globalCorrelation,
+// task_correlation.cu:63-99) of every frame's last-tried state; the
+// statistic of a column covers its leading run of masked rows only.
+// ---------------------------------------------------------------------------
+template <typename TI>
+__device__ void corr_column(const GroupDev &g, int c) {
+    const int f = c / g.nb, b = c % g.nb;
     const int mk = g.lsMk[f], dk = g.lsDk[f];
     const bool same = (mk == dk);
-    float mlx, mrx, mly, mry, dlx, drx, dly, dry;
-    state_knots(g, c, mk, mlx, mrx, mly, mry);
-    state_knots(g, c, dk, dlx, drx, dly, dry);
-    const TI *img = (const TI *)g.img + (size_t)c * g.frameH;
-    const float4 *geo = g.geo + (size_t)b * g.frameH;
-    // pass 1: first unmasked row of the column (the reference's walk stops there)
-    int first = g.frameH;
-    for (int rr = threadIdx.x; rr < g.frameH; rr += kWG) {
-        const float4 q = geo[rr];
-        const float x = q.x + mlx * (1.f - q.z) + mrx * q.z;
-        const float y = q.y + mly * (1.f - q.z) + mry * q.z;
-        if (!in_mask(g, q.w, x, y)) { first = rr; break; }
-    }
-    const int r0 = block_min_int(first);
+    const ColKnots km = state_knots(g, c, mk);
+    const ColKnots kd = state_knots(g, c, dk);
+    const TI *img = column_ptr<TI>(g, c);
+    const float4 *geo = g.geo ? g.geo + (size_t)b * g.frameH : nullptr;
+    const int r0 = threadIdx.x * g.seg;
+    const int n = min(g.seg, g.frameH - r0);
+    GeoIter it;
+    it.start(g, b, r0);
     double v[CR_N];
 #pragma unroll
     for (int i = 0; i < CR_N; i++) v[i] = 0.0;
-    for (int rr = threadIdx.x; rr < g.frameH; rr += kWG) {
-        const float4 q = geo[rr];
-        const float cf = 1.f - q.z;
-        const float I = load_px(img, rr);
-        const float x = q.x + mlx * cf + mrx * q.z;
-        const float y = q.y + mly * cf + mry * q.z;
-        const bool mm = in_mask(g, q.w, x, y);
-        int t;
-        float fx, fy;
-        float dbuf = 0.f;
-        if (same) {
-            if (mm) dbuf = I - bilinear(g.tmpl, g.tw, x, y, t, fx, fy);
-        } else {
-            const float xd = q.x + dlx * cf + drx * q.z;
-            const float yd = q.y + dly * cf + dry * q.z;
-            if (in_mask(g, q.w, xd, yd)) dbuf = I - bilinear(g.tmpl, g.tw, xd, yd, t, fx, fy);
-        }
-        v[CR_SUMI] += (double)I;
-        v[CR_SUMD] += mm ? (double)(I - dbuf) : (double)dbuf;
-        v[CR_NMASK] += mm ? 1.0 : 0.0;
-        if (rr < r0) {
-            const double a = (double)(I - dbuf), bb = (double)I;
-            v[CR_SA] += a;
-            v[CR_SAA] += a * a;
-            v[CR_SB] += bb;
-            v[CR_SBB] += bb * bb;
-            v[CR_SAB] += a * bb;
-            v[CR_NS] += 1.0;
+    bool open = true;  // still inside this thread's leading masked run
+    for (int ch = 0; ch < n; ch += kCH) {
+        float I[kCH];
+        load8(img + r0 + ch, g.vec, n - ch, I);
+#pragma unroll
+        for (int q = 0; q < kCH; q++) {
+            if (ch + q < n) {
+                float bx, by, f1;
+                bool m;
+                it.at(g, geo, r0 + ch + q, bx, by, f1, m);
+                it.next();
+                const float x = fmaf(km.dx, f1, bx + km.ax);
+                const float y = fmaf(km.dy, f1, by + km.ay);
+                const bool mm = m && in_bounds(g, x, y);
+                float dbuf = 0.f;
+                Cell cl;
+                if (same) {
+                    if (mm) dbuf = I[q] - sample(g, x, y, cl);
+                } else {
+                    const float xd = fmaf(kd.dx, f1, bx + kd.ax);
+                    const float yd = fmaf(kd.dy, f1, by + kd.ay);
+                    if (m && in_bounds(g, xd, yd)) dbuf = I[q] - sample(g, xd, yd, cl);
+                }
+                v[CR_SUMD] += mm ? (double)(I[q] - dbuf) : (double)dbuf;
+                v[CR_NMASK] += mm ? 1.0 : 0.0;
+                if (!mm) open = false;
+                if (open) {
+                    const double a = (double)(I[q] - dbuf), bb = (double)I[q];
+                    v[CR_SA] += a;
+                    v[CR_SAA] += a * a;
+                    v[CR_SB] += bb;
+                    v[CR_SBB] += bb * bb;
+                    v[CR_SAB] += a * bb;
+                    v[CR_NS] += 1.0;
+                }
+            }
         }
     }
-    block_reduce_store<CR_N>(v, g.corrsum + (size_t)c * CR_N);
+    // only threads up to the first one that met an unmasked row contribute
+    const int firstBroken = block_min(open ? kWG : (int)threadIdx.x);
+    if ((int)threadIdx.x > firstBroken) {
+        for (int i = CR_SA; i < CR_N; i++) v[i] = 0.0;
+    }
+    __shared__ double out[CR_N];
+    block_sum<CR_N>(v, out);
+    if (threadIdx.x < CR_N) g.corrsum[(size_t)c * CR_N + threadIdx.x] = out[threadIdx.x];
 }
 
-__global__ __launch_bounds__(kSolveWG) void k_corr_reduce(GroupDev g) {
-    __shared__ double sh[CR_N];
-    if (threadIdx.x < CR_N) {
-        double s = 0.0;
-        for (int c = 0; c < g.fframeW; c++) s += g.corrsum[(size_t)c * CR_N + threadIdx.x];
-        sh[threadIdx.x] = s;
+template <typename TI>
+__global__ __launch_bounds__(kWG) void k_corr(Batch B) {
+    for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
+        const int gi = B.itemGroup[item];
+        if (B.ctl[gi].phase != PH_CORR) continue;
+        const GroupDev &g = B.groups[gi];
+        if (g.imgType != (sizeof(TI) == 4 ? IMG_F32 : (std::is_signed<TI>::value ? IMG_I16 : IMG_U16))) continue;
+        corr_column<TI>(g, item - g.itemStart);
+    }
+}
+
+// Global correlation of the group and the stop tests of Task::alignFrames
+// (task.cpp:281-342).
+__global__ __launch_bounds__(kSolveWG) void k_corr_finish(Batch B) {
+    const int gi = blockIdx.x;
+    GroupCtl &ctl = B.ctl[gi];
+    if (ctl.phase != PH_CORR) return;
+    const GroupDev &g = B.groups[gi];
+    double v[CR_N + 1];
+#pragma unroll
+    for (int i = 0; i <= CR_N; i++) v[i] = 0.0;
+    for (int c = threadIdx.x; c < g.fframeW; c += kSolveWG) {
+        for (int i = 0; i < CR_N; i++) v[i] += g.corrsum[(size_t)c * CR_N + i];
+        v[CR_N] += g.colsum[(size_t)c * CS_N + CS_SUMI];  // image over all pixels
+    }
+    __shared__ double tot[CR_N + 1];
+    block_sum<CR_N + 1>(v, tot);
+    const int P = 2 * g.N;
+    const int it = ctl.iteration;
+    if (g.allParams && it <= g.maxIter)
+        for (int i = threadIdx.x; i < P; i += kSolveWG) g.allParams[(size_t)(it - 1) * P + i] = g.p[i];
+    __shared__ int s_copy;
+    if (threadIdx.x == 0) {
+        const double nm = tot[CR_NMASK];
+        const double mA = tot[CR_N] / nm, mB = tot[CR_SUMD] / nm, ns = tot[CR_NS];
+        const double assq = tot[CR_SAA] - 2 * mA * tot[CR_SA] + ns * mA * mA;
+        const double bssq = tot[CR_SBB] - 2 * mB * tot[CR_SB] + ns * mB * mB;
+        const double q = tot[CR_SAB] - mB * tot[CR_SA] - mA * tot[CR_SB] + ns * mA * mB;
+        const double corr = q / sqrt(assq * bssq);
+        ctl.corr = corr;
+        if (g.iterCorr && it <= g.maxIter) g.iterCorr[it - 1] = corr;
+        const double correlation = ctl.correlation;
+        int copy = 0;
+        if (corr - correlation < g.cit) {
+            ctl.retv = 0;  // CorrelationIncreaseThreshold
+            ctl.phase = PH_FINAL;
+        } else if (corr - correlation < g.fct && ctl.filterID != 0) {
+            ctl.solverFilter = 0;
+            ctl.nFilterIterations = 0;
+            ctl.filterID = 0;
+            if (it >= g.maxIter) {  // never run past max_iterations
+                ctl.retv = 3;
+                ctl.phase = PH_FINAL;
+            } else {
+                ctl.iteration = it + 1;
+                ctl.improved = 1;
+                ctl.shift = 0;
+                ctl.phase = PH_ASSEMBLE;
+            }
+        } else if (!ctl.iterImproved) {
+            ctl.retv = 2;  // MovementThreshold
+            ctl.phase = PH_FINAL;
+        } else if (g.maxIter == it) {
+            ctl.retv = 3;  // IterationLimit
+            ctl.phase = PH_FINAL;
+        } else if (correlation >= g.halt) {
+            ctl.retv = 1;  // HaltCorrelationReached
+            ctl.phase = PH_FINAL;
+        } else {
+            ctl.correlation = corr;
+            copy = 1;
+            ctl.iteration = it + 1;
+            ctl.improved = 1;
+            ctl.shift = 0;
+            ctl.phase = PH_ASSEMBLE;
+        }
+        if (ctl.phase == PH_FINAL && it == 1) copy = 1;  // task.cpp:381-383
+        s_copy = copy;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const double n = sh[CR_NMASK];
-        const double mA = sh[CR_SUMI] / n, mB = sh[CR_SUMD] / n, ns = sh[CR_NS];
-        const double assq = sh[CR_SAA] - 2 * mA * sh[CR_SA] + ns * mA * mA;
-        const double bssq = sh[CR_SBB] - 2 * mB * sh[CR_SB] + ns * mB * mB;
-        const double q = sh[CR_SAB] - mB * sh[CR_SA] - mA * sh[CR_SB] + ns * mA * mB;
-        g.status->corr = q / sqrt(assq * bssq);
-    }
+    if (s_copy)
+        for (int i = threadIdx.x; i < P; i += kSolveWG) g.pLast[i] = g.p[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -692,131 +1119,113 @@ __global__ __launch_bounds__(kSolveWG) void k_corr_reduce(GroupDev g) {
 // (hd_calculateGroupErrvals(0) + hd_frameCorrelation, task.cpp:389-405)
 // ---------------------------------------------------------------------------
 template <typename TI>
-__global__ __launch_bounds__(kWG) void k_final(GroupDev g) {
-    const int c = blockIdx.x, f = c / g.nb, b = c % g.nb;
-    float lx, rx, ly, ry;
-    state_knots(g, c, g.lsMk[f], lx, rx, ly, ry);
-    const TI *img = (const TI *)g.img + (size_t)c * g.frameH;
-    const float4 *geo = g.geo + (size_t)b * g.frameH;
+__device__ void final_column(const GroupDev &g, int c) {
+    const int f = c / g.nb, b = c % g.nb;
+    const ColKnots kn = state_knots(g, c, g.lsMk[f]);
+    const TI *img = column_ptr<TI>(g, c);
+    const float4 *geo = g.geo ? g.geo + (size_t)b * g.frameH : nullptr;
     uint8_t *mo = g.maskOut ? g.maskOut + (size_t)c * g.frameH : nullptr;
+    const int r0 = threadIdx.x * g.seg;
+    const int n = min(g.seg, g.frameH - r0);
+    GeoIter it;
+    it.start(g, b, r0);
     double v[FS_N];
 #pragma unroll
     for (int i = 0; i < FS_N; i++) v[i] = 0.0;
-    for (int rr = threadIdx.x; rr < g.frameH; rr += kWG) {
-        const float4 q = geo[rr];
-        const float cf = 1.f - q.z;
-        const float x = q.x + lx * cf + rx * q.z;
-        const float y = q.y + ly * cf + ry * q.z;
-        const bool mm = in_mask(g, q.w, x, y);
-        if (mo) mo[rr] = mm ? 1 : 0;
-        if (mm) {
-            int t;
-            float fx, fy;
-            const float I = load_px(img, rr);
-            const float wI = bilinear(g.tmpl, g.tw, x, y, t, fx, fy);
-            const double d = (double)(I - wI);
-            v[FS_D2] += d * d;
-            v[FS_CNT] += 1.0;
-            v[FS_SA] += (double)wI;
-            v[FS_SAA] += (double)wI * (double)wI;
-            v[FS_SB] += (double)I;
-            v[FS_SBB] += (double)I * (double)I;
-            v[FS_SAB] += (double)wI * (double)I;
+    for (int ch = 0; ch < n; ch += kCH) {
+        float I[kCH];
+        load8(img + r0 + ch, g.vec, n - ch, I);
+#pragma unroll
+        for (int q = 0; q < kCH; q++) {
+            if (ch + q < n) {
+                float bx, by, f1;
+                bool m;
+                it.at(g, geo, r0 + ch + q, bx, by, f1, m);
+                it.next();
+                const float x = fmaf(kn.dx, f1, bx + kn.ax);
+                const float y = fmaf(kn.dy, f1, by + kn.ay);
+                const bool mm = m && in_bounds(g, x, y);
+                if (mo) mo[r0 + ch + q] = mm ? 1 : 0;
+                if (mm) {
+                    Cell cl;
+                    const float wI = sample(g, x, y, cl);
+                    const double d = (double)(I[q] - wI);
+                    v[FS_D2] += d * d;
+                    v[FS_CNT] += 1.0;
+                    v[FS_SA] += (double)wI;
+                    v[FS_SAA] += (double)wI * (double)wI;
+                    v[FS_SB] += (double)I[q];
+                    v[FS_SBB] += (double)I[q] * (double)I[q];
+                    v[FS_SAB] += (double)wI * (double)I[q];
+                }
+            }
         }
     }
-    block_reduce_store<FS_N>(v, g.finalsum + (size_t)c * FS_N);
+    __shared__ double out[FS_N];
+    block_sum<FS_N>(v, out);
+    if (threadIdx.x < FS_N) g.finalsum[(size_t)c * FS_N + threadIdx.x] = out[threadIdx.x];
 }
 
-__global__ __launch_bounds__(kSolveWG) void k_final_reduce(GroupDev g) {
+template <typename TI>
+__global__ __launch_bounds__(kWG) void k_final(Batch B) {
+    for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
+        const int gi = B.itemGroup[item];
+        if (B.ctl[gi].phase != PH_FINAL) continue;
+        const GroupDev &g = B.groups[gi];
+        if (g.imgType != (sizeof(TI) == 4 ? IMG_F32 : (std::is_signed<TI>::value ? IMG_I16 : IMG_U16))) continue;
+        final_column<TI>(g, item - g.itemStart);
+    }
+}
+
+__global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
+    const int gi = blockIdx.x;
+    GroupCtl &ctl = B.ctl[gi];
+    if (ctl.phase != PH_FINAL) return;
+    const GroupDev &g = B.groups[gi];
     for (int f = threadIdx.x; f < g.nF; f += kSolveWG) {
         double s[FS_N];
         for (int i = 0; i < FS_N; i++) s[i] = 0.0;
         for (int bb = 0; bb < g.nb; bb++)
             for (int i = 0; i < FS_N; i++) s[i] += g.finalsum[(size_t)(f * g.nb + bb) * FS_N + i];
-        const double n = s[FS_CNT];
-        g.resid[f] = sqrt(s[FS_D2] / n);
-        const double mA = s[FS_SA] / n, mB = s[FS_SB] / n;
-        const double assq = s[FS_SAA] - n * mA * mA;
-        const double bssq = s[FS_SBB] - n * mB * mB;
-        const double q = s[FS_SAB] - n * mA * mB;
+        const double nn = s[FS_CNT];
+        g.resid[f] = sqrt(s[FS_D2] / nn);
+        const double mA = s[FS_SA] / nn, mB = s[FS_SB] / nn;
+        const double assq = s[FS_SAA] - nn * mA * mA;
+        const double bssq = s[FS_SBB] - nn * mB * mB;
+        const double q = s[FS_SAB] - nn * mA * mB;
         g.fcorr[f] = q / sqrt(assq * bssq);
     }
+    __syncthreads();
+    if (threadIdx.x == 0) ctl.phase = PH_DONE;
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename TI>
-static hipError_t launch_assemble_t(const GroupDev &g, hipStream_t s) {
-    hipLaunchKernelGGL(k_assemble<TI>, dim3(g.fframeW), dim3(kWG), 0, s, g);
-    return hipGetLastError();
-}
-hipError_t launch_assemble(const GroupDev &g, hipStream_t s) {
-    switch (g.imgType) {
-    case IMG_U16: return launch_assemble_t<uint16_t>(g, s);
-    case IMG_I16: return launch_assemble_t<int16_t>(g, s);
-    default: return launch_assemble_t<float>(g, s);
-    }
-}
+#define SS_LAUNCH_TYPED(kernel, grid, B, s)                                              \
+    do {                                                                                 \
+        if ((B).hasType[IMG_F32]) hipLaunchKernelGGL(kernel<float>, grid, dim3(kWG), 0, s, (B).b);    \
+        if ((B).hasType[IMG_U16]) hipLaunchKernelGGL(kernel<uint16_t>, grid, dim3(kWG), 0, s, (B).b); \
+        if ((B).hasType[IMG_I16]) hipLaunchKernelGGL(kernel<int16_t>, grid, dim3(kWG), 0, s, (B).b);  \
+    } while (0)
 
-hipError_t launch_solve(const GroupDev &g, int shift, int filterKind, double strength,
-                        double lambda, hipStream_t s) {
-    hipLaunchKernelGGL(k_solve, dim3(1), dim3(kSolveWG), 0, s, g, shift, filterKind, strength, lambda);
-    return hipGetLastError();
-}
-
-template <typename TI>
-static hipError_t launch_ls_round_t(const GroupDev &g, int K, hipStream_t s) {
-    switch (K) {
-    case 1: hipLaunchKernelGGL((k_ls_round<TI, 1>), dim3(g.fframeW), dim3(kWG), 0, s, g); break;
-    case 2: hipLaunchKernelGGL((k_ls_round<TI, 2>), dim3(g.fframeW), dim3(kWG), 0, s, g); break;
-    case 4: hipLaunchKernelGGL((k_ls_round<TI, 4>), dim3(g.fframeW), dim3(kWG), 0, s, g); break;
-    default: hipLaunchKernelGGL((k_ls_round<TI, 8>), dim3(g.fframeW), dim3(kWG), 0, s, g); break;
+hipError_t launch_stage(const LaunchBatch &L, int stage, hipStream_t s) {
+    const dim3 items(L.itemGrid), groups(L.b.nGroups);
+    switch (stage) {
+    case ST_ASSEMBLE: SS_LAUNCH_TYPED(k_assemble, items, L, s); break;
+    case ST_SOLVE: hipLaunchKernelGGL(k_solve, groups, dim3(kSolveWG), 0, s, L.b); break;
+    case ST_LS: SS_LAUNCH_TYPED(k_ls_round, items, L, s); break;
+    case ST_CORR: SS_LAUNCH_TYPED(k_corr, items, L, s); break;
+    case ST_CORR_FINISH: hipLaunchKernelGGL(k_corr_finish, groups, dim3(kSolveWG), 0, s, L.b); break;
+    case ST_FINAL: SS_LAUNCH_TYPED(k_final, items, L, s); break;
+    case ST_FINAL_FINISH: hipLaunchKernelGGL(k_final_finish, groups, dim3(kSolveWG), 0, s, L.b); break;
     }
-    return hipGetLastError();
-}
-hipError_t launch_ls_round(const GroupDev &g, int K, hipStream_t s) {
-    switch (g.imgType) {
-    case IMG_U16: return launch_ls_round_t<uint16_t>(g, K, s);
-    case IMG_I16: return launch_ls_round_t<int16_t>(g, K, s);
-    default: return launch_ls_round_t<float>(g, K, s);
-    }
-}
-
-hipError_t launch_ls_decide(const GroupDev &g, int shift, int K, double thr, hipStream_t s) {
-    hipLaunchKernelGGL(k_ls_decide, dim3(1), dim3(kSolveWG), 0, s, g, shift, K, thr);
-    return hipGetLastError();
-}
-
-hipError_t launch_corr(const GroupDev &g, hipStream_t s) {
-    switch (g.imgType) {
-    case IMG_U16: hipLaunchKernelGGL(k_corr<uint16_t>, dim3(g.fframeW), dim3(kWG), 0, s, g); break;
-    case IMG_I16: hipLaunchKernelGGL(k_corr<int16_t>, dim3(g.fframeW), dim3(kWG), 0, s, g); break;
-    default: hipLaunchKernelGGL(k_corr<float>, dim3(g.fframeW), dim3(kWG), 0, s, g); break;
-    }
-    hipLaunchKernelGGL(k_corr_reduce, dim3(1), dim3(kSolveWG), 0, s, g);
-    return hipGetLastError();
-}
-
-hipError_t launch_final(const GroupDev &g, hipStream_t s) {
-    switch (g.imgType) {
-    case IMG_U16: hipLaunchKernelGGL(k_final<uint16_t>, dim3(g.fframeW), dim3(kWG), 0, s, g); break;
-    case IMG_I16: hipLaunchKernelGGL(k_final<int16_t>, dim3(g.fframeW), dim3(kWG), 0, s, g); break;
-    default: hipLaunchKernelGGL(k_final<float>, dim3(g.fframeW), dim3(kWG), 0, s, g); break;
-    }
-    hipLaunchKernelGGL(k_final_reduce, dim3(1), dim3(kSolveWG), 0, s, g);
     return hipGetLastError();
 }
 
 size_t solve_workspace_bytes(int n) {
-    size_t eqs = 0;
-    int m = n;
-    for (;;) {
-        eqs += (size_t)(m > 0 ? m : 1);
-        if (m <= 1) break;
-        m >>= 1;
-    }
-    return eqs * sizeof(Eq);
+    // Eq0 + InteriorRow + solution per knot, first-interior forms per partition
+    return (size_t)n * (sizeof(Eq0) + sizeof(InteriorRow) + sizeof(double2)) + kSolveWG * sizeof(Aff) + 256;
 }
 
 }  // namespace ss
