@@ -46,7 +46,15 @@ enum Phase {
     PH_ERROR = 5
 };
 
-enum ImgType { IMG_F32 = 0, IMG_U16 = 1, IMG_I16 = 2 };
+enum ImgType { IMG_F32 = 0, IMG_U16 = 1, IMG_I16 = 2, IMG_F64 = 3, IMG_NTYPES = 4 };
+
+// Affine scan geometry of a block, in the block's (column j, line i) pixel
+// coordinates: base_x = gx0 + gxj j + gxi i, base_y = gy0 + gyb b + gyj j +
+// gyi i, frac = f0 + fj j + fi i.
+template <typename R>
+struct AffineGeo {
+    R gx0, gxj, gxi, gy0, gyb, gyj, gyi, f0, fj, fi;
+};
 
 // Static description of one group (written once per call by the host).
 struct GroupDev {
@@ -55,17 +63,24 @@ struct GroupDev {
     int itemStart;        // first work item (= column) of this group in the batch
     int seg;              // pixels per thread segment (ceil(frameH / 256))
     // settings
-    float xmax, ymax;     // template_w - 1 - eps, template_h - 1 - eps
+    double xmax, ymax;    // template_w - 1 - eps, template_h - 1 - eps
+    int fp64;             // pixel arithmetic in FP64 (precision 'D') or FP32 ('S')
     double rmult, thr, halt, cit, fct, strength, lambda;
     int maxIter, K1, Kn, maxRounds;
     // geometry: affine fast path or per-pixel table
     int affine, L;
-    float gx0, gxj, gxi, gy0, gyb, gyj, gyi, f0, fj, fi;
+    AffineGeo<float> afF;
+    AffineGeo<double> afD;
     int fracUniform;
-    const float4 *geo;    // frameH x nb: {base_x, base_y, frac, base_mask}
-    const float *fnext;   // frameH x nb: frac of block (b+1) % nb (only if !fracUniform)
+    // per-pixel table, frameH x nb: {base_x, base_y, frac, base_mask}, and the
+    // frac of block (b+1) % nb (only if !fracUniform); one of the two widths
+    const float4 *geo;
+    const double4 *geoD;
+    const float *fnext;
+    const double *fnextD;
     // template: pairs (T[t], T[t+1]) row-major; gradient pairs unless derived
     const float2 *tp, *gxv, *gyh;
+    const double2 *tpD, *gxvD, *gyhD;
     int gradFD;           // gradients are forward differences of the template
     // frame data, frameH x fframeW column-contiguous
     const void *img;
@@ -102,7 +117,7 @@ struct Batch {
 
 struct LaunchBatch {
     Batch b;
-    int hasType[3];
+    int hasType[2 * IMG_NTYPES];  // [imgType + IMG_NTYPES * fp64]
     int itemGrid;
 };
 

@@ -21,18 +21,36 @@
 
 namespace {
 
+// Device copies of a template in the pixel precisions it serves (resident
+// templates hold both, per-call templates the one of the call).
 struct DevTemplate {
     int th = 0, tw = 0, gradFD = 1;
     float2 *tp = nullptr, *gxv = nullptr, *gyh = nullptr;
+    double2 *tpD = nullptr, *gxvD = nullptr, *gyhD = nullptr;
     void *mem = nullptr;  // owning allocation (resident only)
 };
 
 struct DevGeometry {
     int frameH = 0, nb = 0, fracUniform = 1, affine = 0, L = 0;
-    float gx0 = 0, gxj = 0, gxi = 0, gy0 = 0, gyb = 0, gyj = 0, gyi = 0, f0 = 0, fj = 0, fi = 0;
+    ss::AffineGeo<float> afF{};
+    ss::AffineGeo<double> afD{};
     float4 *geo = nullptr;
+    double4 *geoD = nullptr;
     float *fnext = nullptr;
+    double *fnextD = nullptr;
     void *mem = nullptr;
+};
+
+// Host-side converted arrays, per precision.
+struct TemplateHost {
+    std::vector<float2> tp, gxv, gyh;
+    std::vector<double2> tpD, gxvD, gyhD;
+};
+struct GeometryHost {
+    std::vector<float4> tab;
+    std::vector<double4> tabD;
+    std::vector<float> fn;
+    std::vector<double> fnD;
 };
 
 struct Profile {
@@ -178,23 +196,43 @@ ss_status ensure_arena(ss_handle *h, size_t bytes, size_t hostBytes) {
 // ---------------------------------------------------------------------------
 // host-side conversions
 // ---------------------------------------------------------------------------
-// Template (column-major doubles) -> row-major float pairs (T[t], T[t+1]);
-// gradient pairs only when the gradients are not the template's forward
-// differences (steady_scan.m:178-179 makes them so).
-void convert_template(int th, int tw, const double *t, const double *xg, const double *yg,
-                      std::vector<float2> &tp, std::vector<float2> &gxv, std::vector<float2> &gyh,
-                      int &gradFD) {
+// Template (column-major doubles) -> row-major pairs (T[t], T[t+1]) in the
+// requested precisions; gradient pairs only when the gradients are not the
+// template's forward differences (steady_scan.m:178-179 makes them so).
+template <typename P, typename S>
+void template_pairs(int th, int tw, const double *t, const double *xg, const double *yg, bool fd,
+                    std::vector<P> &tp, std::vector<P> &gxv, std::vector<P> &gyh) {
     const size_t n = (size_t)th * tw;
+    auto at = [&](const double *a, int iy, int ix) { return (S)a[(size_t)ix * th + iy]; };
+    tp.assign(n, P{});
+    for (int iy = 0; iy < th; iy++)
+        for (int ix = 0; ix < tw; ix++) {
+            P &q = tp[(size_t)iy * tw + ix];
+            q.x = at(t, iy, ix);
+            q.y = ix + 1 < tw ? at(t, iy, ix + 1) : (S)0;
+        }
+    gxv.clear();
+    gyh.clear();
+    if (fd) return;
+    gxv.assign(n, P{});
+    gyh.assign(n, P{});
+    for (int iy = 0; iy < th; iy++)
+        for (int ix = 0; ix < tw; ix++) {
+            const size_t o = (size_t)iy * tw + ix;
+            gxv[o].x = at(xg, iy, ix);
+            gxv[o].y = iy + 1 < th ? at(xg, iy + 1, ix) : (S)0;
+            gyh[o].x = at(yg, iy, ix);
+            gyh[o].y = ix + 1 < tw ? at(yg, iy, ix + 1) : (S)0;
+        }
+}
+
+void convert_template(int th, int tw, const double *t, const double *xg, const double *yg, bool wantF,
+                      bool wantD, TemplateHost &out, int &gradFD) {
     auto T = [&](int iy, int ix) { return t[(size_t)ix * th + iy]; };
     auto GX = [&](int iy, int ix) { return xg[(size_t)ix * th + iy]; };
     auto GY = [&](int iy, int ix) { return yg[(size_t)ix * th + iy]; };
-    tp.assign(n, make_float2(0.f, 0.f));
     double tmax = 0;
-    for (int iy = 0; iy < th; iy++)
-        for (int ix = 0; ix < tw; ix++) {
-            tp[(size_t)iy * tw + ix] = make_float2((float)T(iy, ix), ix + 1 < tw ? (float)T(iy, ix + 1) : 0.f);
-            tmax = std::max(tmax, std::fabs(T(iy, ix)));
-        }
+    for (size_t i = 0; i < (size_t)th * tw; i++) tmax = std::max(tmax, std::fabs(t[i]));
     // every gradient value the warp can read: xgrad(:, 1:end-1), ygrad(1:end-1, :)
     bool fd = true;
     const double tol = 1e-12 * std::max(1.0, tmax);
@@ -205,18 +243,8 @@ void convert_template(int th, int tw, const double *t, const double *xg, const d
         for (int ix = 0; ix < tw; ix++)
             if (!(std::fabs(GY(iy, ix) - (T(iy + 1, ix) - T(iy, ix))) <= tol)) { fd = false; break; }
     gradFD = fd ? 1 : 0;
-    gxv.clear();
-    gyh.clear();
-    if (!fd) {
-        gxv.assign(n, make_float2(0.f, 0.f));
-        gyh.assign(n, make_float2(0.f, 0.f));
-        for (int iy = 0; iy < th; iy++)
-            for (int ix = 0; ix < tw; ix++) {
-                const size_t o = (size_t)iy * tw + ix;
-                gxv[o] = make_float2((float)GX(iy, ix), iy + 1 < th ? (float)GX(iy + 1, ix) : 0.f);
-                gyh[o] = make_float2((float)GY(iy, ix), ix + 1 < tw ? (float)GY(iy, ix + 1) : 0.f);
-            }
-    }
+    if (wantF) template_pairs<float2, float>(th, tw, t, xg, yg, fd, out.tp, out.gxv, out.gyh);
+    if (wantD) template_pairs<double2, double>(th, tw, t, xg, yg, fd, out.tpD, out.gxvD, out.gyhD);
 }
 
 // Geometry block matrices -> affine description (when the base coordinates
@@ -224,8 +252,7 @@ void convert_template(int th, int tw, const double *t, const double *xg, const d
 // the case of every unidirectional integer-spacing scan) or the per-pixel
 // {base_x, base_y, frac, base_mask} table.
 void convert_geometry(int fh, int nb, const double *frac, const double *bx, const double *by,
-                      const uint8_t *bm, DevGeometry &g, std::vector<float4> &tab,
-                      std::vector<float> &fn) {
+                      const uint8_t *bm, bool wantF, bool wantD, DevGeometry &g, GeometryHost &out) {
     const size_t n = (size_t)fh * nb;
     g.frameH = fh;
     g.nb = nb;
@@ -251,7 +278,7 @@ void convert_geometry(int fh, int nb, const double *frac, const double *bx, cons
                          gyj = J > 1 ? at(by, 0, L) - gy0 : 0.0, gyb = nb > 1 ? at(by, 1, 0) - gy0 : 0.0;
             const double f0 = frac[0], fi = L > 1 ? frac[1] - f0 : 0.0, fj = J > 1 ? frac[L] - f0 : 0.0;
             bool ok = true;
-            const double tol = 1e-9;
+            const double tol = 1e-12;
             for (int b = 0; b < nb && ok; b++)
                 for (int r = 0; r < fh; r++) {
                     const int j = r / L, i = r % L;
@@ -265,23 +292,33 @@ void convert_geometry(int fh, int nb, const double *frac, const double *bx, cons
             if (ok) {
                 g.affine = 1;
                 g.L = L;
-                g.gx0 = (float)gx0; g.gxj = (float)gxj; g.gxi = (float)gxi;
-                g.gy0 = (float)gy0; g.gyb = (float)gyb; g.gyj = (float)gyj; g.gyi = (float)gyi;
-                g.f0 = (float)f0; g.fj = (float)fj; g.fi = (float)fi;
+                g.afD = ss::AffineGeo<double>{gx0, gxj, gxi, gy0, gyb, gyj, gyi, f0, fj, fi};
+                g.afF = ss::AffineGeo<float>{(float)gx0, (float)gxj, (float)gxi, (float)gy0, (float)gyb,
+                                             (float)gyj, (float)gyi, (float)f0, (float)fj, (float)fi};
             }
         }
     }
-    tab.clear();
-    fn.clear();
+    out = GeometryHost{};
     if (!g.affine) {
-        tab.resize(n);
-        for (size_t i = 0; i < n; i++)
-            tab[i] = make_float4((float)bx[i], (float)by[i], (float)frac[i], bm[i] ? 1.f : 0.f);
+        if (wantF) {
+            out.tab.resize(n);
+            for (size_t i = 0; i < n; i++)
+                out.tab[i] = make_float4((float)bx[i], (float)by[i], (float)frac[i], bm[i] ? 1.f : 0.f);
+        }
+        if (wantD) {
+            out.tabD.resize(n);
+            for (size_t i = 0; i < n; i++) out.tabD[i] = make_double4(bx[i], by[i], frac[i], bm[i] ? 1.0 : 0.0);
+        }
     }
     if (!uniform) {
-        fn.resize(n);
+        if (wantF) out.fn.resize(n);
+        if (wantD) out.fnD.resize(n);
         for (int b = 0; b < nb; b++)
-            for (int r = 0; r < fh; r++) fn[(size_t)b * fh + r] = (float)frac[(size_t)((b + 1) % nb) * fh + r];
+            for (int r = 0; r < fh; r++) {
+                const double v = frac[(size_t)((b + 1) % nb) * fh + r];
+                if (wantF) out.fn[(size_t)b * fh + r] = (float)v;
+                if (wantD) out.fnD[(size_t)b * fh + r] = v;
+            }
     }
 }
 
@@ -320,6 +357,10 @@ void knots_out(const double *p, int nF, int nb, double *px, double *py) {
     }
 }
 
+size_t img_size(int imgType) {
+    return imgType == ss::IMG_F64 ? 8 : imgType == ss::IMG_F32 ? 4 : 2;
+}
+
 ss_status validate(ss_handle *h, const ss_group_in *in, const ss_settings *s) {
     if (!in || !s) return fail(h, SS_ERR_INVALID_ARGUMENT, "not enough input arguments");
     if (in->frame_h < 1 || in->n_blocks < 1 || in->n_frames < 1)
@@ -331,8 +372,6 @@ ss_status validate(ss_handle *h, const ss_group_in *in, const ss_settings *s) {
         return fail(h, SS_ERR_INVALID_ARGUMENT, "NULL input array");
     if (in->data_type < SS_F64 || in->data_type > SS_I16)
         return fail(h, SS_ERR_INVALID_ARGUMENT, "invalid data_type");
-    if (in->data_on_device && in->data_type == SS_F64)
-        return fail(h, SS_ERR_INVALID_ARGUMENT, "device-resident data must be f32, u16 or i16");
     if (in->template_id <= 0 && (!in->template_image || !in->xgrad || !in->ygrad || in->template_h < 2 || in->template_w < 2))
         return fail(h, SS_ERR_INVALID_ARGUMENT, "template: bad size or NULL array");
     if (in->geometry_id <= 0 && (!in->frac || !in->base_x || !in->base_y || !in->base_mask))
@@ -362,10 +401,9 @@ struct Plan {
     ss_group_out *out;
     DevTemplate tmpl;
     DevGeometry geom;
-    std::vector<float2> tp, gxv, gyh;
-    std::vector<float4> tab;
-    std::vector<float> fn;
-    std::vector<double> p0;
+    TemplateHost th;
+    GeometryHost gh;
+    int fp64 = 0;
     size_t oTp = 0, oGxv = 0, oGyh = 0, oGeo = 0, oFn = 0, oImg = 0, oMin = 0;
     size_t oP = 0, oPL = 0, oP0 = 0, oD0 = 0, oDP = 0, oErr = 0, oCol = 0, oLs = 0, oCorr = 0,
            oFin = 0, oWs = 0, oUsed = 0, oAct = 0, oNext = 0, oMk = 0, oDk = 0, oArr = 0, oMask = 0,
@@ -392,6 +430,8 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
         if (st) return st;
         const ss_group_in &in = *P.in;
         const int fh = in.frame_h, nb = in.n_blocks, nF = in.n_frames, fw = nb * nF, N = fw + 1;
+        P.fp64 = P.s->precision == 'D' ? 1 : 0;
+        const bool wF = !P.fp64, wD = P.fp64;
         if (in.template_id > 0) {
             auto it = h->templates.find(in.template_id);
             if (it == h->templates.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown template_id");
@@ -399,13 +439,14 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
         } else {
             P.tmpl.th = in.template_h;
             P.tmpl.tw = in.template_w;
-            convert_template(in.template_h, in.template_w, in.template_image, in.xgrad, in.ygrad, P.tp,
-                             P.gxv, P.gyh, P.tmpl.gradFD);
+            convert_template(in.template_h, in.template_w, in.template_image, in.xgrad, in.ygrad, wF, wD,
+                             P.th, P.tmpl.gradFD);
             const size_t tn = (size_t)in.template_h * in.template_w;
-            P.oTp = A.take<float2>(tn);
+            const size_t pb = P.fp64 ? sizeof(double2) : sizeof(float2);
+            P.oTp = A.take<char>(tn * pb);
             if (!P.tmpl.gradFD) {
-                P.oGxv = A.take<float2>(tn);
-                P.oGyh = A.take<float2>(tn);
+                P.oGxv = A.take<char>(tn * pb);
+                P.oGyh = A.take<char>(tn * pb);
             }
         }
         if (in.geometry_id > 0) {
@@ -415,12 +456,19 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
             if (P.geom.frameH != fh || P.geom.nb != nb)
                 return fail(h, SS_ERR_INVALID_ARGUMENT, "resident geometry does not match frame_h x n_blocks");
         } else {
-            convert_geometry(fh, nb, in.frac, in.base_x, in.base_y, in.base_mask, P.geom, P.tab, P.fn);
-            if (!P.tab.empty()) P.oGeo = A.take<float4>(P.tab.size());
-            if (!P.fn.empty()) P.oFn = A.take<float>(P.fn.size());
+            convert_geometry(fh, nb, in.frac, in.base_x, in.base_y, in.base_mask, wF, wD, P.geom, P.gh);
+            if (!P.gh.tab.empty()) P.oGeo = A.take<float4>(P.gh.tab.size());
+            if (!P.gh.tabD.empty()) P.oGeo = A.take<double4>(P.gh.tabD.size());
+            if (!P.gh.fn.empty()) P.oFn = A.take<float>(P.gh.fn.size());
+            if (!P.gh.fnD.empty()) P.oFn = A.take<double>(P.gh.fnD.size());
         }
-        P.imgType = in.data_type == SS_U16 ? ss::IMG_U16 : in.data_type == SS_I16 ? ss::IMG_I16 : ss::IMG_F32;
-        P.imgBytes = (size_t)fh * fw * (P.imgType == ss::IMG_F32 ? 4 : 2);
+        // f64 host frames are converted to f32 for precision 'S' (as the mex converts them)
+        P.imgType = in.data_type == SS_U16   ? ss::IMG_U16
+                    : in.data_type == SS_I16 ? ss::IMG_I16
+                    : in.data_type == SS_F32 ? ss::IMG_F32
+                    : (in.data_on_device || P.fp64) ? ss::IMG_F64
+                                                    : ss::IMG_F32;
+        P.imgBytes = (size_t)fh * fw * img_size(P.imgType);
         if (!in.data_on_device) {
             P.oImg = A.take<char>(P.imgBytes);
             P.hImg = H.take<char>(P.imgBytes);
@@ -467,7 +515,7 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
     ss::GroupCtl *hc = (ss::GroupCtl *)(HS + hCtl);
     int *hitems = (int *)(HS + hItems);
     int itemStart = 0;
-    int hasType[3] = {0, 0, 0};
+    int hasType[2 * ss::IMG_NTYPES] = {};
     for (int gi = 0; gi < n; gi++) {
         Plan &P = plans[gi];
         const ss_group_in &in = *P.in;
@@ -487,8 +535,9 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
         g.itemStart = itemStart;
         g.seg = (fh + 255) / 256;
         const double eps = s.precision == 'S' ? (double)FLT_EPSILON : DBL_EPSILON;
-        g.xmax = (float)((double)(g.tw - 1) - eps);
-        g.ymax = (float)((double)(g.th - 1) - eps);
+        g.fp64 = P.fp64;
+        g.xmax = (double)(g.tw - 1) - eps;
+        g.ymax = (double)(g.th - 1) - eps;
         g.rmult = s.linesearch_reduction_multiplier;
         g.thr = s.move_threshold;
         g.halt = s.halt_correlation;
@@ -505,14 +554,32 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
             g.tp = P.tmpl.tp;
             g.gxv = P.tmpl.gxv;
             g.gyh = P.tmpl.gyh;
+            g.tpD = P.tmpl.tpD;
+            g.gxvD = P.tmpl.gxvD;
+            g.gyhD = P.tmpl.gyhD;
         } else {
-            g.tp = (const float2 *)dptr(P.oTp);
-            HIPCHK(hipMemcpyAsync(dptr(P.oTp), P.tp.data(), P.tp.size() * sizeof(float2), hipMemcpyHostToDevice, h->stream));
-            if (!P.tmpl.gradFD) {
-                g.gxv = (const float2 *)dptr(P.oGxv);
-                g.gyh = (const float2 *)dptr(P.oGyh);
-                HIPCHK(hipMemcpyAsync(dptr(P.oGxv), P.gxv.data(), P.gxv.size() * sizeof(float2), hipMemcpyHostToDevice, h->stream));
-                HIPCHK(hipMemcpyAsync(dptr(P.oGyh), P.gyh.data(), P.gyh.size() * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+            auto up = [&](size_t o, const void *src, size_t bytes) {
+                return hipMemcpyAsync(dptr(o), src, bytes, hipMemcpyHostToDevice, h->stream);
+            };
+            const TemplateHost &T = P.th;
+            if (P.fp64) {
+                g.tpD = (const double2 *)dptr(P.oTp);
+                HIPCHK(up(P.oTp, T.tpD.data(), T.tpD.size() * sizeof(double2)));
+                if (!P.tmpl.gradFD) {
+                    g.gxvD = (const double2 *)dptr(P.oGxv);
+                    g.gyhD = (const double2 *)dptr(P.oGyh);
+                    HIPCHK(up(P.oGxv, T.gxvD.data(), T.gxvD.size() * sizeof(double2)));
+                    HIPCHK(up(P.oGyh, T.gyhD.data(), T.gyhD.size() * sizeof(double2)));
+                }
+            } else {
+                g.tp = (const float2 *)dptr(P.oTp);
+                HIPCHK(up(P.oTp, T.tp.data(), T.tp.size() * sizeof(float2)));
+                if (!P.tmpl.gradFD) {
+                    g.gxv = (const float2 *)dptr(P.oGxv);
+                    g.gyh = (const float2 *)dptr(P.oGyh);
+                    HIPCHK(up(P.oGxv, T.gxv.data(), T.gxv.size() * sizeof(float2)));
+                    HIPCHK(up(P.oGyh, T.gyh.data(), T.gyh.size() * sizeof(float2)));
+                }
             }
         }
         g.gradFD = P.tmpl.gradFD;
@@ -520,32 +587,42 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
         const DevGeometry &G = P.geom;
         g.affine = G.affine;
         g.L = G.L;
-        g.gx0 = G.gx0; g.gxj = G.gxj; g.gxi = G.gxi;
-        g.gy0 = G.gy0; g.gyb = G.gyb; g.gyj = G.gyj; g.gyi = G.gyi;
-        g.f0 = G.f0; g.fj = G.fj; g.fi = G.fi;
+        g.afF = G.afF;
+        g.afD = G.afD;
         g.fracUniform = G.fracUniform;
         if (in.geometry_id > 0) {
             g.geo = G.geo;
+            g.geoD = G.geoD;
             g.fnext = G.fnext;
+            g.fnextD = G.fnextD;
         } else {
-            if (!P.tab.empty()) {
+            const GeometryHost &Q = P.gh;
+            if (!Q.tab.empty()) {
                 g.geo = (const float4 *)dptr(P.oGeo);
-                HIPCHK(hipMemcpyAsync(dptr(P.oGeo), P.tab.data(), P.tab.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+                HIPCHK(hipMemcpyAsync(dptr(P.oGeo), Q.tab.data(), Q.tab.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream));
             }
-            if (!P.fn.empty()) {
+            if (!Q.tabD.empty()) {
+                g.geoD = (const double4 *)dptr(P.oGeo);
+                HIPCHK(hipMemcpyAsync(dptr(P.oGeo), Q.tabD.data(), Q.tabD.size() * sizeof(double4), hipMemcpyHostToDevice, h->stream));
+            }
+            if (!Q.fn.empty()) {
                 g.fnext = (const float *)dptr(P.oFn);
-                HIPCHK(hipMemcpyAsync(dptr(P.oFn), P.fn.data(), P.fn.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+                HIPCHK(hipMemcpyAsync(dptr(P.oFn), Q.fn.data(), Q.fn.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            }
+            if (!Q.fnD.empty()) {
+                g.fnextD = (const double *)dptr(P.oFn);
+                HIPCHK(hipMemcpyAsync(dptr(P.oFn), Q.fnD.data(), Q.fnD.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
             }
         }
         // frame data
         g.imgType = P.imgType;
-        hasType[P.imgType] = 1;
+        hasType[P.imgType + ss::IMG_NTYPES * P.fp64] = 1;
         if (in.data_on_device) {
             g.img = in.data;
         } else {
             void *stage = HS + P.hImg;
             const size_t npix = (size_t)fh * fw;
-            if (in.data_type == SS_F64) {
+            if (in.data_type == SS_F64 && P.imgType == ss::IMG_F32) {
                 const double *src = (const double *)in.data;
                 float *dst = (float *)stage;
                 for (size_t i = 0; i < npix; i++) dst[i] = (float)src[i];
@@ -555,7 +632,7 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
             HIPCHK(hipMemcpyAsync(dptr(P.oImg), stage, P.imgBytes, hipMemcpyHostToDevice, h->stream));
             g.img = dptr(P.oImg);
         }
-        const size_t esz = P.imgType == ss::IMG_F32 ? 4 : 2;
+        const size_t esz = img_size(P.imgType);
         g.vec = ((g.seg % 8) == 0 && (fh % 8) == 0 && ((uintptr_t)g.img % 16) == 0 && (esz == 2 || (fh % 4) == 0)) ? 1 : 0;
         // min points (first n_blocks entries are read: minPointsPerBlock[c % frameW])
         HIPCHK(hipMemcpyAsync(dptr(P.oMin), in.min_points, nb * sizeof(int), hipMemcpyHostToDevice, h->stream));
@@ -623,7 +700,7 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
     L.b.groups = (ss::GroupDev *)dptr(oGroups);
     L.b.ctl = (ss::GroupCtl *)dptr(oCtl);
     L.b.itemGroup = (int *)dptr(oItems);
-    for (int i = 0; i < 3; i++) L.hasType[i] = hasType[i];
+    for (int i = 0; i < 2 * ss::IMG_NTYPES; i++) L.hasType[i] = hasType[i];
     L.itemGrid = std::max(1, std::min(nItems, 8 * h->numCU));
     static const int tick[] = {ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS,
                                ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS,
@@ -675,7 +752,7 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
         o.device_init_time = initTime;
         // algorithmic frame bytes per stage class for the roofline
         if (h->prof.on) {
-            const double fb = (double)in.frame_h * nb * (P.imgType == ss::IMG_F32 ? 4.0 : 2.0);
+            const double fb = (double)in.frame_h * nb * (double)img_size(P.imgType);
             h->prof.bytes[SS_K_ASSEMBLE] += fb * nF * c.halfIterations;
             h->prof.bytes[SS_K_LS_ROUND] += fb * c.lsFramePasses;
             h->prof.bytes[SS_K_CORR] += fb * nF * c.iteration;
@@ -744,19 +821,29 @@ ss_status ss_upload_template(ss_handle *h, int32_t th, int32_t tw, const double 
     if (th < 2 || tw < 2 || !t || !xg || !yg) return fail(h, SS_ERR_INVALID_ARGUMENT, "template: bad size or NULL array");
     HIPCHK(hipSetDevice(h->device));
     DevTemplate dt;
-    std::vector<float2> tp, gxv, gyh;
+    TemplateHost T;
     dt.th = th;
     dt.tw = tw;
-    convert_template(th, tw, t, xg, yg, tp, gxv, gyh, dt.gradFD);
-    const size_t tn = (size_t)th * tw, pb = (tn * sizeof(float2) + 255) & ~(size_t)255;
-    HIPCHK(hipMalloc(&dt.mem, dt.gradFD ? pb : 3 * pb));
-    dt.tp = (float2 *)dt.mem;
-    ss_status st = upload_owned(h, tp.data(), tn * sizeof(float2), dt.mem, 0);
+    convert_template(th, tw, t, xg, yg, true, true, T, dt.gradFD);  // both precisions
+    const size_t tn = (size_t)th * tw;
+    const size_t fb = (tn * sizeof(float2) + 255) & ~(size_t)255, db = (tn * sizeof(double2) + 255) & ~(size_t)255;
+    const int planes = dt.gradFD ? 1 : 3;
+    HIPCHK(hipMalloc(&dt.mem, planes * (fb + db)));
+    char *m = (char *)dt.mem;
+    ss_status st = SS_OK;
+    dt.tp = (float2 *)m;
+    dt.tpD = (double2 *)(m + planes * fb);
+    st = upload_owned(h, T.tp.data(), tn * sizeof(float2), m, 0);
+    if (!st) st = upload_owned(h, T.tpD.data(), tn * sizeof(double2), m, planes * fb);
     if (!st && !dt.gradFD) {
-        dt.gxv = (float2 *)((char *)dt.mem + pb);
-        dt.gyh = (float2 *)((char *)dt.mem + 2 * pb);
-        st = upload_owned(h, gxv.data(), tn * sizeof(float2), dt.mem, pb);
-        if (!st) st = upload_owned(h, gyh.data(), tn * sizeof(float2), dt.mem, 2 * pb);
+        dt.gxv = (float2 *)(m + fb);
+        dt.gyh = (float2 *)(m + 2 * fb);
+        dt.gxvD = (double2 *)(m + 3 * fb + db);
+        dt.gyhD = (double2 *)(m + 3 * fb + 2 * db);
+        st = upload_owned(h, T.gxv.data(), tn * sizeof(float2), m, fb);
+        if (!st) st = upload_owned(h, T.gyh.data(), tn * sizeof(float2), m, 2 * fb);
+        if (!st) st = upload_owned(h, T.gxvD.data(), tn * sizeof(double2), m, 3 * fb + db);
+        if (!st) st = upload_owned(h, T.gyhD.data(), tn * sizeof(double2), m, 3 * fb + 2 * db);
     }
     if (st) {
         (void)hipFree(dt.mem);
@@ -773,19 +860,27 @@ ss_status ss_upload_geometry(ss_handle *h, int32_t fh, int32_t nb, const double 
     if (fh < 1 || nb < 1 || !frac || !bx || !by || !bm) return fail(h, SS_ERR_INVALID_ARGUMENT, "geometry: bad size or NULL array");
     HIPCHK(hipSetDevice(h->device));
     DevGeometry g;
-    std::vector<float4> tab;
-    std::vector<float> fn;
-    convert_geometry(fh, nb, frac, bx, by, bm, g, tab, fn);
-    const size_t tb = (tab.size() * sizeof(float4) + 255) & ~(size_t)255, fb = fn.size() * sizeof(float);
-    HIPCHK(hipMalloc(&g.mem, tb + fb + 256));
+    GeometryHost Q;
+    convert_geometry(fh, nb, frac, bx, by, bm, true, true, g, Q);  // both precisions
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o1 = al(Q.tab.size() * sizeof(float4));
+    const size_t o2 = o1 + al(Q.tabD.size() * sizeof(double4));
+    const size_t o3 = o2 + al(Q.fn.size() * sizeof(float));
+    const size_t total = o3 + al(Q.fnD.size() * sizeof(double));
+    HIPCHK(hipMalloc(&g.mem, total + 256));
+    char *m = (char *)g.mem;
     ss_status st = SS_OK;
-    if (!tab.empty()) {
-        g.geo = (float4 *)g.mem;
-        st = upload_owned(h, tab.data(), tab.size() * sizeof(float4), g.mem, 0);
+    if (!Q.tab.empty()) {
+        g.geo = (float4 *)m;
+        g.geoD = (double4 *)(m + o1);
+        st = upload_owned(h, Q.tab.data(), Q.tab.size() * sizeof(float4), m, 0);
+        if (!st) st = upload_owned(h, Q.tabD.data(), Q.tabD.size() * sizeof(double4), m, o1);
     }
-    if (!st && !fn.empty()) {
-        g.fnext = (float *)((char *)g.mem + tb);
-        st = upload_owned(h, fn.data(), fb, g.mem, tb);
+    if (!st && !Q.fn.empty()) {
+        g.fnext = (float *)(m + o2);
+        g.fnextD = (double *)(m + o3);
+        st = upload_owned(h, Q.fn.data(), Q.fn.size() * sizeof(float), m, o2);
+        if (!st) st = upload_owned(h, Q.fnD.data(), Q.fnD.size() * sizeof(double), m, o3);
     }
     if (st) {
         (void)hipFree(g.mem);

@@ -26,9 +26,55 @@ constexpr int kSolveWG = 256;  // per-group kernels
 constexpr int kCH = 8;         // pixels per unrolled chunk
 
 // ---------------------------------------------------------------------------
+// precision: pixel arithmetic in R = float (precision 'S') or double ('D');
+// every reduction is FP64 in both
+// ---------------------------------------------------------------------------
+template <typename R>
+struct Prec;
+template <>
+struct Prec<float> {
+    using V2 = float2;
+    using V4 = float4;
+    static __device__ __forceinline__ const AffineGeo<float> &aff(const GroupDev &g) { return g.afF; }
+    static __device__ __forceinline__ const float4 *geo(const GroupDev &g) { return g.geo; }
+    static __device__ __forceinline__ const float *fnext(const GroupDev &g) { return g.fnext; }
+    static __device__ __forceinline__ const float2 *tp(const GroupDev &g) { return g.tp; }
+    static __device__ __forceinline__ const float2 *gxv(const GroupDev &g) { return g.gxv; }
+    static __device__ __forceinline__ const float2 *gyh(const GroupDev &g) { return g.gyh; }
+};
+template <>
+struct Prec<double> {
+    using V2 = double2;
+    using V4 = double4;
+    static __device__ __forceinline__ const AffineGeo<double> &aff(const GroupDev &g) { return g.afD; }
+    static __device__ __forceinline__ const double4 *geo(const GroupDev &g) { return g.geoD; }
+    static __device__ __forceinline__ const double *fnext(const GroupDev &g) { return g.fnextD; }
+    static __device__ __forceinline__ const double2 *tp(const GroupDev &g) { return g.tpD; }
+    static __device__ __forceinline__ const double2 *gxv(const GroupDev &g) { return g.gxvD; }
+    static __device__ __forceinline__ const double2 *gyh(const GroupDev &g) { return g.gyhD; }
+};
+
+__device__ __forceinline__ float mad(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double mad(double a, double b, double c) { return fma(a, b, c); }
+
+template <typename TI>
+__host__ __device__ constexpr int img_type() {
+    return std::is_same<TI, float>::value      ? IMG_F32
+           : std::is_same<TI, uint16_t>::value ? IMG_U16
+           : std::is_same<TI, int16_t>::value  ? IMG_I16
+                                               : IMG_F64;
+}
+// true when group g runs in the instantiation <TI, R>
+template <typename TI, typename R>
+__device__ __forceinline__ bool runs_here(const GroupDev &g) {
+    return g.imgType == img_type<TI>() && g.fp64 == (sizeof(R) == 8 ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void load8(const float *p, bool vec, int n, float (&v)[kCH]) {
+template <typename R>
+__device__ __forceinline__ void load8(const float *p, bool vec, int n, R (&v)[kCH]) {
     if (vec && n >= kCH) {
         const float4 a = *reinterpret_cast<const float4 *>(p);
         const float4 b = *reinterpret_cast<const float4 *>(p + 4);
@@ -36,35 +82,51 @@ __device__ __forceinline__ void load8(const float *p, bool vec, int n, float (&v
         v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
     } else {
 #pragma unroll
-        for (int i = 0; i < kCH; i++) v[i] = i < n ? p[i] : 0.f;
+        for (int i = 0; i < kCH; i++) v[i] = i < n ? (R)p[i] : (R)0;
     }
 }
-__device__ __forceinline__ void load8(const uint16_t *p, bool vec, int n, float (&v)[kCH]) {
+template <typename R>
+__device__ __forceinline__ void load8(const double *p, bool vec, int n, R (&v)[kCH]) {
+    if (vec && n >= kCH) {
+#pragma unroll
+        for (int i = 0; i < kCH / 2; i++) {
+            const double2 a = *reinterpret_cast<const double2 *>(p + 2 * i);
+            v[2 * i] = (R)a.x;
+            v[2 * i + 1] = (R)a.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kCH; i++) v[i] = i < n ? (R)p[i] : (R)0;
+    }
+}
+template <typename R>
+__device__ __forceinline__ void load8(const uint16_t *p, bool vec, int n, R (&v)[kCH]) {
     if (vec && n >= kCH) {
         const uint4 a = *reinterpret_cast<const uint4 *>(p);
         const unsigned w[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            v[2 * i] = (float)(w[i] & 0xffffu);
-            v[2 * i + 1] = (float)(w[i] >> 16);
+            v[2 * i] = (R)(w[i] & 0xffffu);
+            v[2 * i + 1] = (R)(w[i] >> 16);
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < kCH; i++) v[i] = i < n ? (float)p[i] : 0.f;
+        for (int i = 0; i < kCH; i++) v[i] = i < n ? (R)p[i] : (R)0;
     }
 }
-__device__ __forceinline__ void load8(const int16_t *p, bool vec, int n, float (&v)[kCH]) {
+template <typename R>
+__device__ __forceinline__ void load8(const int16_t *p, bool vec, int n, R (&v)[kCH]) {
     if (vec && n >= kCH) {
         const uint4 a = *reinterpret_cast<const uint4 *>(p);
         const unsigned w[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            v[2 * i] = (float)(int16_t)(w[i] & 0xffffu);
-            v[2 * i + 1] = (float)((int)w[i] >> 16);
+            v[2 * i] = (R)(int16_t)(w[i] & 0xffffu);
+            v[2 * i + 1] = (R)((int)w[i] >> 16);
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < kCH; i++) v[i] = i < n ? (float)p[i] : 0.f;
+        for (int i = 0; i < kCH; i++) v[i] = i < n ? (R)p[i] : (R)0;
     }
 }
 
@@ -117,70 +179,97 @@ __device__ __forceinline__ double reduce_pow(double v, double r, int k) {
 // Geometry of the pixels of one thread segment: affine in the block's
 // (column j, line i) coordinates -- the block matrix lists a block's pixels
 // column-major, row r = j * L + i -- or read from the per-pixel table.
+template <typename R>
 struct GeoIter {
-    float fj, fi, Lf, byb;
+    R fj, fi, Lf, byb;
     __device__ __forceinline__ void start(const GroupDev &g, int b, int r0) {
         if (g.affine) {
+            const AffineGeo<R> &a = Prec<R>::aff(g);
             const int j = r0 / g.L;
-            fj = (float)j;
-            fi = (float)(r0 - j * g.L);
-            Lf = (float)g.L;
-            byb = g.gy0 + g.gyb * (float)b;
+            fj = (R)j;
+            fi = (R)(r0 - j * g.L);
+            Lf = (R)g.L;
+            byb = a.gy0 + a.gyb * (R)b;
         }
     }
-    __device__ __forceinline__ void at(const GroupDev &g, const float4 *geo, int r, float &bx,
-                                       float &by, float &f, bool &m) const {
+    __device__ __forceinline__ void at(const GroupDev &g, const typename Prec<R>::V4 *geo, int r, R &bx,
+                                       R &by, R &f, bool &m) const {
         if (g.affine) {
-            f = fmaf(g.fi, fi, fmaf(g.fj, fj, g.f0));
-            bx = fmaf(g.gxi, fi, fmaf(g.gxj, fj, g.gx0));
-            by = fmaf(g.gyi, fi, fmaf(g.gyj, fj, byb));
+            const AffineGeo<R> &a = Prec<R>::aff(g);
+            f = mad(a.fi, fi, mad(a.fj, fj, a.f0));
+            bx = mad(a.gxi, fi, mad(a.gxj, fj, a.gx0));
+            by = mad(a.gyi, fi, mad(a.gyj, fj, byb));
             m = true;
         } else {
-            const float4 q = geo[r];
+            const typename Prec<R>::V4 q = geo[r];
             bx = q.x;
             by = q.y;
             f = q.z;
-            m = q.w != 0.f;
+            m = q.w != (R)0;
         }
     }
     __device__ __forceinline__ void next() {
-        fi += 1.f;
+        fi += (R)1;
         if (fi == Lf) {
-            fi = 0.f;
-            fj += 1.f;
+            fi = (R)0;
+            fj += (R)1;
         }
     }
 };
 
-__device__ __forceinline__ bool in_bounds(const GroupDev &g, float x, float y) {
-    return x >= 0.f && x < g.xmax && y >= 0.f && y < g.ymax;
+template <typename R>
+__device__ __forceinline__ bool in_bounds(const GroupDev &g, R x, R y) {
+    return x >= (R)0 && x < (R)g.xmax && y >= (R)0 && y < (R)g.ymax;
 }
 
 // Bilinear template sample at (x, y) inside the mask bounds
 // (task_differenceimage.cuh:13-32) from the paired layout.
+template <typename R>
 struct Cell {
-    float fx, fy;
-    float2 top, bot;
+    R fx, fy;
+    typename Prec<R>::V2 top, bot;
     int t;
 };
-__device__ __forceinline__ float sample(const GroupDev &g, float x, float y, Cell &cl) {
+template <typename R>
+__device__ __forceinline__ R sample(const GroupDev &g, R x, R y, Cell<R> &cl) {
     const int ix = (int)x, iy = (int)y;
-    cl.fx = x - (float)ix;
-    cl.fy = y - (float)iy;
+    cl.fx = x - (R)ix;
+    cl.fy = y - (R)iy;
     cl.t = iy * g.tw + ix;
-    cl.top = g.tp[cl.t];
-    cl.bot = g.tp[cl.t + g.tw];
-    const float a = fmaf(cl.fx, cl.top.y - cl.top.x, cl.top.x);
-    const float b = fmaf(cl.fx, cl.bot.y - cl.bot.x, cl.bot.x);
-    return fmaf(cl.fy, b - a, a);
+    const typename Prec<R>::V2 *tp = Prec<R>::tp(g);
+    cl.top = tp[cl.t];
+    cl.bot = tp[cl.t + g.tw];
+    const R a = mad(cl.fx, cl.top.y - cl.top.x, cl.top.x);
+    const R b = mad(cl.fx, cl.bot.y - cl.bot.x, cl.bot.x);
+    return mad(cl.fy, b - a, a);
+}
+
+// Template gradients at the sample of cl (task_updategradients.cu):
+// derived from the cell corners when they are forward differences.
+template <typename R>
+__device__ __forceinline__ void gradients(const GroupDev &g, const Cell<R> &cl, R &wx, R &wy) {
+    if (g.gradFD) {
+        wx = mad(cl.fy, (cl.bot.y - cl.bot.x) - (cl.top.y - cl.top.x), cl.top.y - cl.top.x);
+        wy = mad(cl.fx, (cl.bot.y - cl.top.y) - (cl.bot.x - cl.top.x), cl.bot.x - cl.top.x);
+    } else {
+        const typename Prec<R>::V2 gx = Prec<R>::gxv(g)[cl.t], gy = Prec<R>::gyh(g)[cl.t];
+        wx = mad(cl.fy, gx.y - gx.x, gx.x);
+        wy = mad(cl.fx, gy.y - gy.x, gy.x);
+    }
 }
 
 // Knots of a column for candidate k of the line search (or the half-
 // iteration start state for k < 0), as x = base_x + a + d * frac.
+template <typename R>
 struct ColKnots {
-    float ax, dx, ay, dy;
+    R ax, dx, ay, dy;
 };
-__device__ __forceinline__ ColKnots state_knots(const GroupDev &g, int c, int k) {
+template <typename R>
+__device__ __forceinline__ ColKnots<R> make_knots(double lx, double rx, double ly, double ry) {
+    return ColKnots<R>{(R)lx, (R)(rx - lx), (R)ly, (R)(ry - ly)};
+}
+template <typename R>
+__device__ __forceinline__ ColKnots<R> state_knots(const GroupDev &g, int c, int k) {
     const int N = g.N;
     double lx, ly, rx, ry;
     if (k < 0) {  // half-iteration start: p0 with the real right knot
@@ -200,19 +289,14 @@ __device__ __forceinline__ ColKnots state_knots(const GroupDev &g, int c, int k)
             ry = g.p0[c + 1 + N] + reduce_pow(g.d0[c + 1 + N], r, k);
         }
     }
-    ColKnots kn;
-    kn.ax = (float)lx;
-    kn.dx = (float)(rx - lx);
-    kn.ay = (float)ly;
-    kn.dy = (float)(ry - ly);
-    return kn;
+    return make_knots<R>(lx, rx, ly, ry);
 }
 
 // Knots of column c from the current parameter vector p (applyParameters on p).
-__device__ __forceinline__ ColKnots current_knots(const GroupDev &g, int c) {
+template <typename R>
+__device__ __forceinline__ ColKnots<R> current_knots(const GroupDev &g, int c) {
     const int N = g.N;
-    const double lx = g.p[c], ly = g.p[c + N], rx = g.p[c + 1], ry = g.p[c + 1 + N];
-    return ColKnots{(float)lx, (float)(rx - lx), (float)ly, (float)(ry - ly)};
+    return make_knots<R>(g.p[c], g.p[c + 1], g.p[c + N], g.p[c + 1 + N]);
 }
 
 template <typename TI>
@@ -225,79 +309,75 @@ __device__ __forceinline__ const TI *column_ptr(const GroupDev &g, int c) {
 // (d_applyParameters, d_createDifferenceImage, d_calcBaseErrval,
 //  d_updateGradients, d_fillDiagonals, d_fillJTr in one pass)
 // ---------------------------------------------------------------------------
-template <typename TI>
+template <typename TI, typename R>
 __device__ void assemble_column(const GroupDev &g, int c) {
     const int b = c % g.nb;
-    const ColKnots kn = current_knots(g, c);
+    const ColKnots<R> kn = current_knots<R>(g, c);
     const TI *img = column_ptr<TI>(g, c);
-    const float4 *geo = g.geo ? g.geo + (size_t)b * g.frameH : nullptr;
-    const float *fnx = g.fracUniform ? nullptr : g.fnext + (size_t)b * g.frameH;
+    const typename Prec<R>::V4 *geo = Prec<R>::geo(g) ? Prec<R>::geo(g) + (size_t)b * g.frameH : nullptr;
+    const R *fnx = g.fracUniform ? nullptr : Prec<R>::fnext(g) + (size_t)b * g.frameH;
     const int r0 = threadIdx.x * g.seg;
     const int n = min(g.seg, g.frameH - r0);
-    GeoIter it;
+    GeoIter<R> it;
     it.start(g, b, r0);
-    // h[0] = sum d^2, h[1] = sum I, h[i - 1] = column sum i for i >= CS_AXX
-    float h[CS_N - 1];
+    // h[1] = sum I, h[i - 1] = column sum i for i >= CS_AXX; the errval term
+    // is accumulated in FP64 per pixel: it decides line-search acceptance
+    // against near-equal candidates
+    double d2 = 0.0;
+    R h[CS_N - 1];
 #pragma unroll
-    for (int i = 0; i < CS_N - 1; i++) h[i] = 0.f;
+    for (int i = 0; i < CS_N - 1; i++) h[i] = (R)0;
     int cnt = 0;
     for (int ch = 0; ch < n; ch += kCH) {
-        float I[kCH];
+        R I[kCH];
         load8(img + r0 + ch, g.vec, n - ch, I);
 #pragma unroll
         for (int k = 0; k < kCH; k++) {
             if (ch + k < n) {
-                float bx, by, f;
+                R bx, by, f;
                 bool m;
                 it.at(g, geo, r0 + ch + k, bx, by, f, m);
                 it.next();
                 h[CS_SUMI] += I[k];
-                const float x = fmaf(kn.dx, f, bx + kn.ax);
-                const float y = fmaf(kn.dy, f, by + kn.ay);
+                const R x = mad(kn.dx, f, bx + kn.ax);
+                const R y = mad(kn.dy, f, by + kn.ay);
                 if (m && in_bounds(g, x, y)) {
-                    Cell cl;
-                    const float d = I[k] - sample(g, x, y, cl);
-                    float wx, wy;
-                    if (g.gradFD) {
-                        wx = fmaf(cl.fy, (cl.bot.y - cl.bot.x) - (cl.top.y - cl.top.x), cl.top.y - cl.top.x);
-                        wy = fmaf(cl.fx, (cl.bot.y - cl.top.y) - (cl.bot.x - cl.top.x), cl.bot.x - cl.top.x);
-                    } else {
-                        const float2 gx = g.gxv[cl.t], gy = g.gyh[cl.t];
-                        wx = fmaf(cl.fy, gx.y - gx.x, gx.x);
-                        wy = fmaf(cl.fx, gy.y - gy.x, gy.x);
-                    }
-                    const float cf = 1.f - f;
-                    const float xx = wx * wx, yy = wy * wy, xy = wx * wy;
-                    const float c2 = cf * cf, f2 = f * f, fc = f - f2;
+                    Cell<R> cl;
+                    const R d = I[k] - sample(g, x, y, cl);
+                    R wx, wy;
+                    gradients(g, cl, wx, wy);
+                    const R cf = (R)1 - f;
+                    const R xx = wx * wx, yy = wy * wy, xy = wx * wy;
+                    const R c2 = cf * cf, f2 = f * f, fc = f - f2;
                     cnt++;
-                    h[CS_D2] = fmaf(d, d, h[CS_D2]);
-                    h[CS_AXX - 1] = fmaf(xx, c2, h[CS_AXX - 1]);
-                    h[CS_AYY - 1] = fmaf(yy, c2, h[CS_AYY - 1]);
-                    h[CS_AXY - 1] = fmaf(xy, c2, h[CS_AXY - 1]);
-                    h[CS_BXX - 1] = fmaf(xx, f2, h[CS_BXX - 1]);
-                    h[CS_BYY - 1] = fmaf(yy, f2, h[CS_BYY - 1]);
-                    h[CS_BXY - 1] = fmaf(xy, f2, h[CS_BXY - 1]);
-                    h[CS_CXX - 1] = fmaf(xx, fc, h[CS_CXX - 1]);
-                    h[CS_CYY - 1] = fmaf(yy, fc, h[CS_CYY - 1]);
-                    h[CS_CXY - 1] = fmaf(xy, fc, h[CS_CXY - 1]);
-                    const float dc = d * cf, df = d * f;
-                    h[CS_JXA - 1] = fmaf(dc, wx, h[CS_JXA - 1]);
-                    h[CS_JYA - 1] = fmaf(dc, wy, h[CS_JYA - 1]);
-                    h[CS_JXB - 1] = fmaf(df, wx, h[CS_JXB - 1]);
-                    h[CS_JYB - 1] = fmaf(df, wy, h[CS_JYB - 1]);
+                    d2 = fma((double)d, (double)d, d2);
+                    h[CS_AXX - 1] = mad(xx, c2, h[CS_AXX - 1]);
+                    h[CS_AYY - 1] = mad(yy, c2, h[CS_AYY - 1]);
+                    h[CS_AXY - 1] = mad(xy, c2, h[CS_AXY - 1]);
+                    h[CS_BXX - 1] = mad(xx, f2, h[CS_BXX - 1]);
+                    h[CS_BYY - 1] = mad(yy, f2, h[CS_BYY - 1]);
+                    h[CS_BXY - 1] = mad(xy, f2, h[CS_BXY - 1]);
+                    h[CS_CXX - 1] = mad(xx, fc, h[CS_CXX - 1]);
+                    h[CS_CYY - 1] = mad(yy, fc, h[CS_CYY - 1]);
+                    h[CS_CXY - 1] = mad(xy, fc, h[CS_CXY - 1]);
+                    const R dc = d * cf, df = d * f;
+                    h[CS_JXA - 1] = mad(dc, wx, h[CS_JXA - 1]);
+                    h[CS_JYA - 1] = mad(dc, wy, h[CS_JYA - 1]);
+                    h[CS_JXB - 1] = mad(df, wx, h[CS_JXB - 1]);
+                    h[CS_JYB - 1] = mad(df, wy, h[CS_JYB - 1]);
                     if (fnx) {
-                        const float fn = fnx[r0 + ch + k];
-                        const float n2 = fn * fn;
-                        h[CS_QXX - 1] = fmaf(xx, n2, h[CS_QXX - 1]);
-                        h[CS_QYY - 1] = fmaf(yy, n2, h[CS_QYY - 1]);
-                        h[CS_QXY - 1] = fmaf(xy, n2, h[CS_QXY - 1]);
+                        const R fn = fnx[r0 + ch + k];
+                        const R n2 = fn * fn;
+                        h[CS_QXX - 1] = mad(xx, n2, h[CS_QXX - 1]);
+                        h[CS_QYY - 1] = mad(yy, n2, h[CS_QYY - 1]);
+                        h[CS_QXY - 1] = mad(xy, n2, h[CS_QXY - 1]);
                     }
                 }
             }
         }
     }
     double v[CS_N];
-    v[CS_D2] = (double)h[CS_D2];
+    v[CS_D2] = d2;
     v[CS_SUMI] = (double)h[CS_SUMI];
     v[CS_CNT] = (double)cnt;
 #pragma unroll
@@ -307,14 +387,14 @@ __device__ void assemble_column(const GroupDev &g, int c) {
     if (threadIdx.x < CS_N) g.colsum[(size_t)c * CS_N + threadIdx.x] = out[threadIdx.x];
 }
 
-template <typename TI>
+template <typename TI, typename R>
 __global__ __launch_bounds__(kWG) void k_assemble(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         if (B.ctl[gi].phase != PH_ASSEMBLE) continue;
         const GroupDev &g = B.groups[gi];
-        if (g.imgType != (sizeof(TI) == 4 ? IMG_F32 : (std::is_signed<TI>::value ? IMG_I16 : IMG_U16))) continue;
-        assemble_column<TI>(g, item - g.itemStart);
+        if (!runs_here<TI, R>(g)) continue;
+        assemble_column<TI, R>(g, item - g.itemStart);
     }
 }
 
@@ -730,9 +810,9 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
 // (the tries of d_linesearch, task_linesearch.cu:35-106, in one pass), then
 // the frame's last column to finish resolves the tries sequentially.
 // ---------------------------------------------------------------------------
-template <typename TI, int K>
+template <typename TI, typename R, int K>
 __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
-    ColKnots kn[K];
+    ColKnots<R> kn[K];
     {
         const int N = g.N;
         const double r = g.rmult;
@@ -744,7 +824,7 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
         for (int k = 0; k < K; k++) {
             const double lx = plx + dlx, ly = ply + dly;
             const double rx = lastKnot ? 1e6 : prx + drx, ry = lastKnot ? 1e6 : pry + dry;
-            kn[k] = ColKnots{(float)lx, (float)(rx - lx), (float)ly, (float)(ry - ly)};
+            kn[k] = make_knots<R>(lx, rx, ly, ry);
             dlx *= r;
             dly *= r;
             drx *= r;
@@ -752,37 +832,37 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
         }
     }
     const TI *img = column_ptr<TI>(g, c);
-    const float4 *geo = g.geo ? g.geo + (size_t)b * g.frameH : nullptr;
+    const typename Prec<R>::V4 *geo = Prec<R>::geo(g) ? Prec<R>::geo(g) + (size_t)b * g.frameH : nullptr;
     const int r0 = threadIdx.x * g.seg;
     const int n = min(g.seg, g.frameH - r0);
-    GeoIter it;
+    GeoIter<R> it;
     it.start(g, b, r0);
-    float e2[K];
+    double e2[K];  // FP64: compared against the half-iteration's errval
     int cnt[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        e2[k] = 0.f;
+        e2[k] = 0.0;
         cnt[k] = 0;
     }
     for (int ch = 0; ch < n; ch += kCH) {
-        float I[kCH];
+        R I[kCH];
         load8(img + r0 + ch, g.vec, n - ch, I);
 #pragma unroll
         for (int q = 0; q < kCH; q++) {
             if (ch + q < n) {
-                float bx, by, f;
+                R bx, by, f;
                 bool m;
                 it.at(g, geo, r0 + ch + q, bx, by, f, m);
                 it.next();
                 if (m) {
 #pragma unroll
                     for (int k = 0; k < K; k++) {
-                        const float x = fmaf(kn[k].dx, f, bx + kn[k].ax);
-                        const float y = fmaf(kn[k].dy, f, by + kn[k].ay);
+                        const R x = mad(kn[k].dx, f, bx + kn[k].ax);
+                        const R y = mad(kn[k].dy, f, by + kn[k].ay);
                         if (in_bounds(g, x, y)) {
-                            Cell cl;
-                            const float d = I[q] - sample(g, x, y, cl);
-                            e2[k] = fmaf(d, d, e2[k]);
+                            Cell<R> cl;
+                            const R d = I[q] - sample(g, x, y, cl);
+                            e2[k] = fma((double)d, (double)d, e2[k]);
                             cnt[k]++;
                         }
                     }
@@ -793,7 +873,7 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
     double v[2 * K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        v[2 * k] = (double)e2[k];
+        v[2 * k] = e2[k];
         v[2 * k + 1] = (double)cnt[k];
     }
     __shared__ double out[2 * kMaxCand];
@@ -930,7 +1010,7 @@ __device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f, int K) 
     ctl.phase = PH_CORR;
 }
 
-template <typename TI>
+template <typename TI, typename R>
 __global__ __launch_bounds__(kWG) void k_ls_round(Batch B) {
     __shared__ int s_last;
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
@@ -938,15 +1018,15 @@ __global__ __launch_bounds__(kWG) void k_ls_round(Batch B) {
         GroupCtl &ctl = B.ctl[gi];
         if (ctl.phase != PH_LS) continue;
         const GroupDev &g = B.groups[gi];
-        if (g.imgType != (sizeof(TI) == 4 ? IMG_F32 : (std::is_signed<TI>::value ? IMG_I16 : IMG_U16))) continue;
+        if (!runs_here<TI, R>(g)) continue;
         const int c = item - g.itemStart, f = c / g.nb, b = c % g.nb;
         if (!g.lsActive[f]) continue;
         const int K = ctl.K, k0 = g.lsNextK[f];
         switch (K) {
-        case 1: ls_column<TI, 1>(g, c, b, k0); break;
-        case 2: ls_column<TI, 2>(g, c, b, k0); break;
-        case 4: ls_column<TI, 4>(g, c, b, k0); break;
-        default: ls_column<TI, 8>(g, c, b, k0); break;
+        case 1: ls_column<TI, R, 1>(g, c, b, k0); break;
+        case 2: ls_column<TI, R, 2>(g, c, b, k0); break;
+        case 4: ls_column<TI, R, 4>(g, c, b, k0); break;
+        default: ls_column<TI, R, 8>(g, c, b, k0); break;
         }
         __threadfence();
         __syncthreads();
@@ -966,43 +1046,43 @@ __global__ __launch_bounds__(kWG) void k_ls_round(Batch B) {
 // task_correlation.cu:63-99) of every frame's last-tried state; the
 // statistic of a column covers its leading run of masked rows only.
 // ---------------------------------------------------------------------------
-template <typename TI>
+template <typename TI, typename R>
 __device__ void corr_column(const GroupDev &g, int c) {
     const int f = c / g.nb, b = c % g.nb;
     const int mk = g.lsMk[f], dk = g.lsDk[f];
     const bool same = (mk == dk);
-    const ColKnots km = state_knots(g, c, mk);
-    const ColKnots kd = state_knots(g, c, dk);
+    const ColKnots<R> km = state_knots<R>(g, c, mk);
+    const ColKnots<R> kd = state_knots<R>(g, c, dk);
     const TI *img = column_ptr<TI>(g, c);
-    const float4 *geo = g.geo ? g.geo + (size_t)b * g.frameH : nullptr;
+    const typename Prec<R>::V4 *geo = Prec<R>::geo(g) ? Prec<R>::geo(g) + (size_t)b * g.frameH : nullptr;
     const int r0 = threadIdx.x * g.seg;
     const int n = min(g.seg, g.frameH - r0);
-    GeoIter it;
+    GeoIter<R> it;
     it.start(g, b, r0);
     double v[CR_N];
 #pragma unroll
     for (int i = 0; i < CR_N; i++) v[i] = 0.0;
     bool open = true;  // still inside this thread's leading masked run
     for (int ch = 0; ch < n; ch += kCH) {
-        float I[kCH];
+        R I[kCH];
         load8(img + r0 + ch, g.vec, n - ch, I);
 #pragma unroll
         for (int q = 0; q < kCH; q++) {
             if (ch + q < n) {
-                float bx, by, f1;
+                R bx, by, f1;
                 bool m;
                 it.at(g, geo, r0 + ch + q, bx, by, f1, m);
                 it.next();
-                const float x = fmaf(km.dx, f1, bx + km.ax);
-                const float y = fmaf(km.dy, f1, by + km.ay);
+                const R x = mad(km.dx, f1, bx + km.ax);
+                const R y = mad(km.dy, f1, by + km.ay);
                 const bool mm = m && in_bounds(g, x, y);
-                float dbuf = 0.f;
-                Cell cl;
+                R dbuf = (R)0;
+                Cell<R> cl;
                 if (same) {
                     if (mm) dbuf = I[q] - sample(g, x, y, cl);
                 } else {
-                    const float xd = fmaf(kd.dx, f1, bx + kd.ax);
-                    const float yd = fmaf(kd.dy, f1, by + kd.ay);
+                    const R xd = mad(kd.dx, f1, bx + kd.ax);
+                    const R yd = mad(kd.dy, f1, by + kd.ay);
                     if (m && in_bounds(g, xd, yd)) dbuf = I[q] - sample(g, xd, yd, cl);
                 }
                 v[CR_SUMD] += mm ? (double)(I[q] - dbuf) : (double)dbuf;
@@ -1030,14 +1110,14 @@ __device__ void corr_column(const GroupDev &g, int c) {
     if (threadIdx.x < CR_N) g.corrsum[(size_t)c * CR_N + threadIdx.x] = out[threadIdx.x];
 }
 
-template <typename TI>
+template <typename TI, typename R>
 __global__ __launch_bounds__(kWG) void k_corr(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         if (B.ctl[gi].phase != PH_CORR) continue;
         const GroupDev &g = B.groups[gi];
-        if (g.imgType != (sizeof(TI) == 4 ? IMG_F32 : (std::is_signed<TI>::value ? IMG_I16 : IMG_U16))) continue;
-        corr_column<TI>(g, item - g.itemStart);
+        if (!runs_here<TI, R>(g)) continue;
+        corr_column<TI, R>(g, item - g.itemStart);
     }
 }
 
@@ -1118,37 +1198,37 @@ __global__ __launch_bounds__(kSolveWG) void k_corr_finish(Batch B) {
 // final state: errvals, frame correlations and mask of the last-tried state
 // (hd_calculateGroupErrvals(0) + hd_frameCorrelation, task.cpp:389-405)
 // ---------------------------------------------------------------------------
-template <typename TI>
+template <typename TI, typename R>
 __device__ void final_column(const GroupDev &g, int c) {
     const int f = c / g.nb, b = c % g.nb;
-    const ColKnots kn = state_knots(g, c, g.lsMk[f]);
+    const ColKnots<R> kn = state_knots<R>(g, c, g.lsMk[f]);
     const TI *img = column_ptr<TI>(g, c);
-    const float4 *geo = g.geo ? g.geo + (size_t)b * g.frameH : nullptr;
+    const typename Prec<R>::V4 *geo = Prec<R>::geo(g) ? Prec<R>::geo(g) + (size_t)b * g.frameH : nullptr;
     uint8_t *mo = g.maskOut ? g.maskOut + (size_t)c * g.frameH : nullptr;
     const int r0 = threadIdx.x * g.seg;
     const int n = min(g.seg, g.frameH - r0);
-    GeoIter it;
+    GeoIter<R> it;
     it.start(g, b, r0);
     double v[FS_N];
 #pragma unroll
     for (int i = 0; i < FS_N; i++) v[i] = 0.0;
     for (int ch = 0; ch < n; ch += kCH) {
-        float I[kCH];
+        R I[kCH];
         load8(img + r0 + ch, g.vec, n - ch, I);
 #pragma unroll
         for (int q = 0; q < kCH; q++) {
             if (ch + q < n) {
-                float bx, by, f1;
+                R bx, by, f1;
                 bool m;
                 it.at(g, geo, r0 + ch + q, bx, by, f1, m);
                 it.next();
-                const float x = fmaf(kn.dx, f1, bx + kn.ax);
-                const float y = fmaf(kn.dy, f1, by + kn.ay);
+                const R x = mad(kn.dx, f1, bx + kn.ax);
+                const R y = mad(kn.dy, f1, by + kn.ay);
                 const bool mm = m && in_bounds(g, x, y);
                 if (mo) mo[r0 + ch + q] = mm ? 1 : 0;
                 if (mm) {
-                    Cell cl;
-                    const float wI = sample(g, x, y, cl);
+                    Cell<R> cl;
+                    const R wI = sample(g, x, y, cl);
                     const double d = (double)(I[q] - wI);
                     v[FS_D2] += d * d;
                     v[FS_CNT] += 1.0;
@@ -1166,14 +1246,14 @@ __device__ void final_column(const GroupDev &g, int c) {
     if (threadIdx.x < FS_N) g.finalsum[(size_t)c * FS_N + threadIdx.x] = out[threadIdx.x];
 }
 
-template <typename TI>
+template <typename TI, typename R>
 __global__ __launch_bounds__(kWG) void k_final(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         if (B.ctl[gi].phase != PH_FINAL) continue;
         const GroupDev &g = B.groups[gi];
-        if (g.imgType != (sizeof(TI) == 4 ? IMG_F32 : (std::is_signed<TI>::value ? IMG_I16 : IMG_U16))) continue;
-        final_column<TI>(g, item - g.itemStart);
+        if (!runs_here<TI, R>(g)) continue;
+        final_column<TI, R>(g, item - g.itemStart);
     }
 }
 
@@ -1202,11 +1282,22 @@ __global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-#define SS_LAUNCH_TYPED(kernel, grid, B, s)                                              \
-    do {                                                                                 \
-        if ((B).hasType[IMG_F32]) hipLaunchKernelGGL(kernel<float>, grid, dim3(kWG), 0, s, (B).b);    \
-        if ((B).hasType[IMG_U16]) hipLaunchKernelGGL(kernel<uint16_t>, grid, dim3(kWG), 0, s, (B).b); \
-        if ((B).hasType[IMG_I16]) hipLaunchKernelGGL(kernel<int16_t>, grid, dim3(kWG), 0, s, (B).b);  \
+// one launch per (frame type, precision) present in the batch
+#define SS_LAUNCH_ONE(kernel, TI, R, grid, L, s)                                                  \
+    do {                                                                                          \
+        if ((L).hasType[img_type<TI>() + IMG_NTYPES * (sizeof(R) == 8 ? 1 : 0)])                  \
+            hipLaunchKernelGGL((kernel<TI, R>), grid, dim3(kWG), 0, s, (L).b);                   \
+    } while (0)
+#define SS_LAUNCH_TYPED(kernel, grid, L, s)                  \
+    do {                                                     \
+        SS_LAUNCH_ONE(kernel, float, float, grid, L, s);     \
+        SS_LAUNCH_ONE(kernel, uint16_t, float, grid, L, s);  \
+        SS_LAUNCH_ONE(kernel, int16_t, float, grid, L, s);   \
+        SS_LAUNCH_ONE(kernel, double, float, grid, L, s);    \
+        SS_LAUNCH_ONE(kernel, float, double, grid, L, s);    \
+        SS_LAUNCH_ONE(kernel, uint16_t, double, grid, L, s); \
+        SS_LAUNCH_ONE(kernel, int16_t, double, grid, L, s);  \
+        SS_LAUNCH_ONE(kernel, double, double, grid, L, s);   \
     } while (0)
 
 hipError_t launch_stage(const LaunchBatch &L, int stage, hipStream_t s) {
