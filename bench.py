@@ -6,6 +6,10 @@ settings (steady_scan.m:836-867), synthetic data (seeded generator).  One
 step aligns all 20 groups, submitted together through the C ABI
 (ss_align_batch), with frames, template and geometry resident in HBM.
 
+Precision: 'D' (the reference's default, FP64 pixel arithmetic) is the
+headline; 'S' (FP32 pixel arithmetic, FP64 reductions and solve) is timed
+beside it.
+
 Printed JSON line: frames/s (whole job); group latency p50/p99 of
 single-group submissions (the plain path) and the batch latency; knot RMS
 and corrected-frame error of one group against the FP64 oracle; the
@@ -36,9 +40,14 @@ def parse():
     ap.add_argument("--frames", type=int, default=1000)
     ap.add_argument("--group", type=int, default=50)
     ap.add_argument("--candidates", type=int, default=2)
+    ap.add_argument("--precision", choices=["D", "S", "both"], default="both",
+                    help="pixel precision of the timed runs; the first listed is the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-latency", action="store_true", help="skip the single-group latency leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="simulated acquisition time of the serving leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of known size (rocprof counter calibration)")
     return ap.parse_args()
@@ -54,6 +63,81 @@ def calibrate_copy():
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"calibration": "copy", "bytes_read": n, "bytes_written": n, "launches": 5}))
+
+
+def serve_leg(args, torch, ssa, W):
+    """Continuous batching of acquisition streams (16 x 512x512 at 30 Hz +
+    2 x 1024x1024 at 15 Hz, int16, groups of 16-64 frames, jittered
+    arrivals, 64 GiB budget): groups are submitted to a server as they
+    arrive in real time; latency = completion - arrival."""
+    t0 = time.time()
+    streams, groups = W.config4(duration=args.serve_seconds)
+    print(f"[bench] serving: {len(streams)} streams, {len(groups)} groups generated in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    h = ssa.Handle(0)
+    geoms, calls = {}, []
+    tids = []
+    for st in streams:
+        T = st["T"]
+        xg, yg = W.gradients(T)
+        tids.append((h.upload_template(T, xg, yg), xg, yg))
+        size = st["size"]
+        if size not in geoms:
+            g = ssa.init_settings(size, size, T.shape, nknots=size // 16 + 1)
+            geoms[size] = (g, h.upload_geometry(g["tfrac_bm"], g["basex_bm"], g["basey_bm"], g["base_mask_bm"]))
+    dev = []
+    for gr in groups:
+        st = streams[gr["stream"]]
+        geom, gid = geoms[st["size"]]
+        nk = geom["nknots"]
+        bm = ssa.image2blockmatrix(st["frames"][:, :, gr["first"]:gr["first"] + gr["count"]], geom)
+        d = torch.from_numpy(bm.ravel(order="F")).to("cuda")
+        dev.append(d)
+        tid, xg, yg = tids[gr["stream"]]
+        a = dict(flags="D", template_image=tid, xgrad=xg, ygrad=yg, tfrac=gid,
+                 basex=geom["basex_bm"], basey=geom["basey_bm"], base_mask=geom["base_mask_bm"],
+                 bmData=(d.data_ptr(), np.int16), est_dx_knots=np.zeros((nk, gr["count"])),
+                 est_dy_knots=np.zeros((nk, gr["count"])), group_size=2 * (nk - 1), nsetframes=gr["count"],
+                 min_points_per_block=geom["min_points_per_block"].astype(np.float64))
+        a.update(W.default_args())
+        calls.append(a)
+    torch.cuda.synchronize()
+    srv = ssa.Server(handle=h, hbm_budget_bytes=64 << 30, linesearch_candidates=args.candidates)
+    lat, done, nxt, steps, resident_max = {}, 0, 0, 0, 0
+    tickets = {}
+    start = time.perf_counter()
+    busy = 0.0
+    while done < len(groups):
+        now = time.perf_counter() - start
+        while nxt < len(groups) and groups[nxt]["arrival"] <= now:
+            tickets[srv.submit(calls[nxt])] = nxt
+            nxt += 1
+        if len(tickets) > done:
+            t = time.perf_counter()
+            nr, nq = srv.step()
+            busy += time.perf_counter() - t
+            steps += 1
+            resident_max = max(resident_max, nr)
+            end = time.perf_counter() - start
+            for tk, r in srv.completed().items():
+                if isinstance(r, Exception):
+                    raise r
+                gi = tickets[tk]
+                lat[gi] = end - groups[gi]["arrival"]
+                done += 1
+        elif nxt < len(groups):
+            time.sleep(max(0.0, min(0.002, groups[nxt]["arrival"] - now)))
+    makespan = time.perf_counter() - start
+    srv.close()
+    ms = np.array([lat[i] for i in range(len(groups))]) * 1e3
+    nfr = sum(g["count"] for g in groups)
+    px = sum(g["count"] * streams[g["stream"]]["size"] ** 2 for g in groups)
+    return {"streams": "16 x 512x512 @ 30 Hz + 2 x 1024x1024 @ 15 Hz, int16, precision D",
+            "simulated_seconds": args.serve_seconds, "groups": len(groups), "frames": nfr,
+            "p50_group_ms": float(np.percentile(ms, 50)), "p99_group_ms": float(np.percentile(ms, 99)),
+            "max_group_ms": float(ms.max()), "frames_per_s": nfr / makespan,
+            "megapixels_per_s": px / makespan / 1e6, "offered_megapixels_per_s": px / args.serve_seconds / 1e6,
+            "steps": steps, "max_resident_groups": resident_max, "device_busy_fraction": busy / makespan}
 
 
 def main():
@@ -94,8 +178,8 @@ def main():
     torch.cuda.synchronize()
     zeros = np.zeros((nk, G))
 
-    def call(g):
-        a = dict(flags="D", template_image=tid, xgrad=xg, ygrad=yg, tfrac=gid,
+    def call(g, prec):
+        a = dict(flags=prec, template_image=tid, xgrad=xg, ygrad=yg, tfrac=gid,
                  basex=geom["basex_bm"], basey=geom["basey_bm"], base_mask=geom["base_mask_bm"],
                  bmData=(dev[g].data_ptr(), np.uint16), est_dx_knots=zeros, est_dy_knots=zeros,
                  group_size=2 * nb, nsetframes=G,
@@ -103,104 +187,128 @@ def main():
         a.update(W.default_args())
         return a
 
-    calls = [call(g) for g in range(ngroups)]
+    def timed(prec):
+        calls = [call(g, prec) for g in range(ngroups)]
 
-    def step():
-        return ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+        def step():
+            return ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
 
-    results = None
-    for _ in range(args.warmup):
-        results = step()
-    torch.cuda.synchronize()
-    N.lib().ss_profile_enable(h.ptr, 1)
-    batch_lat = []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t = time.perf_counter()
-        results = step()
-        batch_lat.append(time.perf_counter() - t)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    launches = (ctypes.c_int64 * 6)()
-    kms = (ctypes.c_double * 6)()
-    kbytes = (ctypes.c_double * 6)()
-    N.lib().ss_profile_read(h.ptr, launches, kms, kbytes)
-    N.lib().ss_profile_enable(h.ptr, 0)
-    print(f"[bench] timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
+        res = None
+        for _ in range(args.warmup):
+            res = step()
+        torch.cuda.synchronize()
+        N.lib().ss_profile_enable(h.ptr, 1)
+        lat = []
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            res = step()
+            lat.append(time.perf_counter() - t)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t_start
+        launches = (ctypes.c_int64 * 6)()
+        kms = (ctypes.c_double * 6)()
+        kbytes = (ctypes.c_double * 6)()
+        N.lib().ss_profile_read(h.ptr, launches, kms, kbytes)
+        N.lib().ss_profile_enable(h.ptr, 0)
+        print(f"[bench] precision {prec}: timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
+        per_kernel = {}
+        for i in range(6):
+            if launches[i]:
+                per_kernel[N.KERNEL_CLASSES[i]] = {
+                    "launches": int(launches[i]), "ms": kms[i], "avg_us": kms[i] / launches[i] * 1e3,
+                    "frame_GBps": (kbytes[i] / (kms[i] / 1e3) / 1e9) if kms[i] > 0 else 0.0}
+        dom = max(range(6), key=lambda i: kms[i])
+        avg_s = kms[dom] / launches[dom] / 1e3 if launches[dom] else float("nan")
+        bpl = kbytes[dom] / launches[dom] if launches[dom] else 0.0
+        return dict(calls=calls, results=res, elapsed=elapsed, lat=lat, per_kernel=per_kernel, dom=dom,
+                    avg_s=avg_s, bytes_per_launch=bpl,
+                    achieved=bpl / avg_s / 1e9 if launches[dom] else 0.0,
+                    fps=ngroups * G * args.steps / elapsed)
 
-    frames_done = ngroups * G * args.steps
-    fps = frames_done / elapsed
-    per_kernel = {}
-    for i in range(6):
-        if launches[i]:
-            per_kernel[N.KERNEL_CLASSES[i]] = {
-                "launches": int(launches[i]), "ms": kms[i], "avg_us": kms[i] / launches[i] * 1e3,
-                "frame_GBps": (kbytes[i] / (kms[i] / 1e3) / 1e9) if kms[i] > 0 else 0.0}
-    dom = max(range(6), key=lambda i: kms[i])
-    avg_s = kms[dom] / launches[dom] / 1e3 if launches[dom] else float("nan")
-    bytes_per_launch = kbytes[dom] / launches[dom] if launches[dom] else 0.0
-    achieved = bytes_per_launch / avg_s / 1e9 if launches[dom] else 0.0
+    precs = ["D", "S"] if args.precision == "both" else [args.precision]
+    serving = None
+    runs = {p: timed(p) for p in precs}
+    head = runs[precs[0]]
+    results = head["results"]
 
     # latency of the plain path: one group per submission
     single = []
     if not args.no_latency:
         for g in range(ngroups):
             t = time.perf_counter()
-            ssa.align_groups([calls[g]], handle=h, linesearch_candidates=args.candidates, with_mask=False)
+            ssa.align_groups([head["calls"][g]], handle=h, linesearch_candidates=args.candidates, with_mask=False)
             single.append(time.perf_counter() - t)
     single_ms = np.array(single) * 1e3
 
-    # parity of one group against the FP64 oracle, and the CPU baseline
-    rms = None
+    if not args.no_serve:
+        serving = serve_leg(args, torch, ssa, W)
+        print(f"[bench] serving: p50 {serving['p50_group_ms']:.1f} ms p99 {serving['p99_group_ms']:.1f} ms "
+              f"{serving['frames_per_s']:.0f} frames/s", file=sys.stderr, flush=True)
+
+    # parity of group 0 against the FP64 oracle (same precision letter, which
+    # only sets the mask-bound epsilon there), and the CPU baseline
+    parity = {}
     cpu = None
-    corr_err = None
-    corr_rms = None
     if not args.no_cpu:
         from oracle import oracle as O
         g = 0
-        a = W.group_args(T, frames, geom, first=g * G, count=G)
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        t = time.perf_counter()
-        ref = O.align_group_to_template(a, threads=threads, with_mask=False)
-        tcpu = time.perf_counter() - t
-        out = results[g]
-        dx = (out["knots_x"] - ref["knots_x"]).reshape(G, nk)[:-1]
-        dy = (out["knots_y"] - ref["knots_y"]).reshape(G, nk)[:-1]
-        rms = float(np.sqrt(np.mean(np.concatenate([dx.ravel(), dy.ravel()]) ** 2)))
-        # corrected frames from the GPU knots vs the oracle knots (same splat)
         fr = frames[:, :, g * G:g * G + G - 1]
         m = nk * (G - 1)
-        cg = ssa.resample_frames(fr, geom, out["knots_x"][:m], out["knots_y"][:m], handle=h)
-        co = ssa.resample_frames(fr, geom, ref["knots_x"][:m], ref["knots_y"][:m], handle=h)
-        both = np.isfinite(cg) & np.isfinite(co)
-        diff = np.abs(cg[both] - co[both])
-        corr_err = float(diff.max()) if diff.size else 0.0
-        corr_rms = float(np.sqrt(np.mean(diff ** 2))) if diff.size else 0.0
-        cpu = {"value": G / tcpu, "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"FP64 oracle (C restatement, OpenMP) on 1 group of {G} frames 512x512 "
-                         f"(group 0 of the workload), {ref['iterations']} GN iterations, {tcpu:.1f} s"}
-        print(f"[bench] oracle group 0: iterations {ref['iterations']} gpu {out['iterations']} "
-              f"rms {rms:.2e} px", file=sys.stderr, flush=True)
+        for prec in precs:
+            a = W.group_args(T, frames, geom, first=g * G, count=G, flags=prec)
+            t = time.perf_counter()
+            ref = O.align_group_to_template(a, threads=threads, with_mask=False)
+            tcpu = time.perf_counter() - t
+            out = runs[prec]["results"][g]
+            dx = (out["knots_x"] - ref["knots_x"]).reshape(G, nk)[:-1]
+            dy = (out["knots_y"] - ref["knots_y"]).reshape(G, nk)[:-1]
+            rms = float(np.sqrt(np.mean(np.concatenate([dx.ravel(), dy.ravel()]) ** 2)))
+            # corrected frames from the GPU knots vs the oracle knots (same splat)
+            cg = ssa.resample_frames(fr, geom, out["knots_x"][:m], out["knots_y"][:m], handle=h)
+            co = ssa.resample_frames(fr, geom, ref["knots_x"][:m], ref["knots_y"][:m], handle=h)
+            both = np.isfinite(cg) & np.isfinite(co)
+            diff = np.abs(cg[both] - co[both])
+            parity[prec] = dict(knot_rms=rms, corr_max=float(diff.max()) if diff.size else 0.0,
+                                corr_rms=float(np.sqrt(np.mean(diff ** 2))) if diff.size else 0.0,
+                                iters_gpu=int(out["iterations"]), iters_oracle=int(ref["iterations"]))
+            print(f"[bench] oracle group 0 [{prec}]: iterations {ref['iterations']} gpu {out['iterations']} "
+                  f"rms {rms:.2e} px ({tcpu:.1f} s)", file=sys.stderr, flush=True)
+            if cpu is None:
+                cpu = {"value": G / tcpu, "unit": "frames/s", "cores": threads, "kind": "port",
+                       "sample": f"FP64 oracle (C restatement, OpenMP) on 1 group of {G} frames 512x512 "
+                                 f"(group 0 of the workload), {ref['iterations']} GN iterations, {tcpu:.1f} s"}
 
     iters = [r["iterations"] for r in results]
-    lat_ms = np.array(batch_lat) * 1e3
+    lat_ms = np.array(head["lat"]) * 1e3
+    hp = precs[0]
+    dom = head["dom"]
+    other = {}
+    for prec in precs[1:]:
+        r = runs[prec]
+        other[prec] = {"frames_per_s": r["fps"], "ms_per_step": r["elapsed"] / args.steps * 1e3,
+                       "gn_iterations_mean": float(np.mean([x["iterations"] for x in r["results"]])),
+                       "kernels": r["per_kernel"], **{k: v for k, v in parity.get(prec, {}).items()}}
+    hpar = parity.get(hp, {})
     line = {
         "metric": METRIC,
-        "value": fps,
+        "value": head["fps"],
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": head["elapsed"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64" if hp == "D" else "f32",
         "data": "synthetic",
         "config": {
             "workload": f"{args.frames} frames 512x512 uint16 in groups of {G}, knots every 16 lines, "
                         "default settings (max_iter 120, smearing filter, exact solve); "
                         "all groups of a step in one batched submission",
+            "precision": hp,
             "frames_per_step": ngroups * G,
             "groups_per_step": ngroups,
             "p50_group_ms": float(np.percentile(single_ms, 50)) if single else None,
@@ -210,22 +318,25 @@ def main():
             "gn_iterations_mean": float(np.mean(iters)),
             "gn_iterations_max": int(np.max(iters)),
             "linesearch_candidates_first_round": args.candidates,
-            "knot_rms_vs_fp64_px": rms,
-            "corrected_frame_max_abs_err": corr_err,
-            "corrected_frame_rms_err": corr_rms,
-            "kernels": per_kernel,
-            "storage": "u16 frames; f32 warp/residual/accumulation; f64 reductions and solve",
+            "knot_rms_vs_fp64_px": hpar.get("knot_rms"),
+            "corrected_frame_max_abs_err": hpar.get("corr_max"),
+            "corrected_frame_rms_err": hpar.get("corr_rms"),
+            "kernels": head["per_kernel"],
+            "storage": "u16 frames; " + ("f64 warp/residual/accumulation" if hp == "D" else
+                                         "f32 warp/residual/accumulation") + "; f64 reductions and solve",
+            "other_precisions": other,
+            "serving": serving,
         },
         "roofline": {
             "kernel": N.KERNEL_CLASSES[dom],
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": head["achieved"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": head["achieved"] / HBM_PEAK_GBS,
             "traffic": None,
-            "bytes_per_launch": bytes_per_launch,
-            "avg_launch_us": avg_s * 1e6,
+            "bytes_per_launch": head["bytes_per_launch"],
+            "avg_launch_us": head["avg_s"] * 1e6,
         },
         "cpu_baseline": cpu,
     }

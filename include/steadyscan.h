@@ -171,6 +171,39 @@ ss_status ss_profile_enable(ss_handle *h, int32_t on);
  * bytes (frame pixels read x element size) per kernel class. */
 ss_status ss_profile_read(ss_handle *h, int64_t *launches, double *ms, double *frame_bytes);
 
+/* ---- serving: continuous batching of groups ----
+ * No reference counterpart: the mex aligns one group per call and resets
+ * the device afterwards (task.cpp:164-197).  A server keeps a running batch
+ * on the handle's device: each step advances every resident group by one
+ * tick (about one Gauss-Newton iteration), retires the groups that finished
+ * (their outputs are written) and admits queued groups while the group-slot
+ * count and the HBM budget allow.  A group's result is the same alone and in
+ * any mix (every group's arithmetic is independent and deterministic).
+ * A server is used by one thread at a time, like its handle. */
+typedef struct ss_server ss_server;
+typedef struct {
+    int64_t hbm_budget_bytes;    /* device memory for admitted groups (0: 64 GiB) */
+    int32_t max_groups;          /* groups resident at once (0: 256)            */
+} ss_server_config;
+
+ss_status ss_server_create(ss_handle *h, const ss_server_config *cfg, ss_server **out);
+/* Frees the server's device memory; queued and resident groups are dropped. */
+void ss_server_destroy(ss_server *s);
+/* Queues one group (the arguments of ss_align_group).  The structs are
+ * copied; the arrays they point to, device frames included, and the out
+ * buffers must stay valid until the ticket is reported by
+ * ss_server_completed. */
+ss_status ss_server_submit(ss_server *s, const ss_group_in *in, const ss_settings *st,
+                           ss_group_out *out, int64_t *ticket);
+/* Admits, advances one tick, retires.  Reports the resident and queued
+ * group counts after the step. */
+ss_status ss_server_step(ss_server *s, int32_t *n_resident, int32_t *n_queued);
+/* Moves up to cap finished tickets (with their status) to the caller;
+ * returns how many. */
+int32_t ss_server_completed(ss_server *s, int64_t *tickets, ss_status *status, int32_t cap);
+/* Device bytes reserved by the server's pool and held by resident groups. */
+ss_status ss_server_memory(const ss_server *s, int64_t *reserved, int64_t *in_use);
+
 /* ---- host-side geometry: init_settings + image2blockmatrix ---- */
 
 /* Scan geometry (settings fields of steady_scan.m:748-828, 836-867). */

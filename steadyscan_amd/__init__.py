@@ -4,9 +4,9 @@ The compute runs in the HIP library ``libsteadyscan_amd.so`` (C ABI in
 include/steadyscan.h); this package is its host-side mirror of the
 reference's mex interface.  Importing the package does not touch the GPU.
 """
-from .api import (Handle, SteadyScanError, align_group_to_template, align_groups,  # noqa: F401
+from .api import (Handle, Server, SteadyScanError, align_group_to_template, align_groups,  # noqa: F401
                   default_settings, geometry_desc, image2blockmatrix, init_settings, parse_flags,
                   resample_frames)
 
-__all__ = ["Handle", "SteadyScanError", "align_group_to_template", "align_groups", "default_settings",
+__all__ = ["Handle", "Server", "SteadyScanError", "align_group_to_template", "align_groups", "default_settings",
            "geometry_desc", "image2blockmatrix", "init_settings", "parse_flags", "resample_frames"]

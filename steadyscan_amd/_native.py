@@ -68,6 +68,10 @@ class SSGroupOut(ctypes.Structure):
     ]
 
 
+class SSServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_groups", c_int32)]
+
+
 class SSGeometryDesc(ctypes.Structure):
     _fields_ = [
         ("nlines", c_int32), ("linewidth", c_int32),
@@ -135,6 +139,13 @@ API = [
     ("ss_resample", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(ctypes.c_float)]),
     ("ss_profile_enable", c_int32, [c_void_p, c_int32]),
     ("ss_profile_read", c_int32, [c_void_p, POINTER(ctypes.c_int64), POINTER(c_double), POINTER(c_double)]),
+    ("ss_server_create", c_int32, [c_void_p, POINTER(SSServerConfig), POINTER(c_void_p)]),
+    ("ss_server_destroy", None, [c_void_p]),
+    ("ss_server_submit", c_int32, [c_void_p, POINTER(SSGroupIn), POINTER(SSSettings), POINTER(SSGroupOut),
+                                   POINTER(ctypes.c_int64)]),
+    ("ss_server_step", c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
+    ("ss_server_completed", c_int32, [c_void_p, POINTER(ctypes.c_int64), POINTER(c_int32), c_int32]),
+    ("ss_server_memory", c_int32, [c_void_p, POINTER(ctypes.c_int64), POINTER(ctypes.c_int64)]),
 ]
 
 KERNEL_CLASSES = ["assemble", "solve", "ls_round", "ls_decide", "corr", "final"]

@@ -246,6 +246,83 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     return tuple(outs[:nargout])
 
 
+def _prepare_call(a, h, s, gi, go, with_mask, linesearch_candidates, keep):
+    """Fills settings s, group input gi and output go for one call (a dict
+    keyed by the mex argument names); returns the result dict whose arrays
+    go points into.  Arrays that must outlive the call are appended to keep."""
+    precision, _ = parse_flags(a.get("flags", "D"))
+    ds = default_settings()
+    ctypes.memmove(ctypes.addressof(s), ctypes.addressof(ds), ctypes.sizeof(ds))
+    s.max_iterations = int(a["max_iter"])
+    s.move_threshold = float(a["move_thresh"])
+    s.halt_correlation = float(a["halt_correlation"])
+    s.correlation_increase_threshold = float(a["correlation_increase_threshold"])
+    s.filter_strength = float(a["filter_strength"])
+    s.max_filter_iterations = int(a["max_filter_iterations"])
+    s.filter_correlation_threshold = float(a["filter_deactivation_threshold"])
+    s.lambda_ = float(a["lambda_"])
+    s.linesearch_reduction_multiplier = float(a["linesearch_reduction_multiplier"])
+    s.solver_id = int(a["solver_id"])
+    s.filter_id = int(a["filter_id"])
+    s.device_id = h.device
+    s.precision = precision.encode()
+    s.linesearch_candidates = int(a.get("linesearch_candidates", linesearch_candidates))
+    if isinstance(a["template_image"], int):  # resident template id
+        gi.template_id = a["template_image"]
+        gi.template_h, gi.template_w = np.shape(a["xgrad"])
+    else:
+        t, gx, gy = _f64(a["template_image"]), _f64(a["xgrad"]), _f64(a["ygrad"])
+        keep += [t, gx, gy]
+        gi.template_h, gi.template_w = t.shape
+        gi.template_image, gi.xgrad, gi.ygrad = _ptr(t, c_double), _ptr(gx, c_double), _ptr(gy, c_double)
+    if isinstance(a["tfrac"], int):  # resident geometry id
+        gi.geometry_id = a["tfrac"]
+        fh, nb = np.shape(a["basex"])
+    else:
+        f, bx, by = _f64(a["tfrac"]), _f64(a["basex"]), _f64(a["basey"])
+        bm = np.asfortranarray(np.asarray(a["base_mask"], dtype=np.uint8))
+        keep += [f, bx, by, bm]
+        gi.frac, gi.base_x, gi.base_y, gi.base_mask = (_ptr(f, c_double), _ptr(bx, c_double),
+                                                       _ptr(by, c_double), _ptr(bm, c_uint8))
+        fh, nb = bx.shape
+    if isinstance(a["bmData"], tuple):  # (device pointer, dtype)
+        ptr, dt = a["bmData"]
+        gi.data = c_void_p(int(ptr))
+        gi.data_type = _DTYPES[np.dtype(dt)]
+        gi.data_on_device = 1
+    else:
+        d = np.asarray(a["bmData"])
+        if d.dtype not in _DTYPES:
+            d = d.astype(np.float64)
+        d = np.asfortranarray(d)
+        keep.append(d)
+        gi.data = c_void_p(d.ctypes.data)
+        gi.data_type = _DTYPES[d.dtype]
+    ex, ey = _f64(a["est_dx_knots"]), _f64(a["est_dy_knots"])
+    mp = np.ascontiguousarray(np.trunc(np.asarray(a["min_points_per_block"], dtype=np.float64)).astype(np.int32).ravel())
+    keep += [ex, ey, mp]
+    nF = int(a["nsetframes"])
+    gi.frame_h, gi.n_blocks, gi.n_frames, gi.group_size = fh, nb, nF, int(a["group_size"])
+    gi.init_x, gi.init_y, gi.min_points = _ptr(ex, c_double), _ptr(ey, c_double), _ptr(mp, c_int32)
+    r = dict(knots_x=np.zeros((nb + 1) * nF), knots_y=np.zeros((nb + 1) * nF),
+             frame_corr=np.zeros(nF), residrms=np.zeros(nF),
+             mask=np.zeros((fh, nb * nF), dtype=np.uint8, order="F") if with_mask else None)
+    go.knots_x, go.knots_y = _ptr(r["knots_x"], c_double), _ptr(r["knots_y"], c_double)
+    go.frame_corr, go.residrms = _ptr(r["frame_corr"], c_double), _ptr(r["residrms"], c_double)
+    if with_mask:
+        go.mask = _ptr(r["mask"], c_uint8)
+    return r
+
+
+def _collect(r, go):
+    if r["mask"] is not None:
+        r["mask"] = r["mask"].astype(bool)
+    r.update(iterations=go.iterations, stop_criterion=N.STOP_NAMES[go.stop_criterion],
+             half_iterations=go.half_iterations, linesearch_rounds=go.linesearch_rounds,
+             linesearch_tries=go.linesearch_tries)
+    return r
+
+
 def align_groups(calls, handle=None, device=0, linesearch_candidates=2, with_mask=True):
     """Several independent align_group_to_template calls in one submission
     (ss_align_batch).  `calls` is a list of dicts keyed by the mex argument
@@ -257,81 +334,80 @@ def align_groups(calls, handle=None, device=0, linesearch_candidates=2, with_mas
     sets = (N.SSSettings * n)()
     gouts = (N.SSGroupOut * n)()
     keep = []
-    results = []
-    for i, a in enumerate(calls):
-        precision, _ = parse_flags(a.get("flags", "D"))
-        s = default_settings()
-        s.max_iterations = int(a["max_iter"])
-        s.move_threshold = float(a["move_thresh"])
-        s.halt_correlation = float(a["halt_correlation"])
-        s.correlation_increase_threshold = float(a["correlation_increase_threshold"])
-        s.filter_strength = float(a["filter_strength"])
-        s.max_filter_iterations = int(a["max_filter_iterations"])
-        s.filter_correlation_threshold = float(a["filter_deactivation_threshold"])
-        s.lambda_ = float(a["lambda_"])
-        s.linesearch_reduction_multiplier = float(a["linesearch_reduction_multiplier"])
-        s.solver_id = int(a["solver_id"])
-        s.filter_id = int(a["filter_id"])
-        s.device_id = h.device
-        s.precision = precision.encode()
-        s.linesearch_candidates = int(a.get("linesearch_candidates", linesearch_candidates))
-        sets[i] = s
-        gi = gins[i]
-        if isinstance(a["template_image"], int):  # resident template id
-            gi.template_id = a["template_image"]
-            gi.template_h, gi.template_w = np.shape(a["xgrad"])
-        else:
-            t, gx, gy = _f64(a["template_image"]), _f64(a["xgrad"]), _f64(a["ygrad"])
-            keep += [t, gx, gy]
-            gi.template_h, gi.template_w = t.shape
-            gi.template_image, gi.xgrad, gi.ygrad = _ptr(t, c_double), _ptr(gx, c_double), _ptr(gy, c_double)
-        if isinstance(a["tfrac"], int):  # resident geometry id
-            gi.geometry_id = a["tfrac"]
-            fh, nb = np.shape(a["basex"])
-        else:
-            f, bx, by = _f64(a["tfrac"]), _f64(a["basex"]), _f64(a["basey"])
-            bm = np.asfortranarray(np.asarray(a["base_mask"], dtype=np.uint8))
-            keep += [f, bx, by, bm]
-            gi.frac, gi.base_x, gi.base_y, gi.base_mask = (_ptr(f, c_double), _ptr(bx, c_double),
-                                                           _ptr(by, c_double), _ptr(bm, c_uint8))
-            fh, nb = bx.shape
-        if isinstance(a["bmData"], tuple):  # (device pointer, dtype)
-            ptr, dt = a["bmData"]
-            gi.data = c_void_p(int(ptr))
-            gi.data_type = _DTYPES[np.dtype(dt)]
-            gi.data_on_device = 1
-        else:
-            d = np.asarray(a["bmData"])
-            if d.dtype not in _DTYPES:
-                d = d.astype(np.float64)
-            d = np.asfortranarray(d)
-            keep.append(d)
-            gi.data = c_void_p(d.ctypes.data)
-            gi.data_type = _DTYPES[d.dtype]
-        ex, ey = _f64(a["est_dx_knots"]), _f64(a["est_dy_knots"])
-        mp = np.ascontiguousarray(np.trunc(np.asarray(a["min_points_per_block"], dtype=np.float64)).astype(np.int32).ravel())
-        keep += [ex, ey, mp]
-        nF = int(a["nsetframes"])
-        gi.frame_h, gi.n_blocks, gi.n_frames, gi.group_size = fh, nb, nF, int(a["group_size"])
-        gi.init_x, gi.init_y, gi.min_points = _ptr(ex, c_double), _ptr(ey, c_double), _ptr(mp, c_int32)
-        r = dict(knots_x=np.zeros((nb + 1) * nF), knots_y=np.zeros((nb + 1) * nF),
-                 frame_corr=np.zeros(nF), residrms=np.zeros(nF),
-                 mask=np.zeros((fh, nb * nF), dtype=np.uint8, order="F") if with_mask else None)
-        go = gouts[i]
-        go.knots_x, go.knots_y = _ptr(r["knots_x"], c_double), _ptr(r["knots_y"], c_double)
-        go.frame_corr, go.residrms = _ptr(r["frame_corr"], c_double), _ptr(r["residrms"], c_double)
-        if with_mask:
-            go.mask = _ptr(r["mask"], c_uint8)
-        results.append(r)
+    results = [_prepare_call(a, h, sets[i], gins[i], gouts[i], with_mask, linesearch_candidates, keep)
+               for i, a in enumerate(calls)]
     _check(N.lib().ss_align_batch(h.ptr, n, gins, sets, gouts), h)
-    for r, go in zip(results, gouts):
-        if with_mask:
-            r["mask"] = r["mask"].astype(bool)
-        r.update(iterations=go.iterations, stop_criterion=N.STOP_NAMES[go.stop_criterion],
-                 half_iterations=go.half_iterations, linesearch_rounds=go.linesearch_rounds,
-                 linesearch_tries=go.linesearch_tries)
+    out = [_collect(r, go) for r, go in zip(results, gouts)]
     del keep
-    return results
+    return out
+
+
+class Server:
+    """Continuous batching of alignment groups on one device (ss_server_*):
+    submit() queues a group (a dict of mex arguments, as for align_groups),
+    step() admits queued groups as slots and the HBM budget allow, advances
+    every resident group by one tick and retires finished ones; completed()
+    returns {ticket: result dict} for the groups finished since the last
+    call."""
+
+    def __init__(self, handle=None, device=0, hbm_budget_bytes=0, max_groups=0, linesearch_candidates=2,
+                 with_mask=False):
+        self.h = handle if handle is not None else _default_handle(device)
+        cfg = N.SSServerConfig()
+        cfg.hbm_budget_bytes = int(hbm_budget_bytes)
+        cfg.max_groups = int(max_groups)
+        self.ptr = c_void_p()
+        _check(N.lib().ss_server_create(self.h.ptr, ctypes.byref(cfg), ctypes.byref(self.ptr)), self.h)
+        self.K = linesearch_candidates
+        self.with_mask = with_mask
+        self._pending = {}  # ticket -> (structs, arrays, result)
+
+    def submit(self, call):
+        gi, s, go = N.SSGroupIn(), N.SSSettings(), N.SSGroupOut()
+        keep = []
+        r = _prepare_call(call, self.h, s, gi, go, self.with_mask, self.K, keep)
+        t = ctypes.c_int64()
+        _check(N.lib().ss_server_submit(self.ptr, ctypes.byref(gi), ctypes.byref(s), ctypes.byref(go),
+                                        ctypes.byref(t)), self.h)
+        self._pending[t.value] = (gi, s, go, keep, r)
+        return t.value
+
+    def step(self):
+        nr, nq = c_int32(), c_int32()
+        _check(N.lib().ss_server_step(self.ptr, ctypes.byref(nr), ctypes.byref(nq)), self.h)
+        return nr.value, nq.value
+
+    def completed(self):
+        out = {}
+        cap = 64
+        tickets = (ctypes.c_int64 * cap)()
+        status = (c_int32 * cap)()
+        while True:
+            n = N.lib().ss_server_completed(self.ptr, tickets, status, cap)
+            for i in range(n):
+                gi, s, go, keep, r = self._pending.pop(tickets[i])
+                if status[i] != 0:
+                    out[tickets[i]] = SteadyScanError(status[i], N.lib().ss_last_error(self.h.ptr).decode())
+                else:
+                    out[tickets[i]] = _collect(r, go)
+            if n < cap:
+                return out
+
+    def memory(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(N.lib().ss_server_memory(self.ptr, ctypes.byref(a), ctypes.byref(b)), self.h)
+        return a.value, b.value
+
+    def close(self):
+        if self.ptr:
+            N.lib().ss_server_destroy(self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

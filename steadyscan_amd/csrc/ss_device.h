@@ -61,7 +61,7 @@ struct GroupDev {
     // sizes (mexgateway.cpp:176-198)
     int frameH, nb, nF, fframeW, N, th, tw, groupSize;
     int itemStart;        // first work item (= column) of this group in the batch
-    int seg;              // pixels per thread segment (ceil(frameH / 256))
+    int frameStart;       // first frame item of this group in the batch
     // settings
     double xmax, ymax;    // template_w - 1 - eps, template_h - 1 - eps
     int fp64;             // pixel arithmetic in FP64 (precision 'D') or FP32 ('S')
@@ -90,7 +90,7 @@ struct GroupDev {
     double *p, *pLast, *p0, *d0, *deltaP, *errval;
     double *colsum, *lssum, *corrsum, *finalsum, *ws;
     uint8_t *used;
-    int *lsActive, *lsNextK, *lsMk, *lsDk, *frameArrive;
+    int *lsActive, *lsNextK, *lsMk, *lsDk;
     uint8_t *maskOut;
     double *resid, *fcorr;
     double *allParams, *iterCorr;  // optional diagnostics (or null)
@@ -102,27 +102,28 @@ struct GroupCtl {
     int solverFilter, filterID, nFilterIterations;
     int retv, error;
     int halfIterations, rounds, tries, roundsThisLS;
-    int activeFrames, roundArrive, roundFrames, K;
+    int roundFrames, K;   // frames evaluated in the current round, candidates per frame
     int lsFramePasses;    // frames evaluated over all line-search rounds
     int pad;
     double correlation, corr;
 };
 
 struct Batch {
-    int nGroups, nItems;
+    int nGroups, nItems, nFrames;
     GroupDev *groups;
     GroupCtl *ctl;
-    int *itemGroup;       // work item -> group
+    int *itemGroup;       // work item (column) -> group
+    int *frameGroup;      // frame item -> group
 };
 
 struct LaunchBatch {
     Batch b;
     int hasType[2 * IMG_NTYPES];  // [imgType + IMG_NTYPES * fp64]
-    int itemGrid;
+    int itemGrid, frameGrid;
 };
 
 // Stages of one host submission, in order; see launch_stage.
-enum Stage { ST_ASSEMBLE = 0, ST_SOLVE, ST_LS, ST_CORR, ST_CORR_FINISH, ST_FINAL, ST_FINAL_FINISH };
+enum Stage { ST_ASSEMBLE = 0, ST_SOLVE, ST_LS, ST_LS_DECIDE, ST_CORR, ST_CORR_FINISH, ST_FINAL, ST_FINAL_FINISH };
 
 hipError_t launch_stage(const LaunchBatch &L, int stage, hipStream_t s);
 size_t solve_workspace_bytes(int n);

@@ -11,7 +11,10 @@
 #include <chrono>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <iterator>
 #include <map>
 #include <string>
 #include <vector>
@@ -83,6 +86,10 @@ struct ss_handle {
     void *harena = nullptr;  // pinned host staging
     size_t harenaBytes = 0;
     Profile prof;
+    // the tick's launches, captured once per group table and replayed
+    bool useGraph = true;
+    hipGraphExec_t tickExec = nullptr;
+    ss::LaunchBatch tickFor{};
 };
 
 namespace {
@@ -135,6 +142,7 @@ int stage_class(int stage) {
     case ss::ST_ASSEMBLE: return SS_K_ASSEMBLE;
     case ss::ST_SOLVE: return SS_K_SOLVE;
     case ss::ST_LS: return SS_K_LS_ROUND;
+    case ss::ST_LS_DECIDE: return SS_K_LS_DECIDE;
     case ss::ST_CORR:
     case ss::ST_CORR_FINISH: return SS_K_CORR;
     default: return SS_K_FINAL;
@@ -394,7 +402,8 @@ ss_status validate(ss_handle *h, const ss_group_in *in, const ss_settings *s) {
     return SS_OK;
 }
 
-// Per-group host-side plan of the arena.
+// Per-group host-side plan: the group's device block and host staging, as
+// offsets from the group's own base.
 struct Plan {
     const ss_group_in *in;
     const ss_settings *s;
@@ -406,362 +415,772 @@ struct Plan {
     int fp64 = 0;
     size_t oTp = 0, oGxv = 0, oGyh = 0, oGeo = 0, oFn = 0, oImg = 0, oMin = 0;
     size_t oP = 0, oPL = 0, oP0 = 0, oD0 = 0, oDP = 0, oErr = 0, oCol = 0, oLs = 0, oCorr = 0,
-           oFin = 0, oWs = 0, oUsed = 0, oAct = 0, oNext = 0, oMk = 0, oDk = 0, oArr = 0, oMask = 0,
+           oFin = 0, oWs = 0, oUsed = 0, oAct = 0, oNext = 0, oMk = 0, oDk = 0, oMask = 0,
            oRes = 0, oFc = 0, oAll = 0, oIc = 0;
     size_t hImg = 0, hP = 0;  // host staging offsets
+    size_t dBytes = 0, hBytes = 0;
     int imgType = 0;
     size_t imgBytes = 0;
 };
+
+// Validates one call and lays out its device block and host staging.
+ss_status plan_group(ss_handle *h, Plan &P) {
+    ss_status st = validate(h, P.in, P.s);
+    if (st) return st;
+    Arena A, H;
+    const ss_group_in &in = *P.in;
+    const int fh = in.frame_h, nb = in.n_blocks, nF = in.n_frames, fw = nb * nF, N = fw + 1;
+    P.fp64 = P.s->precision == 'D' ? 1 : 0;
+    const bool wF = !P.fp64, wD = P.fp64;
+    if (in.template_id > 0) {
+        auto it = h->templates.find(in.template_id);
+        if (it == h->templates.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown template_id");
+        P.tmpl = it->second;
+    } else {
+        P.tmpl.th = in.template_h;
+        P.tmpl.tw = in.template_w;
+        convert_template(in.template_h, in.template_w, in.template_image, in.xgrad, in.ygrad, wF, wD,
+                         P.th, P.tmpl.gradFD);
+        const size_t tn = (size_t)in.template_h * in.template_w;
+        const size_t pb = P.fp64 ? sizeof(double2) : sizeof(float2);
+        P.oTp = A.take<char>(tn * pb);
+        if (!P.tmpl.gradFD) {
+            P.oGxv = A.take<char>(tn * pb);
+            P.oGyh = A.take<char>(tn * pb);
+        }
+    }
+    if (in.geometry_id > 0) {
+        auto it = h->geometries.find(in.geometry_id);
+        if (it == h->geometries.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown geometry_id");
+        P.geom = it->second;
+        if (P.geom.frameH != fh || P.geom.nb != nb)
+            return fail(h, SS_ERR_INVALID_ARGUMENT, "resident geometry does not match frame_h x n_blocks");
+    } else {
+        convert_geometry(fh, nb, in.frac, in.base_x, in.base_y, in.base_mask, wF, wD, P.geom, P.gh);
+        if (!P.gh.tab.empty()) P.oGeo = A.take<float4>(P.gh.tab.size());
+        if (!P.gh.tabD.empty()) P.oGeo = A.take<double4>(P.gh.tabD.size());
+        if (!P.gh.fn.empty()) P.oFn = A.take<float>(P.gh.fn.size());
+        if (!P.gh.fnD.empty()) P.oFn = A.take<double>(P.gh.fnD.size());
+    }
+    // f64 host frames are converted to f32 for precision 'S' (as the mex converts them)
+    P.imgType = in.data_type == SS_U16   ? ss::IMG_U16
+                : in.data_type == SS_I16 ? ss::IMG_I16
+                : in.data_type == SS_F32 ? ss::IMG_F32
+                : (in.data_on_device || P.fp64) ? ss::IMG_F64
+                                                : ss::IMG_F32;
+    P.imgBytes = (size_t)fh * fw * img_size(P.imgType);
+    if (!in.data_on_device) {
+        P.oImg = A.take<char>(P.imgBytes);
+        P.hImg = H.take<char>(P.imgBytes);
+    }
+    P.oMin = A.take<int>(nb);
+    const size_t Pn = 2 * (size_t)N;
+    P.oP = A.take<double>(Pn);
+    P.oPL = A.take<double>(Pn);
+    P.oP0 = A.take<double>(Pn);
+    P.oD0 = A.take<double>(Pn);
+    P.oDP = A.take<double>(Pn);
+    P.oErr = A.take<double>(2 * (size_t)nF);
+    P.oCol = A.take<double>((size_t)fw * ss::CS_N);
+    P.oLs = A.take<double>((size_t)fw * 2 * ss::kMaxCand);
+    P.oCorr = A.take<double>((size_t)fw * ss::CR_N);
+    P.oFin = A.take<double>((size_t)fw * ss::FS_N);
+    P.oWs = A.take<char>(ss::solve_workspace_bytes(fw));
+    P.oUsed = A.take<uint8_t>(N);
+    P.oAct = A.take<int>(nF);
+    P.oNext = A.take<int>(nF);
+    P.oMk = A.take<int>(nF);
+    P.oDk = A.take<int>(nF);
+    if (P.out->mask) P.oMask = A.take<uint8_t>((size_t)fh * fw);
+    P.oRes = A.take<double>(nF);
+    P.oFc = A.take<double>(nF);
+    if (P.out->all_params) P.oAll = A.take<double>(Pn * P.s->max_iterations);
+    if (P.out->iteration_corr) P.oIc = A.take<double>(P.s->max_iterations);
+    P.hP = H.take<double>(Pn + 2 * (size_t)nF);
+    P.dBytes = A.off;
+    P.hBytes = H.off;
+    return SS_OK;
+}
+
+// Fills the group's device description and initial control state and
+// enqueues its uploads on the handle's stream (D: the group's device block,
+// HS: its pinned staging, which must stay untouched until the stream syncs).
+ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g, ss::GroupCtl &c) {
+    auto dptr = [&](size_t o) { return (void *)(D + o); };
+    const ss_group_in &in = *P.in;
+    const ss_settings &s = *P.s;
+    const int fh = in.frame_h, nb = in.n_blocks, nF = in.n_frames, fw = nb * nF, N = fw + 1;
+    const size_t Pn = 2 * (size_t)N;
+    std::memset(&g, 0, sizeof(g));
+    g.frameH = fh;
+    g.nb = nb;
+    g.nF = nF;
+    g.fframeW = fw;
+    g.N = N;
+    g.th = P.tmpl.th;
+    g.tw = P.tmpl.tw;
+    g.groupSize = in.group_size;
+    const double eps = s.precision == 'S' ? (double)FLT_EPSILON : DBL_EPSILON;
+    g.fp64 = P.fp64;
+    g.xmax = (double)(g.tw - 1) - eps;
+    g.ymax = (double)(g.th - 1) - eps;
+    g.rmult = s.linesearch_reduction_multiplier;
+    g.thr = s.move_threshold;
+    g.halt = s.halt_correlation;
+    g.cit = s.correlation_increase_threshold;
+    g.fct = s.filter_correlation_threshold;
+    g.strength = s.filter_strength;
+    g.lambda = s.lambda;
+    g.maxIter = s.max_iterations;
+    g.K1 = s.linesearch_candidates <= 0 ? 2 : s.linesearch_candidates;
+    g.Kn = std::max(g.K1, 4);
+    g.maxRounds = 4096 / g.Kn + 1;
+    // template
+    auto up = [&](size_t o, const void *src, size_t bytes) {
+        return hipMemcpyAsync(dptr(o), src, bytes, hipMemcpyHostToDevice, h->stream);
+    };
+    if (in.template_id > 0) {
+        g.tp = P.tmpl.tp;
+        g.gxv = P.tmpl.gxv;
+        g.gyh = P.tmpl.gyh;
+        g.tpD = P.tmpl.tpD;
+        g.gxvD = P.tmpl.gxvD;
+        g.gyhD = P.tmpl.gyhD;
+    } else {
+        const TemplateHost &T = P.th;
+        if (P.fp64) {
+            g.tpD = (const double2 *)dptr(P.oTp);
+            HIPCHK(up(P.oTp, T.tpD.data(), T.tpD.size() * sizeof(double2)));
+            if (!P.tmpl.gradFD) {
+                g.gxvD = (const double2 *)dptr(P.oGxv);
+                g.gyhD = (const double2 *)dptr(P.oGyh);
+                HIPCHK(up(P.oGxv, T.gxvD.data(), T.gxvD.size() * sizeof(double2)));
+                HIPCHK(up(P.oGyh, T.gyhD.data(), T.gyhD.size() * sizeof(double2)));
+            }
+        } else {
+            g.tp = (const float2 *)dptr(P.oTp);
+            HIPCHK(up(P.oTp, T.tp.data(), T.tp.size() * sizeof(float2)));
+            if (!P.tmpl.gradFD) {
+                g.gxv = (const float2 *)dptr(P.oGxv);
+                g.gyh = (const float2 *)dptr(P.oGyh);
+                HIPCHK(up(P.oGxv, T.gxv.data(), T.gxv.size() * sizeof(float2)));
+                HIPCHK(up(P.oGyh, T.gyh.data(), T.gyh.size() * sizeof(float2)));
+            }
+        }
+    }
+    g.gradFD = P.tmpl.gradFD;
+    // geometry
+    const DevGeometry &G = P.geom;
+    g.affine = G.affine;
+    g.L = G.L;
+    g.afF = G.afF;
+    g.afD = G.afD;
+    g.fracUniform = G.fracUniform;
+    if (in.geometry_id > 0) {
+        g.geo = G.geo;
+        g.geoD = G.geoD;
+        g.fnext = G.fnext;
+        g.fnextD = G.fnextD;
+    } else {
+        const GeometryHost &Q = P.gh;
+        if (!Q.tab.empty()) {
+            g.geo = (const float4 *)dptr(P.oGeo);
+            HIPCHK(up(P.oGeo, Q.tab.data(), Q.tab.size() * sizeof(float4)));
+        }
+        if (!Q.tabD.empty()) {
+            g.geoD = (const double4 *)dptr(P.oGeo);
+            HIPCHK(up(P.oGeo, Q.tabD.data(), Q.tabD.size() * sizeof(double4)));
+        }
+        if (!Q.fn.empty()) {
+            g.fnext = (const float *)dptr(P.oFn);
+            HIPCHK(up(P.oFn, Q.fn.data(), Q.fn.size() * sizeof(float)));
+        }
+        if (!Q.fnD.empty()) {
+            g.fnextD = (const double *)dptr(P.oFn);
+            HIPCHK(up(P.oFn, Q.fnD.data(), Q.fnD.size() * sizeof(double)));
+        }
+    }
+    // frame data
+    g.imgType = P.imgType;
+    if (in.data_on_device) {
+        g.img = in.data;
+    } else {
+        void *stage = HS + P.hImg;
+        const size_t npix = (size_t)fh * fw;
+        if (in.data_type == SS_F64 && P.imgType == ss::IMG_F32) {
+            const double *src = (const double *)in.data;
+            float *dst = (float *)stage;
+            for (size_t i = 0; i < npix; i++) dst[i] = (float)src[i];
+        } else {
+            std::memcpy(stage, in.data, P.imgBytes);
+        }
+        HIPCHK(hipMemcpyAsync(dptr(P.oImg), stage, P.imgBytes, hipMemcpyHostToDevice, h->stream));
+        g.img = dptr(P.oImg);
+    }
+    // chunks of rows start 16-byte aligned when the columns do
+    g.vec = ((fh % 8) == 0 && ((uintptr_t)g.img % 16) == 0) ? 1 : 0;
+    // min points (first n_blocks entries are read: minPointsPerBlock[c % frameW])
+    HIPCHK(up(P.oMin, in.min_points, nb * sizeof(int)));
+    g.minPts = (const int *)dptr(P.oMin);
+    g.p = (double *)dptr(P.oP);
+    g.pLast = (double *)dptr(P.oPL);
+    g.p0 = (double *)dptr(P.oP0);
+    g.d0 = (double *)dptr(P.oD0);
+    g.deltaP = (double *)dptr(P.oDP);
+    g.errval = (double *)dptr(P.oErr);
+    g.colsum = (double *)dptr(P.oCol);
+    g.lssum = (double *)dptr(P.oLs);
+    g.corrsum = (double *)dptr(P.oCorr);
+    g.finalsum = (double *)dptr(P.oFin);
+    g.ws = (double *)dptr(P.oWs);
+    g.used = (uint8_t *)dptr(P.oUsed);
+    g.lsActive = (int *)dptr(P.oAct);
+    g.lsNextK = (int *)dptr(P.oNext);
+    g.lsMk = (int *)dptr(P.oMk);
+    g.lsDk = (int *)dptr(P.oDk);
+    g.maskOut = P.out->mask ? (uint8_t *)dptr(P.oMask) : nullptr;
+    g.resid = (double *)dptr(P.oRes);
+    g.fcorr = (double *)dptr(P.oFc);
+    g.allParams = P.out->all_params ? (double *)dptr(P.oAll) : nullptr;
+    g.iterCorr = P.out->iteration_corr ? (double *)dptr(P.oIc) : nullptr;
+    // initial knots (task_allocate.cu:130-134), deltaP = 0, errval = 1e6 (task_allocate.cu:150-184)
+    double *hp = (double *)(HS + P.hP);
+    std::memset(hp, 0, sizeof(double) * Pn);
+    knots_in(in.init_x, in.init_y, nb + 1, nF, nb, hp);
+    for (int i = 0; i < 2 * nF; i++) hp[Pn + i] = 1e6;
+    HIPCHK(hipMemcpyAsync(g.p, hp, Pn * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(g.errval, hp + Pn, 2 * nF * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemsetAsync(g.deltaP, 0, Pn * sizeof(double), h->stream));
+    // control state (task.cpp:54-70 filter selection)
+    std::memset(&c, 0, sizeof(c));
+    c.phase = ss::PH_ASSEMBLE;
+    c.shift = 0;
+    c.iteration = 1;
+    c.improved = 1;
+    c.filterID = s.filter_id;
+    c.nFilterIterations = s.max_filter_iterations;
+    switch (s.max_filter_iterations != 0 ? s.filter_id : 0) {
+    case 0: c.solverFilter = 0; c.nFilterIterations = 0; break;
+    case 1: c.solverFilter = 1; break;
+    default: c.solverFilter = 2; break;
+    }
+    c.retv = SS_STOP_ERROR;
+    return SS_OK;
+}
+
+// Copies a finished group's outputs to the caller's buffers.
+ss_status finish_group(ss_handle *h, const Plan &P, const ss::GroupDev &g, const ss::GroupCtl &c,
+                       double initTime) {
+    const ss_group_in &in = *P.in;
+    ss_group_out &o = *P.out;
+    const int nb = in.n_blocks, nF = in.n_frames, fw = nb * nF;
+    const size_t Pn = 2 * (size_t)(fw + 1);
+    if (c.phase == ss::PH_ERROR)
+        return fail(h, (ss_status)c.error, "line search did not reach the movement threshold (non-finite step?)");
+    std::vector<double> pl(Pn);
+    HIPCHK(hipMemcpyAsync(pl.data(), g.pLast, Pn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (o.residrms) HIPCHK(hipMemcpyAsync(o.residrms, g.resid, nF * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (o.frame_corr) HIPCHK(hipMemcpyAsync(o.frame_corr, g.fcorr, nF * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (o.mask) HIPCHK(hipMemcpyAsync(o.mask, g.maskOut, (size_t)in.frame_h * fw, hipMemcpyDeviceToHost, h->stream));
+    const int nrec = std::min(c.iteration, P.s->max_iterations);
+    if (o.all_params && nrec > 0)
+        HIPCHK(hipMemcpyAsync(o.all_params, g.allParams, Pn * nrec * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (o.iteration_corr && nrec > 0)
+        HIPCHK(hipMemcpyAsync(o.iteration_corr, g.iterCorr, nrec * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (o.knots_x && o.knots_y) knots_out(pl.data(), nF, nb, o.knots_x, o.knots_y);
+    o.iterations = c.iteration;
+    o.stop_criterion = c.retv;
+    o.half_iterations = c.halfIterations;
+    o.linesearch_rounds = c.rounds;
+    o.linesearch_tries = c.tries;
+    o.device_init_time = initTime;
+    // algorithmic frame bytes per stage class for the roofline
+    if (h->prof.on) {
+        const double fb = (double)in.frame_h * nb * (double)img_size(P.imgType);
+        h->prof.bytes[SS_K_ASSEMBLE] += fb * nF * c.halfIterations;
+        h->prof.bytes[SS_K_LS_ROUND] += fb * c.lsFramePasses;
+        h->prof.bytes[SS_K_CORR] += fb * nF * c.iteration;
+        h->prof.bytes[SS_K_FINAL] += fb * nF;
+    }
+    return SS_OK;
+}
+
+// Device table of the groups in flight (GroupDev, GroupCtl, work-item maps).
+struct Table {
+    void *mem = nullptr;
+    size_t bytes = 0;
+};
+
+// Uploads the group table (rows in gd / gc order) and describes the launches.
+ss_status upload_table(ss_handle *h, Table &T, std::vector<ss::GroupDev> &gd, const ss::GroupCtl *gc,
+                       void *hstage, size_t hstageBytes, ss::LaunchBatch &L) {
+    const int n = (int)gd.size();
+    int nItems = 0, nFrames = 0;
+    for (const auto &g : gd) {
+        nItems += g.fframeW;
+        nFrames += g.nF;
+    }
+    Arena A;
+    const size_t oG = A.take<ss::GroupDev>(n), oC = A.take<ss::GroupCtl>(n), oI = A.take<int>(nItems),
+                 oF = A.take<int>(nFrames);
+    if (A.off > hstageBytes) return fail(h, SS_ERR_OUT_OF_MEMORY, "group table staging too small");
+    if (T.bytes < A.off) {
+        if (T.mem) {
+            HIPCHK(hipStreamSynchronize(h->stream));
+            HIPCHK(hipFree(T.mem));
+            T.mem = nullptr;
+            T.bytes = 0;
+        }
+        const size_t want = std::max(A.off, T.bytes * 2);
+        if (hipMalloc(&T.mem, want) != hipSuccess) return fail(h, SS_ERR_OUT_OF_MEMORY, "unable to allocate device memory");
+        T.bytes = want;
+    }
+    char *HS = (char *)hstage;
+    ss::GroupDev *hg = (ss::GroupDev *)(HS + oG);
+    int *hi = (int *)(HS + oI), *hf = (int *)(HS + oF);
+    int item = 0, frame = 0;
+    for (int i = 0; i < 2 * ss::IMG_NTYPES; i++) L.hasType[i] = 0;
+    for (int gi = 0; gi < n; gi++) {
+        ss::GroupDev &g = gd[gi];
+        g.itemStart = item;
+        g.frameStart = frame;
+        hg[gi] = g;
+        for (int i = 0; i < g.fframeW; i++) hi[item + i] = gi;
+        for (int i = 0; i < g.nF; i++) hf[frame + i] = gi;
+        item += g.fframeW;
+        frame += g.nF;
+        L.hasType[g.imgType + ss::IMG_NTYPES * g.fp64] = 1;
+    }
+    std::memcpy(HS + oC, gc, n * sizeof(ss::GroupCtl));
+    char *D = (char *)T.mem;
+    HIPCHK(hipMemcpyAsync(D, HS, A.off, hipMemcpyHostToDevice, h->stream));
+    L.b.nGroups = n;
+    L.b.nItems = nItems;
+    L.b.nFrames = nFrames;
+    L.b.groups = (ss::GroupDev *)(D + oG);
+    L.b.ctl = (ss::GroupCtl *)(D + oC);
+    L.b.itemGroup = (int *)(D + oI);
+    L.b.frameGroup = (int *)(D + oF);
+    L.itemGrid = std::max(1, std::min(nItems, 8 * h->numCU));
+    L.frameGrid = std::max(1, std::min(nFrames, 16 * h->numCU));
+    return SS_OK;
+}
+
+size_t table_bytes(int nGroups, int nItems, int nFrames) {
+    Arena A;
+    A.take<ss::GroupDev>(nGroups);
+    A.take<ss::GroupCtl>(nGroups);
+    A.take<int>(nItems);
+    A.take<int>(nFrames);
+    return A.off;
+}
+
+// One tick: every group advances through the stage sequence (a kernel skips
+// groups in other phases); the control blocks come back to hc.
+ss_status run_tick(ss_handle *h, const ss::LaunchBatch &L, ss::GroupCtl *hc) {
+    static const int tick[] = {ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE, ss::ST_LS, ss::ST_LS_DECIDE,
+                               ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE, ss::ST_LS, ss::ST_LS_DECIDE,
+                               ss::ST_CORR, ss::ST_CORR_FINISH, ss::ST_FINAL, ss::ST_FINAL_FINISH};
+    if (h->prof.on || !h->useGraph) {  // per-stage events need plain launches
+        for (int stg : tick) HIPCHK(run_stage(h, L, stg));
+    } else {
+        // the tick holds kernel launches only (no memset nodes)
+        if (!h->tickExec || std::memcmp(&h->tickFor, &L, sizeof(L)) != 0) {
+            if (h->tickExec) {
+                HIPCHK(hipGraphExecDestroy(h->tickExec));
+                h->tickExec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            for (int stg : tick) {
+                const hipError_t e = ss::launch_stage(L, stg, h->stream);
+                if (e != hipSuccess) {
+                    (void)hipStreamEndCapture(h->stream, &graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return fail(h, SS_ERR_DEVICE, std::string("tick capture: ") + hipGetErrorString(e));
+                }
+            }
+            HIPCHK(hipStreamEndCapture(h->stream, &graph));
+            const hipError_t e = hipGraphInstantiate(&h->tickExec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                h->tickExec = nullptr;
+                return fail(h, SS_ERR_DEVICE, std::string("tick instantiate: ") + hipGetErrorString(e));
+            }
+            h->tickFor = L;
+        }
+        HIPCHK(hipGraphLaunch(h->tickExec, h->stream));
+    }
+    HIPCHK(hipMemcpyAsync(hc, L.b.ctl, L.b.nGroups * sizeof(ss::GroupCtl), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return SS_OK;
+}
+
+bool finished(const ss::GroupCtl &c) { return c.phase == ss::PH_DONE || c.phase == ss::PH_ERROR; }
 
 ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settings *ss_,
                     ss_group_out *outs) {
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<Plan> plans(n);
-    Arena A, H;
-    const size_t oGroups = A.take<ss::GroupDev>(n);
-    const size_t oCtl = A.take<ss::GroupCtl>(n);
-    int nItems = 0;
+    int nItems = 0, nFrames = 0;
+    size_t dTotal = 0, hTotal = 0;
     for (int gi = 0; gi < n; gi++) {
         Plan &P = plans[gi];
         P.in = ins + gi;
         P.s = ss_ + gi;
         P.out = outs + gi;
-        ss_status st = validate(h, P.in, P.s);
+        ss_status st = plan_group(h, P);
         if (st) return st;
-        const ss_group_in &in = *P.in;
-        const int fh = in.frame_h, nb = in.n_blocks, nF = in.n_frames, fw = nb * nF, N = fw + 1;
-        P.fp64 = P.s->precision == 'D' ? 1 : 0;
-        const bool wF = !P.fp64, wD = P.fp64;
-        if (in.template_id > 0) {
-            auto it = h->templates.find(in.template_id);
-            if (it == h->templates.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown template_id");
-            P.tmpl = it->second;
-        } else {
-            P.tmpl.th = in.template_h;
-            P.tmpl.tw = in.template_w;
-            convert_template(in.template_h, in.template_w, in.template_image, in.xgrad, in.ygrad, wF, wD,
-                             P.th, P.tmpl.gradFD);
-            const size_t tn = (size_t)in.template_h * in.template_w;
-            const size_t pb = P.fp64 ? sizeof(double2) : sizeof(float2);
-            P.oTp = A.take<char>(tn * pb);
-            if (!P.tmpl.gradFD) {
-                P.oGxv = A.take<char>(tn * pb);
-                P.oGyh = A.take<char>(tn * pb);
-            }
-        }
-        if (in.geometry_id > 0) {
-            auto it = h->geometries.find(in.geometry_id);
-            if (it == h->geometries.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown geometry_id");
-            P.geom = it->second;
-            if (P.geom.frameH != fh || P.geom.nb != nb)
-                return fail(h, SS_ERR_INVALID_ARGUMENT, "resident geometry does not match frame_h x n_blocks");
-        } else {
-            convert_geometry(fh, nb, in.frac, in.base_x, in.base_y, in.base_mask, wF, wD, P.geom, P.gh);
-            if (!P.gh.tab.empty()) P.oGeo = A.take<float4>(P.gh.tab.size());
-            if (!P.gh.tabD.empty()) P.oGeo = A.take<double4>(P.gh.tabD.size());
-            if (!P.gh.fn.empty()) P.oFn = A.take<float>(P.gh.fn.size());
-            if (!P.gh.fnD.empty()) P.oFn = A.take<double>(P.gh.fnD.size());
-        }
-        // f64 host frames are converted to f32 for precision 'S' (as the mex converts them)
-        P.imgType = in.data_type == SS_U16   ? ss::IMG_U16
-                    : in.data_type == SS_I16 ? ss::IMG_I16
-                    : in.data_type == SS_F32 ? ss::IMG_F32
-                    : (in.data_on_device || P.fp64) ? ss::IMG_F64
-                                                    : ss::IMG_F32;
-        P.imgBytes = (size_t)fh * fw * img_size(P.imgType);
-        if (!in.data_on_device) {
-            P.oImg = A.take<char>(P.imgBytes);
-            P.hImg = H.take<char>(P.imgBytes);
-        }
-        P.oMin = A.take<int>(nb);
-        const size_t Pn = 2 * (size_t)N;
-        P.oP = A.take<double>(Pn);
-        P.oPL = A.take<double>(Pn);
-        P.oP0 = A.take<double>(Pn);
-        P.oD0 = A.take<double>(Pn);
-        P.oDP = A.take<double>(Pn);
-        P.oErr = A.take<double>(2 * (size_t)nF);
-        P.oCol = A.take<double>((size_t)fw * ss::CS_N);
-        P.oLs = A.take<double>((size_t)fw * 2 * ss::kMaxCand);
-        P.oCorr = A.take<double>((size_t)fw * ss::CR_N);
-        P.oFin = A.take<double>((size_t)fw * ss::FS_N);
-        P.oWs = A.take<char>(ss::solve_workspace_bytes(fw));
-        P.oUsed = A.take<uint8_t>(N);
-        P.oAct = A.take<int>(nF);
-        P.oNext = A.take<int>(nF);
-        P.oMk = A.take<int>(nF);
-        P.oDk = A.take<int>(nF);
-        P.oArr = A.take<int>(nF);
-        if (P.out->mask) P.oMask = A.take<uint8_t>((size_t)fh * fw);
-        P.oRes = A.take<double>(nF);
-        P.oFc = A.take<double>(nF);
-        if (P.out->all_params) P.oAll = A.take<double>(Pn * P.s->max_iterations);
-        if (P.out->iteration_corr) P.oIc = A.take<double>(P.s->max_iterations);
-        P.hP = H.take<double>(Pn + 2 * (size_t)nF);
-        nItems += fw;
+        dTotal += (P.dBytes + 255) & ~(size_t)255;
+        hTotal += (P.hBytes + 255) & ~(size_t)255;
+        nItems += P.in->n_blocks * P.in->n_frames;
+        nFrames += P.in->n_frames;
     }
-    const size_t oItems = A.take<int>(nItems);
-    const size_t hGroups = H.take<ss::GroupDev>(n);
-    const size_t hCtl = H.take<ss::GroupCtl>(n);
-    const size_t hItems = H.take<int>(nItems);
-    ss_status st = ensure_arena(h, A.off, H.off);
+    const size_t tb = (table_bytes(n, nItems, nFrames) + 255) & ~(size_t)255;
+    const size_t cb = (n * sizeof(ss::GroupCtl) + 255) & ~(size_t)255;
+    ss_status st = ensure_arena(h, dTotal + tb, hTotal + tb + cb);
     if (st) return st;
-    char *D = (char *)h->arena;
+    // [table][group 0][group 1]... in the arena; the same order in staging
+    char *D = (char *)h->arena + tb;
     char *HS = (char *)h->harena;
-    auto dptr = [&](size_t o) { return (void *)(D + o); };
-    // fill host staging and enqueue uploads
+    ss::GroupCtl *hc = (ss::GroupCtl *)(HS + tb);
+    char *HG = HS + tb + cb;
     std::vector<ss::GroupDev> gd(n);
-    ss::GroupDev *hg = (ss::GroupDev *)(HS + hGroups);
-    ss::GroupCtl *hc = (ss::GroupCtl *)(HS + hCtl);
-    int *hitems = (int *)(HS + hItems);
-    int itemStart = 0;
-    int hasType[2 * ss::IMG_NTYPES] = {};
     for (int gi = 0; gi < n; gi++) {
         Plan &P = plans[gi];
-        const ss_group_in &in = *P.in;
-        const ss_settings &s = *P.s;
-        const int fh = in.frame_h, nb = in.n_blocks, nF = in.n_frames, fw = nb * nF, N = fw + 1;
-        const size_t Pn = 2 * (size_t)N;
-        ss::GroupDev g;
-        std::memset(&g, 0, sizeof(g));
-        g.frameH = fh;
-        g.nb = nb;
-        g.nF = nF;
-        g.fframeW = fw;
-        g.N = N;
-        g.th = P.tmpl.th;
-        g.tw = P.tmpl.tw;
-        g.groupSize = in.group_size;
-        g.itemStart = itemStart;
-        g.seg = (fh + 255) / 256;
-        const double eps = s.precision == 'S' ? (double)FLT_EPSILON : DBL_EPSILON;
-        g.fp64 = P.fp64;
-        g.xmax = (double)(g.tw - 1) - eps;
-        g.ymax = (double)(g.th - 1) - eps;
-        g.rmult = s.linesearch_reduction_multiplier;
-        g.thr = s.move_threshold;
-        g.halt = s.halt_correlation;
-        g.cit = s.correlation_increase_threshold;
-        g.fct = s.filter_correlation_threshold;
-        g.strength = s.filter_strength;
-        g.lambda = s.lambda;
-        g.maxIter = s.max_iterations;
-        g.K1 = s.linesearch_candidates <= 0 ? 2 : s.linesearch_candidates;
-        g.Kn = std::max(g.K1, 4);
-        g.maxRounds = 4096 / g.Kn + 1;
-        // template
-        if (in.template_id > 0) {
-            g.tp = P.tmpl.tp;
-            g.gxv = P.tmpl.gxv;
-            g.gyh = P.tmpl.gyh;
-            g.tpD = P.tmpl.tpD;
-            g.gxvD = P.tmpl.gxvD;
-            g.gyhD = P.tmpl.gyhD;
-        } else {
-            auto up = [&](size_t o, const void *src, size_t bytes) {
-                return hipMemcpyAsync(dptr(o), src, bytes, hipMemcpyHostToDevice, h->stream);
-            };
-            const TemplateHost &T = P.th;
-            if (P.fp64) {
-                g.tpD = (const double2 *)dptr(P.oTp);
-                HIPCHK(up(P.oTp, T.tpD.data(), T.tpD.size() * sizeof(double2)));
-                if (!P.tmpl.gradFD) {
-                    g.gxvD = (const double2 *)dptr(P.oGxv);
-                    g.gyhD = (const double2 *)dptr(P.oGyh);
-                    HIPCHK(up(P.oGxv, T.gxvD.data(), T.gxvD.size() * sizeof(double2)));
-                    HIPCHK(up(P.oGyh, T.gyhD.data(), T.gyhD.size() * sizeof(double2)));
-                }
-            } else {
-                g.tp = (const float2 *)dptr(P.oTp);
-                HIPCHK(up(P.oTp, T.tp.data(), T.tp.size() * sizeof(float2)));
-                if (!P.tmpl.gradFD) {
-                    g.gxv = (const float2 *)dptr(P.oGxv);
-                    g.gyh = (const float2 *)dptr(P.oGyh);
-                    HIPCHK(up(P.oGxv, T.gxv.data(), T.gxv.size() * sizeof(float2)));
-                    HIPCHK(up(P.oGyh, T.gyh.data(), T.gyh.size() * sizeof(float2)));
-                }
-            }
-        }
-        g.gradFD = P.tmpl.gradFD;
-        // geometry
-        const DevGeometry &G = P.geom;
-        g.affine = G.affine;
-        g.L = G.L;
-        g.afF = G.afF;
-        g.afD = G.afD;
-        g.fracUniform = G.fracUniform;
-        if (in.geometry_id > 0) {
-            g.geo = G.geo;
-            g.geoD = G.geoD;
-            g.fnext = G.fnext;
-            g.fnextD = G.fnextD;
-        } else {
-            const GeometryHost &Q = P.gh;
-            if (!Q.tab.empty()) {
-                g.geo = (const float4 *)dptr(P.oGeo);
-                HIPCHK(hipMemcpyAsync(dptr(P.oGeo), Q.tab.data(), Q.tab.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream));
-            }
-            if (!Q.tabD.empty()) {
-                g.geoD = (const double4 *)dptr(P.oGeo);
-                HIPCHK(hipMemcpyAsync(dptr(P.oGeo), Q.tabD.data(), Q.tabD.size() * sizeof(double4), hipMemcpyHostToDevice, h->stream));
-            }
-            if (!Q.fn.empty()) {
-                g.fnext = (const float *)dptr(P.oFn);
-                HIPCHK(hipMemcpyAsync(dptr(P.oFn), Q.fn.data(), Q.fn.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
-            }
-            if (!Q.fnD.empty()) {
-                g.fnextD = (const double *)dptr(P.oFn);
-                HIPCHK(hipMemcpyAsync(dptr(P.oFn), Q.fnD.data(), Q.fnD.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
-            }
-        }
-        // frame data
-        g.imgType = P.imgType;
-        hasType[P.imgType + ss::IMG_NTYPES * P.fp64] = 1;
-        if (in.data_on_device) {
-            g.img = in.data;
-        } else {
-            void *stage = HS + P.hImg;
-            const size_t npix = (size_t)fh * fw;
-            if (in.data_type == SS_F64 && P.imgType == ss::IMG_F32) {
-                const double *src = (const double *)in.data;
-                float *dst = (float *)stage;
-                for (size_t i = 0; i < npix; i++) dst[i] = (float)src[i];
-            } else {
-                std::memcpy(stage, in.data, P.imgBytes);
-            }
-            HIPCHK(hipMemcpyAsync(dptr(P.oImg), stage, P.imgBytes, hipMemcpyHostToDevice, h->stream));
-            g.img = dptr(P.oImg);
-        }
-        const size_t esz = img_size(P.imgType);
-        g.vec = ((g.seg % 8) == 0 && (fh % 8) == 0 && ((uintptr_t)g.img % 16) == 0 && (esz == 2 || (fh % 4) == 0)) ? 1 : 0;
-        // min points (first n_blocks entries are read: minPointsPerBlock[c % frameW])
-        HIPCHK(hipMemcpyAsync(dptr(P.oMin), in.min_points, nb * sizeof(int), hipMemcpyHostToDevice, h->stream));
-        g.minPts = (const int *)dptr(P.oMin);
-        g.p = (double *)dptr(P.oP);
-        g.pLast = (double *)dptr(P.oPL);
-        g.p0 = (double *)dptr(P.oP0);
-        g.d0 = (double *)dptr(P.oD0);
-        g.deltaP = (double *)dptr(P.oDP);
-        g.errval = (double *)dptr(P.oErr);
-        g.colsum = (double *)dptr(P.oCol);
-        g.lssum = (double *)dptr(P.oLs);
-        g.corrsum = (double *)dptr(P.oCorr);
-        g.finalsum = (double *)dptr(P.oFin);
-        g.ws = (double *)dptr(P.oWs);
-        g.used = (uint8_t *)dptr(P.oUsed);
-        g.lsActive = (int *)dptr(P.oAct);
-        g.lsNextK = (int *)dptr(P.oNext);
-        g.lsMk = (int *)dptr(P.oMk);
-        g.lsDk = (int *)dptr(P.oDk);
-        g.frameArrive = (int *)dptr(P.oArr);
-        g.maskOut = P.out->mask ? (uint8_t *)dptr(P.oMask) : nullptr;
-        g.resid = (double *)dptr(P.oRes);
-        g.fcorr = (double *)dptr(P.oFc);
-        g.allParams = P.out->all_params ? (double *)dptr(P.oAll) : nullptr;
-        g.iterCorr = P.out->iteration_corr ? (double *)dptr(P.oIc) : nullptr;
-        // initial knots (task_allocate.cu:130-134), errval = 1e6 (task_allocate.cu:327)
-        double *hp = (double *)(HS + P.hP);
-        std::memset(hp, 0, sizeof(double) * Pn);
-        knots_in(in.init_x, in.init_y, nb + 1, nF, nb, hp);
-        for (int i = 0; i < 2 * nF; i++) hp[Pn + i] = 1e6;
-        HIPCHK(hipMemcpyAsync(g.p, hp, Pn * sizeof(double), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(g.errval, hp + Pn, 2 * nF * sizeof(double), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(hipMemsetAsync(g.deltaP, 0, Pn * sizeof(double), h->stream));
-        HIPCHK(hipMemsetAsync(g.frameArrive, 0, nF * sizeof(int), h->stream));
-        hg[gi] = g;
-        // control state (task.cpp:54-70 filter selection)
-        ss::GroupCtl c;
-        std::memset(&c, 0, sizeof(c));
-        c.phase = ss::PH_ASSEMBLE;
-        c.shift = 0;
-        c.iteration = 1;
-        c.improved = 1;
-        c.filterID = s.filter_id;
-        c.nFilterIterations = s.max_filter_iterations;
-        switch (s.max_filter_iterations != 0 ? s.filter_id : 0) {
-        case 0: c.solverFilter = 0; c.nFilterIterations = 0; break;
-        case 1: c.solverFilter = 1; break;
-        default: c.solverFilter = 2; break;
-        }
-        c.retv = SS_STOP_ERROR;
-        hc[gi] = c;
-        for (int i = 0; i < fw; i++) hitems[itemStart + i] = gi;
-        itemStart += fw;
+        st = setup_group(h, P, D, HG, gd[gi], hc[gi]);
+        if (st) return st;
+        D += (P.dBytes + 255) & ~(size_t)255;
+        HG += (P.hBytes + 255) & ~(size_t)255;
     }
-    HIPCHK(hipMemcpyAsync(dptr(oGroups), hg, n * sizeof(ss::GroupDev), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(dptr(oCtl), hc, n * sizeof(ss::GroupCtl), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(dptr(oItems), hitems, nItems * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    Table T;
+    T.mem = h->arena;
+    T.bytes = tb;
+    ss::LaunchBatch L{};
+    st = upload_table(h, T, gd, hc, HS, tb, L);
+    if (st) return st;
     HIPCHK(hipStreamSynchronize(h->stream));
     const double initTime = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-
-    ss::LaunchBatch L;
-    L.b.nGroups = n;
-    L.b.nItems = nItems;
-    L.b.groups = (ss::GroupDev *)dptr(oGroups);
-    L.b.ctl = (ss::GroupCtl *)dptr(oCtl);
-    L.b.itemGroup = (int *)dptr(oItems);
-    for (int i = 0; i < 2 * ss::IMG_NTYPES; i++) L.hasType[i] = hasType[i];
-    L.itemGrid = std::max(1, std::min(nItems, 8 * h->numCU));
-    static const int tick[] = {ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS,
-                               ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS,
-                               ss::ST_CORR, ss::ST_CORR_FINISH, ss::ST_FINAL, ss::ST_FINAL_FINISH};
     bool done = false;
     for (int t = 0; t < kTickCap && !done; t++) {
-        for (int stg : tick) HIPCHK(run_stage(h, L, stg));
-        HIPCHK(hipMemcpyAsync(hc, L.b.ctl, n * sizeof(ss::GroupCtl), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
+        st = run_tick(h, L, hc);
+        if (st) return st;
         done = true;
         for (int gi = 0; gi < n; gi++)
-            if (hc[gi].phase != ss::PH_DONE && hc[gi].phase != ss::PH_ERROR) done = false;
+            if (!finished(hc[gi])) done = false;
     }
     if (!done) return fail(h, SS_ERR_NUMERICAL, "alignment did not terminate");
-    // outputs
     ss_status result = SS_OK;
+    std::string firstErr;
     for (int gi = 0; gi < n; gi++) {
-        Plan &P = plans[gi];
-        const ss_group_in &in = *P.in;
-        ss_group_out &o = *P.out;
-        const ss::GroupCtl &c = hc[gi];
-        const ss::GroupDev &g = hg[gi];
-        const int nb = in.n_blocks, nF = in.n_frames, fw = nb * nF;
-        const size_t Pn = 2 * (size_t)(fw + 1);
-        if (c.phase == ss::PH_ERROR) {
-            if (result == SS_OK) {
-                result = (ss_status)c.error;
-                h->err = "line search did not reach the movement threshold (non-finite step?) in group " + std::to_string(gi);
-            }
-            continue;
-        }
-        std::vector<double> pl(Pn);
-        HIPCHK(hipMemcpyAsync(pl.data(), g.pLast, Pn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        if (o.residrms) HIPCHK(hipMemcpyAsync(o.residrms, g.resid, nF * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        if (o.frame_corr) HIPCHK(hipMemcpyAsync(o.frame_corr, g.fcorr, nF * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        if (o.mask) HIPCHK(hipMemcpyAsync(o.mask, g.maskOut, (size_t)in.frame_h * fw, hipMemcpyDeviceToHost, h->stream));
-        const int nrec = std::min(c.iteration, P.s->max_iterations);
-        if (o.all_params && nrec > 0)
-            HIPCHK(hipMemcpyAsync(o.all_params, g.allParams, Pn * nrec * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        if (o.iteration_corr && nrec > 0)
-            HIPCHK(hipMemcpyAsync(o.iteration_corr, g.iterCorr, nrec * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        if (o.knots_x && o.knots_y) knots_out(pl.data(), nF, nb, o.knots_x, o.knots_y);
-        o.iterations = c.iteration;
-        o.stop_criterion = c.retv;
-        o.half_iterations = c.halfIterations;
-        o.linesearch_rounds = c.rounds;
-        o.linesearch_tries = c.tries;
-        o.device_init_time = initTime;
-        // algorithmic frame bytes per stage class for the roofline
-        if (h->prof.on) {
-            const double fb = (double)in.frame_h * nb * (double)img_size(P.imgType);
-            h->prof.bytes[SS_K_ASSEMBLE] += fb * nF * c.halfIterations;
-            h->prof.bytes[SS_K_LS_ROUND] += fb * c.lsFramePasses;
-            h->prof.bytes[SS_K_CORR] += fb * nF * c.iteration;
-            h->prof.bytes[SS_K_FINAL] += fb * nF;
+        st = finish_group(h, plans[gi], gd[gi], hc[gi], initTime);
+        if (st && result == SS_OK) {
+            result = st;
+            firstErr = h->err + " (group " + std::to_string(gi) + ")";
         }
     }
+    if (result) h->err = firstErr;
     if (h->prof.on) prof_collect(h);
     return result;
 }
+
+// ---------------------------------------------------------------------------
+// serving: continuous admission of groups into the running batch
+// ---------------------------------------------------------------------------
+// First-fit sub-allocator over device chunks, bounded by a byte budget.
+struct DevicePool {
+    struct Chunk {
+        char *base = nullptr;
+        size_t size = 0;
+        std::map<size_t, size_t> free;  // offset -> length
+    };
+    std::vector<Chunk> chunks;
+    size_t budget = 0, reserved = 0, inUse = 0;
+    static constexpr size_t kAlign = 256, kChunk = (size_t)1 << 30;
+
+    char *alloc(size_t n) {
+        n = (n + kAlign - 1) & ~(kAlign - 1);
+        for (auto &c : chunks)
+            for (auto it = c.free.begin(); it != c.free.end(); ++it)
+                if (it->second >= n) {
+                    const size_t off = it->first, len = it->second;
+                    c.free.erase(it);
+                    if (len > n) c.free[off + n] = len - n;
+                    inUse += n;
+                    return c.base + off;
+                }
+        const size_t want = std::max(std::min(kChunk, budget - reserved), n);
+        if (reserved + want > budget) return nullptr;
+        Chunk c;
+        if (hipMalloc(&c.base, want) != hipSuccess) return nullptr;
+        c.size = want;
+        if (want > n) c.free[n] = want - n;
+        reserved += want;
+        inUse += n;
+        chunks.push_back(std::move(c));
+        return chunks.back().base;
+    }
+    void release(char *p, size_t n) {
+        n = (n + kAlign - 1) & ~(kAlign - 1);
+        for (auto &c : chunks) {
+            if (p < c.base || p >= c.base + c.size) continue;
+            size_t off = (size_t)(p - c.base);
+            inUse -= n;
+            auto next = c.free.lower_bound(off);
+            if (next != c.free.end() && next->first == off + n) {  // merge with the following block
+                n += next->second;
+                next = c.free.erase(next);
+            }
+            if (next != c.free.begin()) {  // and with the preceding one
+                auto prev = std::prev(next);
+                if (prev->first + prev->second == off) {
+                    prev->second += n;
+                    return;
+                }
+            }
+            c.free[off] = n;
+            return;
+        }
+    }
+    void destroy() {
+        for (auto &c : chunks) (void)hipFree(c.base);
+        chunks.clear();
+        reserved = inUse = 0;
+    }
+};
+
+}  // namespace
+
+struct ss_server {
+    ss_handle *h = nullptr;
+    int maxGroups = 256;
+    DevicePool pool;
+    struct Request {
+        int64_t ticket;
+        ss_group_in in;
+        ss_settings s;
+        ss_group_out *out;
+    };
+    struct Slot {
+        Request req;
+        Plan P;
+        char *mem = nullptr;
+        ss::GroupDev g;
+        ss::GroupCtl c;
+        double admitTime = 0;
+    };
+    std::deque<Request> queue;
+    std::vector<Slot> resident;
+    std::vector<std::pair<int64_t, ss_status>> completed;
+    Table table;
+    void *stage = nullptr;  // pinned: group staging + table
+    size_t stageBytes = 0;
+    int64_t nextTicket = 1;
+    bool dirty = true;      // membership changed since the table was uploaded
+    ss::LaunchBatch L{};
+    std::vector<ss::GroupCtl> hc;
+};
+
+namespace {
+
+ss_status server_stage(ss_server *S, size_t bytes) {
+    ss_handle *h = S->h;
+    if (S->stageBytes >= bytes) return SS_OK;
+    if (S->stage) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipHostFree(S->stage));
+        S->stage = nullptr;
+        S->stageBytes = 0;
+    }
+    const size_t want = std::max(bytes, S->stageBytes * 2);
+    if (hipHostMalloc(&S->stage, want, hipHostMallocDefault) != hipSuccess)
+        return fail(h, SS_ERR_OUT_OF_MEMORY, "unable to allocate pinned host memory");
+    S->stageBytes = want;
+    return SS_OK;
+}
+
+// Admits queued requests in order while slots and the memory budget allow.
+ss_status server_admit(ss_server *S) {
+    ss_handle *h = S->h;
+    std::vector<size_t> newSlots;
+    size_t hNeed = 0;
+    while (!S->queue.empty() && (int)S->resident.size() < S->maxGroups) {
+        ss_server::Slot slot;
+        slot.req = S->queue.front();
+        slot.P.in = &slot.req.in;
+        slot.P.s = &slot.req.s;
+        slot.P.out = slot.req.out;
+        ss_status st = plan_group(h, slot.P);
+        if (st) {  // a bad request completes with its error; the others go on
+            S->completed.push_back({slot.req.ticket, st});
+            S->queue.pop_front();
+            continue;
+        }
+        slot.mem = S->pool.alloc(slot.P.dBytes);
+        if (!slot.mem) {
+            if (S->resident.empty() && newSlots.empty()) {  // can never fit
+                S->completed.push_back({slot.req.ticket, SS_ERR_OUT_OF_MEMORY});
+                S->queue.pop_front();
+                continue;
+            }
+            break;  // wait for memory to free up
+        }
+        S->queue.pop_front();
+        hNeed += (slot.P.hBytes + 255) & ~(size_t)255;
+        S->resident.push_back(std::move(slot));
+        newSlots.push_back(S->resident.size() - 1);
+    }
+    if (newSlots.empty()) return SS_OK;
+    // the table is staged after the groups
+    int nItems = 0, nFrames = 0;
+    for (auto &r : S->resident) {
+        nItems += r.req.in.n_blocks * r.req.in.n_frames;
+        nFrames += r.req.in.n_frames;
+    }
+    const size_t tb = table_bytes((int)S->resident.size(), nItems, nFrames);
+    ss_status st = server_stage(S, hNeed + tb + 256);
+    if (st) return st;
+    char *HS = (char *)S->stage;
+    for (size_t k : newSlots) {
+        auto &slot = S->resident[k];
+        // the Plan points at the slot's own request copy
+        slot.P.in = &slot.req.in;
+        slot.P.s = &slot.req.s;
+        st = setup_group(h, slot.P, slot.mem, HS, slot.g, slot.c);
+        if (st) return st;
+        HS += (slot.P.hBytes + 255) & ~(size_t)255;
+    }
+    // staged uploads must land before the staging is reused for the table
+    HIPCHK(hipStreamSynchronize(h->stream));
+    S->dirty = true;
+    return SS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ss_status ss_server_create(ss_handle *h, const ss_server_config *cfg, ss_server **out) {
+    if (!h || !out) return SS_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    ss_server *S = new ss_server();
+    S->h = h;
+    const int64_t budget = (cfg && cfg->hbm_budget_bytes > 0) ? cfg->hbm_budget_bytes : ((int64_t)64 << 30);
+    S->pool.budget = (size_t)budget;
+    S->maxGroups = (cfg && cfg->max_groups > 0) ? cfg->max_groups : 256;
+    *out = S;
+    return SS_OK;
+}
+
+void ss_server_destroy(ss_server *S) {
+    if (!S) return;
+    ss_handle *h = S->h;
+    hipError_t e = hipSetDevice(h->device);
+    e = hipStreamSynchronize(h->stream);
+    S->pool.destroy();
+    if (S->table.mem) e = hipFree(S->table.mem);
+    if (S->stage) e = hipHostFree(S->stage);
+    (void)e;
+    delete S;
+}
+
+ss_status ss_server_submit(ss_server *S, const ss_group_in *in, const ss_settings *s, ss_group_out *out,
+                           int64_t *ticket) {
+    if (!S || !in || !s || !out || !ticket) return SS_ERR_INVALID_ARGUMENT;
+    ss_handle *h = S->h;
+    if (s->device_id != h->device)
+        return fail(h, SS_ERR_INVALID_ARGUMENT, "settings.device_id differs from the handle's device");
+    ss_server::Request r;
+    r.ticket = S->nextTicket++;
+    r.in = *in;
+    r.s = *s;
+    r.out = out;
+    S->queue.push_back(r);
+    *ticket = r.ticket;
+    return SS_OK;
+}
+
+ss_status ss_server_step(ss_server *S, int32_t *n_resident, int32_t *n_queued) {
+    if (!S) return SS_ERR_INVALID_ARGUMENT;
+    ss_handle *h = S->h;
+    HIPCHK(hipSetDevice(h->device));
+    h->err.clear();
+    ss_status st = server_admit(S);
+    if (st) return st;
+    if (!S->resident.empty()) {
+        const int n = (int)S->resident.size();
+        S->hc.resize(n);
+        if (S->dirty) {
+            std::vector<ss::GroupDev> gd(n);
+            for (int i = 0; i < n; i++) {
+                gd[i] = S->resident[i].g;
+                S->hc[i] = S->resident[i].c;
+            }
+            int nItems = 0, nFrames = 0;
+            for (auto &g : gd) {
+                nItems += g.fframeW;
+                nFrames += g.nF;
+            }
+            const size_t tb = table_bytes(n, nItems, nFrames);
+            st = server_stage(S, tb);
+            if (st) return st;
+            st = upload_table(h, S->table, gd, S->hc.data(), S->stage, S->stageBytes, S->L);
+            if (st) return st;
+            for (int i = 0; i < n; i++) S->resident[i].g = gd[i];
+            S->dirty = false;
+        }
+        st = run_tick(h, S->L, S->hc.data());
+        if (st) return st;
+        // retire finished groups
+        std::vector<ss_server::Slot> keep;
+        keep.reserve(n);
+        for (int i = 0; i < n; i++) {
+            auto &slot = S->resident[i];
+            slot.c = S->hc[i];
+            if (finished(slot.c)) {
+                slot.P.in = &slot.req.in;
+                slot.P.s = &slot.req.s;
+                const ss_status fs = finish_group(h, slot.P, slot.g, slot.c, 0.0);
+                S->pool.release(slot.mem, slot.P.dBytes);
+                S->completed.push_back({slot.req.ticket, fs});
+                S->dirty = true;
+            } else {
+                keep.push_back(std::move(slot));
+            }
+        }
+        S->resident.swap(keep);
+        for (auto &slot : S->resident) {  // plans point into the moved requests
+            slot.P.in = &slot.req.in;
+            slot.P.s = &slot.req.s;
+        }
+    }
+    if (n_resident) *n_resident = (int32_t)S->resident.size();
+    if (n_queued) *n_queued = (int32_t)S->queue.size();
+    return SS_OK;
+}
+
+int32_t ss_server_completed(ss_server *S, int64_t *tickets, ss_status *status, int32_t cap) {
+    if (!S || cap <= 0) return 0;
+    const int32_t n = std::min<int32_t>(cap, (int32_t)S->completed.size());
+    for (int32_t i = 0; i < n; i++) {
+        if (tickets) tickets[i] = S->completed[i].first;
+        if (status) status[i] = S->completed[i].second;
+    }
+    S->completed.erase(S->completed.begin(), S->completed.begin() + n);
+    return n;
+}
+
+ss_status ss_server_memory(const ss_server *S, int64_t *reserved, int64_t *in_use) {
+    if (!S) return SS_ERR_INVALID_ARGUMENT;
+    if (reserved) *reserved = (int64_t)S->pool.reserved;
+    if (in_use) *in_use = (int64_t)S->pool.inUse;
+    return SS_OK;
+}
+
+}  // extern "C"
+
+namespace {
 
 ss_status upload_owned(ss_handle *h, const void *src, size_t bytes, void *base, size_t off) {
     if (!bytes) return SS_OK;
@@ -782,6 +1201,7 @@ ss_status ss_create(int32_t device_id, ss_handle **out) {
     if (hipSetDevice(device_id) != hipSuccess) return SS_ERR_DEVICE;
     ss_handle *h = new ss_handle();
     h->device = device_id;
+    if (const char *g = std::getenv("SS_TICK_GRAPH")) h->useGraph = std::atoi(g) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device_id) == hipSuccess && prop.multiProcessorCount > 0)
         h->numCU = prop.multiProcessorCount;
@@ -806,6 +1226,7 @@ void ss_destroy(ss_handle *h) {
         e = hipEventDestroy(p.b);
     }
     for (auto ev : h->prof.pool) e = hipEventDestroy(ev);
+    if (h->tickExec) e = hipGraphExecDestroy(h->tickExec);
     if (h->arena) e = hipFree(h->arena);
     if (h->harena) e = hipHostFree(h->harena);
     if (h->stream) e = hipStreamDestroy(h->stream);

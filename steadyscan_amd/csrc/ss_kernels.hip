@@ -9,10 +9,11 @@
 // Each kernel skips the groups that are not in its phase.
 //
 // Pixel work: one workgroup (4 wave64) per line-block column (frame f,
-// block b = column c = f * nb + b of the block matrix).  Each thread owns a
-// contiguous segment of the column, read with 16-byte loads; neighbouring
-// lanes walk along the scan line, so template gathers of a wave touch a few
-// cache lines.  Per-column partial sums are reduced in registers, then
+// block b = column c = f * nb + b of the block matrix).  The column is cut
+// into chunks of 8 rows dealt round-robin to the threads, so a wave reads
+// 64 consecutive chunks (fully coalesced 16-byte loads) and, since a block
+// lists its pixels column by column, neighbouring lanes sit on neighbouring
+// scan-line pixels: a wave's template gathers touch a few cache lines.  Per-column partial sums are reduced in registers, then
 // across waves through LDS in FP64.  No floating-point atomics: every result
 // is run-to-run deterministic.
 #include "ss_device.h"
@@ -130,6 +131,42 @@ __device__ __forceinline__ void load8(const int16_t *p, bool vec, int n, R (&v)[
     }
 }
 
+// CH consecutive frame values from p (16-byte loads when the chunk is
+// aligned, i.e. vec and a full chunk); values past n read as 0.
+template <int CH, typename TI>
+struct alignas(CH * sizeof(TI) >= 16 ? 16 : CH * sizeof(TI)) Pack {
+    TI v[CH];
+};
+template <int CH, typename TI, typename R>
+__device__ __forceinline__ void load_chunk(const TI *p, bool vec, int n, R (&v)[CH]) {
+    if constexpr (CH == kCH) {
+        load8(p, vec, n, v);
+    } else {
+        if (vec && n >= CH) {
+            constexpr int per = (CH * (int)sizeof(TI) <= 16) ? CH : 16 / (int)sizeof(TI);
+#pragma unroll
+            for (int j = 0; j < CH / per; j++) {
+                const Pack<per, TI> q = *reinterpret_cast<const Pack<per, TI> *>(p + j * per);
+#pragma unroll
+                for (int i = 0; i < per; i++) v[j * per + i] = (R)q.v[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < CH; i++) v[i] = i < n ? (R)p[i] : (R)0;
+        }
+    }
+}
+
+// Rows per chunk: enough independent template gathers per wave to hide
+// their latency, few enough that the unrolled chunk stays in registers
+// (about 16 samples in FP32, 8 in FP64).
+template <typename R>
+__host__ __device__ constexpr int chunk_rows(int samplesPerRow) {
+    const int budget = sizeof(R) == 8 ? 8 : 16;
+    const int c = budget / samplesPerRow;
+    return c < 1 ? 1 : (c > kCH ? kCH : c);
+}
+
 // Sum over the workgroup of NV per-thread doubles; thread i < NV gets sum i
 // in out (LDS-backed, returns after a barrier).
 template <int NV>
@@ -222,8 +259,10 @@ __device__ __forceinline__ bool in_bounds(const GroupDev &g, R x, R y) {
     return x >= (R)0 && x < (R)g.xmax && y >= (R)0 && y < (R)g.ymax;
 }
 
-// Bilinear template sample at (x, y) inside the mask bounds
-// (task_differenceimage.cuh:13-32) from the paired layout.
+// Bilinear template sample at (x, y) (task_differenceimage.cuh:13-32) from
+// the paired layout.  Branch-free: a point outside the mask reads cell 0 (the
+// caller discards the value), so a wave can issue all gathers of a chunk
+// before waiting on any.
 template <typename R>
 struct Cell {
     R fx, fy;
@@ -231,10 +270,11 @@ struct Cell {
     int t;
 };
 template <typename R>
-__device__ __forceinline__ R sample(const GroupDev &g, R x, R y, Cell<R> &cl) {
-    const int ix = (int)x, iy = (int)y;
-    cl.fx = x - (R)ix;
-    cl.fy = y - (R)iy;
+__device__ __forceinline__ R sample(const GroupDev &g, R x, R y, bool valid, Cell<R> &cl) {
+    const R xs = valid ? x : (R)0, ys = valid ? y : (R)0;
+    const int ix = (int)xs, iy = (int)ys;
+    cl.fx = xs - (R)ix;
+    cl.fy = ys - (R)iy;
     cl.t = iy * g.tw + ix;
     const typename Prec<R>::V2 *tp = Prec<R>::tp(g);
     cl.top = tp[cl.t];
@@ -316,10 +356,6 @@ __device__ void assemble_column(const GroupDev &g, int c) {
     const TI *img = column_ptr<TI>(g, c);
     const typename Prec<R>::V4 *geo = Prec<R>::geo(g) ? Prec<R>::geo(g) + (size_t)b * g.frameH : nullptr;
     const R *fnx = g.fracUniform ? nullptr : Prec<R>::fnext(g) + (size_t)b * g.frameH;
-    const int r0 = threadIdx.x * g.seg;
-    const int n = min(g.seg, g.frameH - r0);
-    GeoIter<R> it;
-    it.start(g, b, r0);
     // h[1] = sum I, h[i - 1] = column sum i for i >= CS_AXX; the errval term
     // is accumulated in FP64 per pixel: it decides line-search acceptance
     // against near-equal candidates
@@ -328,51 +364,57 @@ __device__ void assemble_column(const GroupDev &g, int c) {
 #pragma unroll
     for (int i = 0; i < CS_N - 1; i++) h[i] = (R)0;
     int cnt = 0;
-    for (int ch = 0; ch < n; ch += kCH) {
-        R I[kCH];
-        load8(img + r0 + ch, g.vec, n - ch, I);
+    constexpr int CH = chunk_rows<R>(2);
+    for (int r0 = threadIdx.x * CH; r0 < g.frameH; r0 += kWG * CH) {
+        const int n = min(CH, g.frameH - r0);
+        GeoIter<R> it;
+        it.start(g, b, r0);
+        R I[CH];
+        load_chunk<CH>(img + r0, g.vec, n, I);
 #pragma unroll
-        for (int k = 0; k < kCH; k++) {
-            if (ch + k < n) {
-                R bx, by, f;
-                bool m;
-                it.at(g, geo, r0 + ch + k, bx, by, f, m);
-                it.next();
-                h[CS_SUMI] += I[k];
-                const R x = mad(kn.dx, f, bx + kn.ax);
-                const R y = mad(kn.dy, f, by + kn.ay);
-                if (m && in_bounds(g, x, y)) {
-                    Cell<R> cl;
-                    const R d = I[k] - sample(g, x, y, cl);
-                    R wx, wy;
-                    gradients(g, cl, wx, wy);
-                    const R cf = (R)1 - f;
-                    const R xx = wx * wx, yy = wy * wy, xy = wx * wy;
-                    const R c2 = cf * cf, f2 = f * f, fc = f - f2;
-                    cnt++;
-                    d2 = fma((double)d, (double)d, d2);
-                    h[CS_AXX - 1] = mad(xx, c2, h[CS_AXX - 1]);
-                    h[CS_AYY - 1] = mad(yy, c2, h[CS_AYY - 1]);
-                    h[CS_AXY - 1] = mad(xy, c2, h[CS_AXY - 1]);
-                    h[CS_BXX - 1] = mad(xx, f2, h[CS_BXX - 1]);
-                    h[CS_BYY - 1] = mad(yy, f2, h[CS_BYY - 1]);
-                    h[CS_BXY - 1] = mad(xy, f2, h[CS_BXY - 1]);
-                    h[CS_CXX - 1] = mad(xx, fc, h[CS_CXX - 1]);
-                    h[CS_CYY - 1] = mad(yy, fc, h[CS_CYY - 1]);
-                    h[CS_CXY - 1] = mad(xy, fc, h[CS_CXY - 1]);
-                    const R dc = d * cf, df = d * f;
-                    h[CS_JXA - 1] = mad(dc, wx, h[CS_JXA - 1]);
-                    h[CS_JYA - 1] = mad(dc, wy, h[CS_JYA - 1]);
-                    h[CS_JXB - 1] = mad(df, wx, h[CS_JXB - 1]);
-                    h[CS_JYB - 1] = mad(df, wy, h[CS_JYB - 1]);
-                    if (fnx) {
-                        const R fn = fnx[r0 + ch + k];
-                        const R n2 = fn * fn;
-                        h[CS_QXX - 1] = mad(xx, n2, h[CS_QXX - 1]);
-                        h[CS_QYY - 1] = mad(yy, n2, h[CS_QYY - 1]);
-                        h[CS_QXY - 1] = mad(xy, n2, h[CS_QXY - 1]);
-                    }
-                }
+        for (int k = 0; k < CH; k++) {
+            const int rr = min(r0 + k, g.frameH - 1);
+            R bx, by, f;
+            bool m;
+            it.at(g, geo, rr, bx, by, f, m);
+            it.next();
+            h[CS_SUMI] += I[k];
+            const R x = mad(kn.dx, f, bx + kn.ax);
+            const R y = mad(kn.dy, f, by + kn.ay);
+            const bool ok = k < n && m && in_bounds(g, x, y);
+            Cell<R> cl;
+            const R s = sample(g, x, y, ok, cl);
+            R wx, wy;
+            gradients(g, cl, wx, wy);
+            // outside the mask every term below is zero
+            const R d = ok ? I[k] - s : (R)0;
+            wx = ok ? wx : (R)0;
+            wy = ok ? wy : (R)0;
+            const R cf = (R)1 - f;
+            const R xx = wx * wx, yy = wy * wy, xy = wx * wy;
+            const R c2 = cf * cf, f2 = f * f, fc = f - f2;
+            cnt += ok ? 1 : 0;
+            d2 = fma((double)d, (double)d, d2);
+            h[CS_AXX - 1] = mad(xx, c2, h[CS_AXX - 1]);
+            h[CS_AYY - 1] = mad(yy, c2, h[CS_AYY - 1]);
+            h[CS_AXY - 1] = mad(xy, c2, h[CS_AXY - 1]);
+            h[CS_BXX - 1] = mad(xx, f2, h[CS_BXX - 1]);
+            h[CS_BYY - 1] = mad(yy, f2, h[CS_BYY - 1]);
+            h[CS_BXY - 1] = mad(xy, f2, h[CS_BXY - 1]);
+            h[CS_CXX - 1] = mad(xx, fc, h[CS_CXX - 1]);
+            h[CS_CYY - 1] = mad(yy, fc, h[CS_CYY - 1]);
+            h[CS_CXY - 1] = mad(xy, fc, h[CS_CXY - 1]);
+            const R dc = d * cf, df = d * f;
+            h[CS_JXA - 1] = mad(dc, wx, h[CS_JXA - 1]);
+            h[CS_JYA - 1] = mad(dc, wy, h[CS_JYA - 1]);
+            h[CS_JXB - 1] = mad(df, wx, h[CS_JXB - 1]);
+            h[CS_JYB - 1] = mad(df, wy, h[CS_JYB - 1]);
+            if (fnx) {
+                const R fn = fnx[rr];
+                const R n2 = fn * fn;
+                h[CS_QXX - 1] = mad(xx, n2, h[CS_QXX - 1]);
+                h[CS_QYY - 1] = mad(yy, n2, h[CS_QYY - 1]);
+                h[CS_QXY - 1] = mad(xy, n2, h[CS_QXY - 1]);
             }
         }
     }
@@ -793,12 +835,9 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
         g.lsNextK[f] = 0;
         g.lsMk[f] = -1;
         g.lsDk[f] = -1;
-        g.frameArrive[f] = 0;
     }
     if (tid == 0) {
-        ctl.activeFrames = nF;
         ctl.roundFrames = nF;
-        ctl.roundArrive = 0;
         ctl.roundsThisLS = 0;
         ctl.K = g.K1;
         ctl.phase = PH_LS;
@@ -808,7 +847,9 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
 // ---------------------------------------------------------------------------
 // line search: K fused candidates p0 + r^k * delta of every active frame
 // (the tries of d_linesearch, task_linesearch.cu:35-106, in one pass), then
-// the frame's last column to finish resolves the tries sequentially.
+// one wave per frame resolves the tries sequentially and one per group does
+// the round's bookkeeping.  Three kernels, so that no device-scope fence is
+// needed: on gfx950 each one writes back and invalidates the XCD's L2.
 // ---------------------------------------------------------------------------
 template <typename TI, typename R, int K>
 __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
@@ -833,10 +874,6 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
     }
     const TI *img = column_ptr<TI>(g, c);
     const typename Prec<R>::V4 *geo = Prec<R>::geo(g) ? Prec<R>::geo(g) + (size_t)b * g.frameH : nullptr;
-    const int r0 = threadIdx.x * g.seg;
-    const int n = min(g.seg, g.frameH - r0);
-    GeoIter<R> it;
-    it.start(g, b, r0);
     double e2[K];  // FP64: compared against the half-iteration's errval
     int cnt[K];
 #pragma unroll
@@ -844,29 +881,30 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
         e2[k] = 0.0;
         cnt[k] = 0;
     }
-    for (int ch = 0; ch < n; ch += kCH) {
-        R I[kCH];
-        load8(img + r0 + ch, g.vec, n - ch, I);
+    constexpr int CH = chunk_rows<R>(K);
+    for (int r0 = threadIdx.x * CH; r0 < g.frameH; r0 += kWG * CH) {
+        const int n = min(CH, g.frameH - r0);
+        GeoIter<R> it;
+        it.start(g, b, r0);
+        R I[CH];
+        load_chunk<CH>(img + r0, g.vec, n, I);
 #pragma unroll
-        for (int q = 0; q < kCH; q++) {
-            if (ch + q < n) {
-                R bx, by, f;
-                bool m;
-                it.at(g, geo, r0 + ch + q, bx, by, f, m);
-                it.next();
-                if (m) {
+        for (int q = 0; q < CH; q++) {
+            R bx, by, f;
+            bool m;
+            it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f, m);
+            it.next();
+            m = m && q < n;
 #pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        const R x = mad(kn[k].dx, f, bx + kn[k].ax);
-                        const R y = mad(kn[k].dy, f, by + kn[k].ay);
-                        if (in_bounds(g, x, y)) {
-                            Cell<R> cl;
-                            const R d = I[q] - sample(g, x, y, cl);
-                            e2[k] = fma((double)d, (double)d, e2[k]);
-                            cnt[k]++;
-                        }
-                    }
-                }
+            for (int k = 0; k < K; k++) {
+                const R x = mad(kn[k].dx, f, bx + kn[k].ax);
+                const R y = mad(kn[k].dy, f, by + kn[k].ay);
+                const bool ok = m && in_bounds(g, x, y);
+                Cell<R> cl;
+                const R s = sample(g, x, y, ok, cl);
+                const R d = ok ? I[q] - s : (R)0;
+                e2[k] = fma((double)d, (double)d, e2[k]);
+                cnt[k] += ok ? 1 : 0;
             }
         }
     }
@@ -881,52 +919,99 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
     if (threadIdx.x < 2 * K) g.lssum[(size_t)c * 2 * kMaxCand + threadIdx.x] = out[threadIdx.x];
 }
 
-// Sequential resolution of the candidates of frame f (reproduces the
-// reference's try / reduce / movement-threshold order), commit, and the
-// group's end-of-round bookkeeping.  Runs in the frame's last workgroup.
-__device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f, int K) {
-    const int nb = g.nb, N = g.N, nF = g.nF;
-    const int shift = ctl.shift;
-    __shared__ double se[kMaxCand], sn[kMaxCand];
-    __shared__ int sp[kMaxCand];
-    __shared__ int s_fin, s_acc, s_last;
-    if (threadIdx.x < K) {
-        const int kk = threadIdx.x;
-        double s = 0.0, c = 0.0;
-        int pres = 0;
-        for (int bb = 0; bb < nb; bb++) {
-            const double *ls = g.lssum + (size_t)(f * nb + bb) * 2 * kMaxCand + 2 * kk;
-            if (ls[1] >= (double)g.minPts[bb]) pres = 1;
-            s += ls[0];
-            c += ls[1];
+template <typename TI, typename R>
+__global__ __launch_bounds__(kWG) void k_ls_round(Batch B) {
+    for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
+        const int gi = B.itemGroup[item];
+        const GroupCtl &ctl = B.ctl[gi];
+        if (ctl.phase != PH_LS) continue;
+        const GroupDev &g = B.groups[gi];
+        if (!runs_here<TI, R>(g)) continue;
+        const int c = item - g.itemStart, f = c / g.nb, b = c % g.nb;
+        if (!g.lsActive[f]) continue;
+        const int K = ctl.K, k0 = g.lsNextK[f];
+        switch (K) {
+        case 1: ls_column<TI, R, 1>(g, c, b, k0); break;
+        case 2: ls_column<TI, R, 2>(g, c, b, k0); break;
+        case 4: ls_column<TI, R, 4>(g, c, b, k0); break;
+        default: ls_column<TI, R, 8>(g, c, b, k0); break;
         }
-        se[kk] = s;
-        sn[kk] = c;
-        sp[kk] = pres;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    return x;
+}
+__device__ __forceinline__ int wave_or(int x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x |= __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Sequential resolution of the candidates of frame f (reproduces the
+// reference's try / reduce / movement-threshold order) and commit.  One
+// wave per frame.
+__device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f) {
+    const int nb = g.nb, N = g.N, nF = g.nF, K = ctl.K;
+    const int lane = threadIdx.x;
+    double se[kMaxCand], sn[kMaxCand];
+#pragma unroll
+    for (int k = 0; k < kMaxCand; k++) se[k] = sn[k] = 0.0;
+    int pres = 0;  // bit k: some column of candidate k reaches its block's min points
+    double dmax = 0.0;
+    for (int bb = lane; bb < nb; bb += 64) {
+        const double *ls = g.lssum + (size_t)(f * nb + bb) * 2 * kMaxCand;
+#pragma unroll
+        for (int k = 0; k < kMaxCand; k++) {
+            if (k < K) {
+                se[k] += ls[2 * k];
+                sn[k] += ls[2 * k + 1];
+                if (ls[2 * k + 1] >= (double)g.minPts[bb]) pres |= 1 << k;
+            }
+        }
+        dmax = fmax(dmax, fmax(fabs(g.d0[f * nb + bb]), fabs(g.d0[f * nb + bb + N])));
+    }
+#pragma unroll
+    for (int k = 0; k < kMaxCand; k++) {
+        if (k < K) {
+            se[k] = wave_sum(se[k]);
+            sn[k] = wave_sum(sn[k]);
+        }
+    }
+    pres = wave_or(pres);
+    dmax = wave_max(dmax);
+    int fin = -1, acc = -1;
+    if (lane == 0) {
+        const int shift = ctl.shift;
         const double r = g.rmult, thr = g.thr;
         const int k0 = g.lsNextK[f];
-        double dmax = 0.0;
-        for (int t = 0; t < nb; t++) {
-            const double a = fabs(g.d0[f * nb + t]), bq = fabs(g.d0[f * nb + t + N]);
-            if (a > dmax) dmax = a;
-            if (bq > dmax) dmax = bq;
-        }
         double dm = reduce_pow(dmax, r, k0);
-        int fin = -1, acc = -1, mk = -1, tries = 0;
+        int mk = -1, tries = 0;
         for (int kk = 0; kk < K && fin < 0; kk++) {
             const int k = k0 + kk;
             tries++;
             mk = k;
-            if (!sp[kk]) {
+            if (!((pres >> kk) & 1)) {
                 const double dm1 = dm * r;
                 if (!(dm1 > thr)) fin = k + 1;
                 dm = dm1;
             } else {
                 g.lsDk[f] = k;
-                const double e = se[kk] / sn[kk];
+                double e = 0.0, c = 0.0;
+#pragma unroll
+                for (int q = 0; q < kMaxCand; q++)
+                    if (q == kk) {
+                        e = se[q];
+                        c = sn[q];
+                    }
+                e /= c;
                 if (e < g.errval[shift * nF + f]) {
                     acc = k;
                     fin = k;
@@ -941,39 +1026,50 @@ __device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f, int K) 
         g.lsMk[f] = mk;
         atomicAdd(&ctl.tries, tries);
         if (fin < 0) g.lsNextK[f] = k0 + K;
-        s_fin = fin;
-        s_acc = acc;
+        else g.lsActive[f] = 0;
     }
-    __syncthreads();
-    const int fin = s_fin, acc = s_acc;
+    fin = __shfl(fin, 0, 64);
+    acc = __shfl(acc, 0, 64);
     if (fin >= 0) {
         const double r = g.rmult;
-        for (int t = threadIdx.x; t < 2 * nb; t += kWG) {
+        for (int t = lane; t < 2 * nb; t += 64) {
             const int i = f * nb + (t % nb) + (t >= nb ? N : 0);
             const double d0 = g.d0[i];
             g.deltaP[i] = reduce_pow(d0, r, fin);
             if (acc >= 0) g.p[i] = g.p0[i] + reduce_pow(d0, r, acc);
         }
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (fin >= 0) {
-            g.lsActive[f] = 0;
-            atomicSub(&ctl.activeFrames, 1);
-        }
-        __threadfence();
-        s_last = (atomicAdd(&ctl.roundArrive, 1) == ctl.roundFrames - 1);
+}
+
+__global__ __launch_bounds__(64) void k_ls_decide(Batch B) {
+    for (int fi = blockIdx.x; fi < B.nFrames; fi += gridDim.x) {
+        const int gi = B.frameGroup[fi];
+        GroupCtl &ctl = B.ctl[gi];
+        if (ctl.phase != PH_LS) continue;
+        const GroupDev &g = B.groups[gi];
+        const int f = fi - g.frameStart;
+        if (!g.lsActive[f]) continue;
+        ls_decide_frame(g, ctl, f);
     }
-    __syncthreads();
-    if (!s_last || threadIdx.x != 0) return;
-    // end of the round for this group
-    __threadfence();
+}
+
+// End of a line-search round of a group: the next round's frames, or the
+// end of the half-iteration (`improved`, task_linesearch.cu:117-125) and of
+// MainIteration::execute (mainiteration.cpp:103-118).
+__global__ __launch_bounds__(64) void k_ls_finish(Batch B) {
+    const int gi = blockIdx.x;
+    GroupCtl &ctl = B.ctl[gi];
+    if (ctl.phase != PH_LS) return;
+    const GroupDev &g = B.groups[gi];
+    const int nF = g.nF;
+    int active = 0;
+    for (int f = threadIdx.x; f < nF; f += 64) active += g.lsActive[f];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) active += __shfl_xor(active, off, 64);
+    if (threadIdx.x != 0) return;
     ctl.rounds++;
     ctl.roundsThisLS++;
-    ctl.roundArrive = 0;
     ctl.lsFramePasses += ctl.roundFrames;
-    const int active = atomicAdd(&ctl.activeFrames, 0);
     if (active > 0) {
         ctl.roundFrames = active;
         ctl.K = g.Kn;
@@ -983,12 +1079,13 @@ __device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f, int K) 
         }
         return;
     }
-    // half-iteration complete: `improved` (task_linesearch.cu:117-125)
+    // half-iteration complete
     double e0 = 0.0, e1 = 0.0;
     for (int ff = 0; ff < nF; ff++) {
         e0 += g.errval[ff];
         e1 += g.errval[ff + nF];
     }
+    const int shift = ctl.shift;
     const int imp = (shift && e0 > e1) || (!shift && e0 < e1);
     ctl.improved &= imp;
     ctl.halfIterations++;
@@ -997,7 +1094,6 @@ __device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f, int K) 
         ctl.phase = PH_ASSEMBLE;
         return;
     }
-    // end of MainIteration::execute (mainiteration.cpp:103-118)
     if (!ctl.improved && ctl.nFilterIterations != 0) {
         ctl.solverFilter = 0;
         ctl.nFilterIterations = 0;
@@ -1008,37 +1104,6 @@ __device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f, int K) 
     }
     ctl.iterImproved = ctl.improved;
     ctl.phase = PH_CORR;
-}
-
-template <typename TI, typename R>
-__global__ __launch_bounds__(kWG) void k_ls_round(Batch B) {
-    __shared__ int s_last;
-    for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
-        const int gi = B.itemGroup[item];
-        GroupCtl &ctl = B.ctl[gi];
-        if (ctl.phase != PH_LS) continue;
-        const GroupDev &g = B.groups[gi];
-        if (!runs_here<TI, R>(g)) continue;
-        const int c = item - g.itemStart, f = c / g.nb, b = c % g.nb;
-        if (!g.lsActive[f]) continue;
-        const int K = ctl.K, k0 = g.lsNextK[f];
-        switch (K) {
-        case 1: ls_column<TI, R, 1>(g, c, b, k0); break;
-        case 2: ls_column<TI, R, 2>(g, c, b, k0); break;
-        case 4: ls_column<TI, R, 4>(g, c, b, k0); break;
-        default: ls_column<TI, R, 8>(g, c, b, k0); break;
-        }
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) s_last = (atomicAdd(&g.frameArrive[f], 1) == g.nb - 1);
-        __syncthreads();
-        if (s_last) {
-            __threadfence();
-            if (threadIdx.x == 0) g.frameArrive[f] = 0;
-            ls_decide_frame(g, ctl, f, K);
-        }
-        __syncthreads();
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1055,55 +1120,68 @@ __device__ void corr_column(const GroupDev &g, int c) {
     const ColKnots<R> kd = state_knots<R>(g, c, dk);
     const TI *img = column_ptr<TI>(g, c);
     const typename Prec<R>::V4 *geo = Prec<R>::geo(g) ? Prec<R>::geo(g) + (size_t)b * g.frameH : nullptr;
-    const int r0 = threadIdx.x * g.seg;
-    const int n = min(g.seg, g.frameH - r0);
-    GeoIter<R> it;
-    it.start(g, b, r0);
+    // first row outside the mask of the last-tried state: the moments cover
+    // the rows before it
+    int firstOut = g.frameH;
+    constexpr int CH = chunk_rows<R>(1);
+    for (int r0 = threadIdx.x * CH; r0 < g.frameH && r0 < firstOut; r0 += kWG * CH) {
+        const int n = min(CH, g.frameH - r0);
+        GeoIter<R> it;
+        it.start(g, b, r0);
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+            R bx, by, f1;
+            bool m;
+            it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f1, m);
+            it.next();
+            const R x = mad(km.dx, f1, bx + km.ax);
+            const R y = mad(km.dy, f1, by + km.ay);
+            if (q < n && !(m && in_bounds(g, x, y))) firstOut = min(firstOut, r0 + q);
+        }
+    }
+    firstOut = block_min(firstOut);
     double v[CR_N];
 #pragma unroll
     for (int i = 0; i < CR_N; i++) v[i] = 0.0;
-    bool open = true;  // still inside this thread's leading masked run
-    for (int ch = 0; ch < n; ch += kCH) {
-        R I[kCH];
-        load8(img + r0 + ch, g.vec, n - ch, I);
+    for (int r0 = threadIdx.x * CH; r0 < g.frameH; r0 += kWG * CH) {
+        const int n = min(CH, g.frameH - r0);
+        GeoIter<R> it;
+        it.start(g, b, r0);
+        R I[CH];
+        load_chunk<CH>(img + r0, g.vec, n, I);
 #pragma unroll
-        for (int q = 0; q < kCH; q++) {
-            if (ch + q < n) {
-                R bx, by, f1;
-                bool m;
-                it.at(g, geo, r0 + ch + q, bx, by, f1, m);
-                it.next();
-                const R x = mad(km.dx, f1, bx + km.ax);
-                const R y = mad(km.dy, f1, by + km.ay);
-                const bool mm = m && in_bounds(g, x, y);
-                R dbuf = (R)0;
-                Cell<R> cl;
-                if (same) {
-                    if (mm) dbuf = I[q] - sample(g, x, y, cl);
-                } else {
-                    const R xd = mad(kd.dx, f1, bx + kd.ax);
-                    const R yd = mad(kd.dy, f1, by + kd.ay);
-                    if (m && in_bounds(g, xd, yd)) dbuf = I[q] - sample(g, xd, yd, cl);
-                }
-                v[CR_SUMD] += mm ? (double)(I[q] - dbuf) : (double)dbuf;
-                v[CR_NMASK] += mm ? 1.0 : 0.0;
-                if (!mm) open = false;
-                if (open) {
-                    const double a = (double)(I[q] - dbuf), bb = (double)I[q];
-                    v[CR_SA] += a;
-                    v[CR_SAA] += a * a;
-                    v[CR_SB] += bb;
-                    v[CR_SBB] += bb * bb;
-                    v[CR_SAB] += a * bb;
-                    v[CR_NS] += 1.0;
-                }
+        for (int q = 0; q < CH; q++) {
+            R bx, by, f1;
+            bool m;
+            it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f1, m);
+            it.next();
+            m = m && q < n;
+            const R x = mad(km.dx, f1, bx + km.ax);
+            const R y = mad(km.dy, f1, by + km.ay);
+            const bool mm = m && in_bounds(g, x, y);
+            // the difference image holds the last-decided state's residual
+            R xd = x, yd = y;
+            bool okd = mm;
+            if (!same) {
+                xd = mad(kd.dx, f1, bx + kd.ax);
+                yd = mad(kd.dy, f1, by + kd.ay);
+                okd = m && in_bounds(g, xd, yd);
             }
+            Cell<R> cl;
+            const R s = sample(g, xd, yd, okd, cl);
+            const R dbuf = okd ? I[q] - s : (R)0;
+            const double Iq = q < n ? (double)I[q] : 0.0;
+            v[CR_SUMD] += mm ? (double)(I[q] - dbuf) : (double)dbuf;
+            v[CR_NMASK] += mm ? 1.0 : 0.0;
+            const bool run = r0 + q < firstOut;  // rows before it are all inside the mask
+            const double a = run ? (double)(I[q] - dbuf) : 0.0, bb = run ? Iq : 0.0;
+            v[CR_SA] += a;
+            v[CR_SAA] += a * a;
+            v[CR_SB] += bb;
+            v[CR_SBB] += bb * bb;
+            v[CR_SAB] += a * bb;
+            v[CR_NS] += run ? 1.0 : 0.0;
         }
-    }
-    // only threads up to the first one that met an unmasked row contribute
-    const int firstBroken = block_min(open ? kWG : (int)threadIdx.x);
-    if ((int)threadIdx.x > firstBroken) {
-        for (int i = CR_SA; i < CR_N; i++) v[i] = 0.0;
     }
     __shared__ double out[CR_N];
     block_sum<CR_N>(v, out);
@@ -1205,40 +1283,37 @@ __device__ void final_column(const GroupDev &g, int c) {
     const TI *img = column_ptr<TI>(g, c);
     const typename Prec<R>::V4 *geo = Prec<R>::geo(g) ? Prec<R>::geo(g) + (size_t)b * g.frameH : nullptr;
     uint8_t *mo = g.maskOut ? g.maskOut + (size_t)c * g.frameH : nullptr;
-    const int r0 = threadIdx.x * g.seg;
-    const int n = min(g.seg, g.frameH - r0);
-    GeoIter<R> it;
-    it.start(g, b, r0);
     double v[FS_N];
 #pragma unroll
     for (int i = 0; i < FS_N; i++) v[i] = 0.0;
-    for (int ch = 0; ch < n; ch += kCH) {
-        R I[kCH];
-        load8(img + r0 + ch, g.vec, n - ch, I);
+    constexpr int CH = chunk_rows<R>(1);
+    for (int r0 = threadIdx.x * CH; r0 < g.frameH; r0 += kWG * CH) {
+        const int n = min(CH, g.frameH - r0);
+        GeoIter<R> it;
+        it.start(g, b, r0);
+        R I[CH];
+        load_chunk<CH>(img + r0, g.vec, n, I);
 #pragma unroll
-        for (int q = 0; q < kCH; q++) {
-            if (ch + q < n) {
-                R bx, by, f1;
-                bool m;
-                it.at(g, geo, r0 + ch + q, bx, by, f1, m);
-                it.next();
-                const R x = mad(kn.dx, f1, bx + kn.ax);
-                const R y = mad(kn.dy, f1, by + kn.ay);
-                const bool mm = m && in_bounds(g, x, y);
-                if (mo) mo[r0 + ch + q] = mm ? 1 : 0;
-                if (mm) {
-                    Cell<R> cl;
-                    const R wI = sample(g, x, y, cl);
-                    const double d = (double)(I[q] - wI);
-                    v[FS_D2] += d * d;
-                    v[FS_CNT] += 1.0;
-                    v[FS_SA] += (double)wI;
-                    v[FS_SAA] += (double)wI * (double)wI;
-                    v[FS_SB] += (double)I[q];
-                    v[FS_SBB] += (double)I[q] * (double)I[q];
-                    v[FS_SAB] += (double)wI * (double)I[q];
-                }
-            }
+        for (int q = 0; q < CH; q++) {
+            R bx, by, f1;
+            bool m;
+            it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f1, m);
+            it.next();
+            const R x = mad(kn.dx, f1, bx + kn.ax);
+            const R y = mad(kn.dy, f1, by + kn.ay);
+            const bool mm = q < n && m && in_bounds(g, x, y);
+            if (mo && q < n) mo[r0 + q] = mm ? 1 : 0;
+            Cell<R> cl;
+            const R s = sample(g, x, y, mm, cl);
+            const double wI = mm ? (double)s : 0.0, Iq = mm ? (double)I[q] : 0.0;
+            const double d = Iq - wI;
+            v[FS_D2] += d * d;
+            v[FS_CNT] += mm ? 1.0 : 0.0;
+            v[FS_SA] += wI;
+            v[FS_SAA] += wI * wI;
+            v[FS_SB] += Iq;
+            v[FS_SBB] += Iq * Iq;
+            v[FS_SAB] += wI * Iq;
         }
     }
     __shared__ double out[FS_N];
@@ -1306,6 +1381,10 @@ hipError_t launch_stage(const LaunchBatch &L, int stage, hipStream_t s) {
     case ST_ASSEMBLE: SS_LAUNCH_TYPED(k_assemble, items, L, s); break;
     case ST_SOLVE: hipLaunchKernelGGL(k_solve, groups, dim3(kSolveWG), 0, s, L.b); break;
     case ST_LS: SS_LAUNCH_TYPED(k_ls_round, items, L, s); break;
+    case ST_LS_DECIDE:
+        hipLaunchKernelGGL(k_ls_decide, dim3(L.frameGrid), dim3(64), 0, s, L.b);
+        hipLaunchKernelGGL(k_ls_finish, groups, dim3(64), 0, s, L.b);
+        break;
     case ST_CORR: SS_LAUNCH_TYPED(k_corr, items, L, s); break;
     case ST_CORR_FINISH: hipLaunchKernelGGL(k_corr_finish, groups, dim3(kSolveWG), 0, s, L.b); break;
     case ST_FINAL: SS_LAUNCH_TYPED(k_final, items, L, s); break;

@@ -134,3 +134,34 @@ def config1(n_frames=1000, size=512, seed=200, dtype=np.uint16):
     fr = make_frames(T, dx, dy, size, size, dtype, True, seed=seed + 3)
     geom = init_settings(size, size, T.shape, nknots=size // 16 + 1)
     return T, fr, geom, (dx, dy)
+
+
+def config4(duration=4.0, seed=400, n512=16, n1024=2, dtype=np.int16):
+    """Acquisition streams for the serving workload: n512 streams of 512x512
+    at 30 Hz and n1024 of 1024x1024 at 15 Hz, int16, each with its own
+    template and motion (as config1), cut into groups of U{16..64} frames.
+    A group arrives when its last frame has been scanned, plus an
+    exponential jitter of mean 10 ms.  Returns (streams, groups) where a
+    group is a dict(stream, first, count, arrival)."""
+    rng = np.random.default_rng(seed)
+    streams, groups = [], []
+    for k, (size, rate) in enumerate([(512, 30.0)] * n512 + [(1024, 15.0)] * n1024):
+        s = seed + 10 * (k + 1)
+        n = max(16, int(round(duration * rate)))
+        T = make_template(size, size, 400 * (size // 512) ** 2, (1.5, 8.0), n_background=4,
+                          background_sigma=150.0 * size / 512, target_mean=20.0, seed=s + 1)
+        amp = (float(rng.uniform(2, 8)), float(rng.uniform(2, 8)))
+        dx, dy = make_motion(n, size, 0.1, 5.6e-4 * 512 / size, amp, jump_prob=0.002, jump_size=5.0,
+                             frame_rate=rate, seed=s + 2)
+        fr = make_frames(T, dx, dy, size, size, dtype, True, seed=s + 3)
+        streams.append(dict(T=T, frames=fr, size=size, rate=rate))
+        first = 0
+        while first < n:
+            count = min(int(rng.integers(16, 65)), n - first)
+            if count < 2:
+                break
+            arrival = (first + count) / rate + float(rng.exponential(0.010))
+            groups.append(dict(stream=k, first=first, count=count, arrival=arrival))
+            first += count
+    groups.sort(key=lambda g: g["arrival"])
+    return streams, groups

@@ -41,7 +41,8 @@ def test_ctypes_table_covers_header():
 
 @pytest.mark.parametrize("cname,pyt", [("ss_settings", N.SSSettings), ("ss_group_in", N.SSGroupIn),
                                        ("ss_group_out", N.SSGroupOut), ("ss_geometry_desc", N.SSGeometryDesc),
-                                       ("ss_resample_in", N.SSResampleIn)])
+                                       ("ss_resample_in", N.SSResampleIn),
+                                       ("ss_server_config", N.SSServerConfig)])
 def test_struct_layout_matches_header(cname, pyt, tmp_path):
     """Offsets of every field of the ctypes mirror equal the C compiler's."""
     import shutil

@@ -1,9 +1,20 @@
 """GPU parity: the HIP path against the FP64 oracle on the same inputs.
 
-Tolerances: knot RMS <= 0.01 px (x and y together, excluding each group's
-last frame, whose right knot the reference leaves at 1e6 in the line
-search); residrms and frame correlations within 1e-3 relative (last frame
-excluded); masks agree on >= 99.9% of pixels.
+Knot RMS is over x and y together, excluding each group's last frame (whose
+right knot the reference leaves at 1e6 in the line search).
+
+Precision 'D' (the reference's default, FP64 pixel arithmetic): knot RMS
+<= 1e-9 px, residrms and frame correlations within 1e-9 relative, masks
+agree on >= 99.999% of pixels.  The measured knot RMS is ~1e-14 px: the GPU
+follows the oracle's trajectory decision for decision.
+
+Precision 'S' (FP32 pixel arithmetic, FP64 reductions and solve): knot RMS
+<= 0.01 px, residrms and frame correlations within 1e-3 relative, masks
+agree on >= 99.9% of pixels.  On the 512x512 workload group the method
+itself is sensitive to perturbations far below FP32 rounding (the FP64
+oracle moves by ~0.04 px RMS when the initial knots move by 1e-6 px), so
+there the 'S' result is held to 0.01 px after two iterations and to the
+oracle's own perturbation spread at convergence.
 """
 import numpy as np
 import pytest
@@ -30,28 +41,40 @@ def knot_rms(a, kx_gpu, ky_gpu, ref):
     return float(np.sqrt(np.mean(np.concatenate([dx.ravel(), dy.ravel()]) ** 2)))
 
 
-def check(a, ref, kx, ky, resid, fcorr, mask, tol=0.01, label=""):
+TOL = {"D": dict(knots=1e-9, rtol=1e-9, atol=1e-12, mask=0.99999),
+       "S": dict(knots=0.01, rtol=1e-3, atol=1e-4, mask=0.999)}
+
+
+def precision_of(a):
+    return ssa.parse_flags(a.get("flags", "D"))[0]
+
+
+def check(a, ref, kx, ky, resid, fcorr, mask, label=""):
+    t = TOL[precision_of(a)]
     rms = knot_rms(a, kx, ky, ref)
-    print(f"{label} knot RMS vs FP64 oracle = {rms:.3e} px", flush=True)
-    assert rms <= tol
-    assert np.allclose(np.ravel(resid)[:-1], ref["residrms"][:-1], rtol=1e-3)
-    assert np.allclose(np.ravel(fcorr)[:-1], ref["frame_corr"][:-1], rtol=1e-3, atol=1e-4)
-    agree = np.mean(np.asarray(mask, bool) == ref["mask"].astype(bool))
-    assert agree >= 0.999, agree
+    print(f"{label} [{precision_of(a)}] knot RMS vs FP64 oracle = {rms:.3e} px", flush=True)
+    assert rms <= t["knots"]
+    assert np.allclose(np.ravel(resid)[:-1], ref["residrms"][:-1], rtol=t["rtol"], atol=t["atol"])
+    assert np.allclose(np.ravel(fcorr)[:-1], ref["frame_corr"][:-1], rtol=t["rtol"], atol=t["atol"])
+    if mask is not None:
+        agree = np.mean(np.asarray(mask, bool) == ref["mask"].astype(bool))
+        assert agree >= t["mask"], agree
 
 
-def test_config0_small():
+@pytest.mark.parametrize("flags", ["D", "S"])
+def test_config0_small(flags):
     T, fr, geom, _ = W.config0(n_frames=16, size=64, seed=11)
-    a = W.group_args(T, fr, geom, max_iter=20)
+    a = W.group_args(T, fr, geom, max_iter=20, flags=flags)
     ref = O.align_group_to_template(a, threads=8)
     out, info = run_gpu(a)
     print(f"iterations gpu={int(out[3])} oracle={ref['iterations']}")
     check(a, ref, out[0], out[1], out[5], out[2], out[4], label="config0 small")
 
 
-def test_config0_full():
+@pytest.mark.parametrize("flags", ["D", "S"])
+def test_config0_full(flags):
     T, fr, geom, _ = W.config0()
-    a = W.group_args(T, fr, geom, max_iter=20)
+    a = W.group_args(T, fr, geom, max_iter=20, flags=flags)
     ref = O.align_group_to_template(a, threads=8)
     out, info = run_gpu(a)
     print(f"iterations gpu={int(out[3])} oracle={ref['iterations']} stop={info['stop_criterion']}")
@@ -60,10 +83,11 @@ def test_config0_full():
 
 
 @pytest.mark.parametrize("K", [1, 4, 8])
-def test_candidate_count_does_not_change_results(K):
+@pytest.mark.parametrize("flags", ["D", "S"])
+def test_candidate_count_does_not_change_results(K, flags):
     # the fused rounds resolve the same sequential backtracking for any K
     T, fr, geom, _ = W.config0(n_frames=12, size=64, seed=13)
-    a = W.group_args(T, fr, geom, max_iter=15)
+    a = W.group_args(T, fr, geom, max_iter=15, flags=flags)
     base, _ = run_gpu(a, linesearch_candidates=2)
     out, _ = run_gpu(a, linesearch_candidates=K)
     assert np.array_equal(out[0], base[0]) and np.array_equal(out[1], base[1])
@@ -71,28 +95,31 @@ def test_candidate_count_does_not_change_results(K):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.uint16, np.int16, np.float64])
-def test_data_types(dtype):
+@pytest.mark.parametrize("flags", ["D", "S"])
+def test_data_types(dtype, flags):
     T, fr, geom, _ = W.config0(n_frames=10, size=64, seed=17)
-    a = W.group_args(T, fr, geom, max_iter=10)
+    a = W.group_args(T, fr, geom, max_iter=10, flags=flags)
     a["bmData"] = np.asfortranarray(np.rint(a["bmData"]).astype(dtype))
     ref = O.align_group_to_template(a, threads=8)
     out, _ = run_gpu(a)
     check(a, ref, out[0], out[1], out[5], out[2], out[4], label=str(dtype))
 
 
-def test_table_geometry_bidirectional():
+@pytest.mark.parametrize("flags", ["D", "S"])
+def test_table_geometry_bidirectional(flags):
     # bidirectional scanning: the time fractions are not affine, table path
     T = W.make_template(64, 64, 20, (2.0, 5.0), offset=10.0, seed=31)
     dx, dy = W.make_motion(10, 64, 0.05, 0.0, (1.5, 1.5), seed=32)
     fr = W.make_frames(T, dx, dy, 64, 64, np.float32, True, seed=33)
     geom = ssa.init_settings(64, 64, T.shape, nknots=9, isbidirectional=True)
-    a = W.group_args(T, fr, geom, max_iter=10)
+    a = W.group_args(T, fr, geom, max_iter=10, flags=flags)
     ref = O.align_group_to_template(a, threads=8)
     out, _ = run_gpu(a)
     check(a, ref, out[0], out[1], out[5], out[2], out[4], label="bidirectional")
 
 
-def test_table_geometry_noninteger_spacing():
+@pytest.mark.parametrize("flags", ["D", "S"])
+def test_table_geometry_noninteger_spacing(flags):
     # 60 lines in 7 knot intervals: padded blocks (base_mask false) and
     # frac columns that differ (the block c-1 diagonal term reads column c)
     T = W.make_template(60, 48, 20, (2.0, 5.0), offset=10.0, seed=41)
@@ -100,16 +127,17 @@ def test_table_geometry_noninteger_spacing():
     fr = W.make_frames(T, dx, dy, 60, 48, np.float32, True, seed=43)
     geom = ssa.init_settings(60, 48, T.shape, nknots=8)
     assert not geom["base_mask_bm"].all()
-    a = W.group_args(T, fr, geom, max_iter=10)
+    a = W.group_args(T, fr, geom, max_iter=10, flags=flags)
     ref = O.align_group_to_template(a, threads=8)
     out, _ = run_gpu(a)
     check(a, ref, out[0], out[1], out[5], out[2], out[4], label="non-integer spacing")
 
 
-def test_custom_gradients():
+@pytest.mark.parametrize("flags", ["D", "S"])
+def test_custom_gradients(flags):
     # gradients that are not the template's forward differences are read
     T, fr, geom, _ = W.config0(n_frames=8, size=64, seed=19)
-    a = W.group_args(T, fr, geom, max_iter=10)
+    a = W.group_args(T, fr, geom, max_iter=10, flags=flags)
     a["xgrad"] = np.asfortranarray(a["xgrad"] * 0.9)
     a["ygrad"] = np.asfortranarray(a["ygrad"] * 1.1)
     ref = O.align_group_to_template(a, threads=8)
@@ -127,11 +155,12 @@ def test_lambda_and_range_filter():
 
 
 def test_batch_of_uneven_groups_equals_single_calls():
+    # uneven sizes, knot spacings and precisions in one submission
     calls = []
-    for i, (n, size, lpk) in enumerate([(9, 64, 8), (16, 64, 16), (5, 96, 8), (13, 64, 8)]):
+    for i, (n, size, lpk, fl) in enumerate([(9, 64, 8, "D"), (16, 64, 16, "S"), (5, 96, 8, "D"), (13, 64, 8, "S")]):
         T, fr, geom, _ = W.config0(n_frames=n, size=size, seed=50 + i)
         geom = ssa.init_settings(size, size, T.shape, nknots=size // lpk + 1)
-        calls.append(W.group_args(T, fr, geom, max_iter=12))
+        calls.append(W.group_args(T, fr, geom, max_iter=12, flags=fl))
     res = ssa.align_groups(calls)
     for a, r in zip(calls, res):
         single, _ = run_gpu(a)
@@ -142,11 +171,44 @@ def test_batch_of_uneven_groups_equals_single_calls():
         check(a, ref, r["knots_x"], r["knots_y"], r["residrms"], r["frame_corr"], r["mask"], label="batch")
 
 
-def test_config1_group():
+@pytest.fixture(scope="module")
+def config1_group():
     # one 50-frame 512x512 uint16 group of the headline workload
     T, fr, geom, _ = W.config1(n_frames=50)
-    a = W.group_args(T, fr, geom)
+    return T, fr, geom
+
+
+def test_config1_group(config1_group):
+    T, fr, geom = config1_group
+    a = W.group_args(T, fr, geom, flags="D")
     ref = O.align_group_to_template(a, threads=16)
     out, info = run_gpu(a)
     print(f"iterations gpu={int(out[3])} oracle={ref['iterations']}")
+    assert int(out[3]) == ref["iterations"]
+    assert info["linesearch_tries"] > 0
     check(a, ref, out[0], out[1], out[5], out[2], out[4], label="config1")
+
+
+def test_config1_group_single_precision(config1_group):
+    T, fr, geom = config1_group
+    # two iterations: the FP32 pixel path stays on the oracle's trajectory
+    a2 = W.group_args(T, fr, geom, flags="S", max_iter=2)
+    ref2 = O.align_group_to_template(a2, threads=16)
+    out2, _ = run_gpu(a2)
+    check(a2, ref2, out2[0], out2[1], out2[5], out2[2], out2[4], label="config1, 2 iterations")
+    # at convergence: within the oracle's own spread under a 1e-6 px
+    # perturbation of the initial knots (its FP64 trajectory is not stable
+    # to that either)
+    a = W.group_args(T, fr, geom, flags="S")
+    ref = O.align_group_to_template(a, threads=16, with_mask=False)
+    rng = np.random.default_rng(1)
+    ap = dict(a)
+    ap["est_dx_knots"] = a["est_dx_knots"] + 1e-6 * rng.standard_normal(a["est_dx_knots"].shape)
+    ap["est_dy_knots"] = a["est_dy_knots"] + 1e-6 * rng.standard_normal(a["est_dy_knots"].shape)
+    refp = O.align_group_to_template(ap, threads=16, with_mask=False)
+    spread = knot_rms(a, refp["knots_x"], refp["knots_y"], ref)
+    out, _ = run_gpu(a)
+    rms = knot_rms(a, out[0], out[1], ref)
+    print(f"config1 [S] converged knot RMS vs FP64 oracle = {rms:.3e} px; "
+          f"oracle spread under 1e-6 px init perturbation = {spread:.3e} px", flush=True)
+    assert rms <= max(0.01, 2.0 * spread)
