@@ -69,6 +69,7 @@ struct GroupDev {
     int maxIter, K1, Kn, maxRounds;
     // geometry: affine fast path or per-pixel table
     int affine, L;
+    int runLen;           // rows a thread walks in order (a block's lines of one image column)
     AffineGeo<float> afF;
     AffineGeo<double> afD;
     int fracUniform;

@@ -34,7 +34,7 @@ struct DevTemplate {
 };
 
 struct DevGeometry {
-    int frameH = 0, nb = 0, fracUniform = 1, affine = 0, L = 0;
+    int frameH = 0, nb = 0, fracUniform = 1, affine = 0, L = 0, runLen = 8;
     ss::AffineGeo<float> afF{};
     ss::AffineGeo<double> afD{};
     float4 *geo = nullptr;
@@ -306,6 +306,17 @@ void convert_geometry(int fh, int nb, const double *frac, const double *bx, cons
             }
         }
     }
+    // rows a thread walks in order: the lines of one image column of a block
+    // (image2blockmatrix lists a block column by column), so that a wave's
+    // lanes sit on neighbouring pixels of a scan line
+    g.runLen = 8;
+    {
+        int L = fh;
+        for (int r = 1; r < fh; r++)
+            if (bx[r] != bx[0]) { L = r; break; }
+        if (g.affine) L = g.L;
+        if (L >= 1 && L <= 64 && fh % L == 0) g.runLen = L;
+    }
     out = GeometryHost{};
     if (!g.affine) {
         if (wantF) {
@@ -573,6 +584,7 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     const DevGeometry &G = P.geom;
     g.affine = G.affine;
     g.L = G.L;
+    g.runLen = G.runLen;
     g.afF = G.afF;
     g.afD = G.afD;
     g.fracUniform = G.fracUniform;
@@ -618,7 +630,7 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
         g.img = dptr(P.oImg);
     }
     // chunks of rows start 16-byte aligned when the columns do
-    g.vec = ((fh % 8) == 0 && ((uintptr_t)g.img % 16) == 0) ? 1 : 0;
+    g.vec = ((fh % 8) == 0 && (g.runLen % 8) == 0 && ((uintptr_t)g.img % 16) == 0) ? 1 : 0;
     // min points (first n_blocks entries are read: minPointsPerBlock[c % frameW])
     HIPCHK(up(P.oMin, in.min_points, nb * sizeof(int)));
     g.minPts = (const int *)dptr(P.oMin);

@@ -365,56 +365,59 @@ __device__ void assemble_column(const GroupDev &g, int c) {
     for (int i = 0; i < CS_N - 1; i++) h[i] = (R)0;
     int cnt = 0;
     constexpr int CH = chunk_rows<R>(2);
-    for (int r0 = threadIdx.x * CH; r0 < g.frameH; r0 += kWG * CH) {
-        const int n = min(CH, g.frameH - r0);
+    for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
+        const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
-        it.start(g, b, r0);
-        R I[CH];
-        load_chunk<CH>(img + r0, g.vec, n, I);
+        it.start(g, b, run * g.runLen);
+        for (int r0 = run * g.runLen; r0 < rend; r0 += CH) {
+            const int n = min(CH, rend - r0);
+            R I[CH];
+            load_chunk<CH>(img + r0, g.vec, n, I);
 #pragma unroll
-        for (int k = 0; k < CH; k++) {
-            const int rr = min(r0 + k, g.frameH - 1);
-            R bx, by, f;
-            bool m;
-            it.at(g, geo, rr, bx, by, f, m);
-            it.next();
-            h[CS_SUMI] += I[k];
-            const R x = mad(kn.dx, f, bx + kn.ax);
-            const R y = mad(kn.dy, f, by + kn.ay);
-            const bool ok = k < n && m && in_bounds(g, x, y);
-            Cell<R> cl;
-            const R s = sample(g, x, y, ok, cl);
-            R wx, wy;
-            gradients(g, cl, wx, wy);
-            // outside the mask every term below is zero
-            const R d = ok ? I[k] - s : (R)0;
-            wx = ok ? wx : (R)0;
-            wy = ok ? wy : (R)0;
-            const R cf = (R)1 - f;
-            const R xx = wx * wx, yy = wy * wy, xy = wx * wy;
-            const R c2 = cf * cf, f2 = f * f, fc = f - f2;
-            cnt += ok ? 1 : 0;
-            d2 = fma((double)d, (double)d, d2);
-            h[CS_AXX - 1] = mad(xx, c2, h[CS_AXX - 1]);
-            h[CS_AYY - 1] = mad(yy, c2, h[CS_AYY - 1]);
-            h[CS_AXY - 1] = mad(xy, c2, h[CS_AXY - 1]);
-            h[CS_BXX - 1] = mad(xx, f2, h[CS_BXX - 1]);
-            h[CS_BYY - 1] = mad(yy, f2, h[CS_BYY - 1]);
-            h[CS_BXY - 1] = mad(xy, f2, h[CS_BXY - 1]);
-            h[CS_CXX - 1] = mad(xx, fc, h[CS_CXX - 1]);
-            h[CS_CYY - 1] = mad(yy, fc, h[CS_CYY - 1]);
-            h[CS_CXY - 1] = mad(xy, fc, h[CS_CXY - 1]);
-            const R dc = d * cf, df = d * f;
-            h[CS_JXA - 1] = mad(dc, wx, h[CS_JXA - 1]);
-            h[CS_JYA - 1] = mad(dc, wy, h[CS_JYA - 1]);
-            h[CS_JXB - 1] = mad(df, wx, h[CS_JXB - 1]);
-            h[CS_JYB - 1] = mad(df, wy, h[CS_JYB - 1]);
-            if (fnx) {
-                const R fn = fnx[rr];
-                const R n2 = fn * fn;
-                h[CS_QXX - 1] = mad(xx, n2, h[CS_QXX - 1]);
-                h[CS_QYY - 1] = mad(yy, n2, h[CS_QYY - 1]);
-                h[CS_QXY - 1] = mad(xy, n2, h[CS_QXY - 1]);
+            for (int k = 0; k < CH; k++) {
+                const int rr = min(r0 + k, g.frameH - 1);
+                R bx, by, f;
+                bool m;
+                it.at(g, geo, rr, bx, by, f, m);
+                it.next();
+                h[CS_SUMI] += I[k];
+                const R x = mad(kn.dx, f, bx + kn.ax);
+                const R y = mad(kn.dy, f, by + kn.ay);
+                const bool ok = k < n && m && in_bounds(g, x, y);
+                Cell<R> cl;
+                const R s = sample(g, x, y, ok, cl);
+                R wx, wy;
+                gradients(g, cl, wx, wy);
+                // outside the mask every term below is zero
+                const R d = ok ? I[k] - s : (R)0;
+                wx = ok ? wx : (R)0;
+                wy = ok ? wy : (R)0;
+                const R cf = (R)1 - f;
+                const R xx = wx * wx, yy = wy * wy, xy = wx * wy;
+                const R c2 = cf * cf, f2 = f * f, fc = f - f2;
+                cnt += ok ? 1 : 0;
+                d2 = fma((double)d, (double)d, d2);
+                h[CS_AXX - 1] = mad(xx, c2, h[CS_AXX - 1]);
+                h[CS_AYY - 1] = mad(yy, c2, h[CS_AYY - 1]);
+                h[CS_AXY - 1] = mad(xy, c2, h[CS_AXY - 1]);
+                h[CS_BXX - 1] = mad(xx, f2, h[CS_BXX - 1]);
+                h[CS_BYY - 1] = mad(yy, f2, h[CS_BYY - 1]);
+                h[CS_BXY - 1] = mad(xy, f2, h[CS_BXY - 1]);
+                h[CS_CXX - 1] = mad(xx, fc, h[CS_CXX - 1]);
+                h[CS_CYY - 1] = mad(yy, fc, h[CS_CYY - 1]);
+                h[CS_CXY - 1] = mad(xy, fc, h[CS_CXY - 1]);
+                const R dc = d * cf, df = d * f;
+                h[CS_JXA - 1] = mad(dc, wx, h[CS_JXA - 1]);
+                h[CS_JYA - 1] = mad(dc, wy, h[CS_JYA - 1]);
+                h[CS_JXB - 1] = mad(df, wx, h[CS_JXB - 1]);
+                h[CS_JYB - 1] = mad(df, wy, h[CS_JYB - 1]);
+                if (fnx) {
+                    const R fn = fnx[rr];
+                    const R n2 = fn * fn;
+                    h[CS_QXX - 1] = mad(xx, n2, h[CS_QXX - 1]);
+                    h[CS_QYY - 1] = mad(yy, n2, h[CS_QYY - 1]);
+                    h[CS_QXY - 1] = mad(xy, n2, h[CS_QXY - 1]);
+                }
             }
         }
     }
@@ -882,29 +885,32 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
         cnt[k] = 0;
     }
     constexpr int CH = chunk_rows<R>(K);
-    for (int r0 = threadIdx.x * CH; r0 < g.frameH; r0 += kWG * CH) {
-        const int n = min(CH, g.frameH - r0);
+    for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
+        const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
-        it.start(g, b, r0);
-        R I[CH];
-        load_chunk<CH>(img + r0, g.vec, n, I);
+        it.start(g, b, run * g.runLen);
+        for (int r0 = run * g.runLen; r0 < rend; r0 += CH) {
+            const int n = min(CH, rend - r0);
+            R I[CH];
+            load_chunk<CH>(img + r0, g.vec, n, I);
 #pragma unroll
-        for (int q = 0; q < CH; q++) {
-            R bx, by, f;
-            bool m;
-            it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f, m);
-            it.next();
-            m = m && q < n;
+            for (int q = 0; q < CH; q++) {
+                R bx, by, f;
+                bool m;
+                it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f, m);
+                it.next();
+                m = m && q < n;
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const R x = mad(kn[k].dx, f, bx + kn[k].ax);
-                const R y = mad(kn[k].dy, f, by + kn[k].ay);
-                const bool ok = m && in_bounds(g, x, y);
-                Cell<R> cl;
-                const R s = sample(g, x, y, ok, cl);
-                const R d = ok ? I[q] - s : (R)0;
-                e2[k] = fma((double)d, (double)d, e2[k]);
-                cnt[k] += ok ? 1 : 0;
+                for (int k = 0; k < K; k++) {
+                    const R x = mad(kn[k].dx, f, bx + kn[k].ax);
+                    const R y = mad(kn[k].dy, f, by + kn[k].ay);
+                    const bool ok = m && in_bounds(g, x, y);
+                    Cell<R> cl;
+                    const R s = sample(g, x, y, ok, cl);
+                    const R d = ok ? I[q] - s : (R)0;
+                    e2[k] = fma((double)d, (double)d, e2[k]);
+                    cnt[k] += ok ? 1 : 0;
+                }
             }
         }
     }
@@ -1124,63 +1130,69 @@ __device__ void corr_column(const GroupDev &g, int c) {
     // the rows before it
     int firstOut = g.frameH;
     constexpr int CH = chunk_rows<R>(1);
-    for (int r0 = threadIdx.x * CH; r0 < g.frameH && r0 < firstOut; r0 += kWG * CH) {
-        const int n = min(CH, g.frameH - r0);
+    for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
+        const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
-        it.start(g, b, r0);
+        it.start(g, b, run * g.runLen);
+        for (int r0 = run * g.runLen; r0 < rend && r0 < firstOut; r0 += CH) {
+            const int n = min(CH, rend - r0);
 #pragma unroll
-        for (int q = 0; q < CH; q++) {
-            R bx, by, f1;
-            bool m;
-            it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f1, m);
-            it.next();
-            const R x = mad(km.dx, f1, bx + km.ax);
-            const R y = mad(km.dy, f1, by + km.ay);
-            if (q < n && !(m && in_bounds(g, x, y))) firstOut = min(firstOut, r0 + q);
+            for (int q = 0; q < CH; q++) {
+                R bx, by, f1;
+                bool m;
+                it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f1, m);
+                it.next();
+                const R x = mad(km.dx, f1, bx + km.ax);
+                const R y = mad(km.dy, f1, by + km.ay);
+                if (q < n && !(m && in_bounds(g, x, y))) firstOut = min(firstOut, r0 + q);
+            }
         }
     }
     firstOut = block_min(firstOut);
     double v[CR_N];
 #pragma unroll
     for (int i = 0; i < CR_N; i++) v[i] = 0.0;
-    for (int r0 = threadIdx.x * CH; r0 < g.frameH; r0 += kWG * CH) {
-        const int n = min(CH, g.frameH - r0);
+    for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
+        const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
-        it.start(g, b, r0);
-        R I[CH];
-        load_chunk<CH>(img + r0, g.vec, n, I);
+        it.start(g, b, run * g.runLen);
+        for (int r0 = run * g.runLen; r0 < rend; r0 += CH) {
+            const int n = min(CH, rend - r0);
+            R I[CH];
+            load_chunk<CH>(img + r0, g.vec, n, I);
 #pragma unroll
-        for (int q = 0; q < CH; q++) {
-            R bx, by, f1;
-            bool m;
-            it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f1, m);
-            it.next();
-            m = m && q < n;
-            const R x = mad(km.dx, f1, bx + km.ax);
-            const R y = mad(km.dy, f1, by + km.ay);
-            const bool mm = m && in_bounds(g, x, y);
-            // the difference image holds the last-decided state's residual
-            R xd = x, yd = y;
-            bool okd = mm;
-            if (!same) {
-                xd = mad(kd.dx, f1, bx + kd.ax);
-                yd = mad(kd.dy, f1, by + kd.ay);
-                okd = m && in_bounds(g, xd, yd);
+            for (int q = 0; q < CH; q++) {
+                R bx, by, f1;
+                bool m;
+                it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f1, m);
+                it.next();
+                m = m && q < n;
+                const R x = mad(km.dx, f1, bx + km.ax);
+                const R y = mad(km.dy, f1, by + km.ay);
+                const bool mm = m && in_bounds(g, x, y);
+                // the difference image holds the last-decided state's residual
+                R xd = x, yd = y;
+                bool okd = mm;
+                if (!same) {
+                    xd = mad(kd.dx, f1, bx + kd.ax);
+                    yd = mad(kd.dy, f1, by + kd.ay);
+                    okd = m && in_bounds(g, xd, yd);
+                }
+                Cell<R> cl;
+                const R s = sample(g, xd, yd, okd, cl);
+                const R dbuf = okd ? I[q] - s : (R)0;
+                const double Iq = q < n ? (double)I[q] : 0.0;
+                v[CR_SUMD] += mm ? (double)(I[q] - dbuf) : (double)dbuf;
+                v[CR_NMASK] += mm ? 1.0 : 0.0;
+                const bool run = r0 + q < firstOut;  // rows before it are all inside the mask
+                const double a = run ? (double)(I[q] - dbuf) : 0.0, bb = run ? Iq : 0.0;
+                v[CR_SA] += a;
+                v[CR_SAA] += a * a;
+                v[CR_SB] += bb;
+                v[CR_SBB] += bb * bb;
+                v[CR_SAB] += a * bb;
+                v[CR_NS] += run ? 1.0 : 0.0;
             }
-            Cell<R> cl;
-            const R s = sample(g, xd, yd, okd, cl);
-            const R dbuf = okd ? I[q] - s : (R)0;
-            const double Iq = q < n ? (double)I[q] : 0.0;
-            v[CR_SUMD] += mm ? (double)(I[q] - dbuf) : (double)dbuf;
-            v[CR_NMASK] += mm ? 1.0 : 0.0;
-            const bool run = r0 + q < firstOut;  // rows before it are all inside the mask
-            const double a = run ? (double)(I[q] - dbuf) : 0.0, bb = run ? Iq : 0.0;
-            v[CR_SA] += a;
-            v[CR_SAA] += a * a;
-            v[CR_SB] += bb;
-            v[CR_SBB] += bb * bb;
-            v[CR_SAB] += a * bb;
-            v[CR_NS] += run ? 1.0 : 0.0;
         }
     }
     __shared__ double out[CR_N];
@@ -1287,33 +1299,36 @@ __device__ void final_column(const GroupDev &g, int c) {
 #pragma unroll
     for (int i = 0; i < FS_N; i++) v[i] = 0.0;
     constexpr int CH = chunk_rows<R>(1);
-    for (int r0 = threadIdx.x * CH; r0 < g.frameH; r0 += kWG * CH) {
-        const int n = min(CH, g.frameH - r0);
+    for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
+        const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
-        it.start(g, b, r0);
-        R I[CH];
-        load_chunk<CH>(img + r0, g.vec, n, I);
+        it.start(g, b, run * g.runLen);
+        for (int r0 = run * g.runLen; r0 < rend; r0 += CH) {
+            const int n = min(CH, rend - r0);
+            R I[CH];
+            load_chunk<CH>(img + r0, g.vec, n, I);
 #pragma unroll
-        for (int q = 0; q < CH; q++) {
-            R bx, by, f1;
-            bool m;
-            it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f1, m);
-            it.next();
-            const R x = mad(kn.dx, f1, bx + kn.ax);
-            const R y = mad(kn.dy, f1, by + kn.ay);
-            const bool mm = q < n && m && in_bounds(g, x, y);
-            if (mo && q < n) mo[r0 + q] = mm ? 1 : 0;
-            Cell<R> cl;
-            const R s = sample(g, x, y, mm, cl);
-            const double wI = mm ? (double)s : 0.0, Iq = mm ? (double)I[q] : 0.0;
-            const double d = Iq - wI;
-            v[FS_D2] += d * d;
-            v[FS_CNT] += mm ? 1.0 : 0.0;
-            v[FS_SA] += wI;
-            v[FS_SAA] += wI * wI;
-            v[FS_SB] += Iq;
-            v[FS_SBB] += Iq * Iq;
-            v[FS_SAB] += wI * Iq;
+            for (int q = 0; q < CH; q++) {
+                R bx, by, f1;
+                bool m;
+                it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f1, m);
+                it.next();
+                const R x = mad(kn.dx, f1, bx + kn.ax);
+                const R y = mad(kn.dy, f1, by + kn.ay);
+                const bool mm = q < n && m && in_bounds(g, x, y);
+                if (mo && q < n) mo[r0 + q] = mm ? 1 : 0;
+                Cell<R> cl;
+                const R s = sample(g, x, y, mm, cl);
+                const double wI = mm ? (double)s : 0.0, Iq = mm ? (double)I[q] : 0.0;
+                const double d = Iq - wI;
+                v[FS_D2] += d * d;
+                v[FS_CNT] += mm ? 1.0 : 0.0;
+                v[FS_SA] += wI;
+                v[FS_SAA] += wI * wI;
+                v[FS_SB] += Iq;
+                v[FS_SBB] += Iq * Iq;
+                v[FS_SAB] += wI * Iq;
+            }
         }
     }
     __shared__ double out[FS_N];
