@@ -71,8 +71,11 @@ typedef struct {
     int32_t solver_id;                       /* arg 23: 0 ILU(0) (x-then-y order), 1/2 exact */
     int32_t filter_id;                       /* arg 24: 0 none, 1 smearing, 2 range      */
     int32_t device_id;                       /* arg 25 gpu_device_id                     */
-    char precision;                          /* 'D' or 'S' (arg 0): mask epsilon         */
-    int32_t linesearch_candidates;           /* fused candidates per round (0 = default 4) */
+    char precision;                          /* arg 0: 'D' FP64 or 'S' FP32 pixel arithmetic
+                                                (and the mask epsilon, helper/eps.h)        */
+    int32_t linesearch_candidates;           /* fused candidates in a half-iteration's first
+                                                line-search round: 1, 2 or 4 (0 = 2); later
+                                                rounds use 4.  Results do not depend on it. */
 } ss_settings;
 
 /* Fills the defaults of default_alignment_settings (steady_scan.m:836-867). */

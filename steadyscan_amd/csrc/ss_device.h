@@ -33,7 +33,7 @@ enum CorrSum {
 // Per-column sums of the final pass (FP64).
 enum FinalSum { FS_D2 = 0, FS_CNT, FS_SA, FS_SAA, FS_SB, FS_SBB, FS_SAB, FS_N };
 
-constexpr int kMaxCand = 8;  // fused line-search candidates per round
+constexpr int kMaxCand = 4;  // fused line-search candidates per round
 
 // Phases of the per-group state machine (Task<T>::alignFrames +
 // MainIteration<T>::execute).

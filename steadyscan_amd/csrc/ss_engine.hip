@@ -406,8 +406,8 @@ ss_status validate(ss_handle *h, const ss_group_in *in, const ss_settings *s) {
     if (s->precision != 'S' && s->precision != 'D')
         return fail(h, SS_ERR_INVALID_ARGUMENT, "invalid precision type! use 'S' or 'D'");
     const int K = s->linesearch_candidates <= 0 ? 2 : s->linesearch_candidates;
-    if (K != 1 && K != 2 && K != 4 && K != 8)
-        return fail(h, SS_ERR_INVALID_ARGUMENT, "linesearch_candidates must be 1, 2, 4 or 8");
+    if (K != 1 && K != 2 && K != 4)
+        return fail(h, SS_ERR_INVALID_ARGUMENT, "linesearch_candidates must be 1, 2 or 4");
     const long long npix = (long long)in->frame_h * in->n_blocks * in->n_frames;
     if (npix >= (1ll << 31)) return fail(h, SS_ERR_INVALID_ARGUMENT, "group too large");
     return SS_OK;

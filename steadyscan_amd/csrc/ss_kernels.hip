@@ -939,8 +939,7 @@ __global__ __launch_bounds__(kWG) void k_ls_round(Batch B) {
         switch (K) {
         case 1: ls_column<TI, R, 1>(g, c, b, k0); break;
         case 2: ls_column<TI, R, 2>(g, c, b, k0); break;
-        case 4: ls_column<TI, R, 4>(g, c, b, k0); break;
-        default: ls_column<TI, R, 8>(g, c, b, k0); break;
+        default: ls_column<TI, R, 4>(g, c, b, k0); break;
         }
     }
 }

@@ -82,7 +82,7 @@ def test_config0_full(flags):
     assert out[5].shape == (a["nsetframes"], 1)
 
 
-@pytest.mark.parametrize("K", [1, 4, 8])
+@pytest.mark.parametrize("K", [1, 4])
 @pytest.mark.parametrize("flags", ["D", "S"])
 def test_candidate_count_does_not_change_results(K, flags):
     # the fused rounds resolve the same sequential backtracking for any K
