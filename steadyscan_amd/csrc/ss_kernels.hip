@@ -17,6 +17,7 @@
 // across waves through LDS in FP64.  No floating-point atomics: every result
 // is run-to-run deterministic.
 #include "ss_device.h"
+#include "ss_tuning.h"
 
 #include <type_traits>
 
@@ -159,10 +160,9 @@ __device__ __forceinline__ void load_chunk(const TI *p, bool vec, int n, R (&v)[
 
 // Rows per chunk: enough independent template gathers per wave to hide
 // their latency, few enough that the unrolled chunk stays in registers
-// (about 16 samples in FP32, 8 in FP64).
+// (the kernel's *_SAMPLES tuning value).
 template <typename R>
-__host__ __device__ constexpr int chunk_rows(int samplesPerRow) {
-    const int budget = sizeof(R) == 8 ? 8 : 16;
+__host__ __device__ constexpr int chunk_rows(int samplesPerRow, int budget) {
     const int c = budget / samplesPerRow;
     return c < 1 ? 1 : (c > kCH ? kCH : c);
 }
@@ -364,7 +364,7 @@ __device__ void assemble_column(const GroupDev &g, int c) {
 #pragma unroll
     for (int i = 0; i < CS_N - 1; i++) h[i] = (R)0;
     int cnt = 0;
-    constexpr int CH = chunk_rows<R>(2);
+    constexpr int CH = chunk_rows<R>(2, SS_TUNE(ASSEMBLE, R, SAMPLES));
     for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
         const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
@@ -433,7 +433,7 @@ __device__ void assemble_column(const GroupDev &g, int c) {
 }
 
 template <typename TI, typename R>
-__global__ __launch_bounds__(kWG) void k_assemble(Batch B) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(SS_TUNE(ASSEMBLE, R, WAVES)))) void k_assemble(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         if (B.ctl[gi].phase != PH_ASSEMBLE) continue;
@@ -884,7 +884,7 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
         e2[k] = 0.0;
         cnt[k] = 0;
     }
-    constexpr int CH = chunk_rows<R>(K);
+    constexpr int CH = chunk_rows<R>(K, SS_TUNE(LS, R, SAMPLES));
     for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
         const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
@@ -926,7 +926,7 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
 }
 
 template <typename TI, typename R>
-__global__ __launch_bounds__(kWG) void k_ls_round(Batch B) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(SS_TUNE(LS, R, WAVES)))) void k_ls_round(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         const GroupCtl &ctl = B.ctl[gi];
@@ -1128,7 +1128,7 @@ __device__ void corr_column(const GroupDev &g, int c) {
     // first row outside the mask of the last-tried state: the moments cover
     // the rows before it
     int firstOut = g.frameH;
-    constexpr int CH = chunk_rows<R>(1);
+    constexpr int CH = chunk_rows<R>(1, SS_TUNE(CORR, R, SAMPLES));
     for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
         const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
@@ -1200,7 +1200,7 @@ __device__ void corr_column(const GroupDev &g, int c) {
 }
 
 template <typename TI, typename R>
-__global__ __launch_bounds__(kWG) void k_corr(Batch B) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(SS_TUNE(CORR, R, WAVES)))) void k_corr(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         if (B.ctl[gi].phase != PH_CORR) continue;
@@ -1297,7 +1297,7 @@ __device__ void final_column(const GroupDev &g, int c) {
     double v[FS_N];
 #pragma unroll
     for (int i = 0; i < FS_N; i++) v[i] = 0.0;
-    constexpr int CH = chunk_rows<R>(1);
+    constexpr int CH = chunk_rows<R>(1, SS_TUNE(FINAL, R, SAMPLES));
     for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
         const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
@@ -1336,7 +1336,7 @@ __device__ void final_column(const GroupDev &g, int c) {
 }
 
 template <typename TI, typename R>
-__global__ __launch_bounds__(kWG) void k_final(Batch B) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(SS_TUNE(FINAL, R, WAVES)))) void k_final(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         if (B.ctl[gi].phase != PH_FINAL) continue;
