@@ -30,6 +30,22 @@ sys.path.insert(0, ROOT)
 
 METRIC = "frames/s aligned at 512² + p50/p99 group latency, at ≤0.01 px RMS vs FP64"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "align_512_u16_traffic.md")
+
+
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) on this workload, from the
+    committed profile summary; None when absent."""
+    key = f"hbm_bytes_per_launch[{kernel},{precision}]:"
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.strip().startswith(key):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -337,7 +353,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": head["achieved"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(N.KERNEL_CLASSES[dom], hp),
             "bytes_per_launch": head["bytes_per_launch"],
             "avg_launch_us": head["avg_s"] * 1e6,
         },
