@@ -56,12 +56,15 @@ def parse():
     ap.add_argument("--frames", type=int, default=1000)
     ap.add_argument("--group", type=int, default=50)
     ap.add_argument("--candidates", type=int, default=2)
-    ap.add_argument("--precision", choices=["D", "S", "both"], default="both",
-                    help="pixel precision of the timed runs; the first listed is the headline")
+    ap.add_argument("--precision", choices=["D", "S", "SB", "all"], default="all",
+                    help="pixel precision of the timed runs (SB: 'S' with a BF16 template); "
+                         "all = D (the headline), S, SB")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-latency", action="store_true", help="skip the single-group latency leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
+    ap.add_argument("--uneven", action="store_true",
+                    help="also time the batched uneven-groups workload (256 groups, 8-96 frames, 256^2/512^2)")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="simulated acquisition time of the serving leg")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -156,6 +159,60 @@ def serve_leg(args, torch, ssa, W):
             "steps": steps, "max_resident_groups": resident_max, "device_busy_fraction": busy / makespan}
 
 
+def uneven_leg(args, torch, ssa, W):
+    """256 groups of U{8..96} frames, half 256x256 and half 512x512, knots
+    every 8 or 16 lines, each with its own template, all in one batched
+    submission (precision D); group 0 is checked against the FP64 oracle."""
+    t0 = time.time()
+    groups = W.config2()
+    print(f"[bench] uneven: {len(groups)} groups generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    h = ssa.Handle(0)
+    geoms = {}
+    calls, dev = [], []
+    for gr in groups:
+        T, size, lpk = gr["T"], gr["size"], gr["lines_per_knot"]
+        key = (size, lpk)
+        if key not in geoms:
+            g = ssa.init_settings(size, size, T.shape, nknots=size // lpk + 1)
+            geoms[key] = (g, h.upload_geometry(g["tfrac_bm"], g["basex_bm"], g["basey_bm"], g["base_mask_bm"]))
+        geom, gid = geoms[key]
+        xg, yg = W.gradients(T)
+        tid = h.upload_template(T, xg, yg)
+        n = gr["frames"].shape[2]
+        nk = geom["nknots"]
+        bm = ssa.image2blockmatrix(gr["frames"], geom)
+        d = torch.from_numpy(bm.view(np.int16).ravel(order="F")).to("cuda")
+        dev.append(d)
+        a = dict(flags="D", template_image=tid, xgrad=xg, ygrad=yg, tfrac=gid,
+                 basex=geom["basex_bm"], basey=geom["basey_bm"], base_mask=geom["base_mask_bm"],
+                 bmData=(d.data_ptr(), np.uint16), est_dx_knots=np.zeros((nk, n)), est_dy_knots=np.zeros((nk, n)),
+                 group_size=2 * (nk - 1), nsetframes=n,
+                 min_points_per_block=geom["min_points_per_block"].astype(np.float64))
+        a.update(W.default_args())
+        calls.append(a)
+    torch.cuda.synchronize()
+    res = ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+    t = time.perf_counter()
+    res = ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+    el = time.perf_counter() - t
+    nfr = sum(gr["frames"].shape[2] for gr in groups)
+    px = sum(gr["frames"].shape[2] * gr["size"] ** 2 for gr in groups)
+    out = {"groups": len(groups), "frames": nfr, "gigapixels": px / 1e9, "seconds": el,
+           "frames_per_s": nfr / el, "megapixels_per_s": px / el / 1e6,
+           "gn_iterations_mean": float(np.mean([r["iterations"] for r in res]))}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        gr = groups[0]
+        geom = geoms[(gr["size"], gr["lines_per_knot"])][0]
+        a = W.group_args(gr["T"], gr["frames"], geom, flags="D")
+        ref = O.align_group_to_template(a, threads=args.cpu_threads or min(16, os.cpu_count() or 1), with_mask=False)
+        n, nk = gr["frames"].shape[2], geom["nknots"]
+        dx = (res[0]["knots_x"] - ref["knots_x"]).reshape(n, nk)[:-1]
+        dy = (res[0]["knots_y"] - ref["knots_y"]).reshape(n, nk)[:-1]
+        out["group0_knot_rms_vs_fp64_px"] = float(np.sqrt(np.mean(np.concatenate([dx.ravel(), dy.ravel()]) ** 2)))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -195,7 +252,7 @@ def main():
     zeros = np.zeros((nk, G))
 
     def call(g, prec):
-        a = dict(flags=prec, template_image=tid, xgrad=xg, ygrad=yg, tfrac=gid,
+        a = dict(flags=prec[0], template_bf16=(prec == "SB"), template_image=tid, xgrad=xg, ygrad=yg, tfrac=gid,
                  basex=geom["basex_bm"], basey=geom["basey_bm"], base_mask=geom["base_mask_bm"],
                  bmData=(dev[g].data_ptr(), np.uint16), est_dx_knots=zeros, est_dy_knots=zeros,
                  group_size=2 * nb, nsetframes=G,
@@ -245,8 +302,9 @@ def main():
                     achieved=bpl / avg_s / 1e9 if launches[dom] else 0.0,
                     fps=ngroups * G * args.steps / elapsed)
 
-    precs = ["D", "S"] if args.precision == "both" else [args.precision]
+    precs = ["D", "S", "SB"] if args.precision == "all" else [args.precision]
     serving = None
+    uneven = None
     runs = {p: timed(p) for p in precs}
     head = runs[precs[0]]
     results = head["results"]
@@ -260,6 +318,10 @@ def main():
             single.append(time.perf_counter() - t)
     single_ms = np.array(single) * 1e3
 
+    if args.uneven:
+        uneven = uneven_leg(args, torch, ssa, W)
+        print(f"[bench] uneven: {uneven['frames_per_s']:.0f} frames/s, {uneven['megapixels_per_s']:.0f} Mpx/s",
+              file=sys.stderr, flush=True)
     if not args.no_serve:
         serving = serve_leg(args, torch, ssa, W)
         print(f"[bench] serving: p50 {serving['p50_group_ms']:.1f} ms p99 {serving['p99_group_ms']:.1f} ms "
@@ -275,10 +337,13 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         fr = frames[:, :, g * G:g * G + G - 1]
         m = nk * (G - 1)
+        refs = {}
         for prec in precs:
-            a = W.group_args(T, frames, geom, first=g * G, count=G, flags=prec)
+            a = W.group_args(T, frames, geom, first=g * G, count=G, flags=prec[0])
             t = time.perf_counter()
-            ref = O.align_group_to_template(a, threads=threads, with_mask=False)
+            if prec[0] not in refs:  # the oracle is FP64; the letter only sets the mask epsilon
+                refs[prec[0]] = O.align_group_to_template(a, threads=threads, with_mask=False)
+            ref = refs[prec[0]]
             tcpu = time.perf_counter() - t
             out = runs[prec]["results"][g]
             dx = (out["knots_x"] - ref["knots_x"]).reshape(G, nk)[:-1]
@@ -345,6 +410,7 @@ def main():
                                          "f32 warp/residual/accumulation") + "; f64 reductions and solve",
             "other_precisions": other,
             "serving": serving,
+            "uneven_groups": uneven,
         },
         "roofline": {
             "kernel": N.KERNEL_CLASSES[dom],

@@ -76,6 +76,9 @@ typedef struct {
     int32_t linesearch_candidates;           /* fused candidates in a half-iteration's first
                                                 line-search round: 1, 2 or 4 (0 = 2); later
                                                 rounds use 4.  Results do not depend on it. */
+    int32_t template_bf16;                   /* 1: keep the template (and gradients) as BF16
+                                                pairs, FP32 arithmetic; precision 'S' only.
+                                                0 (default): the precision's own width.    */
 } ss_settings;
 
 /* Fills the defaults of default_alignment_settings (steady_scan.m:836-867). */

@@ -39,6 +39,7 @@ class SSSettings(ctypes.Structure):
         ("device_id", c_int32),
         ("precision", c_char),
         ("linesearch_candidates", c_int32),
+        ("template_bf16", c_int32),
     ]
 
 

@@ -133,13 +133,14 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
                             correlation_increase_threshold, filter_strength, max_filter_iterations,
                             filter_deactivation_threshold, lambda_, linesearch_reduction_multiplier,
                             solver_id, filter_id, gpu_device_id, *, nargout=6, handle=None,
-                            linesearch_candidates=4, info=None):
+                            linesearch_candidates=2, template_bf16=False, info=None):
     """The mex call (MexGateway<T>::execute, mexgateway.cpp:373-445).
 
     Returns the first `nargout` of
     (block_dx_knots, block_dy_knots, frame_corr, block_iter_used, block_mask,
      residrms, deviceInitTime, iterationCorrelations, ..., allParams).
     `info`, if a dict, receives the stop criterion and work counters.
+    `template_bf16` keeps the template as BF16 (precision 'S' only).
     """
     if nargout < 1 or nargout > 15:
         raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, "invalid number of lhs arguments specified")
@@ -160,6 +161,7 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     s.device_id = int(gpu_device_id)
     s.precision = precision.encode()
     s.linesearch_candidates = int(linesearch_candidates)
+    s.template_bf16 = int(bool(template_bf16))
 
     gi = N.SSGroupIn()
     keep = []
@@ -267,6 +269,7 @@ def _prepare_call(a, h, s, gi, go, with_mask, linesearch_candidates, keep):
     s.device_id = h.device
     s.precision = precision.encode()
     s.linesearch_candidates = int(a.get("linesearch_candidates", linesearch_candidates))
+    s.template_bf16 = int(bool(a.get("template_bf16", False)))
     if isinstance(a["template_image"], int):  # resident template id
         gi.template_id = a["template_image"]
         gi.template_h, gi.template_w = np.shape(a["xgrad"])

@@ -82,6 +82,8 @@ struct GroupDev {
     // template: pairs (T[t], T[t+1]) row-major; gradient pairs unless derived
     const float2 *tp, *gxv, *gyh;
     const double2 *tpD, *gxvD, *gyhD;
+    const uint32_t *tpB, *gxvB, *gyhB;  // BF16 pairs (FP32 arithmetic only)
+    int tplBF16;          // read the template from the BF16 pairs
     int gradFD;           // gradients are forward differences of the template
     // frame data, frameH x fframeW column-contiguous
     const void *img;

@@ -30,6 +30,7 @@ struct DevTemplate {
     int th = 0, tw = 0, gradFD = 1;
     float2 *tp = nullptr, *gxv = nullptr, *gyh = nullptr;
     double2 *tpD = nullptr, *gxvD = nullptr, *gyhD = nullptr;
+    uint32_t *tpB = nullptr, *gxvB = nullptr, *gyhB = nullptr;
     void *mem = nullptr;  // owning allocation (resident only)
 };
 
@@ -48,7 +49,41 @@ struct DevGeometry {
 struct TemplateHost {
     std::vector<float2> tp, gxv, gyh;
     std::vector<double2> tpD, gxvD, gyhD;
+    std::vector<uint32_t> tpB, gxvB, gyhB;
 };
+
+// Round to nearest even BF16 (through FP32), quiet NaN kept.
+uint32_t to_bf16(double v) {
+    const float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (u >> 16) | 0x40u;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+// Template pairs as packed BF16: low half T[t], high half T[t+1].
+void template_pairs_bf16(int th, int tw, const double *t, const double *xg, const double *yg, bool fd,
+                         std::vector<uint32_t> &tp, std::vector<uint32_t> &gxv, std::vector<uint32_t> &gyh) {
+    const size_t n = (size_t)th * tw;
+    auto at = [&](const double *a, int iy, int ix) { return a[(size_t)ix * th + iy]; };
+    auto pack = [](double lo, double hi) { return to_bf16(lo) | (to_bf16(hi) << 16); };
+    tp.assign(n, 0u);
+    for (int iy = 0; iy < th; iy++)
+        for (int ix = 0; ix < tw; ix++)
+            tp[(size_t)iy * tw + ix] = pack(at(t, iy, ix), ix + 1 < tw ? at(t, iy, ix + 1) : 0.0);
+    gxv.clear();
+    gyh.clear();
+    if (fd) return;
+    gxv.assign(n, 0u);
+    gyh.assign(n, 0u);
+    for (int iy = 0; iy < th; iy++)
+        for (int ix = 0; ix < tw; ix++) {
+            const size_t o = (size_t)iy * tw + ix;
+            gxv[o] = pack(at(xg, iy, ix), iy + 1 < th ? at(xg, iy + 1, ix) : 0.0);
+            gyh[o] = pack(at(yg, iy, ix), ix + 1 < tw ? at(yg, iy, ix + 1) : 0.0);
+        }
+}
 struct GeometryHost {
     std::vector<float4> tab;
     std::vector<double4> tabD;
@@ -235,7 +270,7 @@ void template_pairs(int th, int tw, const double *t, const double *xg, const dou
 }
 
 void convert_template(int th, int tw, const double *t, const double *xg, const double *yg, bool wantF,
-                      bool wantD, TemplateHost &out, int &gradFD) {
+                      bool wantD, bool wantB, TemplateHost &out, int &gradFD) {
     auto T = [&](int iy, int ix) { return t[(size_t)ix * th + iy]; };
     auto GX = [&](int iy, int ix) { return xg[(size_t)ix * th + iy]; };
     auto GY = [&](int iy, int ix) { return yg[(size_t)ix * th + iy]; };
@@ -253,6 +288,7 @@ void convert_template(int th, int tw, const double *t, const double *xg, const d
     gradFD = fd ? 1 : 0;
     if (wantF) template_pairs<float2, float>(th, tw, t, xg, yg, fd, out.tp, out.gxv, out.gyh);
     if (wantD) template_pairs<double2, double>(th, tw, t, xg, yg, fd, out.tpD, out.gxvD, out.gyhD);
+    if (wantB) template_pairs_bf16(th, tw, t, xg, yg, fd, out.tpB, out.gxvB, out.gyhB);
 }
 
 // Geometry block matrices -> affine description (when the base coordinates
@@ -405,6 +441,10 @@ ss_status validate(ss_handle *h, const ss_group_in *in, const ss_settings *s) {
         return fail(h, SS_ERR_INVALID_ARGUMENT, "invalid filter ID\n0: disable filter\n1: smearingfiter\n2:rangefilter");
     if (s->precision != 'S' && s->precision != 'D')
         return fail(h, SS_ERR_INVALID_ARGUMENT, "invalid precision type! use 'S' or 'D'");
+    if (s->template_bf16 != 0 && s->template_bf16 != 1)
+        return fail(h, SS_ERR_INVALID_ARGUMENT, "template_bf16 must be 0 or 1");
+    if (s->template_bf16 && s->precision != 'S')
+        return fail(h, SS_ERR_INVALID_ARGUMENT, "a BF16 template needs precision 'S'");
     const int K = s->linesearch_candidates <= 0 ? 2 : s->linesearch_candidates;
     if (K != 1 && K != 2 && K != 4)
         return fail(h, SS_ERR_INVALID_ARGUMENT, "linesearch_candidates must be 1, 2 or 4");
@@ -423,7 +463,7 @@ struct Plan {
     DevGeometry geom;
     TemplateHost th;
     GeometryHost gh;
-    int fp64 = 0;
+    int fp64 = 0, bf16 = 0;
     size_t oTp = 0, oGxv = 0, oGyh = 0, oGeo = 0, oFn = 0, oImg = 0, oMin = 0;
     size_t oP = 0, oPL = 0, oP0 = 0, oD0 = 0, oDP = 0, oErr = 0, oCol = 0, oLs = 0, oCorr = 0,
            oFin = 0, oWs = 0, oUsed = 0, oAct = 0, oNext = 0, oMk = 0, oDk = 0, oMask = 0,
@@ -442,7 +482,8 @@ ss_status plan_group(ss_handle *h, Plan &P) {
     const ss_group_in &in = *P.in;
     const int fh = in.frame_h, nb = in.n_blocks, nF = in.n_frames, fw = nb * nF, N = fw + 1;
     P.fp64 = P.s->precision == 'D' ? 1 : 0;
-    const bool wF = !P.fp64, wD = P.fp64;
+    P.bf16 = P.s->template_bf16 ? 1 : 0;
+    const bool wF = !P.fp64 && !P.bf16, wD = P.fp64;
     if (in.template_id > 0) {
         auto it = h->templates.find(in.template_id);
         if (it == h->templates.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown template_id");
@@ -450,10 +491,10 @@ ss_status plan_group(ss_handle *h, Plan &P) {
     } else {
         P.tmpl.th = in.template_h;
         P.tmpl.tw = in.template_w;
-        convert_template(in.template_h, in.template_w, in.template_image, in.xgrad, in.ygrad, wF, wD,
+        convert_template(in.template_h, in.template_w, in.template_image, in.xgrad, in.ygrad, wF, wD, P.bf16,
                          P.th, P.tmpl.gradFD);
         const size_t tn = (size_t)in.template_h * in.template_w;
-        const size_t pb = P.fp64 ? sizeof(double2) : sizeof(float2);
+        const size_t pb = P.fp64 ? sizeof(double2) : P.bf16 ? sizeof(uint32_t) : sizeof(float2);
         P.oTp = A.take<char>(tn * pb);
         if (!P.tmpl.gradFD) {
             P.oGxv = A.take<char>(tn * pb);
@@ -550,6 +591,7 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     auto up = [&](size_t o, const void *src, size_t bytes) {
         return hipMemcpyAsync(dptr(o), src, bytes, hipMemcpyHostToDevice, h->stream);
     };
+    g.tplBF16 = P.bf16;
     if (in.template_id > 0) {
         g.tp = P.tmpl.tp;
         g.gxv = P.tmpl.gxv;
@@ -557,9 +599,21 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
         g.tpD = P.tmpl.tpD;
         g.gxvD = P.tmpl.gxvD;
         g.gyhD = P.tmpl.gyhD;
+        g.tpB = P.tmpl.tpB;
+        g.gxvB = P.tmpl.gxvB;
+        g.gyhB = P.tmpl.gyhB;
     } else {
         const TemplateHost &T = P.th;
-        if (P.fp64) {
+        if (P.bf16) {
+            g.tpB = (const uint32_t *)dptr(P.oTp);
+            HIPCHK(up(P.oTp, T.tpB.data(), T.tpB.size() * sizeof(uint32_t)));
+            if (!P.tmpl.gradFD) {
+                g.gxvB = (const uint32_t *)dptr(P.oGxv);
+                g.gyhB = (const uint32_t *)dptr(P.oGyh);
+                HIPCHK(up(P.oGxv, T.gxvB.data(), T.gxvB.size() * sizeof(uint32_t)));
+                HIPCHK(up(P.oGyh, T.gyhB.data(), T.gyhB.size() * sizeof(uint32_t)));
+            }
+        } else if (P.fp64) {
             g.tpD = (const double2 *)dptr(P.oTp);
             HIPCHK(up(P.oTp, T.tpD.data(), T.tpD.size() * sizeof(double2)));
             if (!P.tmpl.gradFD) {
@@ -1257,26 +1311,36 @@ ss_status ss_upload_template(ss_handle *h, int32_t th, int32_t tw, const double 
     TemplateHost T;
     dt.th = th;
     dt.tw = tw;
-    convert_template(th, tw, t, xg, yg, true, true, T, dt.gradFD);  // both precisions
+    convert_template(th, tw, t, xg, yg, true, true, true, T, dt.gradFD);  // every width
     const size_t tn = (size_t)th * tw;
-    const size_t fb = (tn * sizeof(float2) + 255) & ~(size_t)255, db = (tn * sizeof(double2) + 255) & ~(size_t)255;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t fb = al(tn * sizeof(float2)), db = al(tn * sizeof(double2)), bb = al(tn * sizeof(uint32_t));
     const int planes = dt.gradFD ? 1 : 3;
-    HIPCHK(hipMalloc(&dt.mem, planes * (fb + db)));
+    HIPCHK(hipMalloc(&dt.mem, planes * (fb + db + bb)));
     char *m = (char *)dt.mem;
+    // [planes x float2][planes x double2][planes x bf16 pairs]; plane 0 the template,
+    // planes 1, 2 the gradient pairs
+    const size_t oF = 0, oD = planes * fb, oB = planes * (fb + db);
     ss_status st = SS_OK;
-    dt.tp = (float2 *)m;
-    dt.tpD = (double2 *)(m + planes * fb);
-    st = upload_owned(h, T.tp.data(), tn * sizeof(float2), m, 0);
-    if (!st) st = upload_owned(h, T.tpD.data(), tn * sizeof(double2), m, planes * fb);
+    dt.tp = (float2 *)(m + oF);
+    dt.tpD = (double2 *)(m + oD);
+    dt.tpB = (uint32_t *)(m + oB);
+    st = upload_owned(h, T.tp.data(), tn * sizeof(float2), m, oF);
+    if (!st) st = upload_owned(h, T.tpD.data(), tn * sizeof(double2), m, oD);
+    if (!st) st = upload_owned(h, T.tpB.data(), tn * sizeof(uint32_t), m, oB);
     if (!st && !dt.gradFD) {
-        dt.gxv = (float2 *)(m + fb);
-        dt.gyh = (float2 *)(m + 2 * fb);
-        dt.gxvD = (double2 *)(m + 3 * fb + db);
-        dt.gyhD = (double2 *)(m + 3 * fb + 2 * db);
-        st = upload_owned(h, T.gxv.data(), tn * sizeof(float2), m, fb);
-        if (!st) st = upload_owned(h, T.gyh.data(), tn * sizeof(float2), m, 2 * fb);
-        if (!st) st = upload_owned(h, T.gxvD.data(), tn * sizeof(double2), m, 3 * fb + db);
-        if (!st) st = upload_owned(h, T.gyhD.data(), tn * sizeof(double2), m, 3 * fb + 2 * db);
+        dt.gxv = (float2 *)(m + oF + fb);
+        dt.gyh = (float2 *)(m + oF + 2 * fb);
+        dt.gxvD = (double2 *)(m + oD + db);
+        dt.gyhD = (double2 *)(m + oD + 2 * db);
+        dt.gxvB = (uint32_t *)(m + oB + bb);
+        dt.gyhB = (uint32_t *)(m + oB + 2 * bb);
+        st = upload_owned(h, T.gxv.data(), tn * sizeof(float2), m, oF + fb);
+        if (!st) st = upload_owned(h, T.gyh.data(), tn * sizeof(float2), m, oF + 2 * fb);
+        if (!st) st = upload_owned(h, T.gxvD.data(), tn * sizeof(double2), m, oD + db);
+        if (!st) st = upload_owned(h, T.gyhD.data(), tn * sizeof(double2), m, oD + 2 * db);
+        if (!st) st = upload_owned(h, T.gxvB.data(), tn * sizeof(uint32_t), m, oB + bb);
+        if (!st) st = upload_owned(h, T.gyhB.data(), tn * sizeof(uint32_t), m, oB + 2 * bb);
     }
     if (st) {
         (void)hipFree(dt.mem);

@@ -128,6 +128,7 @@ extern "C" void ss_settings_default(ss_settings *s) {
     s->device_id = 0;
     s->precision = 'D';
     s->linesearch_candidates = 2;
+    s->template_bf16 = 0;
 }
 
 extern "C" ss_status ss_geometry_size(const ss_geometry_desc *g, int32_t *hc, int32_t *wc,

@@ -259,6 +259,11 @@ __device__ __forceinline__ bool in_bounds(const GroupDev &g, R x, R y) {
     return x >= (R)0 && x < (R)g.xmax && y >= (R)0 && y < (R)g.ymax;
 }
 
+// Two BF16 values packed in 32 bits (low half first) as a float pair.
+__device__ __forceinline__ float2 bf16_pair(uint32_t u) {
+    return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+}
+
 // Bilinear template sample at (x, y) (task_differenceimage.cuh:13-32) from
 // the paired layout.  Branch-free: a point outside the mask reads cell 0 (the
 // caller discards the value), so a wave can issue all gathers of a chunk
@@ -276,9 +281,19 @@ __device__ __forceinline__ R sample(const GroupDev &g, R x, R y, bool valid, Cel
     cl.fx = xs - (R)ix;
     cl.fy = ys - (R)iy;
     cl.t = iy * g.tw + ix;
-    const typename Prec<R>::V2 *tp = Prec<R>::tp(g);
-    cl.top = tp[cl.t];
-    cl.bot = tp[cl.t + g.tw];
+    bool loaded = false;
+    if constexpr (sizeof(R) == 4) {
+        if (g.tplBF16) {  // BF16 pairs: low half T[t], high half T[t+1]
+            cl.top = bf16_pair(g.tpB[cl.t]);
+            cl.bot = bf16_pair(g.tpB[cl.t + g.tw]);
+            loaded = true;
+        }
+    }
+    if (!loaded) {
+        const typename Prec<R>::V2 *tp = Prec<R>::tp(g);
+        cl.top = tp[cl.t];
+        cl.bot = tp[cl.t + g.tw];
+    }
     const R a = mad(cl.fx, cl.top.y - cl.top.x, cl.top.x);
     const R b = mad(cl.fx, cl.bot.y - cl.bot.x, cl.bot.x);
     return mad(cl.fy, b - a, a);
@@ -291,7 +306,18 @@ __device__ __forceinline__ void gradients(const GroupDev &g, const Cell<R> &cl, 
     if (g.gradFD) {
         wx = mad(cl.fy, (cl.bot.y - cl.bot.x) - (cl.top.y - cl.top.x), cl.top.y - cl.top.x);
         wy = mad(cl.fx, (cl.bot.y - cl.top.y) - (cl.bot.x - cl.top.x), cl.bot.x - cl.top.x);
-    } else {
+        return;
+    }
+    bool loaded = false;
+    if constexpr (sizeof(R) == 4) {
+        if (g.tplBF16) {
+            const float2 gx = bf16_pair(g.gxvB[cl.t]), gy = bf16_pair(g.gyhB[cl.t]);
+            wx = mad(cl.fy, gx.y - gx.x, gx.x);
+            wy = mad(cl.fx, gy.y - gy.x, gy.x);
+            loaded = true;
+        }
+    }
+    if (!loaded) {
         const typename Prec<R>::V2 gx = Prec<R>::gxv(g)[cl.t], gy = Prec<R>::gyh(g)[cl.t];
         wx = mad(cl.fy, gx.y - gx.x, gx.x);
         wy = mad(cl.fx, gy.y - gy.x, gy.x);

@@ -165,3 +165,25 @@ def config4(duration=4.0, seed=400, n512=16, n1024=2, dtype=np.int16):
             first += count
     groups.sort(key=lambda g: g["arrival"])
     return streams, groups
+
+
+def config2(n_groups=256, seed=300, dtype=np.uint16):
+    """Batched uneven groups: frame counts U{8..96}, half 256x256 and half
+    512x512, each group with its own template and motion (as config1) and
+    knots every 8 or 16 lines (coin flip).  Returns a list of dicts
+    (T, frames, size, lines_per_knot)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_groups):
+        s = seed + 10 * (k + 1)
+        size = 256 if k % 2 == 0 else 512
+        n = int(rng.integers(8, 97))
+        lpk = 8 if rng.random() < 0.5 else 16
+        T = make_template(size, size, 400 * size * size // (512 * 512), (1.5, 8.0), n_background=4,
+                          background_sigma=150.0 * size / 512, target_mean=20.0, seed=s + 1)
+        amp = (float(rng.uniform(2, 8)), float(rng.uniform(2, 8)))
+        dx, dy = make_motion(n, size, 0.1, 5.6e-4 * 512 / size, amp, jump_prob=0.002, jump_size=5.0,
+                             seed=s + 2)
+        fr = make_frames(T, dx, dy, size, size, dtype, True, seed=s + 3)
+        out.append(dict(T=T, frames=fr, size=size, lines_per_knot=lpk))
+    return out

@@ -212,3 +212,24 @@ def test_config1_group_single_precision(config1_group):
     print(f"config1 [S] converged knot RMS vs FP64 oracle = {rms:.3e} px; "
           f"oracle spread under 1e-6 px init perturbation = {spread:.3e} px", flush=True)
     assert rms <= max(0.01, 2.0 * spread)
+
+
+def test_bf16_template(config1_group):
+    # int16 frames and a BF16 template with FP32 arithmetic: the error
+    # against the FP64 oracle after two iterations and at convergence is
+    # stated beside the all-FP32 run (no bar beyond the FP32 path's: BF16
+    # keeps 8 significant bits of the template)
+    T, fr, geom = config1_group
+    out = {}
+    for bf in (False, True):
+        for it in (2, 120):
+            a = W.group_args(T, fr[:, :, :20], geom, flags="S", max_iter=it)
+            a["bmData"] = np.asfortranarray(a["bmData"].astype(np.int16))
+            ref = O.align_group_to_template(a, threads=16, with_mask=False)
+            o, _ = run_gpu(a, template_bf16=bf)
+            out[(bf, it)] = knot_rms(a, o[0], o[1], ref)
+    print("BF16 template: knot RMS vs FP64 oracle (2 it / converged) = "
+          f"{out[(True, 2)]:.3e} / {out[(True, 120)]:.3e} px; all-FP32: "
+          f"{out[(False, 2)]:.3e} / {out[(False, 120)]:.3e} px", flush=True)
+    assert out[(True, 2)] < 0.05
+    assert np.isfinite(out[(True, 120)])
