@@ -845,8 +845,12 @@ size_t table_bytes(int nGroups, int nItems, int nFrames) {
 // One tick: every group advances through the stage sequence (a kernel skips
 // groups in other phases); the control blocks come back to hc.
 ss_status run_tick(ss_handle *h, const ss::LaunchBatch &L, ss::GroupCtl *hc) {
+    // three line-search rounds per half-iteration: most half-iterations
+    // finish within them, so a tick is usually one Gauss-Newton iteration
     static const int tick[] = {ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE, ss::ST_LS, ss::ST_LS_DECIDE,
+                               ss::ST_LS, ss::ST_LS_DECIDE,
                                ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE, ss::ST_LS, ss::ST_LS_DECIDE,
+                               ss::ST_LS, ss::ST_LS_DECIDE,
                                ss::ST_CORR, ss::ST_CORR_FINISH, ss::ST_FINAL, ss::ST_FINAL_FINISH};
     if (h->prof.on || !h->useGraph) {  // per-stage events need plain launches
         for (int stg : tick) HIPCHK(run_stage(h, L, stg));
