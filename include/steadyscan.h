@@ -242,6 +242,48 @@ ss_status ss_geometry_init(const ss_geometry_desc *g, int32_t *prevknotind, doub
 ss_status ss_pack_frames(const ss_geometry_desc *g, ss_dtype type, const void *frames,
                          int32_t n, void *out);
 
+/* ---- the steady_scan driver: estimate_displacements (steady_scan.m:93-288) ---- */
+
+/* Driver settings (default_alignment_settings, steady_scan.m:836-867). */
+typedef struct {
+    int32_t frames_per_set;      /* settings.frames_per_set (500)                    */
+    int32_t min_frames_per_set;  /* settings.min_frames_per_set (100); see below     */
+    double corr_thresh;          /* settings.corr_thresh (0.75)                      */
+    double minblockfrac;         /* settings.minblockfrac (0.25)                     */
+    int32_t frame_groupsize;     /* settings.gpu_frame_groupsize (2)                 */
+} ss_driver_settings;
+
+void ss_driver_settings_default(ss_driver_settings *d);
+
+/* Per-frame results of estimate_displacements; caller-owned, NULL skips an
+ * output except the knots.  Frames that are not estimated (the stack's last
+ * frame, and failed frames' corr) hold NaN, as in the reference. */
+typedef struct {
+    double *dx_knots;            /* nknots x n_frames                                */
+    double *dy_knots;            /* nknots x n_frames                                */
+    double *corr;                /* n_frames                                          */
+    double *iter_used;           /* n_frames                                          */
+    double *residrms;            /* n_frames                                          */
+    uint8_t *failed;             /* n_frames                                          */
+} ss_displacements_out;
+
+/* estimate_displacements: frames (nlines x linewidth x n_frames, raw and
+ * uncropped, the frame_list already applied) are split into sets starting
+ * every frames_per_set frames, each extended by the next set's first frame
+ * (steady_scan.m:97-99, 216; min_frames_per_set only rewrites set_size,
+ * which estimate_displacements does not read), every set is aligned against the
+ * template (template_h x template_w, column-major) and the per-frame outputs
+ * are unpacked with the reference's success tests (steady_scan.m:246-284).
+ * All sets go to the device in one batched submission.  est_dx/est_dy
+ * (nknots x n_frames) may be NULL for zero estimates; as in the reference,
+ * every set reads their first columns.  The prefilter (pregauss, mednorm) is
+ * not applied: it is a no-op at its defaults. */
+ss_status ss_estimate_displacements(ss_handle *h, const ss_geometry_desc *geometry,
+                                    const ss_driver_settings *driver, const ss_settings *s,
+                                    const double *template_image, ss_dtype type, const void *frames,
+                                    int32_t n_frames, const double *est_dx, const double *est_dy,
+                                    ss_displacements_out *out);
+
 /* ---- corrected frames: resample_frame / pushgrid (steady_scan.m:562-599) ---- */
 typedef struct {
     const ss_geometry_desc *geometry;

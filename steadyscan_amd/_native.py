@@ -69,6 +69,16 @@ class SSGroupOut(ctypes.Structure):
     ]
 
 
+class SSDriverSettings(ctypes.Structure):
+    _fields_ = [("frames_per_set", c_int32), ("min_frames_per_set", c_int32), ("corr_thresh", c_double),
+                ("minblockfrac", c_double), ("frame_groupsize", c_int32)]
+
+
+class SSDisplacementsOut(ctypes.Structure):
+    _fields_ = [("dx_knots", POINTER(c_double)), ("dy_knots", POINTER(c_double)), ("corr", POINTER(c_double)),
+                ("iter_used", POINTER(c_double)), ("residrms", POINTER(c_double)), ("failed", POINTER(c_uint8))]
+
+
 class SSServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_groups", c_int32)]
 
@@ -140,6 +150,10 @@ API = [
     ("ss_resample", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(ctypes.c_float)]),
     ("ss_profile_enable", c_int32, [c_void_p, c_int32]),
     ("ss_profile_read", c_int32, [c_void_p, POINTER(ctypes.c_int64), POINTER(c_double), POINTER(c_double)]),
+    ("ss_driver_settings_default", None, [POINTER(SSDriverSettings)]),
+    ("ss_estimate_displacements", c_int32, [c_void_p, POINTER(SSGeometryDesc), POINTER(SSDriverSettings),
+                                            POINTER(SSSettings), POINTER(c_double), c_int32, c_void_p, c_int32,
+                                            POINTER(c_double), POINTER(c_double), POINTER(SSDisplacementsOut)]),
     ("ss_server_create", c_int32, [c_void_p, POINTER(SSServerConfig), POINTER(c_void_p)]),
     ("ss_server_destroy", None, [c_void_p]),
     ("ss_server_submit", c_int32, [c_void_p, POINTER(SSGroupIn), POINTER(SSSettings), POINTER(SSGroupOut),

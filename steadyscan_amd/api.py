@@ -413,6 +413,80 @@ class Server:
             pass
 
 
+def estimate_displacements(frames, template, flags="D", nknots=33, cropping=None, subrec=None,
+                           isbidirectional=False, min_blockpoints_ratio=0.3, frames_per_set=500,
+                           min_frames_per_set=100, corr_thresh=0.75, minblockfrac=0.25, frame_groupsize=2,
+                           est_dx_knots=None, est_dy_knots=None, handle=None, device=0,
+                           linesearch_candidates=2, **align):
+    """estimate_displacements of steady_scan.m:157-288 on a frame stack
+    (nlines x linewidth x n, f64/f32/u16/i16, frame_list already applied):
+    sets of frames_per_set frames (+1 overlap) are aligned in one batched
+    submission (ss_estimate_displacements).  `align` overrides the scalar
+    alignment settings (max_iter, move_thresh, ... as in
+    workloads.default_args()).  Returns dict(dx_knots, dy_knots (nknots x n),
+    corr, iter_used, residrms, failed)."""
+    h = handle if handle is not None else _default_handle(device)
+    fr = np.asarray(frames)
+    if fr.dtype not in _DTYPES:
+        fr = fr.astype(np.float64)
+    fr = np.asfortranarray(fr)
+    if fr.ndim == 2:
+        fr = fr[:, :, None]
+    H, Wd, n = fr.shape
+    T = _f64(template)
+    g = geometry_desc(H, Wd, T.shape, nknots=nknots, cropping=cropping, subrec=subrec,
+                      isbidirectional=isbidirectional, min_blockpoints_ratio=min_blockpoints_ratio)
+    d = N.SSDriverSettings()
+    N.lib().ss_driver_settings_default(byref(d))
+    d.frames_per_set, d.min_frames_per_set = int(frames_per_set), int(min_frames_per_set)
+    d.corr_thresh, d.minblockfrac, d.frame_groupsize = float(corr_thresh), float(minblockfrac), int(frame_groupsize)
+    precision, _ = parse_flags(flags)
+    a = default_args_dict()
+    a.update(align)
+    s = default_settings()
+    s.max_iterations = int(a["max_iter"])
+    s.move_threshold = float(a["move_thresh"])
+    s.halt_correlation = float(a["halt_correlation"])
+    s.correlation_increase_threshold = float(a["correlation_increase_threshold"])
+    s.filter_strength = float(a["filter_strength"])
+    s.max_filter_iterations = int(a["max_filter_iterations"])
+    s.filter_correlation_threshold = float(a["filter_deactivation_threshold"])
+    s.lambda_ = float(a["lambda_"])
+    s.linesearch_reduction_multiplier = float(a["linesearch_reduction_multiplier"])
+    s.solver_id = int(a["solver_id"])
+    s.filter_id = int(a["filter_id"])
+    s.device_id = h.device
+    s.precision = precision.encode()
+    s.linesearch_candidates = int(linesearch_candidates)
+    out = dict(dx_knots=np.zeros((nknots, n), order="F"), dy_knots=np.zeros((nknots, n), order="F"),
+               corr=np.zeros(n), iter_used=np.zeros(n), residrms=np.zeros(n), failed=np.zeros(n, np.uint8))
+    o = N.SSDisplacementsOut()
+    o.dx_knots, o.dy_knots = _ptr(out["dx_knots"], c_double), _ptr(out["dy_knots"], c_double)
+    o.corr, o.iter_used, o.residrms = (_ptr(out["corr"], c_double), _ptr(out["iter_used"], c_double),
+                                       _ptr(out["residrms"], c_double))
+    o.failed = _ptr(out["failed"], c_uint8)
+    ex = ey = None
+    if est_dx_knots is not None and est_dy_knots is not None:
+        ex, ey = _f64(est_dx_knots), _f64(est_dy_knots)
+        if ex.shape != (nknots, n) or ey.shape != (nknots, n):
+            raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, "est knots must be nknots x n")
+    _check(N.lib().ss_estimate_displacements(h.ptr, byref(g), byref(d), byref(s), _ptr(T, c_double),
+                                             _DTYPES[fr.dtype], c_void_p(fr.ctypes.data), n,
+                                             _ptr(ex, c_double) if ex is not None else None,
+                                             _ptr(ey, c_double) if ey is not None else None, byref(o)), h)
+    out["failed"] = out["failed"].astype(bool)
+    return out
+
+
+def default_args_dict():
+    """Scalar alignment settings of default_alignment_settings
+    (steady_scan.m:195-208, 836-867) keyed by the mex argument names."""
+    return dict(max_iter=120, move_thresh=0.010, halt_correlation=0.995,
+                correlation_increase_threshold=-1.0, filter_strength=0.8,
+                max_filter_iterations=10, filter_deactivation_threshold=0.01, lambda_=0.0,
+                linesearch_reduction_multiplier=0.5, solver_id=2, filter_id=1, gpu_device_id=0)
+
+
 # ---------------------------------------------------------------------------
 # geometry and corrected frames
 # ---------------------------------------------------------------------------

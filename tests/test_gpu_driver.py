@@ -1,0 +1,55 @@
+"""ss_estimate_displacements (the steady_scan set loop, steady_scan.m:157-288)
+against the oracle's restatement of it (oracle/driver.py) on the same frame
+stacks.  Precision 'D': knots within 1e-9 px, correlations and residuals
+within 1e-9 relative, identical NaN patterns, iteration counts and failure
+flags."""
+import numpy as np
+import pytest
+
+import steadyscan_amd as ssa
+from oracle import driver as D
+from steadyscan_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def compare(r, ref):
+    for k in ("dx_knots", "dy_knots"):
+        a, b = np.asarray(r[k]), np.asarray(ref[k])
+        assert np.array_equal(np.isnan(a), np.isnan(b)), k
+        ok = ~np.isnan(a)
+        assert np.allclose(a[ok], b[ok], rtol=0, atol=1e-9), (k, np.abs(a[ok] - b[ok]).max())
+    for k in ("corr", "residrms"):
+        a, b = np.asarray(r[k]), np.asarray(ref[k])
+        assert np.array_equal(np.isnan(a), np.isnan(b)), k
+        ok = ~np.isnan(a)
+        assert np.allclose(a[ok], b[ok], rtol=1e-9, atol=1e-12), k
+    assert np.array_equal(np.nan_to_num(r["iter_used"], nan=-1), np.nan_to_num(ref["iter_used"], nan=-1))
+    assert np.array_equal(np.asarray(r["failed"], bool), np.asarray(ref["failed"], bool))
+
+
+@pytest.mark.parametrize("case", ["sets", "subrec", "failures", "crop_u16"])
+def test_driver_equals_oracle(case):
+    args = dict(W.default_args(), max_iter=12)
+    kw = dict(nknots=9, frames_per_set=16)
+    if case == "sets":
+        T, fr, geom, _ = W.config0(n_frames=40, size=64, seed=61)
+    elif case == "subrec":
+        # the raster starts 7 lines above the template: the first block of
+        # every frame maps outside it and the knots are extended
+        T, fr, geom, _ = W.config0(n_frames=20, size=64, seed=62)
+        kw.update(subrec=(1.0, 64.0, -7.0, 56.0))
+    elif case == "failures":
+        T, fr, geom, _ = W.config0(n_frames=20, size=64, seed=63)
+        kw.update(corr_thresh=0.999)
+    else:
+        T, fr, geom, _ = W.config0(n_frames=24, size=64, seed=64)
+        fr = np.asfortranarray(np.rint(fr).astype(np.uint16))
+        kw.update(cropping=(2, 3, 4, 0))
+    ref = D.estimate_displacements(fr, T, dict(args, flags="D"), threads=8, **kw)
+    r = ssa.estimate_displacements(fr, T, flags="D", **kw, **args)
+    if case == "subrec":
+        assert np.any(ref["dx_knots"][0, :-1] == ref["dx_knots"][1, :-1])  # extension happened
+    if case == "failures":
+        assert ref["failed"][:-1].any()
+    compare(r, ref)
