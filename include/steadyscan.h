@@ -284,6 +284,18 @@ ss_status ss_estimate_displacements(ss_handle *h, const ss_geometry_desc *geomet
                                     int32_t n_frames, const double *est_dx, const double *est_dy,
                                     ss_displacements_out *out);
 
+/* ---- call capture and replay (the 'F' flag's ROM file) ----
+ * Replaces MexSimulator<T>::saveRomToFile / loadRomFromFile
+ * (mex/mexsimulator/mexsimulator.cpp:52-98), which dump the host struct
+ * image; this format is self-describing and little-endian (header "SSROM001",
+ * byte-order mark, tagged column-major arrays, settings field by field). */
+typedef struct ss_rom ss_rom;
+/* Writes the arrays and settings of one call (host arrays only). */
+ss_status ss_rom_write(const char *path, const ss_group_in *in, const ss_settings *s);
+/* Reads a ROM; on success in/s point into *rom, valid until ss_rom_free. */
+ss_status ss_rom_read(const char *path, ss_rom **rom, ss_group_in *in, ss_settings *s);
+void ss_rom_free(ss_rom *rom);
+
 /* ---- corrected frames: resample_frame / pushgrid (steady_scan.m:562-599) ---- */
 typedef struct {
     const ss_geometry_desc *geometry;

@@ -5,8 +5,9 @@ include/steadyscan.h); this package is its host-side mirror of the
 reference's mex interface.  Importing the package does not touch the GPU.
 """
 from .api import (Handle, Server, SteadyScanError, align_group_to_template, align_groups,  # noqa: F401
-                  default_settings, estimate_displacements, geometry_desc, image2blockmatrix, init_settings, parse_flags,
-                  resample_frames)
+                  default_settings, estimate_displacements, geometry_desc, image2blockmatrix,
+                  init_settings, load_rom, parse_flags, resample_frames, save_rom)
 
-__all__ = ["Handle", "Server", "SteadyScanError", "align_group_to_template", "align_groups", "default_settings", "estimate_displacements",
-           "geometry_desc", "image2blockmatrix", "init_settings", "parse_flags", "resample_frames"]
+__all__ = ["Handle", "Server", "SteadyScanError", "align_group_to_template", "align_groups",
+           "default_settings", "estimate_displacements", "geometry_desc", "image2blockmatrix",
+           "init_settings", "load_rom", "parse_flags", "resample_frames", "save_rom"]

@@ -154,6 +154,9 @@ API = [
     ("ss_estimate_displacements", c_int32, [c_void_p, POINTER(SSGeometryDesc), POINTER(SSDriverSettings),
                                             POINTER(SSSettings), POINTER(c_double), c_int32, c_void_p, c_int32,
                                             POINTER(c_double), POINTER(c_double), POINTER(SSDisplacementsOut)]),
+    ("ss_rom_write", c_int32, [c_char_p, POINTER(SSGroupIn), POINTER(SSSettings)]),
+    ("ss_rom_read", c_int32, [c_char_p, POINTER(c_void_p), POINTER(SSGroupIn), POINTER(SSSettings)]),
+    ("ss_rom_free", None, [c_void_p]),
     ("ss_server_create", c_int32, [c_void_p, POINTER(SSServerConfig), POINTER(c_void_p)]),
     ("ss_server_destroy", None, [c_void_p]),
     ("ss_server_submit", c_int32, [c_void_p, POINTER(SSGroupIn), POINTER(SSSettings), POINTER(SSGroupOut),

@@ -100,6 +100,58 @@ def _default_handle(device):
     return h
 
 
+ROM_FILE = "steadyscan_rom.bin"  # where the 'F' flag saves a call
+
+
+def save_rom(path, call):
+    """Writes one call (a dict of the mex arguments with host arrays) to a
+    ROM file (ss_rom_write)."""
+    h_dummy = type("H", (), {"device": int(call.get("gpu_device_id", 0))})()
+    gi, s, go = N.SSGroupIn(), N.SSSettings(), N.SSGroupOut()
+    keep = []
+    _prepare_call(call, h_dummy, s, gi, go, False, call.get("linesearch_candidates", 2), keep)
+    rc = N.lib().ss_rom_write(str(path).encode(), byref(gi), byref(s))
+    if rc:
+        raise SteadyScanError(rc, "ss_rom_write failed (host arrays needed)")
+
+
+def load_rom(path):
+    """Reads a ROM file back into a call dict (arrays copied out)."""
+    rom = c_void_p()
+    gi, s = N.SSGroupIn(), N.SSSettings()
+    rc = N.lib().ss_rom_read(str(path).encode(), byref(rom), byref(gi), byref(s))
+    if rc:
+        raise SteadyScanError(rc, f"ss_rom_read failed for {path}")
+    try:
+        fh, nb, nF, th, tw = gi.frame_h, gi.n_blocks, gi.n_frames, gi.template_h, gi.template_w
+
+        def arr(p, shape, dt):
+            n = int(np.prod(shape))
+            return np.ctypeslib.as_array(ctypes.cast(p, POINTER(dt)), (n,)).copy().reshape(shape, order="F")
+
+        dtype = {N.SS_F64: (np.float64, c_double), N.SS_F32: (np.float32, c_float),
+                 N.SS_U16: (np.uint16, ctypes.c_uint16), N.SS_I16: (np.int16, ctypes.c_int16)}[gi.data_type]
+        call = dict(
+            flags=s.precision.decode(), template_image=arr(gi.template_image, (th, tw), c_double),
+            xgrad=arr(gi.xgrad, (th, tw), c_double), ygrad=arr(gi.ygrad, (th, tw), c_double),
+            tfrac=arr(gi.frac, (fh, nb), c_double), basex=arr(gi.base_x, (fh, nb), c_double),
+            basey=arr(gi.base_y, (fh, nb), c_double), base_mask=arr(gi.base_mask, (fh, nb), c_uint8).astype(bool),
+            bmData=arr(gi.data, (fh, nb * nF), dtype[1]).astype(dtype[0]),
+            est_dx_knots=arr(gi.init_x, (nb + 1, nF), c_double), est_dy_knots=arr(gi.init_y, (nb + 1, nF), c_double),
+            group_size=gi.group_size, nsetframes=nF,
+            min_points_per_block=arr(gi.min_points, (nb + 1,), c_int32).astype(np.float64),
+            max_iter=s.max_iterations, move_thresh=s.move_threshold, halt_correlation=s.halt_correlation,
+            correlation_increase_threshold=s.correlation_increase_threshold, filter_strength=s.filter_strength,
+            max_filter_iterations=s.max_filter_iterations,
+            filter_deactivation_threshold=s.filter_correlation_threshold, lambda_=s.lambda_,
+            linesearch_reduction_multiplier=s.linesearch_reduction_multiplier, solver_id=s.solver_id,
+            filter_id=s.filter_id, gpu_device_id=s.device_id, linesearch_candidates=s.linesearch_candidates,
+            template_bf16=bool(s.template_bf16))
+    finally:
+        N.lib().ss_rom_free(rom)
+    return call
+
+
 def default_settings():
     s = N.SSSettings()
     N.lib().ss_settings_default(byref(s))
@@ -232,6 +284,9 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     allp = np.zeros((2 * (nb * nF + 1), int(max_iter)), order="F") if nargout >= 15 else None
     if allp is not None:
         go.all_params = _ptr(allp, c_double)
+    if "F" in opts:  # save the call's arguments (the 'F' flag, configuration.h:10-28)
+        if N.lib().ss_rom_write(ROM_FILE.encode(), byref(gi), byref(s)) != 0:
+            raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, f"could not write {ROM_FILE}")
     _check(N.lib().ss_align_group(h.ptr, byref(gi), byref(s), byref(go)), h)
     del keep
     if info is not None:

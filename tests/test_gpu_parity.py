@@ -233,3 +233,16 @@ def test_bf16_template(config1_group):
           f"{out[(False, 2)]:.3e} / {out[(False, 120)]:.3e} px", flush=True)
     assert out[(True, 2)] < 0.05
     assert np.isfinite(out[(True, 120)])
+
+
+def test_rom_replay_gives_the_same_result(tmp_path):
+    # a call saved with the 'F' flag's format and replayed from the file
+    T, fr, geom, _ = W.config0(n_frames=8, size=48, seed=29)
+    a = W.group_args(T, fr, geom, max_iter=8)
+    out, _ = run_gpu(a)
+    p = tmp_path / "call.rom"
+    ssa.save_rom(p, a)
+    b = ssa.load_rom(p)
+    rep, _ = run_gpu(b)
+    assert np.array_equal(out[0], rep[0]) and np.array_equal(out[1], rep[1])
+    assert int(out[3]) == int(rep[3])

@@ -1,0 +1,71 @@
+"""ROM capture/replay format (host only; no GPU needed)."""
+import struct
+
+import numpy as np
+import pytest
+
+import steadyscan_amd as ssa
+from steadyscan_amd import workloads as W
+
+
+@pytest.fixture
+def call():
+    T, fr, geom, _ = W.config0(n_frames=5, size=32, seed=3)
+    a = W.group_args(T, fr, geom, max_iter=7, flags="S", lambda_=0.5)
+    a["bmData"] = np.asfortranarray(np.rint(a["bmData"]).astype(np.uint16))
+    a["est_dx_knots"] = np.asfortranarray(np.arange(a["est_dx_knots"].size, dtype=float).reshape(a["est_dx_knots"].shape))
+    return a
+
+
+def test_round_trip_is_exact(call, tmp_path):
+    p = tmp_path / "call.rom"
+    ssa.save_rom(p, call)
+    back = ssa.load_rom(p)
+    for k in ("template_image", "xgrad", "ygrad", "tfrac", "basex", "basey", "bmData", "est_dx_knots", "est_dy_knots"):
+        a, b = np.asarray(call[k]), back[k]
+        assert a.shape == b.shape and a.dtype == b.dtype, k
+        assert np.array_equal(np.nan_to_num(a, nan=7.5), np.nan_to_num(b, nan=7.5)), k
+        assert np.array_equal(np.isnan(a.astype(float)), np.isnan(b.astype(float))), k
+    assert np.array_equal(np.asarray(call["base_mask"], bool), back["base_mask"])
+    assert np.array_equal(np.trunc(call["min_points_per_block"]), back["min_points_per_block"])
+    for k in ("max_iter", "move_thresh", "halt_correlation", "correlation_increase_threshold", "filter_strength",
+              "max_filter_iterations", "filter_deactivation_threshold", "lambda_",
+              "linesearch_reduction_multiplier", "solver_id", "filter_id", "group_size", "nsetframes"):
+        assert back[k] == call[k], k
+    assert back["flags"] == "S"
+
+
+def test_format_header_and_records(call, tmp_path):
+    p = tmp_path / "call.rom"
+    ssa.save_rom(p, call)
+    raw = p.read_bytes()
+    assert raw[:8] == b"SSROM001"
+    bom, count = struct.unpack("<II", raw[8:16])
+    assert bom == 0x01020304 and count == 28
+    tags, off = [], 16
+    for _ in range(count):
+        tag = raw[off:off + 4].decode()
+        code, rows, cols = struct.unpack("<III", raw[off + 4:off + 16])
+        size = {1: 8, 2: 4, 3: 2, 4: 2, 5: 1, 6: 4}[code] * rows * cols
+        if tag == "TMPL":
+            t = np.frombuffer(raw[off + 16:off + 16 + size], "<f8").reshape(rows, cols, order="F")
+            assert np.array_equal(t, call["template_image"])
+        if tag == "DATA":
+            assert code == 3 and (rows, cols) == call["bmData"].shape
+        tags.append(tag)
+        off += 16 + size
+    assert off == len(raw)
+    assert tags[0] == "PREC" and "DATA" in tags and "MINP" in tags
+
+
+def test_corrupt_file_is_rejected(call, tmp_path):
+    p = tmp_path / "call.rom"
+    ssa.save_rom(p, call)
+    raw = bytearray(p.read_bytes())
+    raw[3] = ord("X")
+    p.write_bytes(bytes(raw))
+    with pytest.raises(ssa.SteadyScanError):
+        ssa.load_rom(p)
+    p.write_bytes(bytes(p.read_bytes()[:100]))
+    with pytest.raises(ssa.SteadyScanError):
+        ssa.load_rom(p)
