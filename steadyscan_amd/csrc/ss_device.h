@@ -66,6 +66,7 @@ struct GroupDev {
     double xmax, ymax;    // template_w - 1 - eps, template_h - 1 - eps
     int fp64;             // pixel arithmetic in FP64 (precision 'D') or FP32 ('S')
     double rmult, thr, halt, cit, fct, strength, lambda;
+    int solverId;         // 0: ILU(0); 1, 2: exact
     int maxIter, K1, Kn, maxRounds;
     // geometry: affine fast path or per-pixel table
     int affine, L;

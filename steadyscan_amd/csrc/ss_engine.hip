@@ -434,8 +434,6 @@ ss_status validate(ss_handle *h, const ss_group_in *in, const ss_settings *s) {
     if (s->max_iterations < 1) return fail(h, SS_ERR_INVALID_ARGUMENT, "max_iterations must be >= 1");
     if (s->solver_id < 0 || s->solver_id > 2)  // task.cpp:74-88
         return fail(h, SS_ERR_INVALID_ARGUMENT, "invalid solver id\n0:CuSparse solver\n1:CuSolver solver\n2:Eigen solver");
-    if (s->solver_id == 0)
-        return fail(h, SS_ERR_NOT_AVAILABLE, "solver_id 0 (ILU(0)) is not available in this build");
     const int fsel = s->max_filter_iterations != 0 ? s->filter_id : 0;  // task.cpp:54
     if (fsel < 0 || fsel > 2)
         return fail(h, SS_ERR_INVALID_ARGUMENT, "invalid filter ID\n0: disable filter\n1: smearingfiter\n2:rangefilter");
@@ -583,6 +581,7 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     g.fct = s.filter_correlation_threshold;
     g.strength = s.filter_strength;
     g.lambda = s.lambda;
+    g.solverId = s.solver_id;
     g.maxIter = s.max_iterations;
     g.K1 = s.linesearch_candidates <= 0 ? 2 : s.linesearch_candidates;
     g.Kn = std::max(g.K1, 4);

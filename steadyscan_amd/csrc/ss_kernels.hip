@@ -527,6 +527,111 @@ struct InteriorRow {  // forward-eliminated interior row: Bh^{-1} and rhs
     Aff R;
 };
 
+// Solver id 0 (SolverCuSparse::solve, cusparsesolver.cu:41-174): incomplete
+// LU with zero fill of the normal equations ordered [x_0..x_{n-1},
+// y_0..y_{n-1}] (the compacted COO order; unused knots are decoupled
+// identity rows and change nothing), then the unit-lower and upper
+// triangular solves.  With this ordering every row of the pattern is known:
+// x_i: {x_{i-1}, x_i, x_{i+1}, y_{i-1}, y_i, y_{i+1}}, y_i likewise, so the
+// elimination reduces to the recurrences below, applied in column order
+// exactly as a row-wise ILU(0) does (separately rounded products).
+struct IluRow {
+    double lx;                        // x row: l(x_i, x_{i-1})
+    double uxx, uxn, uxyp, uxy, uxyn; // u(x_i, x_i), u(x_i, x_{i+1}), u(x_i, y_{i-1}), u(x_i, y_i), u(x_i, y_{i+1})
+    double l1, l2, l3, l4;            // y row: l(y_i, x_{i-1}), l(y_i, x_i), l(y_i, x_{i+1}), l(y_i, y_{i-1})
+    double uyy, uyn;                  // u(y_i, y_i), u(y_i, y_{i+1})
+};
+static_assert(sizeof(IluRow) <= sizeof(InteriorRow), "ILU factors reuse the interior-row workspace");
+
+__device__ void ilu0_solve(const Eq0 *E, IluRow *F, double2 *X, int n) {
+#pragma clang fp contract(off)
+    // x rows: the only lower entry is x_{i-1}
+    for (int i = 0; i < n; i++) {
+        const Eq0 q = E[i];
+        IluRow r;
+        r.uxx = q.b0;
+        r.uxn = q.c0;
+        r.uxy = q.b1;
+        r.uxyn = q.c1;
+        r.uxyp = 0.0;
+        r.lx = 0.0;
+        if (i > 0) {
+            const Eq0 qp = E[i - 1];
+            const IluRow &p = F[i - 1];
+            r.uxyp = qp.c1;
+            r.lx = qp.c0 / p.uxx;
+            r.uxx = r.uxx - r.lx * p.uxn;
+            r.uxyp = r.uxyp - r.lx * p.uxy;
+            r.uxy = r.uxy - r.lx * p.uxyn;
+        }
+        F[i] = r;
+    }
+    // y rows: lower entries x_{i-1}, x_i, x_{i+1}, y_{i-1}
+    for (int i = 0; i < n; i++) {
+        const Eq0 q = E[i];
+        IluRow &r = F[i];
+        double ayxp = 0.0, ayx = q.b1, ayxn = q.c1, ayyp = 0.0, ayy = q.b2, ayyn = q.c2;
+        r.l1 = r.l3 = r.l4 = 0.0;
+        if (i > 0) {
+            const IluRow &p = F[i - 1];
+            ayxp = E[i - 1].c1;
+            ayyp = E[i - 1].c2;
+            r.l1 = ayxp / p.uxx;
+            ayx = ayx - r.l1 * p.uxn;
+            ayyp = ayyp - r.l1 * p.uxy;
+            ayy = ayy - r.l1 * p.uxyn;
+        }
+        r.l2 = ayx / r.uxx;
+        ayxn = ayxn - r.l2 * r.uxn;
+        ayyp = ayyp - r.l2 * r.uxyp;
+        ayy = ayy - r.l2 * r.uxy;
+        ayyn = ayyn - r.l2 * r.uxyn;
+        if (i + 1 < n) {
+            const IluRow &nx = F[i + 1];
+            r.l3 = ayxn / nx.uxx;
+            ayy = ayy - r.l3 * nx.uxyp;
+            ayyn = ayyn - r.l3 * nx.uxy;
+        }
+        if (i > 0) {
+            const IluRow &p = F[i - 1];
+            r.l4 = ayyp / p.uyy;
+            ayy = ayy - r.l4 * p.uyn;
+        }
+        r.uyy = ayy;
+        r.uyn = ayyn;
+    }
+    // forward: z = L^{-1} g (z kept in X)
+    for (int i = 0; i < n; i++) {
+        double zx = E[i].d0;
+        if (i > 0) zx = zx - F[i].lx * X[i - 1].x;
+        X[i].x = zx;
+    }
+    for (int i = 0; i < n; i++) {
+        const IluRow &r = F[i];
+        double zy = E[i].d1;
+        if (i > 0) zy = zy - r.l1 * X[i - 1].x;
+        zy = zy - r.l2 * X[i].x;
+        if (i + 1 < n) zy = zy - r.l3 * X[i + 1].x;
+        if (i > 0) zy = zy - r.l4 * X[i - 1].y;
+        X[i].y = zy;
+    }
+    // backward: y rows from the last, then x rows
+    for (int i = n - 1; i >= 0; i--) {
+        double s = X[i].y;
+        if (i + 1 < n) s = s - F[i].uyn * X[i + 1].y;
+        X[i].y = s / F[i].uyy;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        const IluRow &r = F[i];
+        double s = X[i].x;
+        if (i + 1 < n) s = s - r.uxn * X[i + 1].x;
+        if (i > 0) s = s - r.uxyp * X[i - 1].y;
+        s = s - r.uxy * X[i].y;
+        if (i + 1 < n) s = s - r.uxyn * X[i + 1].y;
+        X[i].x = s / r.uxx;
+    }
+}
+
 __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
     const int gi = blockIdx.x;
     GroupCtl &ctl = B.ctl[gi];
@@ -608,15 +713,22 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
     }
     __threadfence_block();
     __syncthreads();
+    InteriorRow *IR = reinterpret_cast<InteriorRow *>(g.ws + (size_t)n * 8);
+    double2 *X = reinterpret_cast<double2 *>(IR + n);
+    Aff *firstAff = reinterpret_cast<Aff *>(X + n);
+    if (g.solverId == 0) {
+        // 4a. solver id 0: ILU(0) in x-then-y order (one thread, the
+        //     reference's recurrence order)
+        if (tid == 0) ilu0_solve(E, reinterpret_cast<IluRow *>(IR), X, n);
+        __threadfence_block();
+        __syncthreads();
+    } else {
     // 4. partitioned solve: thread t owns knots [s, e); interior rows s..e-2
     //    are eliminated in terms of the interfaces z_{t-1} = x_{s-1} and
     //    z_t = x_{e-1}; the interface system is solved by cyclic reduction.
     const int m = (n + kSolveWG - 1) / kSolveWG;
     const int P = (n + m - 1) / m;
     const int s0 = tid * m, e0 = min(n, s0 + m);
-    InteriorRow *IR = reinterpret_cast<InteriorRow *>(g.ws + (size_t)n * 8);
-    double2 *X = reinterpret_cast<double2 *>(IR + n);
-    Aff *firstAff = reinterpret_cast<Aff *>(X + n);
     __shared__ RedEq red[kSolveWG];
     __shared__ int hasInt[kSolveWG];
     Aff lastA;  // last interior row as a function of (z_prev, z_own)
@@ -767,6 +879,7 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
     }
     __threadfence_block();
     __syncthreads();
+    }
     // 5. step filter over the used knots, x sequence then y sequence
     //    (SmearingFilter::apply / RangeFilter::apply on the compacted halves)
     const int filterKind = s_filter;

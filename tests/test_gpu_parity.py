@@ -246,3 +246,17 @@ def test_rom_replay_gives_the_same_result(tmp_path):
     rep, _ = run_gpu(b)
     assert np.array_equal(out[0], rep[0]) and np.array_equal(out[1], rep[1])
     assert int(out[3]) == int(rep[3])
+
+
+@pytest.mark.parametrize("flags", ["D", "S"])
+def test_ilu0_solver(flags):
+    # solver id 0: ILU(0) of the x-then-y ordered normal equations (an
+    # approximate solve; the trajectory differs from the exact solvers')
+    T, fr, geom, _ = W.config0(n_frames=12, size=64, seed=37)
+    a = W.group_args(T, fr, geom, max_iter=15, flags=flags, solver_id=0)
+    ref = O.align_group_to_template(a, threads=8)
+    out, info = run_gpu(a)
+    print(f"ILU(0) iterations gpu={int(out[3])} oracle={ref['iterations']}")
+    check(a, ref, out[0], out[1], out[5], out[2], out[4], label="ILU(0)")
+    exact = O.align_group_to_template(W.group_args(T, fr, geom, max_iter=15, flags=flags), threads=8)
+    assert not np.array_equal(ref["knots_x"], exact["knots_x"])  # a different solver was run
