@@ -87,8 +87,26 @@ def lib():
         L.orc_geometry.restype = c_int32
         L.orc_image2blockmatrix.argtypes = [PD, POINTER(c_int32), c_int32, c_int32, PD, POINTER(c_uint8)]
         L.orc_image2blockmatrix.restype = c_int32
+        L.orc_invert_bilinear_sampling.argtypes = [PD, c_int32, c_int32, c_int32, PD, PD, c_int32,
+                                                   POINTER(c_int32), PD, PD, PD, c_int32, c_int32, c_double,
+                                                   c_double, POINTER(ctypes.c_float), POINTER(c_uint8),
+                                                   POINTER(c_int32)]
+        L.orc_invert_bilinear_sampling.restype = c_int32
+        L.orc_ibs_neighbour.argtypes = [ctypes.c_long, c_int32, ctypes.c_long, ctypes.c_long,
+                                        POINTER(ctypes.c_long)]
+        L.orc_ibs_neighbour.restype = c_int32
         L.orc_resample_frame.argtypes = [PD, c_int32, PD, PD, PD, PD, POINTER(c_int32), PD, c_int32,
                                          c_int32, POINTER(ctypes.c_float)]
+        PF = POINTER(ctypes.c_float)
+        L.orc_ibs_system.argtypes = [PD, c_int32, PD, PD, POINTER(c_int32), PD, PD, PD, c_int32, c_int32,
+                                     c_double, c_double, PD, PD, PF, POINTER(c_uint8)]
+        L.orc_ibs_system.restype = c_int32
+        L.orc_ibs_energy.argtypes = [PD, PD, PF, POINTER(c_uint8), c_int32, c_int32]
+        L.orc_ibs_energy.restype = ctypes.c_float
+        L.orc_ibs_interactions.argtypes = [PD, PD, POINTER(c_uint8), c_int32, c_int32, PF, PF]
+        L.orc_ibs_sweep.argtypes = [PF, PF, c_int32, c_int32, PF]
+        L.orc_ibs_check_interval.argtypes = [c_int32]
+        L.orc_ibs_check_interval.restype = c_int32
     return _lib
 
 
@@ -224,3 +242,30 @@ class State:
 
     def global_correlation(self):
         return lib().orc_global_correlation(self.ptr)
+
+
+def invert_bilinear_sampling(frames, dx_knots, dy_knots, geometry, template_shape, regfac=3e-3, aspect=1.0):
+    """Oracle of invert_bilinear_sampling (steady_scan.m:290-404) for one call.
+    frames: Hc x Wc x n cropped frames; knots: nknots x n; geometry: dict with
+    image-shaped prevknotind, tfrac, basex, basey (as init_settings gives).
+    Returns (corrected th x tw x n float32 with NaN outside the mask, mask, iterations)."""
+    fr = np.asfortranarray(np.asarray(frames, dtype=np.float64))
+    if fr.ndim == 2:
+        fr = fr[:, :, None]
+    Hc, Wc, n = fr.shape
+    th, tw = template_shape
+    dxk = np.asfortranarray(np.asarray(dx_knots, dtype=np.float64).reshape(-1, n, order="F"))
+    dyk = np.asfortranarray(np.asarray(dy_knots, dtype=np.float64).reshape(-1, n, order="F"))
+    pk = np.asfortranarray(np.asarray(geometry["prevknotind"], dtype=np.int32))
+    tf, bx, by = (np.asfortranarray(np.asarray(geometry[k], dtype=np.float64)) for k in ("tfrac", "basex", "basey"))
+    out = np.zeros((th, tw, n), dtype=np.float32, order="F")
+    mask = np.zeros((th, tw, n), dtype=np.uint8, order="F")
+    iters = np.zeros(n, dtype=np.int32)
+    rc = lib().orc_invert_bilinear_sampling(_p(fr), Hc, Wc, n, _p(dxk), _p(dyk), dxk.shape[0],
+                                            pk.ctypes.data_as(POINTER(c_int32)), _p(tf), _p(bx), _p(by), th, tw,
+                                            float(regfac), float(aspect),
+                                            out.ctypes.data_as(POINTER(ctypes.c_float)),
+                                            mask.ctypes.data_as(POINTER(c_uint8)), iters.ctypes.data_as(POINTER(c_int32)))
+    if rc:
+        raise RuntimeError(f"oracle returned status {rc}")
+    return out, mask.astype(bool), iters

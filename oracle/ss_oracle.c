@@ -22,6 +22,14 @@
  *   - the line-search movement test abs() is the floating-point magnitude.
  * Eigen::SimplicialLDLT is replaced by an exact banded LDL^T of the same
  * matrix in interleaved (x_k, y_k) order (exact solves agree to rounding).
+ *
+ * The file also restates invert_bilinear_sampling (steady_scan.m:290-404)
+ * with its helpers.  Its summation orders are fixed here where MATLAB's
+ * sparse() and BLAS leave them open: constraint sums over data pixels in
+ * ascending index; the stopping energy per template row, then over rows;
+ * (Q x)_i over neighbours in ascending index.  Those rounding-level orders
+ * are parity unpinned against the reference (no MATLAB here); everything
+ * else is pinned by the dense-definition tests.
  */
 #include "ss_oracle.h"
 
@@ -1169,4 +1177,289 @@ void orc_resample_frame(const double *data, int npix, const double *dx_knots,
     }
     for (size_t t = 0; t < tp; t++) out[t] = z[t] / nn[t];
     free(z); free(nn); free(zo); free(sel); free(xf); free(yf); free(mi);
+}
+
+/* ------------------------------------------------------------------ */
+/* invert_bilinear_sampling (steady_scan.m:290-404)                     */
+/* ------------------------------------------------------------------ */
+/* Neighbour slots of a template pixel, in the order of calc_indices'
+ * find_* functions (steady_scan.m:645-723): N NE E SE S SW W NW. */
+enum { IB_N, IB_NE, IB_E, IB_SE, IB_S, IB_SW, IB_W, IB_NW, IB_NB };
+
+/* 0-based neighbour index and validity of pixel i (column-major, nrows rows,
+ * npix pixels) as find_* computes them on 1-based indices.  find_SE's
+ * "I < npixels" drops the one legitimate SE neighbour equal to the last
+ * pixel; find_NW has no top-row test (its wrapped neighbour is never
+ * coupled in Q, so its coefficient is zero either way). */
+int orc_ibs_neighbour(long i0, int dir, long nrows, long npix, long *j0) {
+    long i = i0 + 1, I = 1;
+    int M = 0;
+    switch (dir) {
+    case IB_N: I = i - 1; M = (I % nrows) != 0; break;
+    case IB_NE: I = i + nrows - 1; M = (I % nrows) != 0 && I < npix; break;
+    case IB_E: I = i + nrows; M = I <= npix; break;
+    case IB_SE: I = i + nrows + 1; M = (i % nrows) != 0 && I < npix; break;
+    case IB_S: I = i + 1; M = (i % nrows) != 0; break;
+    case IB_SW: I = i - nrows + 1; M = (i % nrows) != 0 && I > 0; break;
+    case IB_W: I = i - nrows; M = I > 0; break;
+    default: I = i - nrows - 1; M = I > 0; break;
+    }
+    if (!M) I = 1;
+    *j0 = I - 1;
+    return M;
+}
+
+/* Offset of neighbour slot dir as (drow, dcol). */
+static void ib_offset(int dir, int *dr, int *dc) {
+    static const int r[IB_NB] = {-1, -1, 0, 1, 1, 1, 0, -1};
+    static const int c[IB_NB] = {0, 1, 1, 1, 0, -1, -1, -1};
+    *dr = r[dir];
+    *dc = c[dir];
+}
+
+static int ib_dir(int dr, int dc) {
+    for (int d = 0; d < IB_NB; d++) {
+        int a, b;
+        ib_offset(d, &a, &b);
+        if (a == dr && b == dc) return d;
+    }
+    return -1;
+}
+
+/* Q as a 9-point stencil per pixel: q[0] the diagonal, q[1 + dir] the
+ * coupling to the neighbour in slot dir. */
+#define IBQ(p, k) q[(size_t)(p) * 9 + (k)]
+
+/* One frame's masked system (steady_scan.m:327-356).  qm (np x 9) is the
+ * stencil of Q = ww + regmat with the rows and columns outside the mask
+ * zeroed; wy (np) the constraint right-hand side, not yet masked; iv (np)
+ * the resampled frame (resample_frame); mask (np).  Returns whether any
+ * pixel had positive summed coefficients (the reference skips the frame
+ * otherwise).  ww and wy are sums over the data pixels in ascending index
+ * order; MATLAB's sparse() fixes no order, so that rounding is unpinned. */
+int orc_ibs_system(const double *data, int npd_, const double *dx, const double *dy,
+                   const int32_t *prevknotind, const double *tfrac, const double *basex,
+                   const double *basey, int th, int tw, double regfac, double aspect, double *qm,
+                   double *wy, float *iv, uint8_t *mask) {
+    const long npd = npd_, np = (long)th * tw;
+    const double eps = DBL_EPSILON, vfac = 1.0 / aspect, v2 = vfac * vfac;
+    double *q = (double *)xcalloc((size_t)np * 9, sizeof(double));
+    memset(wy, 0, sizeof(double) * (size_t)np);
+    /* bilinear_map_linear_constraints (steady_scan.m:438-499) */
+    for (long p = 0; p < npd; p++) {
+        int k = prevknotind[p] - 1;
+        double tc = 1 - tfrac[p];
+        double xx = basex[p] + dx[k] * tc + dx[k + 1] * tfrac[p];
+        double yy = basey[p] + dy[k] * tc + dy[k + 1] * tfrac[p];
+        if (!((xx > 1 + eps) && (xx < tw - eps) && (yy > 1 + eps) && (yy < th - eps))) continue;
+        double xi = floor(xx), yi = floor(yy);
+        double fx = xx - xi, fy = yy - yi, cx = 1 - fx, cy = 1 - fy;
+        long c0 = (long)(yi - 1) + (long)th * (long)(xi - 1);
+        long cn[4] = {c0, c0 + 1, c0 + th, c0 + th + 1};
+        int cr[4] = {0, 1, 0, 1}, cc[4] = {0, 0, 1, 1};
+        double w[4] = {cx * cy, cx * fy, fx * cy, fx * fy};
+        for (int a = 0; a < 4; a++) wy[cn[a]] += w[a] * data[p];
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                double v = w[a] * w[b];
+                if (a == b) IBQ(cn[a], 0) += v;
+                else IBQ(cn[a], 1 + ib_dir(cr[b] - cr[a], cc[b] - cc[a])) += v;
+            }
+    }
+    /* mask: summed coefficients (sum(ww)) positive and a resampled value;
+     * any(mask) is tested before the resampled value masks pixels out
+     * (steady_scan.m:333-338) */
+    orc_resample_frame(data, (int)npd, dx, dy, basex, basey, prevknotind, tfrac, th, tw, iv);
+    int any = 0;
+    for (long i = 0; i < np; i++) {
+        double s = 0;
+        for (int k = 0; k < 9; k++) s += IBQ(i, k);
+        mask[i] = s > 0;
+        any |= mask[i];
+        if (isnan(iv[i])) mask[i] = 0;
+    }
+    /* Q = ww + regmat (imageadjdiffquadmat, steady_scan.m:406-436): G'G has
+     * -1 (horizontal) and -vfac^2 (vertical) off the diagonal and the sum of
+     * the squared coefficients, W E N S, on it */
+    for (long i = 0; i < np; i++) {
+        long r = i % th, c = i / th;
+        double dreg = 0;
+        if (c > 0) { IBQ(i, 1 + IB_W) += -regfac; dreg += 1; }
+        if (c + 1 < tw) { IBQ(i, 1 + IB_E) += -regfac; dreg += 1; }
+        if (r > 0) { IBQ(i, 1 + IB_N) += -regfac * v2; dreg += v2; }
+        if (r + 1 < th) { IBQ(i, 1 + IB_S) += -regfac * v2; dreg += v2; }
+        IBQ(i, 0) += regfac * dreg;
+    }
+    /* rows and columns outside the mask zeroed */
+    for (long i = 0; i < np; i++) {
+        for (int k = 0; k < 9; k++) qm[i * 9 + k] = 0;
+        if (!mask[i]) continue;
+        qm[i * 9] = IBQ(i, 0);
+        long r = i % th, c = i / th;
+        for (int d = 0; d < IB_NB; d++) {
+            int dr, dc;
+            ib_offset(d, &dr, &dc);
+            long rr = r + dr, cc2 = c + dc;
+            if (rr < 0 || rr >= th || cc2 < 0 || cc2 >= tw) continue;
+            long j = rr + cc2 * th;
+            if (mask[j]) qm[i * 9 + 1 + d] = IBQ(i, 1 + d);
+        }
+    }
+    free(q);
+    return any;
+}
+
+/* The stopping energy single(double(x)' * Qe * double(x) - 2 * double(Le * x))
+ * over the mask (steady_scan.m:370, 383).  The reference's order of summation
+ * is that of MATLAB's sparse product and BLAS dot; this restatement fixes
+ * one: (Q x)_i sums x_j Q_ij over the neighbours j in ascending linear index
+ * (the order of a column of the sparse matrix), and both the quadratic term
+ * (double) and the linear term (single) are summed along each template row
+ * over ascending columns, the row sums then over ascending rows. */
+float orc_ibs_energy(const double *qm, const double *wy, const float *x, const uint8_t *mask, int th,
+                     int tw) {
+    static const int ord[9][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {0, 0},
+                                  {1, 0},   {-1, 1}, {0, 1},  {1, 1}};
+    double quad = 0;
+    float lin = 0;
+    for (long r = 0; r < th; r++) {
+        double qr = 0;
+        float lr = 0;
+        for (long c = 0; c < tw; c++) {
+            long i = r + c * (long)th;
+            if (!mask[i]) continue;
+            double qx = 0;
+            for (int k = 0; k < 9; k++) {
+                long rr = r + ord[k][0], cc = c + ord[k][1];
+                if (rr < 0 || rr >= th || cc < 0 || cc >= tw) continue;
+                long j = rr + cc * (long)th;
+                if (!mask[j]) continue;
+                int slot = k == 4 ? 0 : 1 + ib_dir(ord[k][0], ord[k][1]);
+                qx += (double)x[j] * qm[i * 9 + slot];
+            }
+            qr += (double)x[i] * qx;
+            lr += (float)wy[i] * x[i];
+        }
+        quad += qr;
+        lin += lr;
+    }
+    return (float)(quad - 2.0 * (double)lin);
+}
+
+/* calc_interactions (steady_scan.m:501-523) on one frame's masked system:
+ * R = single(diag), Rm = single(single(Q_ij) / R) times the find_* mask,
+ * L = single(wy) / R with NaN and Inf set to zero (steady_scan.m:358-362). */
+void orc_ibs_interactions(const double *qm, const double *wy, const uint8_t *mask, int th, int tw,
+                          float *Lk, float *Rm) {
+    const long np = (long)th * tw;
+    for (long i = 0; i < np; i++) {
+        float R = (float)qm[i * 9];
+        float Lf = (float)(mask[i] ? wy[i] : 0.0);
+        float L = Lf / R;
+        if (isnan(L) || isinf(L)) L = 0;
+        Lk[i] = L;
+        for (int d = 0; d < IB_NB; d++) {
+            float v = 0;
+            if (R != 0) {
+                long j;
+                int m = orc_ibs_neighbour(i, d, th, np, &j);
+                /* Q(i, Imat) at the (possibly wrapped) neighbour index */
+                long r = i % th, c = i / th, rj = j % th, cj = j / th;
+                double qij = 0;
+                if (labs(rj - r) <= 1 && labs(cj - c) <= 1 && j != i)
+                    qij = qm[i * 9 + 1 + ib_dir((int)(rj - r), (int)(cj - c))];
+                v = ((float)qij / R) * (m ? 1.0f : 0.0f);
+            }
+            Rm[i * IB_NB + d] = v;
+        }
+    }
+}
+
+/* One Gauss-Seidel iteration (steady_scan.m:374-377): the colours A B C D
+ * (1-based odd/odd, odd/even, even/odd, even/even row/column) in turn, each
+ * pixel x_i = L_i - sum over the slots N..NW of x(nbr) * Rm, in single. */
+void orc_ibs_sweep(const float *Lk, const float *Rm, int th, int tw, float *x) {
+    const long np = (long)th * tw;
+    for (int color = 0; color < 4; color++) {
+        int rpar = color >> 1, cpar = color & 1;
+        for (long c = cpar; c < tw; c += 2)
+            for (long r = rpar; r < th; r += 2) {
+                long i = r + c * (long)th;
+                float s = 0;
+                for (int d = 0; d < IB_NB; d++) {
+                    long j;
+                    orc_ibs_neighbour(i, d, th, np, &j);
+                    s += x[j] * Rm[i * IB_NB + d];
+                }
+                x[i] = Lk[i] - s;
+            }
+    }
+}
+
+int orc_invert_bilinear_sampling(const double *frames, int Hc, int Wc, int nframes, const double *dxk,
+                                 const double *dyk, int nknots, const int32_t *prevknotind,
+                                 const double *tfrac, const double *basex, const double *basey, int th,
+                                 int tw, double regfac, double aspect, float *out, uint8_t *maskout,
+                                 int32_t *iters) {
+    const long npd = (long)Hc * Wc, np = (long)th * tw;
+    double *wy = (double *)xcalloc((size_t)np, sizeof(double));
+    double *qm = (double *)xcalloc((size_t)np * 9, sizeof(double));
+    uint8_t *mask = (uint8_t *)xcalloc((size_t)np, 1);
+    float *iv = (float *)xcalloc((size_t)np, sizeof(float));
+    float *x = (float *)xcalloc((size_t)np, sizeof(float));
+    float *xo = (float *)xcalloc((size_t)np, sizeof(float));
+    float *Lk = (float *)xcalloc((size_t)np, sizeof(float));
+    float *Rm = (float *)xcalloc((size_t)np * IB_NB, sizeof(float));
+    int n_iter_error = 1;
+    for (int f = 0; f < nframes; f++) {
+        const double *data = frames + (size_t)f * npd;
+        const double *dx = dxk + (size_t)f * nknots, *dy = dyk + (size_t)f * nknots;
+        int any = orc_ibs_system(data, (int)npd, dx, dy, prevknotind, tfrac, basex, basey, th, tw, regfac,
+                                 aspect, qm, wy, iv, mask);
+        float *of = out + (size_t)f * np;
+        uint8_t *mf = maskout + (size_t)f * np;
+        if (!any) {
+            for (long i = 0; i < np; i++) {
+                of[i] = NAN;
+                mf[i] = 0;
+            }
+            if (iters) iters[f] = 0;
+            continue;
+        }
+        orc_ibs_interactions(qm, wy, mask, th, tw, Lk, Rm);
+        /* initial x from the resampled frame, NaN -> 0; xo starts at zero */
+        for (long i = 0; i < np; i++) {
+            x[i] = isnan(iv[i]) ? 0.0f : iv[i];
+            xo[i] = 0.0f;
+        }
+        float eo = orc_ibs_energy(qm, wy, x, mask, th, tw);
+        int iter = 0;
+        for (;;) {
+            orc_ibs_sweep(Lk, Rm, th, tw, x);
+            iter++;
+            if (iter % n_iter_error == 0) {
+                float e = orc_ibs_energy(qm, wy, x, mask, th, tw);
+                if (eo <= e) break;
+                eo = e;
+                memcpy(xo, x, sizeof(float) * (size_t)np);
+            }
+        }
+        /* the next frame checks the energy every int32(max(1, min(10,
+         * sqrt(single(iter))))) iterations: round half away from zero */
+        n_iter_error = orc_ibs_check_interval(iter);
+        if (iters) iters[f] = iter;
+        for (long i = 0; i < np; i++) {
+            of[i] = mask[i] ? xo[i] : NAN;
+            mf[i] = mask[i];
+        }
+    }
+    free(wy); free(qm); free(mask); free(iv); free(x); free(xo); free(Lk); free(Rm);
+    return 0;
+}
+
+int orc_ibs_check_interval(int iter) {
+    float sq = sqrtf((float)iter);
+    float v = sq < 10.0f ? sq : 10.0f;
+    if (v < 1.0f) v = 1.0f;
+    return (int)lroundf(v);
 }

@@ -147,3 +147,28 @@ void orc_resample_frame(const double *data, int npix, const double *dx_knots,
 }
 #endif
 #endif
+
+/* invert_bilinear_sampling (steady_scan.m:290-404) on one call's frames:
+ * frames Hc x Wc x nframes (cropped, double), knots nknots x nframes,
+ * image-shaped geometry (Hc x Wc), template th x tw; out th x tw x nframes
+ * (NaN outside the mask), maskout likewise; iters per frame (may be NULL). */
+int orc_invert_bilinear_sampling(const double *frames, int Hc, int Wc, int nframes, const double *dxk,
+                                 const double *dyk, int nknots, const int32_t *prevknotind,
+                                 const double *tfrac, const double *basex, const double *basey, int th,
+                                 int tw, double regfac, double aspect, float *out, uint8_t *maskout,
+                                 int32_t *iters);
+int orc_ibs_neighbour(long i0, int dir, long nrows, long npix, long *j0);
+/* The steps of invert_bilinear_sampling, exposed for the tests: one frame's
+ * masked system (stencil qm np x 9: diagonal, then N NE E SE S SW W NW),
+ * the stopping energy, calc_interactions, one Gauss-Seidel iteration and
+ * the energy-check interval that follows a frame of iter iterations. */
+int orc_ibs_system(const double *data, int npd, const double *dx, const double *dy,
+                   const int32_t *prevknotind, const double *tfrac, const double *basex,
+                   const double *basey, int th, int tw, double regfac, double aspect, double *qm,
+                   double *wy, float *iv, uint8_t *mask);
+float orc_ibs_energy(const double *qm, const double *wy, const float *x, const uint8_t *mask, int th,
+                     int tw);
+void orc_ibs_interactions(const double *qm, const double *wy, const uint8_t *mask, int th, int tw,
+                          float *Lk, float *Rm);
+void orc_ibs_sweep(const float *Lk, const float *Rm, int th, int tw, float *x);
+int orc_ibs_check_interval(int iter);
