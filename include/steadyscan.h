@@ -313,6 +313,35 @@ typedef struct {
  * as MATLAB's indexed assignment does. */
 ss_status ss_resample(ss_handle *h, const ss_resample_in *in, float *out);
 
+/* ---- corrected frames: invert_bilinear_sampling (steady_scan.m:290-404) ---- */
+typedef struct {
+    double regfac;          /* settings.ibs_regfac (3e-3)                          */
+    double pixel_aspect;    /* settings.ibs_template_pixel_aspect_ratio (1)        */
+    int32_t inpaint;        /* settings.ibs_inpaint: must be 0 (not supported)     */
+    int32_t max_iterations; /* Gauss-Seidel iterations allowed per frame; 0 = 100000.
+                               The reference has no bound; a frame that needs more
+                               fails the call with SS_ERR_NUMERICAL               */
+    int32_t chunk_frames;   /* frames reconstructed together on the device; 0 = as
+                               many as fit in 8 GiB of work memory               */
+    int32_t pad;
+} ss_ibs_settings;
+
+void ss_ibs_settings_default(ss_ibs_settings *s);
+
+/* Replaces invert_bilinear_sampling(data, settings, corrected_data_size,
+ * frame_list, dx_knots, dy_knots) (steady_scan.m:290, called at :133).
+ * in describes the frames as for ss_resample (raw nlines x linewidth frames,
+ * cropped here by the geometry's cropping; knots nknots x n_frames, one
+ * column per frame, i.e. dx_knots(:, frame_list)).  Frames of any element
+ * type are taken as the double values they hold.  Outputs, template_h x
+ * template_w x n_frames: out (float32, NaN outside the mask) = data_corrected,
+ * mask = mask; iterations (may be NULL) the Gauss-Seidel iterations of each
+ * frame, 0 for a frame with no constraint.  The energy-check interval starts
+ * at 1 for the call and carries from frame to frame as in the reference. */
+ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample_in *in,
+                                      const ss_ibs_settings *s, float *out, uint8_t *mask,
+                                      int32_t *iterations);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,6 +108,13 @@ class SSResampleIn(ctypes.Structure):
     ]
 
 
+class SSIbsSettings(ctypes.Structure):
+    _fields_ = [
+        ("regfac", c_double), ("pixel_aspect", c_double), ("inpaint", c_int32),
+        ("max_iterations", c_int32), ("chunk_frames", c_int32), ("pad", c_int32),
+    ]
+
+
 class SynthTemplateDesc(ctypes.Structure):
     _fields_ = [
         ("h", c_int32), ("w", c_int32), ("n_blobs", c_int32),
@@ -148,6 +155,9 @@ API = [
                                    POINTER(c_uint8), POINTER(c_int32)]),
     ("ss_pack_frames", c_int32, [POINTER(SSGeometryDesc), c_int32, c_void_p, c_int32, c_void_p]),
     ("ss_resample", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(ctypes.c_float)]),
+    ("ss_ibs_settings_default", None, [POINTER(SSIbsSettings)]),
+    ("ss_invert_bilinear_sampling", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(SSIbsSettings),
+                                              POINTER(ctypes.c_float), POINTER(c_uint8), POINTER(c_int32)]),
     ("ss_profile_enable", c_int32, [c_void_p, c_int32]),
     ("ss_profile_read", c_int32, [c_void_p, POINTER(ctypes.c_int64), POINTER(c_double), POINTER(c_double)]),
     ("ss_driver_settings_default", None, [POINTER(SSDriverSettings)]),

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "ss_device.h"
+#include "ss_host.h"
 #include "steadyscan.h"
 
 namespace {
@@ -1258,6 +1259,12 @@ ss_status upload_owned(ss_handle *h, const void *src, size_t bytes, void *base, 
 }
 
 }  // namespace
+
+namespace ss {
+ss_status handle_fail(ss_handle *h, ss_status st, const std::string &msg) { return fail(h, st, msg); }
+int handle_device(const ss_handle *h) { return h->device; }
+hipStream_t handle_stream(const ss_handle *h) { return h->stream; }
+}  // namespace ss
 
 extern "C" {
 

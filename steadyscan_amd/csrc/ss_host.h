@@ -1,0 +1,19 @@
+// ss_host.h -- what the other translation units of the library need from the
+// handle, whose definition lives with the alignment engine.
+#pragma once
+
+#include <string>
+
+#include <hip/hip_runtime.h>
+
+#include "steadyscan.h"
+
+namespace ss {
+
+// Records msg as the handle's last error and returns st.
+ss_status handle_fail(ss_handle *h, ss_status st, const std::string &msg);
+// The handle's device and stream (default priority, non-blocking).
+int handle_device(const ss_handle *h);
+hipStream_t handle_stream(const ss_handle *h);
+
+}  // namespace ss

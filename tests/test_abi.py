@@ -42,7 +42,10 @@ def test_ctypes_table_covers_header():
 @pytest.mark.parametrize("cname,pyt", [("ss_settings", N.SSSettings), ("ss_group_in", N.SSGroupIn),
                                        ("ss_group_out", N.SSGroupOut), ("ss_geometry_desc", N.SSGeometryDesc),
                                        ("ss_resample_in", N.SSResampleIn),
-                                       ("ss_server_config", N.SSServerConfig)])
+                                       ("ss_server_config", N.SSServerConfig),
+                                       ("ss_ibs_settings", N.SSIbsSettings),
+                                       ("ss_driver_settings", N.SSDriverSettings),
+                                       ("ss_displacements_out", N.SSDisplacementsOut)])
 def test_struct_layout_matches_header(cname, pyt, tmp_path):
     """Offsets of every field of the ctypes mirror equal the C compiler's."""
     import shutil

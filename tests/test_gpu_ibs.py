@@ -1,0 +1,119 @@
+"""GPU parity of invert_bilinear_sampling (steady_scan.m:290-404) against the
+oracle's restatement on the same inputs.
+
+The bar is bit-exact: the GPU sums the constraints in the oracle's order
+(ascending data pixel per template pixel), evaluates the stopping energy in
+the same fixed order and runs the same single-precision sweeps, so the
+corrected frames, the masks and the per-frame iteration counts must be
+identical (NaN in the same places).
+"""
+import numpy as np
+import pytest
+
+import steadyscan_amd as ssa
+from oracle import oracle as O
+from oracle import driver as OD
+from steadyscan_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_geometry(nlines, linewidth, th, tw, nknots, cropping=(0, 0, 0, 0), subrec=None):
+    (hc, wc), pk, tf, bx, by = OD._geometry(nlines, linewidth, th, tw, nknots, cropping, subrec, False, 0.3)
+    s = (hc, wc)
+    return dict(prevknotind=pk.reshape(s, order="F"), tfrac=tf.reshape(s, order="F"),
+                basex=bx.reshape(s, order="F"), basey=by.reshape(s, order="F"))
+
+
+def smooth_knots(rng, nk, n, step=0.3, offset=0.0):
+    return offset + np.cumsum(rng.normal(0, step, (nk, n)), axis=0)
+
+
+def compare(frames, dx, dy, th, tw, nknots, cropping=(0, 0, 0, 0), subrec=None, regfac=3e-3, aspect=1.0,
+            chunk_frames=0, label=""):
+    nl, lw, n = frames.shape
+    geom = ssa.init_settings(nl, lw, (th, tw), nknots=nknots, cropping=cropping, subrec=subrec)
+    info = {}
+    out, mask = ssa.invert_bilinear_sampling(frames, geom, dx, dy, ibs_regfac=regfac,
+                                             ibs_template_pixel_aspect_ratio=aspect, chunk_frames=chunk_frames,
+                                             info=info)
+    og = oracle_geometry(nl, lw, th, tw, nknots, cropping, subrec)
+    c = cropping
+    cropped = np.asarray(frames, dtype=np.float64)[c[2]:nl - c[3], c[0]:lw - c[1], :]
+    ref, rmask, rit = O.invert_bilinear_sampling(cropped, dx, dy, og, (th, tw), regfac=regfac, aspect=aspect)
+    diff = np.abs(out.astype(np.float64) - ref)
+    ndiff = int(np.sum(~((out == ref) | (np.isnan(out) & np.isnan(ref)))))
+    print(f"{label}: iterations gpu {info['iterations'].tolist()} oracle {rit.tolist()}; "
+          f"{ndiff} differing pixels, max |diff| {np.nanmax(diff) if ndiff else 0.0}", flush=True)
+    np.testing.assert_array_equal(info["iterations"], rit)
+    np.testing.assert_array_equal(mask, rmask)
+    np.testing.assert_array_equal(out, ref)
+    return out, mask, info["iterations"]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.uint16, np.int16])
+def test_small_stack_all_frame_types(dtype):
+    T, fr, geom, mot = W.config0(n_frames=4, size=64, seed=11)
+    rng = np.random.default_rng(12)
+    nk = geom["nknots"]
+    fr = np.asarray(fr)
+    if dtype in (np.uint16, np.int16):
+        fr = np.clip(np.round(fr), 0, 30000)
+    fr = fr.astype(dtype)
+    compare(fr, smooth_knots(rng, nk, 4), smooth_knots(rng, nk, 4), 64, 64, nk, label=f"64x64 {np.dtype(dtype).name}")
+
+
+def test_strong_regulariser_chains_the_check_interval():
+    # more iterations per frame, so later frames check every 2..5 iterations
+    T, fr, geom, mot = W.config0(n_frames=6, size=48, seed=13)
+    rng = np.random.default_rng(14)
+    nk = geom["nknots"]
+    _, _, it = compare(np.asarray(fr, np.float64), smooth_knots(rng, nk, 6), smooth_knots(rng, nk, 6), 48, 48, nk,
+                       regfac=1.0, label="regfac 1")
+    assert it.max() >= 3
+
+
+def test_chunks_carry_the_check_interval():
+    T, fr, geom, mot = W.config0(n_frames=5, size=40, seed=15)
+    rng = np.random.default_rng(16)
+    nk = geom["nknots"]
+    dx, dy = smooth_knots(rng, nk, 5), smooth_knots(rng, nk, 5)
+    compare(np.asarray(fr, np.float64), dx, dy, 40, 40, nk, regfac=0.3, chunk_frames=2, label="chunks of 2")
+
+
+def test_frame_off_the_template_is_skipped():
+    T, fr, geom, mot = W.config0(n_frames=4, size=32, seed=17)
+    rng = np.random.default_rng(18)
+    nk = geom["nknots"]
+    dx, dy = smooth_knots(rng, nk, 4), smooth_knots(rng, nk, 4)
+    dx[:, 1] = 100.0
+    dy[:, 3] = np.nan                      # a frame whose knots were never estimated
+    out, mask, it = compare(np.asarray(fr, np.float64), dx, dy, 32, 32, nk, regfac=0.5, label="skipped frames")
+    assert it[1] == 0 and it[3] == 0 and not mask[:, :, 1].any() and np.isnan(out[:, :, 3]).all()
+
+
+def test_rectangular_template_aspect_and_cropping():
+    rng = np.random.default_rng(19)
+    nl, lw = 40, 56
+    fr = 20 + 50 * rng.random((nl, lw, 3))
+    nk = 11
+    dx, dy = smooth_knots(rng, nk, 3, 0.2), smooth_knots(rng, nk, 3, 0.2)
+    compare(fr, dx, dy, 40, 50, nk, cropping=(2, 4, 0, 0), aspect=2.0, label="40x50 aspect 2 cropped")
+
+
+def test_subrec_squeezes_several_pixels_per_cell():
+    # a scan mapped onto a quarter-size template region: many data pixels per
+    # cell, repeated pushgrid targets (the last value is kept)
+    rng = np.random.default_rng(20)
+    fr = 10 + 90 * rng.random((32, 32, 2))
+    nk = 5
+    dx, dy = smooth_knots(rng, nk, 2, 0.1), smooth_knots(rng, nk, 2, 0.1)
+    compare(fr, dx, dy, 24, 24, nk, subrec=(6, 14, 6, 14), label="subrec")
+
+
+def test_workload_size_512():
+    T, fr, geom, mot = W.config1(n_frames=2, size=512, seed=21)
+    rng = np.random.default_rng(22)
+    nk = geom["nknots"]
+    dx, dy = smooth_knots(rng, nk, 2, 0.3), smooth_knots(rng, nk, 2, 0.3)
+    compare(np.asarray(fr), dx, dy, 512, 512, nk, label="512x512 uint16")
