@@ -67,6 +67,10 @@ def parse():
                     help="also time the batched uneven-groups workload (256 groups, 8-96 frames, 256^2/512^2)")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="simulated acquisition time of the serving leg")
+    ap.add_argument("--no-ibs", action="store_true",
+                    help="skip the reconstruction (invert_bilinear_sampling) leg")
+    ap.add_argument("--ibs-only", action="store_true",
+                    help="run only the reconstruction leg (knots from one alignment step)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of known size (rocprof counter calibration)")
     return ap.parse_args()
@@ -213,6 +217,58 @@ def uneven_leg(args, torch, ssa, W):
     return out
 
 
+def ibs_leg(args, torch, ssa, frames, geom, results, G, h):
+    """invert_bilinear_sampling (steady_scan.m:290-404) of the workload's
+    frames with the knots the alignment step estimated; frames and outputs
+    resident in HBM.  The first frames are also reconstructed by the oracle
+    (bit-exact check, and the CPU baseline of this leg)."""
+    nk = geom["nknots"]
+    n = min(frames.shape[2], G * len(results))
+    kx = np.concatenate([r["knots_x"].reshape(G, nk).T for r in results], axis=1)[:, :n]
+    ky = np.concatenate([r["knots_y"].reshape(G, nk).T for r in results], axis=1)[:, :n]
+    th, tw = geom["desc"].template_h, geom["desc"].template_w
+    d_in = torch.from_numpy(np.asfortranarray(frames[:, :, :n]).ravel(order="F").view(np.int16)).to("cuda")
+    d_out = torch.empty(th * tw * n, dtype=torch.float32, device="cuda")
+    d_mask = torch.empty(th * tw * n, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    info = {}
+
+    def call():
+        ssa.invert_bilinear_sampling((d_in.data_ptr(), np.uint16, n), geom, kx, ky,
+                                     device_out=(d_out.data_ptr(), d_mask.data_ptr()), handle=h, info=info)
+
+    call()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    call()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    it = info["iterations"]
+    out = {"frames": n, "seconds": el, "frames_per_s": n / el,
+           "gs_iterations_mean": float(it.mean()), "gs_iterations_max": int(it.max()),
+           "mask_fraction": float(d_mask.float().mean().item()),
+           "inputs": "uint16 frames and float32/uint8 outputs resident in HBM; knots of the alignment step"}
+    print(f"[bench] reconstruction: {n} frames in {el:.3f}s ({n / el:.0f} frames/s), "
+          f"Gauss-Seidel iterations mean {it.mean():.1f} max {it.max()}", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        from oracle import oracle as O
+        m = 3
+        gi = {}
+        gout, gmask = ssa.invert_bilinear_sampling(frames[:, :, :m], geom, kx[:, :m], ky[:, :m], handle=h, info=gi)
+        t = time.perf_counter()
+        ref, rmask, rit = O.invert_bilinear_sampling(frames[:, :, :m].astype(np.float64), kx[:, :m], ky[:, :m],
+                                                     geom, (th, tw))
+        tcpu = time.perf_counter() - t
+        same = bool(np.array_equal(gout, ref, equal_nan=True) and np.array_equal(gmask, rmask)
+                    and np.array_equal(gi["iterations"], rit))
+        out["oracle_bit_exact_first_frames"] = same
+        out["cpu_baseline"] = {"value": m / tcpu, "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle (C, one thread) on the first {m} frames, {tcpu:.1f} s"}
+        print(f"[bench] reconstruction oracle: first {m} frames bit-exact {same}, {m / tcpu:.2f} frames/s on 1 core",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -305,6 +361,11 @@ def main():
     precs = ["D", "S", "SB"] if args.precision == "all" else [args.precision]
     serving = None
     uneven = None
+    if args.ibs_only:
+        calls = [call(g, "D") for g in range(ngroups)]
+        res = ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+        print(json.dumps({"reconstruction": ibs_leg(args, torch, ssa, frames, geom, res, G, h)}))
+        return
     runs = {p: timed(p) for p in precs}
     head = runs[precs[0]]
     results = head["results"]
@@ -322,6 +383,9 @@ def main():
         uneven = uneven_leg(args, torch, ssa, W)
         print(f"[bench] uneven: {uneven['frames_per_s']:.0f} frames/s, {uneven['megapixels_per_s']:.0f} Mpx/s",
               file=sys.stderr, flush=True)
+    recon = None
+    if not args.no_ibs:
+        recon = ibs_leg(args, torch, ssa, frames, geom, results, G, h)
     if not args.no_serve:
         serving = serve_leg(args, torch, ssa, W)
         print(f"[bench] serving: p50 {serving['p50_group_ms']:.1f} ms p99 {serving['p99_group_ms']:.1f} ms "
@@ -411,6 +475,7 @@ def main():
             "other_precisions": other,
             "serving": serving,
             "uneven_groups": uneven,
+            "reconstruction": recon,
         },
         "roofline": {
             "kernel": N.KERNEL_CLASSES[dom],

@@ -322,8 +322,10 @@ typedef struct {
                                The reference has no bound; a frame that needs more
                                fails the call with SS_ERR_NUMERICAL               */
     int32_t chunk_frames;   /* frames reconstructed together on the device; 0 = as
-                               many as fit in 8 GiB of work memory               */
-    int32_t pad;
+                               many as fit in work_bytes                          */
+    int32_t outputs_on_device; /* out and mask are device pointers                */
+    int64_t work_bytes;     /* device work memory the handle may keep for this
+                               call; 0 = 32 GiB                                   */
 } ss_ibs_settings;
 
 void ss_ibs_settings_default(ss_ibs_settings *s);

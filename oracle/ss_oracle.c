@@ -26,8 +26,8 @@
  * The file also restates invert_bilinear_sampling (steady_scan.m:290-404)
  * with its helpers.  Its summation orders are fixed here where MATLAB's
  * sparse() and BLAS leave them open: constraint sums over data pixels in
- * ascending index; the stopping energy per template row, then over rows;
- * (Q x)_i over neighbours in ascending index.  Those rounding-level orders
+ * ascending index; the stopping energy in 1024 interleaved partial sums
+ * added by a fixed tree; (Q x)_i over neighbours in ascending index.  Those rounding-level orders
  * are parity unpinned against the reference (no MATLAB here); everything
  * else is pinned by the dense-definition tests.
  */
@@ -1312,22 +1312,26 @@ int orc_ibs_system(const double *data, int npd_, const double *dx, const double 
 /* The stopping energy single(double(x)' * Qe * double(x) - 2 * double(Le * x))
  * over the mask (steady_scan.m:370, 383).  The reference's order of summation
  * is that of MATLAB's sparse product and BLAS dot; this restatement fixes
- * one: (Q x)_i sums x_j Q_ij over the neighbours j in ascending linear index
- * (the order of a column of the sparse matrix), and both the quadratic term
- * (double) and the linear term (single) are summed along each template row
- * over ascending columns, the row sums then over ascending rows. */
+ * one that parallel hardware can follow: (Q x)_i sums x_j Q_ij over the
+ * neighbours j in ascending linear index (the order of a column of the
+ * sparse matrix); the pixel terms x_i (Q x)_i (double) and Le_i x_i (single)
+ * are summed in ORC_IBS_LANES interleaved partial sums, partial t taking the
+ * pixels t, t + LANES, t + 2 LANES, ... in ascending order, and the partials
+ * are then added by a fixed binary tree (t += t + 512, then 256, ... 1). */
+#define ORC_IBS_LANES 1024
 float orc_ibs_energy(const double *qm, const double *wy, const float *x, const uint8_t *mask, int th,
                      int tw) {
     static const int ord[9][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {0, 0},
                                   {1, 0},   {-1, 1}, {0, 1},  {1, 1}};
-    double quad = 0;
-    float lin = 0;
-    for (long r = 0; r < th; r++) {
-        double qr = 0;
-        float lr = 0;
-        for (long c = 0; c < tw; c++) {
-            long i = r + c * (long)th;
+    const long np = (long)th * tw;
+    double pq[ORC_IBS_LANES];
+    float pl[ORC_IBS_LANES];
+    for (int t = 0; t < ORC_IBS_LANES; t++) {
+        pq[t] = 0;
+        pl[t] = 0;
+        for (long i = t; i < np; i += ORC_IBS_LANES) {
             if (!mask[i]) continue;
+            long r = i % th, c = i / th;
             double qx = 0;
             for (int k = 0; k < 9; k++) {
                 long rr = r + ord[k][0], cc = c + ord[k][1];
@@ -1337,13 +1341,16 @@ float orc_ibs_energy(const double *qm, const double *wy, const float *x, const u
                 int slot = k == 4 ? 0 : 1 + ib_dir(ord[k][0], ord[k][1]);
                 qx += (double)x[j] * qm[i * 9 + slot];
             }
-            qr += (double)x[i] * qx;
-            lr += (float)wy[i] * x[i];
+            pq[t] += (double)x[i] * qx;
+            pl[t] += (float)wy[i] * x[i];
         }
-        quad += qr;
-        lin += lr;
     }
-    return (float)(quad - 2.0 * (double)lin);
+    for (int off = ORC_IBS_LANES / 2; off >= 1; off >>= 1)
+        for (int t = 0; t < off; t++) {
+            pq[t] += pq[t + off];
+            pl[t] += pl[t + off];
+        }
+    return (float)(pq[0] - 2.0 * (double)pl[0]);
 }
 
 /* calc_interactions (steady_scan.m:501-523) on one frame's masked system:

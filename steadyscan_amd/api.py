@@ -636,32 +636,41 @@ def resample_frames(frames, geometry, dx_knots, dy_knots, handle=None, device=0)
 
 def invert_bilinear_sampling(frames, geometry, dx_knots, dy_knots, ibs_regfac=3e-3,
                              ibs_template_pixel_aspect_ratio=1.0, ibs_inpaint=False, max_iterations=0,
-                             chunk_frames=0, handle=None, device=0, info=None):
+                             chunk_frames=0, work_bytes=0, device_out=None, handle=None, device=0, info=None):
     """invert_bilinear_sampling (steady_scan.m:290-404) on the GPU.
 
     frames: nlines x linewidth x n raw frames (cropped here by the geometry's
-    cropping, as steady_scan.m:132 crops them before the call); geometry: the
-    dict of init_settings; dx_knots/dy_knots: nknots x n, one column per frame.
-    Returns (data_corrected float32 template_h x template_w x n with NaN
-    outside the mask, mask bool).  info, a dict if given, receives the
+    cropping, as steady_scan.m:132 crops them before the call), or a tuple
+    (device_pointer, dtype, n) for frames resident in HBM (same layout);
+    geometry: the dict of init_settings; dx_knots/dy_knots: nknots x n, one
+    column per frame.  Returns (data_corrected float32 template_h x
+    template_w x n with NaN outside the mask, mask bool); with
+    device_out=(out_pointer, mask_pointer) the results are written to device
+    memory instead and None is returned.  info, a dict if given, receives the
     per-frame Gauss-Seidel iteration counts under "iterations"."""
-    fr = np.asarray(frames)
-    if fr.ndim == 2:
-        fr = fr[:, :, None]
-    if fr.dtype not in _DTYPES:
-        fr = fr.astype(np.float64)
-    fr = np.asfortranarray(fr)
     g = geometry["desc"]
-    n = fr.shape[2]
-    if fr.shape[:2] != (g.nlines, g.linewidth):
-        raise ValueError(f"frames are {fr.shape[:2]}, the geometry expects {(g.nlines, g.linewidth)}")
+    ri = N.SSResampleIn()
+    if isinstance(frames, tuple):
+        ptr, dtype, n = frames
+        ri.frames = c_void_p(int(ptr))
+        ri.data_type = _DTYPES[np.dtype(dtype)]
+        ri.data_on_device = 1
+    else:
+        fr = np.asarray(frames)
+        if fr.ndim == 2:
+            fr = fr[:, :, None]
+        if fr.dtype not in _DTYPES:
+            fr = fr.astype(np.float64)
+        fr = np.asfortranarray(fr)
+        if fr.shape[:2] != (g.nlines, g.linewidth):
+            raise ValueError(f"frames are {fr.shape[:2]}, the geometry expects {(g.nlines, g.linewidth)}")
+        n = fr.shape[2]
+        ri.frames = c_void_p(fr.ctypes.data)
+        ri.data_type = _DTYPES[fr.dtype]
     kx, ky = _f64(np.reshape(dx_knots, (g.nknots, n), order="F")), _f64(np.reshape(dy_knots, (g.nknots, n), order="F"))
     h = handle if handle is not None else _default_handle(device)
-    ri = N.SSResampleIn()
     ri.geometry = ctypes.pointer(g)
     ri.n_frames = n
-    ri.data_type = _DTYPES[fr.dtype]
-    ri.frames = c_void_p(fr.ctypes.data)
     ri.knots_x, ri.knots_y = _ptr(kx, c_double), _ptr(ky, c_double)
     st = N.SSIbsSettings()
     N.lib().ss_ibs_settings_default(byref(st))
@@ -670,9 +679,19 @@ def invert_bilinear_sampling(frames, geometry, dx_knots, dy_knots, ibs_regfac=3e
     st.inpaint = 1 if ibs_inpaint else 0
     st.max_iterations = int(max_iterations)
     st.chunk_frames = int(chunk_frames)
+    st.work_bytes = int(work_bytes)
+    iters = np.zeros(n, dtype=np.int32)
+    if device_out is not None:
+        st.outputs_on_device = 1
+        _check(N.lib().ss_invert_bilinear_sampling(h.ptr, byref(ri), byref(st),
+                                                   ctypes.cast(c_void_p(int(device_out[0])), POINTER(c_float)),
+                                                   ctypes.cast(c_void_p(int(device_out[1])), POINTER(c_uint8)),
+                                                   _ptr(iters, c_int32)), h)
+        if info is not None:
+            info["iterations"] = iters
+        return None
     out = np.zeros((g.template_h, g.template_w, n), dtype=np.float32, order="F")
     mask = np.zeros((g.template_h, g.template_w, n), dtype=np.uint8, order="F")
-    iters = np.zeros(n, dtype=np.int32)
     _check(N.lib().ss_invert_bilinear_sampling(h.ptr, byref(ri), byref(st), out.ctypes.data_as(POINTER(c_float)),
                                                _ptr(mask, c_uint8), _ptr(iters, c_int32)), h)
     if info is not None:

@@ -121,6 +121,8 @@ struct ss_handle {
     size_t arenaBytes = 0;
     void *harena = nullptr;  // pinned host staging
     size_t harenaBytes = 0;
+    void *scratch = nullptr;  // work memory of the reconstruction
+    size_t scratchBytes = 0;
     Profile prof;
     // the tick's launches, captured once per group table and replayed
     bool useGraph = true;
@@ -1264,6 +1266,25 @@ namespace ss {
 ss_status handle_fail(ss_handle *h, ss_status st, const std::string &msg) { return fail(h, st, msg); }
 int handle_device(const ss_handle *h) { return h->device; }
 hipStream_t handle_stream(const ss_handle *h) { return h->stream; }
+void *handle_scratch(ss_handle *h, size_t bytes, ss_status *st) {
+    *st = SS_OK;
+    if (h->scratchBytes >= bytes) return h->scratch;
+    if (h->scratch) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipFree(h->scratch);
+        h->scratch = nullptr;
+        h->scratchBytes = 0;
+    }
+    hipError_t e = hipMalloc(&h->scratch, bytes);
+    if (e != hipSuccess) {
+        h->scratch = nullptr;
+        *st = fail(h, e == hipErrorOutOfMemory ? SS_ERR_OUT_OF_MEMORY : SS_ERR_DEVICE,
+                   std::string("reconstruction work memory: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    h->scratchBytes = bytes;
+    return h->scratch;
+}
 }  // namespace ss
 
 extern "C" {
@@ -1305,6 +1326,7 @@ void ss_destroy(ss_handle *h) {
     if (h->tickExec) e = hipGraphExecDestroy(h->tickExec);
     if (h->arena) e = hipFree(h->arena);
     if (h->harena) e = hipHostFree(h->harena);
+    if (h->scratch) e = hipFree(h->scratch);
     if (h->stream) e = hipStreamDestroy(h->stream);
     (void)e;
     delete h;

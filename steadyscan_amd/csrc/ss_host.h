@@ -15,5 +15,8 @@ ss_status handle_fail(ss_handle *h, ss_status st, const std::string &msg);
 // The handle's device and stream (default priority, non-blocking).
 int handle_device(const ss_handle *h);
 hipStream_t handle_stream(const ss_handle *h);
+// Device work memory of at least `bytes` kept by the handle across calls
+// (grown on demand, freed by ss_destroy); nullptr on failure (st set).
+void *handle_scratch(ss_handle *h, size_t bytes, ss_status *st);
 
 }  // namespace ss

@@ -25,19 +25,23 @@
 //              data pixels, bit for bit, without floating-point atomics;
 //   interact   the masked stencil, calc_interactions' R, L and Rm;
 //   gs         one workgroup per frame runs the Gauss-Seidel loop with the
-//              energy summed in a fixed order (per template row over
-//              ascending columns, then over rows) and tracks the stopping
+//              energy summed in a fixed order (1024 interleaved partial sums
+//              added by a binary tree) and tracks the stopping
 //              rule for all ten possible check intervals at once, keeping a
-//              snapshot per interval; a frame whose interval is already
-//              known (published by the previous frame's workgroup) drops
-//              the others;
+//              snapshot per interval.  Each interval's stop is published as
+//              it happens; a frame walks its predecessors' published stops
+//              and, once its own interval follows from them, drops the
+//              other intervals;
 //   resolve    on the host: walks the frames in order, picks each frame's
 //              interval from its predecessor, and
 //   output     writes the chosen snapshot (NaN outside the mask).
 // Everything is deterministic; all floating-point code is compiled without
 // contraction so that it rounds as the sequential restatement does.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -92,16 +96,17 @@ struct Chunk {
     double *wy;               // B x np
     float *iv;                // B x np
     uint8_t *mask;            // B x np
+    uint8_t *full;            // B x np: masked, interior, all eight neighbours masked
     int *any;                 // B
     double *qs;               // B x QS_COUNT x np
     float *lf, *lk;           // B x np
     float *rm;                // B x np x 8
     float *x;                 // B x np
     float *snap;              // B x kNSlots x np
-    double *rowq;             // B x th
-    float *rowl;              // B x th
-    int *tab;                 // B x 3 x kNSched: iterations, slot, stopped (+1 known)
-    int *pub;                 // B + 1: each frame's check interval once known, else 0
+    int *stop;                // B x kNSched: 1 + the iteration a check interval stopped at, else 0
+    int *slot;                // B x kNSched: the snapshot holding that interval's result
+    int *diag;                // B x 2: iterations run, iteration the interval became known
+    int carry;                // check interval of the chunk's first frame
 };
 
 // int32(max(1, min(10, sqrt(single(iter))))) (steady_scan.m:394): sqrt is
@@ -408,94 +413,183 @@ __global__ __launch_bounds__(256) void k_interact(Chunk C) {
     qs[QS_SE * (size_t)C.np + i] = qm[1 + NB_SE];
     const float iv = C.iv[fo + i];
     C.x[fo + i] = isnan(iv) ? 0.0f : iv;
+    bool full = m && r >= 1 && r + 1 < th && c >= 1 && c + 1 < tw;
+    for (int d = 0; d < NB_COUNT && full; d++) full = mask[(r + kSlotDr[d]) + (c + kSlotDc[d]) * th] != 0;
+    C.full[fo + i] = full ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
 // gs: one workgroup per frame
 // ---------------------------------------------------------------------------
-// The stopping energy single(x'Qx - 2 single(Le x)) over the mask, in the
-// fixed order: per row over ascending columns, (Q x)_i over the neighbours
-// in ascending index, then the rows in ascending order by thread 0.
-__device__ float gs_energy(const Chunk &C, int f, const float *x, const uint8_t *mask,
-                           const double *qs, const float *lf, double *rowq, float *rowl) {
+// (Q x)_i for pixel i at (r, c): x_j Q_ij over the in-bounds, masked
+// neighbours j in ascending index.  Q_ij is read from the symmetric half
+// (the pixel with the smaller index holds the coupling).
+__device__ inline double qx_generic(const float *x, const uint8_t *mask, const double *qD, const double *qS,
+                                    const double *qNE, const double *qE, const double *qSE, int i, int r,
+                                    int c, int th, int tw) {
 #pragma clang fp contract(off)
-    const int th = C.th, tw = C.tw;
-    const size_t np = C.np;
-    const double *qD = qs + QS_D * np, *qS = qs + QS_S * np, *qNE = qs + QS_NE * np,
-                 *qE = qs + QS_E * np, *qSE = qs + QS_SE * np;
-    for (int r = threadIdx.x; r < th; r += blockDim.x) {
-        double qr = 0.0;
-        float lr = 0.f;
-        for (int c = 0; c < tw; c++) {
-            const int i = r + c * th;
+    const bool up = r >= 1, dn = r + 1 < th, lt = c >= 1, rt = c + 1 < tw;
+    double qx = 0.0;
+    if (lt) {
+        if (up && mask[i - th - 1]) qx += (double)x[i - th - 1] * qSE[i - th - 1];  // NW
+        if (mask[i - th]) qx += (double)x[i - th] * qE[i - th];                     // W
+        if (dn && mask[i - th + 1]) qx += (double)x[i - th + 1] * qNE[i - th + 1];  // SW
+    }
+    if (up && mask[i - 1]) qx += (double)x[i - 1] * qS[i - 1];                      // N
+    qx += (double)x[i] * qD[i];
+    if (dn && mask[i + 1]) qx += (double)x[i + 1] * qS[i];                          // S
+    if (rt) {
+        if (up && mask[i + th - 1]) qx += (double)x[i + th - 1] * qNE[i];           // NE
+        if (mask[i + th]) qx += (double)x[i + th] * qE[i];                          // E
+        if (dn && mask[i + th + 1]) qx += (double)x[i + th + 1] * qSE[i];           // SE
+    }
+    return qx;
+}
+
+// The stopping energy single(x'Qx - 2 single(Le x)) over the mask in the
+// fixed order: (Q x)_i over the neighbours in ascending index; thread t sums
+// the pixels t, t + 1024, ... in ascending order; the 1024 partial sums are
+// added by a binary tree (t += t + 512, then 256, ... 1).  The result is
+// returned to every thread.  Pixels whose eight neighbours are all masked
+// take the same sum with no tests.
+__device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, const uint8_t *full,
+                           const double *qs, const float *lf, double *sp, float *sl) {
+#pragma clang fp contract(off)
+    const int th = C.th, tw = C.tw, np = C.np;
+    const double *qD = qs, *qS = qs + QS_S * (size_t)np, *qNE = qs + QS_NE * (size_t)np,
+                 *qE = qs + QS_E * (size_t)np, *qSE = qs + QS_SE * (size_t)np;
+    double pq = 0.0;
+    float pl = 0.f;
+    for (int i = threadIdx.x; i < np; i += kGsThreads) {
+        const uint8_t fl = full[i];
+        double qx;
+        if (fl) {
+            const float xnw = x[i - th - 1], xw = x[i - th], xsw = x[i - th + 1];
+            const float xn = x[i - 1], xc = x[i], xs = x[i + 1];
+            const float xne = x[i + th - 1], xe = x[i + th], xse = x[i + th + 1];
+            qx = 0.0;
+            qx += (double)xnw * qSE[i - th - 1];
+            qx += (double)xw * qE[i - th];
+            qx += (double)xsw * qNE[i - th + 1];
+            qx += (double)xn * qS[i - 1];
+            qx += (double)xc * qD[i];
+            qx += (double)xs * qS[i];
+            qx += (double)xne * qNE[i];
+            qx += (double)xe * qE[i];
+            qx += (double)xse * qSE[i];
+        } else {
             if (!mask[i]) continue;
-            double qx = 0.0;
-            if (c >= 1) {
-                if (r >= 1 && mask[i - th - 1]) qx += (double)x[i - th - 1] * qSE[i - th - 1];  // NW
-                if (mask[i - th]) qx += (double)x[i - th] * qE[i - th];                         // W
-                if (r + 1 < th && mask[i - th + 1]) qx += (double)x[i - th + 1] * qNE[i - th + 1];  // SW
-            }
-            if (r >= 1 && mask[i - 1]) qx += (double)x[i - 1] * qS[i - 1];  // N
-            qx += (double)x[i] * qD[i];
-            if (r + 1 < th && mask[i + 1]) qx += (double)x[i + 1] * qS[i];  // S
-            if (c + 1 < tw) {
-                if (r >= 1 && mask[i + th - 1]) qx += (double)x[i + th - 1] * qNE[i];  // NE
-                if (mask[i + th]) qx += (double)x[i + th] * qE[i];                     // E
-                if (r + 1 < th && mask[i + th + 1]) qx += (double)x[i + th + 1] * qSE[i];  // SE
-            }
-            qr += (double)x[i] * qx;
-            lr += lf[i] * x[i];
+            qx = qx_generic(x, mask, qD, qS, qNE, qE, qSE, i, i % th, i / th, th, tw);
         }
-        rowq[r] = qr;
-        rowl[r] = lr;
+        pq += (double)x[i] * qx;
+        pl += lf[i] * x[i];
     }
+    const int t = threadIdx.x;
+    sp[t] = pq;
+    sl[t] = pl;
     __syncthreads();
-    float e = 0.f;
-    if (threadIdx.x == 0) {
-        double quad = 0.0;
-        float lin = 0.f;
-        for (int r = 0; r < th; r++) {
-            quad += rowq[r];
-            lin += rowl[r];
+    for (int off = kGsThreads / 2; off >= 1; off >>= 1) {
+        if (t < off) {
+            sp[t] += sp[t + off];
+            sl[t] += sl[t + off];
         }
-        e = (float)(quad - 2.0 * (double)lin);
+        __syncthreads();
     }
-    return e;  // valid in thread 0
+    const float e = (float)(sp[0] - 2.0 * (double)sl[0]);
+    __syncthreads();  // sp/sl are reused by the next call
+    return e;
+}
+
+// x_i = L_i - sum over the slots N..NW of x(nbr) * Rm for one pixel at (r, c).
+// The neighbour indices are find_*'s (steady_scan.m:677-723) written out for
+// a pixel at (r, c): a rejected neighbour reads pixel 0 (its Rm is zero),
+// find_SE rejects the last pixel, find_NW wraps from the top row.  Away from
+// the border every neighbour is in the image and accepted except the one
+// find_SE drops; reading the true neighbour there instead of pixel 0 gives
+// the same result, since its coefficient is zero and x is finite.
+__device__ inline float gs_update(const float *x, const float4 a, const float4 b, float lk, int i,
+                                  int r, int c, int th, int tw, int np) {
+#pragma clang fp contract(off)
+    float xn, xne, xe, xse, xs, xsw, xw, xnw;
+    if (r >= 1 && r + 1 < th && c >= 1 && c + 1 < tw) {
+        xnw = x[i - th - 1]; xw = x[i - th]; xsw = x[i - th + 1];
+        xn = x[i - 1]; xs = x[i + 1];
+        xne = x[i + th - 1]; xe = x[i + th]; xse = x[i + th + 1];
+    } else {
+        const bool rTop = r == 0, rBot = r == th - 1, cLast = c == tw - 1;
+        xn = x[rTop ? 0 : i - 1];
+        xne = x[(rTop || cLast) ? 0 : i + th - 1];
+        xe = x[cLast ? 0 : i + th];
+        xse = x[(rBot || i + th + 2 >= np) ? 0 : i + th + 1];
+        xs = x[rBot ? 0 : i + 1];
+        xsw = x[(rBot || i < th - 1) ? 0 : i - th + 1];
+        xw = x[i < th ? 0 : i - th];
+        xnw = x[i <= th ? 0 : i - th - 1];
+    }
+    float s = 0.f;
+    s += xn * a.x;
+    s += xne * a.y;
+    s += xe * a.z;
+    s += xse * a.w;
+    s += xs * b.x;
+    s += xsw * b.y;
+    s += xw * b.z;
+    s += xnw * b.w;
+    return lk - s;
 }
 
 // One iteration: the colours A B C D (1-based odd/odd, odd/even, even/odd,
-// even/even row/column), each x_i = L_i - sum over N..NW of x(nbr) * Rm.
+// even/even row/column) in turn; four pixels per thread and step, all loads
+// before the stores (pixels of one colour do not read each other except
+// through zero coefficients).
 __device__ void gs_sweep(const Chunk &C, float *x, const float *lk, const float *rm) {
-#pragma clang fp contract(off)
-    const int th = C.th, tw = C.tw;
+    const int th = C.th, tw = C.tw, np = C.np;
+    constexpr int P = 4;
     for (int color = 0; color < 4; color++) {
         const int rp = color >> 1, cp = color & 1;
         const int nr = (th - rp + 1) / 2, nc = (tw - cp + 1) / 2;
         const int n = nr * nc;
-        for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            const int r = rp + 2 * (k % nr), c = cp + 2 * (k / nr);
-            const int i = r + c * th;
-            const float4 *rv = (const float4 *)(rm + (size_t)i * 8);
-            const float4 a = rv[0], b = rv[1];
-            const float w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-            float s = 0.f;
-            for (int d = 0; d < NB_COUNT; d++) {
-                bool mm;
-                const long j = neighbour(i, d, th, C.np, &mm);
-                s += x[j] * w[d];
+        for (int k0 = threadIdx.x; k0 < n; k0 += P * kGsThreads) {
+            int ii[P], rr[P], cc[P];
+            float4 a[P], b[P];
+            float l[P], v[P];
+            for (int q = 0; q < P; q++) {
+                const int k = k0 + q * kGsThreads;
+                const int kk = k < n ? k : k0;
+                rr[q] = rp + 2 * (kk % nr);
+                cc[q] = cp + 2 * (kk / nr);
+                ii[q] = rr[q] + cc[q] * th;
+                const float4 *rv = (const float4 *)(rm + (size_t)ii[q] * 8);
+                a[q] = rv[0];
+                b[q] = rv[1];
+                l[q] = lk[ii[q]];
             }
-            x[i] = lk[i] - s;
+            for (int q = 0; q < P; q++) v[q] = gs_update(x, a[q], b[q], l[q], ii[q], rr[q], cc[q], th, tw, np);
+            for (int q = 0; q < P; q++)
+                if (k0 + q * kGsThreads < n) x[ii[q]] = v[q];
         }
         __syncthreads();
     }
 }
 
-// The check interval of frame f if already known: frames skipped for an
-// empty mask pass their interval on unchanged.
-__device__ int known_interval(const Chunk &C, int f) {
-    int j = f;
-    while (j > 0 && !C.any[j - 1]) j--;
-    return __hip_atomic_load(&C.pub[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// The check interval of frame f, found by walking the frames before it from
+// (*wj, *wn): frame j's interval is wn, and its successor's follows from the
+// iteration at which j's schedule wn stopped, once j has published it.
+// Frames skipped for an empty mask pass the interval on unchanged.  Returns
+// 0 while a predecessor's needed schedule is still running; the walk resumes
+// from where it stopped at the next call.
+__device__ int walk_interval(const Chunk &C, int f, int *wj, int *wn) {
+    for (int steps = 0; *wj < f && steps < 64; steps++) {
+        const int j = *wj;
+        if (C.any[j]) {
+            const int v = __hip_atomic_load(&C.stop[(size_t)j * kNSched + *wn - 1], __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+            if (v == 0) return 0;
+            *wn = check_interval(v - 1);
+        }
+        (*wj)++;
+    }
+    return *wj == f ? *wn : 0;
 }
 
 __global__ __launch_bounds__(kGsThreads) void k_gs(Chunk C) {
@@ -504,27 +598,29 @@ __global__ __launch_bounds__(kGsThreads) void k_gs(Chunk C) {
     const size_t fo = (size_t)f * C.np;
     float *x = C.x + fo;
     const float *lk = C.lk + fo, *lf = C.lf + fo, *rm = C.rm + fo * 8;
-    const uint8_t *mask = C.mask + fo;
+    const uint8_t *mask = C.mask + fo, *full = C.full + fo;
     const double *qs = C.qs + (size_t)f * QS_COUNT * C.np;
-    double *rowq = C.rowq + (size_t)f * C.th;
-    float *rowl = C.rowl + (size_t)f * C.th;
     float *snap = C.snap + (size_t)f * kNSlots * C.np;
 
+    __shared__ double s_pq[kGsThreads];
+    __shared__ float s_pl[kGsThreads];
     __shared__ float s_eo[kNSched];
-    __shared__ int s_iter[kNSched], s_slot[kNSched], s_ref[kNSlots];
+    __shared__ int s_slot[kNSched], s_ref[kNSlots];
     __shared__ unsigned s_active, s_stopped;
-    __shared__ int s_known, s_needE, s_copy, s_done;
+    __shared__ int s_known, s_needE, s_copy, s_done, s_wj, s_wn, s_knownAt, s_k;
 
-    float e0 = gs_energy(C, f, x, mask, qs, lf, rowq, rowl);
+    const float e0 = gs_energy(C, x, mask, full, qs, lf, s_pq, s_pl);
     if (threadIdx.x == 0) {
         for (int n = 0; n < kNSched; n++) {
             s_eo[n] = e0;
-            s_iter[n] = 0;
             s_slot[n] = -1;  // xo starts at zero
         }
         for (int s = 0; s < kNSlots; s++) s_ref[s] = 0;
         s_stopped = 0;
-        s_known = known_interval(C, f);
+        s_wj = 0;
+        s_wn = C.carry;
+        s_known = walk_interval(C, f, &s_wj, &s_wn);
+        s_knownAt = s_known ? 0 : -1;
         s_active = s_known ? 1u << (s_known - 1) : (1u << kNSched) - 1;
     }
     __syncthreads();
@@ -538,7 +634,7 @@ __global__ __launch_bounds__(kGsThreads) void k_gs(Chunk C) {
         }
         __syncthreads();
         float e = 0.f;
-        if (s_needE) e = gs_energy(C, f, x, mask, qs, lf, rowq, rowl);
+        if (s_needE) e = gs_energy(C, x, mask, full, qs, lf, s_pq, s_pl);
         if (threadIdx.x == 0) {
             unsigned pass = 0;
             if (s_needE) {
@@ -547,7 +643,8 @@ __global__ __launch_bounds__(kGsThreads) void k_gs(Chunk C) {
                     if (!(s_active & ~s_stopped & bit) || k % n) continue;
                     if (s_eo[n - 1] <= e) {
                         s_stopped |= bit;
-                        s_iter[n - 1] = k;
+                        __hip_atomic_store(&C.stop[(size_t)f * kNSched + n - 1], k + 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
                     } else {
                         s_eo[n - 1] = e;
                         pass |= bit;
@@ -566,33 +663,35 @@ __global__ __launch_bounds__(kGsThreads) void k_gs(Chunk C) {
                     }
             }
             if (!s_known) {
-                const int kn = known_interval(C, f);
+                int wj = s_wj, wn = s_wn;
+                const int kn = walk_interval(C, f, &wj, &wn);
+                s_wj = wj;
+                s_wn = wn;
                 if (kn) {
+                    s_knownAt = k;
                     s_known = kn;
                     s_active = 1u << (kn - 1);
                 }
             }
             s_copy = copy;
+            s_k = k;
             s_done = (s_active & ~s_stopped) == 0 || k >= C.cap;
         }
         __syncthreads();
         if (s_copy >= 0) {
-            float *dst = snap + (size_t)s_copy * C.np;
-            for (int i = threadIdx.x; i < C.np; i += blockDim.x) dst[i] = x[i];
+            float4 *dst = (float4 *)(snap + (size_t)s_copy * C.np);
+            const float4 *src = (const float4 *)x;
+            const int n4 = C.np / 4;
+            for (int i = threadIdx.x; i < n4; i += kGsThreads) dst[i] = src[i];
+            for (int i = 4 * n4 + threadIdx.x; i < C.np; i += kGsThreads) snap[(size_t)s_copy * C.np + i] = x[i];
         }
         __syncthreads();
         if (s_done) break;
     }
     if (threadIdx.x == 0) {
-        int *tab = C.tab + (size_t)f * 3 * kNSched;
-        for (int n = 0; n < kNSched; n++) {
-            tab[n] = s_iter[n];
-            tab[kNSched + n] = s_slot[n];
-            tab[2 * kNSched + n] = ((s_stopped >> n) & 1) | ((s_active >> n) & 1) << 1;
-        }
-        if (s_known && ((s_stopped >> (s_known - 1)) & 1))
-            __hip_atomic_store(&C.pub[f + 1], check_interval(s_iter[s_known - 1]), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        for (int n = 0; n < kNSched; n++) C.slot[(size_t)f * kNSched + n] = s_slot[n];
+        C.diag[2 * f] = s_k;
+        C.diag[2 * f + 1] = s_knownAt;
     }
 }
 
@@ -624,9 +723,6 @@ extern "C" void ss_ibs_settings_default(ss_ibs_settings *s) {
     std::memset(s, 0, sizeof(*s));
     s->regfac = 3e-3;
     s->pixel_aspect = 1.0;
-    s->inpaint = 0;
-    s->max_iterations = 0;
-    s->chunk_frames = 0;
 }
 
 extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample_in *in,
@@ -640,7 +736,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
     if (st->inpaint)
         return handle_fail(h, SS_ERR_INVALID_ARGUMENT, "invert_bilinear_sampling: inpainting is not supported");
     if (!(st->pixel_aspect > 0) || !std::isfinite(st->regfac) || st->max_iterations < 0 ||
-        st->chunk_frames < 0)
+        st->chunk_frames < 0 || st->work_bytes < 0)
         return handle_fail(h, SS_ERR_INVALID_ARGUMENT, "invert_bilinear_sampling: bad settings");
     const ss_geometry_desc &g = *in->geometry;
     int32_t hc, wc, fh;
@@ -661,19 +757,23 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
     for (int i = 0; i < npd; i++) pg[i] = PixGeo{bx[i], by[i], tf[i], pk[i] - 1, 0};
     const size_t esz = in->data_type == SS_F64 ? 8 : in->data_type == SS_F32 ? 4 : 2;
     const size_t raw = (size_t)g.nlines * g.linewidth;
+    const bool devOut = st->outputs_on_device != 0;
 
-    // device memory per frame of a chunk
+    // device memory: the geometry, then per frame of a chunk
+    const size_t geoBytes = align256(sizeof(PixGeo) * (size_t)npd);
     const size_t perFrame =
-        align256(sizeof(double) * 2 * nk) + align256(sizeof(int) * (size_t)npd) * 3 +
+        align256(sizeof(double) * 2 * nk) + align256(sizeof(int) * (size_t)npd) * 2 +
         align256(sizeof(int) * (size_t)np) + align256(sizeof(int) * ((size_t)np + 1)) +
         align256(sizeof(int) * 4 * (size_t)np) + align256(sizeof(double) * 9 * (size_t)np) +
-        align256(sizeof(double) * (size_t)np) + align256(sizeof(float) * (size_t)np) * 5 +
-        align256((size_t)np) + align256(sizeof(double) * QS_COUNT * (size_t)np) +
+        align256(sizeof(double) * (size_t)np) + align256(sizeof(float) * (size_t)np) * 4 +
+        align256((size_t)np) * 2 + align256(sizeof(int)) + align256(sizeof(double) * QS_COUNT * (size_t)np) +
         align256(sizeof(float) * 8 * (size_t)np) + align256(sizeof(float) * kNSlots * (size_t)np) +
-        align256(sizeof(double) * th) + align256(sizeof(float) * th) + align256(sizeof(int) * 3 * kNSched) +
-        align256(sizeof(int) * 2) + (in->data_on_device ? 0 : align256(esz * raw)) +
-        align256(sizeof(float) * (size_t)np) + align256((size_t)np);
-    int B = st->chunk_frames > 0 ? st->chunk_frames : (int)std::max<size_t>(1, ((size_t)8 << 30) / perFrame);
+        align256(sizeof(int) * 2 * kNSched) + align256(sizeof(int)) * 3 +
+        (in->data_on_device ? 0 : align256(esz * raw)) +
+        (devOut ? 0 : align256(sizeof(float) * (size_t)np) + align256((size_t)np));
+    const size_t budget = st->work_bytes > 0 ? (size_t)st->work_bytes : ((size_t)32 << 30);
+    int B = st->chunk_frames > 0 ? st->chunk_frames
+                                 : (int)std::min<size_t>(65535, std::max<size_t>(1, budget / perFrame));
     B = std::min(B, in->n_frames);
     B = std::min(B, 65535);
     const int cap = st->max_iterations > 0 ? st->max_iterations : kDefaultCap;
@@ -681,10 +781,10 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
     hipStream_t s = ss::handle_stream(h);
     if (hipSetDevice(ss::handle_device(h)) != hipSuccess)
         return handle_fail(h, SS_ERR_DEVICE, "invert_bilinear_sampling: hipSetDevice failed");
-    char *base = nullptr;
-    PixGeo *dgeo = nullptr;
+    const size_t total = geoBytes + perFrame * (size_t)B;
+    char *base = (char *)ss::handle_scratch(h, total, &rc);
+    if (!base) return rc;
     std::string err;
-    rc = SS_OK;
 #define ICHK(expr)                                                           \
     do {                                                                     \
         hipError_t e_ = (expr);                                              \
@@ -695,16 +795,14 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
         }                                                                    \
     } while (0)
     {
-        const size_t total = perFrame * (size_t)B + 4096;
-        ICHK(hipMalloc(&dgeo, sizeof(PixGeo) * npd));
-        ICHK(hipMalloc((void **)&base, total));
-        ICHK(hipMemcpyAsync(dgeo, pg.data(), sizeof(PixGeo) * npd, hipMemcpyHostToDevice, s));
         size_t off = 0;
         auto take = [&](size_t bytes) {
             char *p = base + off;
             off += align256(bytes);
             return (void *)p;
         };
+        PixGeo *dgeo = (PixGeo *)take(sizeof(PixGeo) * (size_t)npd);
+        ICHK(hipMemcpyAsync(dgeo, pg.data(), sizeof(PixGeo) * npd, hipMemcpyHostToDevice, s));
         Chunk C{};
         C.th = th; C.tw = tw; C.np = np; C.npd = npd; C.hc = hc; C.nl = g.nlines;
         C.cl = g.cropping[0]; C.cb = g.cropping[2]; C.nk = nk; C.cap = cap;
@@ -725,6 +823,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
         C.wy = (double *)take(sizeof(double) * (size_t)np * B);
         C.iv = (float *)take(sizeof(float) * (size_t)np * B);
         C.mask = (uint8_t *)take((size_t)np * B);
+        C.full = (uint8_t *)take((size_t)np * B);
         C.any = (int *)take(sizeof(int) * (size_t)B);
         C.qs = (double *)take(sizeof(double) * QS_COUNT * (size_t)np * B);
         C.lf = (float *)take(sizeof(float) * (size_t)np * B);
@@ -732,22 +831,25 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
         C.rm = (float *)take(sizeof(float) * 8 * (size_t)np * B);
         C.x = (float *)take(sizeof(float) * (size_t)np * B);
         C.snap = (float *)take(sizeof(float) * kNSlots * (size_t)np * B);
-        C.rowq = (double *)take(sizeof(double) * th * (size_t)B);
-        C.rowl = (float *)take(sizeof(float) * th * (size_t)B);
-        C.tab = (int *)take(sizeof(int) * 3 * kNSched * (size_t)B);
-        C.pub = (int *)take(sizeof(int) * ((size_t)B + 1));
+        C.stop = (int *)take(sizeof(int) * kNSched * (size_t)B);
+        C.slot = (int *)take(sizeof(int) * kNSched * (size_t)B);
+        C.diag = (int *)take(sizeof(int) * 2 * (size_t)B);
         int *dchosen = (int *)take(sizeof(int) * (size_t)B);
-        float *dout = (float *)take(sizeof(float) * (size_t)np * B);
-        uint8_t *dmask = (uint8_t *)take((size_t)np * B);
         void *ddata = in->data_on_device ? nullptr : take(esz * raw * B);
+        float *dout = devOut ? nullptr : (float *)take(sizeof(float) * (size_t)np * B);
+        uint8_t *dmask = devOut ? nullptr : (uint8_t *)take((size_t)np * B);
         if (off > total) {
             rc = SS_ERR_DEVICE;
             err = "invert_bilinear_sampling: internal sizing error";
             goto done;
         }
-        std::vector<int> tab(3 * kNSched * (size_t)B), any(B), chosen(B);
+        std::vector<int> stop(kNSched * (size_t)B), slot(kNSched * (size_t)B), any(B), chosen(B), diag(2 * (size_t)B);
+        // SS_IBS_STATS=1 prints, per chunk, the Gauss-Seidel iterations the
+        // workgroups ran and when they learned their check interval
+        const char *statsEnv = std::getenv("SS_IBS_STATS");
+        const bool stats = statsEnv && statsEnv[0] == '1';
         std::vector<double> kbuf(2 * (size_t)nk * B);
-        int carry = 1;  // n_iter_error starts at 1 for the call
+        int carry = 1;  // n_iter_error starts at 1 for the call (steady_scan.m:326)
         for (int f0 = 0; f0 < in->n_frames; f0 += B) {
             const int nb = std::min(B, in->n_frames - f0);
             C.B = nb;
@@ -768,8 +870,8 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
             ICHK(hipMemsetAsync(C.cnt, 0, sizeof(int) * (size_t)np * nb, s));
             ICHK(hipMemsetAsync(C.win, 0xFF, sizeof(int) * 4 * (size_t)np * nb, s));
             ICHK(hipMemsetAsync(C.any, 0, sizeof(int) * (size_t)nb, s));
-            ICHK(hipMemsetAsync(C.pub, 0, sizeof(int) * ((size_t)nb + 1), s));
-            ICHK(hipMemcpyAsync(C.pub, &carry, sizeof(int), hipMemcpyHostToDevice, s));
+            ICHK(hipMemsetAsync(C.stop, 0, sizeof(int) * kNSched * (size_t)nb, s));
+            C.carry = carry;
             const dim3 gp((npd + 255) / 256, nb), gt((np + 255) / 256, nb);
             hipLaunchKernelGGL(k_points, gp, dim3(256), 0, s, C);
             hipLaunchKernelGGL(k_scan, dim3(nb), dim3(1024), 0, s, C);
@@ -784,7 +886,9 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
             hipLaunchKernelGGL(k_interact, gt, dim3(256), 0, s, C);
             hipLaunchKernelGGL(k_gs, dim3(nb), dim3(kGsThreads), 0, s, C);
             ICHK(hipGetLastError());
-            ICHK(hipMemcpyAsync(tab.data(), C.tab, sizeof(int) * 3 * kNSched * nb, hipMemcpyDeviceToHost, s));
+            ICHK(hipMemcpyAsync(stop.data(), C.stop, sizeof(int) * kNSched * nb, hipMemcpyDeviceToHost, s));
+            ICHK(hipMemcpyAsync(slot.data(), C.slot, sizeof(int) * kNSched * nb, hipMemcpyDeviceToHost, s));
+            if (stats) ICHK(hipMemcpyAsync(diag.data(), C.diag, sizeof(int) * 2 * nb, hipMemcpyDeviceToHost, s));
             ICHK(hipMemcpyAsync(any.data(), C.any, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
             ICHK(hipStreamSynchronize(s));
             // resolve the check intervals in frame order (steady_scan.m:326-395)
@@ -794,31 +898,52 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
                     if (iterations) iterations[f0 + f] = 0;
                     continue;
                 }
-                const int *t = &tab[(size_t)f * 3 * kNSched];
                 const int n = carry;
-                if (!(t[2 * kNSched + n - 1] & 1)) {
+                const int v = stop[(size_t)f * kNSched + n - 1];
+                if (v == 0) {
                     rc = SS_ERR_NUMERICAL;
                     err = "invert_bilinear_sampling: frame " + std::to_string(f0 + f) +
                           ": Gauss-Seidel did not stop within " + std::to_string(cap) + " iterations";
                     goto done;
                 }
-                chosen[f] = t[kNSched + n - 1];
-                if (iterations) iterations[f0 + f] = t[n - 1];
-                carry = check_interval(t[n - 1]);
+                chosen[f] = slot[(size_t)f * kNSched + n - 1];
+                if (iterations) iterations[f0 + f] = v - 1;
+                carry = check_interval(v - 1);
+            }
+            if (stats) {
+                long runSum = 0, knownSum = 0, needSum = 0;
+                int runMax = 0, knownMax = -1, unknown = 0, nf = 0;
+                for (int f = 0; f < nb; f++) {
+                    if (!any[f]) continue;
+                    nf++;
+                    runSum += diag[2 * f];
+                    runMax = std::max(runMax, diag[2 * f]);
+                    if (diag[2 * f + 1] < 0) unknown++;
+                    else { knownSum += diag[2 * f + 1]; knownMax = std::max(knownMax, diag[2 * f + 1]); }
+                }
+                for (int f0i = 0; f0i < nb; f0i++)
+                    if (any[f0i] && iterations) needSum += iterations[f0 + f0i];
+                std::fprintf(stderr,
+                             "[ibs] chunk of %d frames: iterations run mean %.1f max %d (needed mean %.1f); "
+                             "interval known at mean %.1f max %d, never in %d\n",
+                             nf, nf ? (double)runSum / nf : 0.0, runMax, nf ? (double)needSum / nf : 0.0,
+                             nf - unknown ? (double)knownSum / (nf - unknown) : 0.0, knownMax, unknown);
             }
             ICHK(hipMemcpyAsync(dchosen, chosen.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_output, gt, dim3(256), 0, s, C, dchosen, dout, dmask);
+            float *o = devOut ? out + (size_t)np * f0 : dout;
+            uint8_t *m = devOut ? mask_out + (size_t)np * f0 : dmask;
+            hipLaunchKernelGGL(k_output, gt, dim3(256), 0, s, C, dchosen, o, m);
             ICHK(hipGetLastError());
-            ICHK(hipMemcpyAsync(out + (size_t)np * f0, dout, sizeof(float) * (size_t)np * nb,
-                                hipMemcpyDeviceToHost, s));
-            ICHK(hipMemcpyAsync(mask_out + (size_t)np * f0, dmask, (size_t)np * nb, hipMemcpyDeviceToHost, s));
+            if (!devOut) {
+                ICHK(hipMemcpyAsync(out + (size_t)np * f0, dout, sizeof(float) * (size_t)np * nb,
+                                    hipMemcpyDeviceToHost, s));
+                ICHK(hipMemcpyAsync(mask_out + (size_t)np * f0, dmask, (size_t)np * nb, hipMemcpyDeviceToHost, s));
+            }
             ICHK(hipStreamSynchronize(s));
         }
     }
 done:
     (void)hipStreamSynchronize(s);
-    if (base) (void)hipFree(base);
-    if (dgeo) (void)hipFree(dgeo);
 #undef ICHK
     if (rc) return handle_fail(h, rc, err);
     return SS_OK;

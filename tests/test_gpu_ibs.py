@@ -117,3 +117,55 @@ def test_workload_size_512():
     nk = geom["nknots"]
     dx, dy = smooth_knots(rng, nk, 2, 0.3), smooth_knots(rng, nk, 2, 0.3)
     compare(np.asarray(fr), dx, dy, 512, 512, nk, label="512x512 uint16")
+
+
+class DeviceBuffer:
+    """Device memory from the HIP runtime the library itself links (a second
+    runtime, such as torch's, may not share the device with it in one process)."""
+    _hip = None
+
+    def __init__(self, nbytes):
+        import ctypes
+        if DeviceBuffer._hip is None:
+            DeviceBuffer._hip = ctypes.CDLL("libamdhip64.so.7")
+        self.ptr = ctypes.c_void_p()
+        assert DeviceBuffer._hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(nbytes)) == 0
+        self.nbytes = nbytes
+
+    def upload(self, a):
+        import ctypes
+        a = np.ascontiguousarray(a)
+        assert a.nbytes == self.nbytes
+        assert DeviceBuffer._hip.hipMemcpy(self.ptr, ctypes.c_void_p(a.ctypes.data), ctypes.c_size_t(a.nbytes), 1) == 0
+
+    def download(self, dtype, count):
+        import ctypes
+        a = np.empty(count, dtype)
+        assert DeviceBuffer._hip.hipMemcpy(ctypes.c_void_p(a.ctypes.data), self.ptr, ctypes.c_size_t(a.nbytes), 2) == 0
+        return a
+
+    def free(self):
+        DeviceBuffer._hip.hipFree(self.ptr)
+
+
+def test_device_resident_frames_and_outputs_match_the_host_path():
+    T, fr, geom, mot = W.config0(n_frames=3, size=64, seed=23)
+    rng = np.random.default_rng(24)
+    nk = geom["nknots"]
+    fr = np.clip(np.round(np.asarray(fr)), 0, 60000).astype(np.uint16)
+    dx, dy = smooth_knots(rng, nk, 3), smooth_knots(rng, nk, 3)
+    host_out, host_mask = ssa.invert_bilinear_sampling(fr, geom, dx, dy)
+    n = 64 * 64 * 3
+    d_in, d_out, d_mask = DeviceBuffer(2 * n), DeviceBuffer(4 * n), DeviceBuffer(n)
+    try:
+        d_in.upload(np.asfortranarray(fr).ravel(order="F"))
+        ret = ssa.invert_bilinear_sampling((d_in.ptr.value, np.uint16, 3), geom, dx, dy,
+                                           device_out=(d_out.ptr.value, d_mask.ptr.value))
+        assert ret is None
+        out = d_out.download(np.float32, n).reshape(64, 64, 3, order="F")
+        mask = d_mask.download(np.uint8, n).reshape(64, 64, 3, order="F").astype(bool)
+    finally:
+        for b in (d_in, d_out, d_mask):
+            b.free()
+    np.testing.assert_array_equal(mask, host_mask)
+    np.testing.assert_array_equal(out, host_out)
