@@ -317,7 +317,8 @@ ss_status ss_resample(ss_handle *h, const ss_resample_in *in, float *out);
 typedef struct {
     double regfac;          /* settings.ibs_regfac (3e-3)                          */
     double pixel_aspect;    /* settings.ibs_template_pixel_aspect_ratio (1)        */
-    int32_t inpaint;        /* settings.ibs_inpaint: must be 0 (not supported)     */
+    int32_t inpaint;        /* settings.ibs_inpaint: must be 0; inpainting is
+                               ss_fill_gaps on the result                         */
     int32_t max_iterations; /* Gauss-Seidel iterations allowed per frame; 0 = 100000.
                                The reference has no bound; a frame that needs more
                                fails the call with SS_ERR_NUMERICAL               */
@@ -343,6 +344,30 @@ void ss_ibs_settings_default(ss_ibs_settings *s);
 ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample_in *in,
                                       const ss_ibs_settings *s, float *out, uint8_t *mask,
                                       int32_t *iterations);
+
+/* ---- frame preparation and gap filling ---- */
+
+/* Replaces prefilter_foralignment(data, gsd, mednorm) (steady_scan.m:625-643):
+ * frames height x width x n_frames (column-major, element type `type`) are
+ * blurred by the reference's circular Gaussian on the grid padded by
+ * L = ceil(sqrt(2 gsd^2 log(10000 / (sqrt(2 pi) gsd)))) pixels (gsd =
+ * pregauss > 0), then divided by each frame's median (mednorm != 0).  The
+ * result keeps the element type, as MATLAB keeps the class: integer frames are
+ * rounded half away from zero and saturated.  Median normalisation of integer
+ * frames is not supported (SS_ERR_INVALID_ARGUMENT).  in == out is allowed;
+ * on_device: both are device pointers. */
+ss_status ss_prefilter(ss_handle *h, ss_dtype type, const void *in, void *out, int32_t height,
+                       int32_t width, int32_t n_frames, double pregauss, int32_t mednorm,
+                       int32_t on_device);
+
+/* Replaces imgapfill_leastsquares(a) (steady_scan.m:525-560), the inpainting
+ * of invert_bilinear_sampling's ibs_inpaint: the NaN pixels of each frame are
+ * set to the minimiser of the summed squared adjacent differences with the
+ * other pixels fixed (a frame without any known pixel is left as it is).
+ * type is SS_F32 or SS_F64; the solve runs in FP64 (relative residual 1e-13)
+ * and rounds to the type.  in == out is allowed; on_device: device pointers. */
+ss_status ss_fill_gaps(ss_handle *h, ss_dtype type, const void *in, void *out, int32_t height,
+                       int32_t width, int32_t n_frames, int32_t on_device);
 
 #ifdef __cplusplus
 }

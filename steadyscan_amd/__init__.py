@@ -6,10 +6,10 @@ reference's mex interface.  Importing the package does not touch the GPU.
 """
 from .api import (Handle, Server, SteadyScanError, align_group_to_template, align_groups,  # noqa: F401
                   default_settings, estimate_displacements, geometry_desc, image2blockmatrix,
-                  init_settings, invert_bilinear_sampling, load_rom, parse_flags, resample_frames,
-                  save_rom)
+                  imgapfill_leastsquares, init_settings, invert_bilinear_sampling, load_rom, parse_flags,
+                  prefilter_foralignment, resample_frames, save_rom)
 
 __all__ = ["Handle", "Server", "SteadyScanError", "align_group_to_template", "align_groups",
            "default_settings", "estimate_displacements", "geometry_desc", "image2blockmatrix",
-           "init_settings", "invert_bilinear_sampling", "load_rom", "parse_flags", "resample_frames",
-           "save_rom"]
+           "imgapfill_leastsquares", "init_settings", "invert_bilinear_sampling", "load_rom", "parse_flags",
+           "prefilter_foralignment", "resample_frames", "save_rom"]

@@ -157,6 +157,9 @@ API = [
     ("ss_pack_frames", c_int32, [POINTER(SSGeometryDesc), c_int32, c_void_p, c_int32, c_void_p]),
     ("ss_resample", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(ctypes.c_float)]),
     ("ss_ibs_settings_default", None, [POINTER(SSIbsSettings)]),
+    ("ss_prefilter", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_double, c_int32,
+                               c_int32]),
+    ("ss_fill_gaps", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32]),
     ("ss_invert_bilinear_sampling", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(SSIbsSettings),
                                               POINTER(ctypes.c_float), POINTER(c_uint8), POINTER(c_int32)]),
     ("ss_profile_enable", c_int32, [c_void_p, c_int32]),

@@ -676,7 +676,6 @@ def invert_bilinear_sampling(frames, geometry, dx_knots, dy_knots, ibs_regfac=3e
     N.lib().ss_ibs_settings_default(byref(st))
     st.regfac = float(ibs_regfac)
     st.pixel_aspect = float(ibs_template_pixel_aspect_ratio)
-    st.inpaint = 1 if ibs_inpaint else 0
     st.max_iterations = int(max_iterations)
     st.chunk_frames = int(chunk_frames)
     st.work_bytes = int(work_bytes)
@@ -687,6 +686,9 @@ def invert_bilinear_sampling(frames, geometry, dx_knots, dy_knots, ibs_regfac=3e
                                                    ctypes.cast(c_void_p(int(device_out[0])), POINTER(c_float)),
                                                    ctypes.cast(c_void_p(int(device_out[1])), POINTER(c_uint8)),
                                                    _ptr(iters, c_int32)), h)
+        if ibs_inpaint:  # in place, single precision
+            _check(N.lib().ss_fill_gaps(h.ptr, N.SS_F32, c_void_p(int(device_out[0])), c_void_p(int(device_out[0])),
+                                        g.template_h, g.template_w, n, 1), h)
         if info is not None:
             info["iterations"] = iters
         return None
@@ -696,4 +698,45 @@ def invert_bilinear_sampling(frames, geometry, dx_knots, dy_knots, ibs_regfac=3e
                                                _ptr(mask, c_uint8), _ptr(iters, c_int32)), h)
     if info is not None:
         info["iterations"] = iters
+    if ibs_inpaint:
+        # data_corrected has the class of the frames (steady_scan.m:312): single
+        # frames stay single, everything else is filled in double
+        single = ri.data_type == N.SS_F32
+        out = imgapfill_leastsquares(out if single else out.astype(np.float64), handle=h)
     return out, mask.astype(bool)
+
+
+def prefilter_foralignment(data, gsd, mednorm, handle=None, device=0):
+    """prefilter_foralignment (steady_scan.m:625-643) on the GPU: Gaussian blur
+    (gsd > 0) and median normalisation of every frame of an H x W (x n) array;
+    the result keeps the element type (MATLAB's class)."""
+    a = np.asarray(data)
+    squeeze = a.ndim == 2
+    if squeeze:
+        a = a[:, :, None]
+    if a.dtype not in _DTYPES:
+        a = a.astype(np.float64)
+    a = np.asfortranarray(a)
+    out = np.empty_like(a, order="F")
+    h = handle if handle is not None else _default_handle(device)
+    _check(N.lib().ss_prefilter(h.ptr, _DTYPES[a.dtype], c_void_p(a.ctypes.data), c_void_p(out.ctypes.data),
+                                a.shape[0], a.shape[1], a.shape[2], float(gsd), 1 if mednorm else 0, 0), h)
+    return out[:, :, 0] if squeeze else out
+
+
+def imgapfill_leastsquares(a, handle=None, device=0):
+    """imgapfill_leastsquares (steady_scan.m:525-560) on the GPU: the NaN pixels
+    of each frame filled with the minimiser of the summed squared adjacent
+    differences (float32 or float64 frames; other types are taken as float64)."""
+    x = np.asarray(a)
+    squeeze = x.ndim == 2
+    if squeeze:
+        x = x[:, :, None]
+    if x.dtype not in (np.dtype(np.float32), np.dtype(np.float64)):
+        x = x.astype(np.float64)
+    x = np.asfortranarray(x)
+    out = np.empty_like(x, order="F")
+    h = handle if handle is not None else _default_handle(device)
+    _check(N.lib().ss_fill_gaps(h.ptr, _DTYPES[x.dtype], c_void_p(x.ctypes.data), c_void_p(out.ctypes.data),
+                                x.shape[0], x.shape[1], x.shape[2], 0), h)
+    return out[:, :, 0] if squeeze else out

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsteadyscan_amd.so")
 SYNTH_LIB = os.path.join(HERE, "libss_synth.so")
 
-HIP_SOURCES = ["ss_kernels.hip", "ss_engine.hip", "ss_resample.hip", "ss_ibs.hip"]
+HIP_SOURCES = ["ss_kernels.hip", "ss_engine.hip", "ss_resample.hip", "ss_ibs.hip", "ss_prefilter.hip"]
 CXX_SOURCES = ["ss_geometry.cpp", "ss_driver.cpp", "ss_rom.cpp"]
 
 # packed FP32 instructions are disabled (they have produced silent errors on

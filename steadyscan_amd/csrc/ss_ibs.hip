@@ -734,7 +734,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
         !in->knots_y || in->n_frames < 1)
         return handle_fail(h, SS_ERR_INVALID_ARGUMENT, "invert_bilinear_sampling: NULL argument or no frames");
     if (st->inpaint)
-        return handle_fail(h, SS_ERR_INVALID_ARGUMENT, "invert_bilinear_sampling: inpainting is not supported");
+        return handle_fail(h, SS_ERR_INVALID_ARGUMENT, "invert_bilinear_sampling: inpainting is a separate call (ss_fill_gaps on the result)");
     if (!(st->pixel_aspect > 0) || !std::isfinite(st->regfac) || st->max_iterations < 0 ||
         st->chunk_frames < 0 || st->work_bytes < 0)
         return handle_fail(h, SS_ERR_INVALID_ARGUMENT, "invert_bilinear_sampling: bad settings");
