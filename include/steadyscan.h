@@ -251,6 +251,9 @@ typedef struct {
     double corr_thresh;          /* settings.corr_thresh (0.75)                      */
     double minblockfrac;         /* settings.minblockfrac (0.25)                     */
     int32_t frame_groupsize;     /* settings.gpu_frame_groupsize (2)                 */
+    double pregauss;             /* settings.pregauss (0): prefilter Gaussian sigma  */
+    int32_t mednorm;             /* settings.mednorm (0): divide frames by median    */
+    int32_t pad;
 } ss_driver_settings;
 
 void ss_driver_settings_default(ss_driver_settings *d);
@@ -276,8 +279,10 @@ typedef struct {
  * are unpacked with the reference's success tests (steady_scan.m:246-284).
  * All sets go to the device in one batched submission.  est_dx/est_dy
  * (nknots x n_frames) may be NULL for zero estimates; as in the reference,
- * every set reads their first columns.  The prefilter (pregauss, mednorm) is
- * not applied: it is a no-op at its defaults. */
+ * every set reads their first columns.  Each set's frames are prefiltered
+ * (pregauss, mednorm; steady_scan.m:228) before packing, as ss_prefilter does;
+ * the template is used as given (steady_scan.m:94 prefilters it before the
+ * call; ss_prefilter does that step). */
 ss_status ss_estimate_displacements(ss_handle *h, const ss_geometry_desc *geometry,
                                     const ss_driver_settings *driver, const ss_settings *s,
                                     const double *template_image, ss_dtype type, const void *frames,
@@ -368,6 +373,21 @@ ss_status ss_prefilter(ss_handle *h, ss_dtype type, const void *in, void *out, i
  * and rounds to the type.  in == out is allowed; on_device: device pointers. */
 ss_status ss_fill_gaps(ss_handle *h, ss_dtype type, const void *in, void *out, int32_t height,
                        int32_t width, int32_t n_frames, int32_t on_device);
+
+/* The reconstruction loop of steady_scan (steady_scan.m:124-134, and
+ * :136-154 for each extra channel): frames are taken set by set (sets start
+ * every frames_per_set frames, without overlap), cropped, prefiltered
+ * (pregauss, mednorm) and reconstructed by invert_bilinear_sampling with the
+ * frames' knots (nknots x n_frames; a frame with NaN knots, such as the
+ * stack's last frame, comes out NaN), the energy-check interval restarting
+ * at each set.  With ibs->inpaint, each set's result is gap-filled
+ * (ss_fill_gaps) in single precision for SS_F32 frames and in double
+ * otherwise, as the reference fills in the frames' class.  data_corrected
+ * (double) and mask are template_h x template_w x n_frames. */
+ss_status ss_reconstruct(ss_handle *h, const ss_geometry_desc *geometry, const ss_driver_settings *driver,
+                         const ss_ibs_settings *ibs, ss_dtype type, const void *frames, int32_t n_frames,
+                         const double *dx_knots, const double *dy_knots, double *data_corrected,
+                         uint8_t *mask);
 
 #ifdef __cplusplus
 }

@@ -1,5 +1,7 @@
-"""CPU restatement of SteadyScan's estimate_displacements
-(steady_scan.m:157-288) over the FP64 oracle of align_group_to_template.
+"""CPU restatement of SteadyScan's driver: estimate_displacements
+(steady_scan.m:157-288) over the FP64 oracle of align_group_to_template, and
+the reconstruction loop (steady_scan.m:124-134) over the oracle of
+invert_bilinear_sampling.
 
 TEST INFRASTRUCTURE ONLY: used by tests/ to check the library's
 ss_estimate_displacements; never called by the product path.
@@ -20,6 +22,7 @@ from ctypes import POINTER, c_int32
 import numpy as np
 
 from . import oracle as O
+from . import prefilter as P
 
 
 def set_starts(nframes, frames_per_set):
@@ -73,7 +76,7 @@ def _image2blockmatrix(x, pk, nb):
 def estimate_displacements(frames, template, align_args, nknots=33, cropping=(0, 0, 0, 0), subrec=None,
                            bidirectional=False, min_blockpoints_ratio=0.3, frames_per_set=500,
                            corr_thresh=0.75, minblockfrac=0.25, frame_groupsize=2, est_dx=None, est_dy=None,
-                           threads=8):
+                           threads=8, pregauss=0.0, mednorm=False):
     """frames: nlines x linewidth x n; align_args: the scalar mex arguments
     (flags, max_iter, move_thresh, ...) as in workloads.default_args().
     Returns dict(dx_knots, dy_knots, corr, iter_used, residrms, failed)."""
@@ -104,9 +107,12 @@ def estimate_displacements(frames, template, align_args, nknots=33, cropping=(0,
     for iset in range(len(starts) - 1):
         ii = list(range(starts[iset], min(starts[iset + 1], n - 1) + 1))
         ns = len(ii)
+        # setdata = prefilter_foralignment(frames(ii), pregauss, mednorm) (:228)
+        setdata = P.prefilter_foralignment(frames[:, :, ii], pregauss, mednorm) if (pregauss > 0 or mednorm) \
+            else frames[:, :, ii]
         bm = np.zeros((tfrac_bm.shape[0], nb * ns))
         for k, fi in enumerate(ii):
-            img = np.asarray(frames[c2:H - c3, c0:Wd - c1, fi], dtype=np.float64)
+            img = np.asarray(setdata[c2:H - c3, c0:Wd - c1, k], dtype=np.float64)
             bm[:, k * nb:(k + 1) * nb], _ = _image2blockmatrix(img.ravel(order="F"), pk, nb)
         a = dict(align_args)
         a.update(template_image=np.asfortranarray(template, dtype=np.float64), xgrad=np.asfortranarray(xgrad),
@@ -134,3 +140,35 @@ def estimate_displacements(frames, template, align_args, nknots=33, cropping=(0,
                 dy[:, j] = extend_firstandlast(dy[:, j], used)
             failed[j] = (not (r["frame_corr"][sj] >= corr_thresh)) or (len(used) / nb < minblockfrac)
     return dict(dx_knots=dx, dy_knots=dy, corr=corr, iter_used=iters, residrms=resid, failed=failed)
+
+
+def reconstruct(frames, dx_knots, dy_knots, template_shape, nknots=33, cropping=(0, 0, 0, 0), subrec=None,
+                bidirectional=False, frames_per_set=500, pregauss=0.0, mednorm=False, ibs_inpaint=False,
+                ibs_regfac=3e-3, aspect=1.0):
+    """The reconstruction loop of steady_scan (steady_scan.m:124-134): per set
+    ii = set_start:set_start_next - 1, prefilter_foralignment(cropimage(frames))
+    then invert_bilinear_sampling with the frames' knots (the energy-check
+    interval restarts per call), inpainted if ibs_inpaint (in the frames'
+    class: single for float32 frames, double otherwise).  Returns
+    (data_corrected float64 th x tw x n, mask)."""
+    H, Wd, n = frames.shape
+    th, tw = template_shape
+    (hc, wc), pk, tf, bx, by = _geometry(H, Wd, th, tw, nknots, cropping, subrec, bidirectional, 0.3)
+    geom = dict(prevknotind=pk.reshape(hc, wc, order="F"), tfrac=tf.reshape(hc, wc, order="F"),
+                basex=bx.reshape(hc, wc, order="F"), basey=by.reshape(hc, wc, order="F"))
+    c0, c1, c2, c3 = cropping
+    dc = np.zeros((th, tw, n))
+    mk = np.zeros((th, tw, n), bool)
+    starts = set_starts(n, frames_per_set)
+    for iset in range(len(starts) - 1):
+        ii = list(range(starts[iset], starts[iset + 1]))
+        setdata = frames[c2:H - c3, c0:Wd - c1, ii]
+        if pregauss > 0 or mednorm:
+            setdata = P.prefilter_foralignment(setdata, pregauss, mednorm)
+        out, m, _ = O.invert_bilinear_sampling(np.asarray(setdata, dtype=np.float64), dx_knots[:, ii],
+                                               dy_knots[:, ii], geom, (th, tw), regfac=ibs_regfac, aspect=aspect)
+        if ibs_inpaint:
+            out = P.imgapfill_leastsquares(out if frames.dtype == np.float32 else out.astype(np.float64))
+        dc[:, :, ii] = out
+        mk[:, :, ii] = m
+    return dc, mk

@@ -7,9 +7,9 @@ reference's mex interface.  Importing the package does not touch the GPU.
 from .api import (Handle, Server, SteadyScanError, align_group_to_template, align_groups,  # noqa: F401
                   default_settings, estimate_displacements, geometry_desc, image2blockmatrix,
                   imgapfill_leastsquares, init_settings, invert_bilinear_sampling, load_rom, parse_flags,
-                  prefilter_foralignment, resample_frames, save_rom)
+                  prefilter_foralignment, reconstruct, resample_frames, save_rom, steady_scan)
 
 __all__ = ["Handle", "Server", "SteadyScanError", "align_group_to_template", "align_groups",
            "default_settings", "estimate_displacements", "geometry_desc", "image2blockmatrix",
            "imgapfill_leastsquares", "init_settings", "invert_bilinear_sampling", "load_rom", "parse_flags",
-           "prefilter_foralignment", "resample_frames", "save_rom"]
+           "prefilter_foralignment", "reconstruct", "resample_frames", "save_rom", "steady_scan"]

@@ -71,7 +71,8 @@ class SSGroupOut(ctypes.Structure):
 
 class SSDriverSettings(ctypes.Structure):
     _fields_ = [("frames_per_set", c_int32), ("min_frames_per_set", c_int32), ("corr_thresh", c_double),
-                ("minblockfrac", c_double), ("frame_groupsize", c_int32)]
+                ("minblockfrac", c_double), ("frame_groupsize", c_int32), ("pregauss", c_double),
+                ("mednorm", c_int32), ("pad", c_int32)]
 
 
 class SSDisplacementsOut(ctypes.Structure):
@@ -160,6 +161,9 @@ API = [
     ("ss_prefilter", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_double, c_int32,
                                c_int32]),
     ("ss_fill_gaps", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32]),
+    ("ss_reconstruct", c_int32, [c_void_p, POINTER(SSGeometryDesc), POINTER(SSDriverSettings), POINTER(SSIbsSettings),
+                                 c_int32, c_void_p, c_int32, POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                                 POINTER(c_uint8)]),
     ("ss_invert_bilinear_sampling", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(SSIbsSettings),
                                               POINTER(ctypes.c_float), POINTER(c_uint8), POINTER(c_int32)]),
     ("ss_profile_enable", c_int32, [c_void_p, c_int32]),

@@ -472,7 +472,7 @@ def estimate_displacements(frames, template, flags="D", nknots=33, cropping=None
                            isbidirectional=False, min_blockpoints_ratio=0.3, frames_per_set=500,
                            min_frames_per_set=100, corr_thresh=0.75, minblockfrac=0.25, frame_groupsize=2,
                            est_dx_knots=None, est_dy_knots=None, handle=None, device=0,
-                           linesearch_candidates=2, **align):
+                           linesearch_candidates=2, pregauss=0.0, mednorm=False, **align):
     """estimate_displacements of steady_scan.m:157-288 on a frame stack
     (nlines x linewidth x n, f64/f32/u16/i16, frame_list already applied):
     sets of frames_per_set frames (+1 overlap) are aligned in one batched
@@ -495,6 +495,7 @@ def estimate_displacements(frames, template, flags="D", nknots=33, cropping=None
     N.lib().ss_driver_settings_default(byref(d))
     d.frames_per_set, d.min_frames_per_set = int(frames_per_set), int(min_frames_per_set)
     d.corr_thresh, d.minblockfrac, d.frame_groupsize = float(corr_thresh), float(minblockfrac), int(frame_groupsize)
+    d.pregauss, d.mednorm = float(pregauss), 1 if mednorm else 0
     precision, _ = parse_flags(flags)
     a = default_args_dict()
     a.update(align)
@@ -531,6 +532,84 @@ def estimate_displacements(frames, template, flags="D", nknots=33, cropping=None
                                              _ptr(ey, c_double) if ey is not None else None, byref(o)), h)
     out["failed"] = out["failed"].astype(bool)
     return out
+
+
+def reconstruct(frames, geometry, dx_knots, dy_knots, frames_per_set=500, pregauss=0.0, mednorm=False,
+                ibs_inpaint=False, ibs_regfac=3e-3, ibs_template_pixel_aspect_ratio=1.0, handle=None, device=0):
+    """The reconstruction loop of steady_scan (steady_scan.m:124-134; ss_reconstruct):
+    set by set, crop + prefilter + invert_bilinear_sampling (+ inpainting).
+    frames: nlines x linewidth x n raw frames; knots nknots x n.  Returns
+    (data_corrected float64 template_h x template_w x n, mask bool)."""
+    h = handle if handle is not None else _default_handle(device)
+    fr = np.asarray(frames)
+    if fr.ndim == 2:
+        fr = fr[:, :, None]
+    if fr.dtype not in _DTYPES:
+        fr = fr.astype(np.float64)
+    fr = np.asfortranarray(fr)
+    g = geometry["desc"] if isinstance(geometry, dict) else geometry
+    n = fr.shape[2]
+    d = N.SSDriverSettings()
+    N.lib().ss_driver_settings_default(byref(d))
+    d.frames_per_set = int(frames_per_set)
+    d.pregauss, d.mednorm = float(pregauss), 1 if mednorm else 0
+    st = N.SSIbsSettings()
+    N.lib().ss_ibs_settings_default(byref(st))
+    st.regfac = float(ibs_regfac)
+    st.pixel_aspect = float(ibs_template_pixel_aspect_ratio)
+    st.inpaint = 1 if ibs_inpaint else 0
+    kx = _f64(np.reshape(dx_knots, (g.nknots, n), order="F"))
+    ky = _f64(np.reshape(dy_knots, (g.nknots, n), order="F"))
+    out = np.zeros((g.template_h, g.template_w, n), order="F")
+    mask = np.zeros((g.template_h, g.template_w, n), np.uint8, order="F")
+    _check(N.lib().ss_reconstruct(h.ptr, byref(g), byref(d), byref(st), _DTYPES[fr.dtype], c_void_p(fr.ctypes.data),
+                                  n, _ptr(kx, c_double), _ptr(ky, c_double), _ptr(out, c_double),
+                                  _ptr(mask, c_uint8)), h)
+    return out, mask.astype(bool)
+
+
+def steady_scan(data, template_image, est_dx_knots=None, est_dy_knots=None, data_extra_channels=None,
+                flags="D", nknots=33, cropping=None, subrec=None, isbidirectional=False,
+                min_blockpoints_ratio=0.3, frames_per_set=500, min_frames_per_set=100, corr_thresh=0.75,
+                minblockfrac=0.25, frame_groupsize=2, pregauss=0.0, mednorm=False, ibs_inpaint=False,
+                ibs_regfac=3e-3, ibs_template_pixel_aspect_ratio=1.0, handle=None, device=0, **align):
+    """steady_scan (steady_scan.m:1-155) on a frame stack (nlines x linewidth x n,
+    frame_list already applied): the template is prefiltered (:94), the
+    displacements estimated set by set (:111, estimate_displacements) and the
+    frames reconstructed set by set (:124-134); data_extra_channels
+    (nlines x linewidth x n x n_extra) are reconstructed with the same knots
+    (:136-154).  As in the reference, extra channel k (1-based) is written to
+    channel slot k, so the first extra channel replaces the main channel's
+    corrected frames and the last slot stays zero.  Returns dict(dx_knots,
+    dy_knots, iter_used, corr, failed, residrms, data_corrected (th x tw x n x
+    (1 + n_extra), float64), mask (same shape))."""
+    h = handle if handle is not None else _default_handle(device)
+    fr = np.asarray(data)
+    if fr.ndim == 2:
+        fr = fr[:, :, None]
+    T = prefilter_foralignment(_f64(template_image), pregauss, mednorm, handle=h)
+    est = estimate_displacements(fr, T, flags=flags, nknots=nknots, cropping=cropping, subrec=subrec,
+                                 isbidirectional=isbidirectional, min_blockpoints_ratio=min_blockpoints_ratio,
+                                 frames_per_set=frames_per_set, min_frames_per_set=min_frames_per_set,
+                                 corr_thresh=corr_thresh, minblockfrac=minblockfrac, frame_groupsize=frame_groupsize,
+                                 est_dx_knots=est_dx_knots, est_dy_knots=est_dy_knots, handle=h, pregauss=pregauss,
+                                 mednorm=mednorm, **align)
+    g = geometry_desc(fr.shape[0], fr.shape[1], T.shape, nknots=nknots, cropping=cropping, subrec=subrec,
+                      isbidirectional=isbidirectional, min_blockpoints_ratio=min_blockpoints_ratio)
+    extra = None if data_extra_channels is None else np.asarray(data_extra_channels)
+    if extra is not None and extra.ndim == 3:
+        extra = extra[:, :, :, None]
+    nch = 1 + (0 if extra is None else extra.shape[3])
+    n = fr.shape[2]
+    dc = np.zeros(T.shape + (n, nch), order="F")
+    mk = np.zeros(T.shape + (n, nch), bool, order="F")
+    kw = dict(frames_per_set=frames_per_set, pregauss=pregauss, mednorm=mednorm, ibs_inpaint=ibs_inpaint,
+              ibs_regfac=ibs_regfac, ibs_template_pixel_aspect_ratio=ibs_template_pixel_aspect_ratio, handle=h)
+    dc[:, :, :, 0], mk[:, :, :, 0] = reconstruct(fr, g, est["dx_knots"], est["dy_knots"], **kw)
+    for ch in range(nch - 1):
+        dc[:, :, :, ch], mk[:, :, :, ch] = reconstruct(extra[:, :, :, ch], g, est["dx_knots"], est["dy_knots"], **kw)
+    est.update(data_corrected=dc, mask=mk)
+    return est
 
 
 def default_args_dict():

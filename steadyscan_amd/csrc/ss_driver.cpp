@@ -1,11 +1,13 @@
-// ss_driver.cpp -- estimate_displacements (steady_scan.m:93-288): a whole
-// frame stack aligned set by set through the C ABI.
+// ss_driver.cpp -- the steady_scan driver (steady_scan.m:93-288): a whole
+// frame stack aligned set by set (estimate_displacements) and reconstructed
+// set by set (the loop of steady_scan.m:124-154) through the C ABI.
 //
 // The reference aligns its sets one after another; here every set of the
 // stack is one group of a single batched submission (ss_align_batch), with
 // the template and geometry resident.  The set partition, the +1-frame
-// overlap, the unpacking and the per-frame success tests follow the
-// reference line by line.
+// overlap, the prefiltering, the unpacking and the per-frame success tests
+// follow the reference line by line.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -52,6 +54,8 @@ extern "C" void ss_driver_settings_default(ss_driver_settings *d) {
     d->corr_thresh = 0.75;
     d->minblockfrac = 0.25;
     d->frame_groupsize = 2;
+    d->pregauss = 0;
+    d->mednorm = 0;
 }
 
 extern "C" ss_status ss_estimate_displacements(ss_handle *h, const ss_geometry_desc *geom,
@@ -61,7 +65,7 @@ extern "C" ss_status ss_estimate_displacements(ss_handle *h, const ss_geometry_d
                                                ss_displacements_out *out) {
     if (!h || !geom || !ds || !s || !template_image || !frames || !out || n_frames < 1)
         return SS_ERR_INVALID_ARGUMENT;
-    if (ds->frames_per_set < 1 || ds->frame_groupsize < 1) return SS_ERR_INVALID_ARGUMENT;
+    if (ds->frames_per_set < 1 || ds->frame_groupsize < 1 || !(ds->pregauss >= 0)) return SS_ERR_INVALID_ARGUMENT;
     int32_t Hc, Wc, fh;
     ss_status st = ss_geometry_size(geom, &Hc, &Wc, &fh);
     if (st) return st;
@@ -114,6 +118,20 @@ extern "C" ss_status ss_estimate_displacements(ss_handle *h, const ss_geometry_d
         est_dx = est_dy = zeros.data();
     }
     const size_t esz = elem_size(type), rawFrame = (size_t)geom->nlines * geom->linewidth * esz;
+    // prefilter_foralignment of each set's (uncropped) frames (steady_scan.m:228);
+    // a frame's result does not depend on its set, so the stack is filtered once
+    std::vector<char> filtered;
+    if (ds->pregauss > 0 || ds->mednorm) {
+        filtered.resize(rawFrame * n_frames);
+        st = ss_prefilter(h, type, frames, filtered.data(), geom->nlines, geom->linewidth, n_frames, ds->pregauss,
+                          ds->mednorm, 0);
+        if (st) {
+            ss_release(h, tid);
+            ss_release(h, gid);
+            return st;
+        }
+        frames = filtered.data();
+    }
     std::vector<std::vector<char>> packed(nsets);
     std::vector<ss_group_in> gin(nsets);
     std::vector<ss_settings> gs(nsets, *s);
@@ -187,6 +205,78 @@ extern "C" ss_status ss_estimate_displacements(ss_handle *h, const ss_geometry_d
             if (out->corr) out->corr[j] = corr;
             // numel(blocksused) / (nknots - 1) is 1 (steady_scan.m:283)
             if (out->failed) out->failed[j] = (!(corr >= ds->corr_thresh) || 1.0 < ds->minblockfrac) ? 1 : 0;
+        }
+    }
+    return SS_OK;
+}
+
+extern "C" ss_status ss_reconstruct(ss_handle *h, const ss_geometry_desc *geom, const ss_driver_settings *ds,
+                                    const ss_ibs_settings *ibs, ss_dtype type, const void *frames, int32_t n_frames,
+                                    const double *dx_knots, const double *dy_knots, double *data_corrected,
+                                    uint8_t *mask) {
+    if (!h || !geom || !ds || !ibs || !frames || !dx_knots || !dy_knots || !data_corrected || !mask ||
+        n_frames < 1 || ds->frames_per_set < 1 || !(ds->pregauss >= 0))
+        return SS_ERR_INVALID_ARGUMENT;
+    int32_t Hc, Wc, fh;
+    ss_status st = ss_geometry_size(geom, &Hc, &Wc, &fh);
+    if (st) return st;
+    const int nl = geom->nlines, lw = geom->linewidth, nk = geom->nknots;
+    const size_t np = (size_t)geom->template_h * geom->template_w;
+    const size_t esz = elem_size(type), rawPix = (size_t)nl * lw;
+    const int cl = geom->cropping[0], cb = geom->cropping[2];
+    const bool prefilter = ds->pregauss > 0 || ds->mednorm;
+    ss_ibs_settings is = *ibs;
+    is.inpaint = 0;
+    std::vector<int> start;
+    set_starts(n_frames, ds->frames_per_set, start);
+    std::vector<char> cropped, raw;
+    std::vector<float> out;
+    std::vector<double> out64;
+    for (size_t i = 0; i < start.size(); i++) {
+        // ii = set_start(iset):min(set_start(iset + 1) - 1, nframes) (steady_scan.m:131)
+        const int f0 = start[i], f1 = (i + 1 < start.size()) ? start[i + 1] : n_frames, n = f1 - f0;
+        const char *src = (const char *)frames + rawPix * esz * f0;
+        if (prefilter) {
+            // prefilter_foralignment(cropimage(frames)) (steady_scan.m:132): crop,
+            // filter the cropped frames, and put them back into raw-size frames
+            // for the reconstruction, which crops again
+            cropped.assign((size_t)Hc * Wc * n * esz, 0);
+            raw.assign(rawPix * n * esz, 0);
+            for (int f = 0; f < n; f++)
+                for (int c = 0; c < Wc; c++)
+                    std::memcpy(cropped.data() + (((size_t)f * Wc + c) * Hc) * esz,
+                                src + ((size_t)f * rawPix + (size_t)(c + cl) * nl + cb) * esz, (size_t)Hc * esz);
+            st = ss_prefilter(h, type, cropped.data(), cropped.data(), Hc, Wc, n, ds->pregauss, ds->mednorm, 0);
+            if (st) return st;
+            for (int f = 0; f < n; f++)
+                for (int c = 0; c < Wc; c++)
+                    std::memcpy(raw.data() + ((size_t)f * rawPix + (size_t)(c + cl) * nl + cb) * esz,
+                                cropped.data() + (((size_t)f * Wc + c) * Hc) * esz, (size_t)Hc * esz);
+            src = raw.data();
+        }
+        ss_resample_in in;
+        std::memset(&in, 0, sizeof(in));
+        in.geometry = geom;
+        in.n_frames = n;
+        in.data_type = type;
+        in.frames = src;
+        in.knots_x = dx_knots + (size_t)nk * f0;
+        in.knots_y = dy_knots + (size_t)nk * f0;
+        out.resize(np * n);
+        st = ss_invert_bilinear_sampling(h, &in, &is, out.data(), mask + np * f0, nullptr);
+        if (st) return st;
+        double *dc = data_corrected + np * f0;
+        if (ibs->inpaint && type == SS_F32) {
+            // the frames' class is single: filled in double, rounded to single
+            st = ss_fill_gaps(h, SS_F32, out.data(), out.data(), geom->template_h, geom->template_w, n, 0);
+            if (st) return st;
+            for (size_t k = 0; k < np * n; k++) dc[k] = out[k];
+        } else if (ibs->inpaint) {
+            out64.assign(out.begin(), out.end());
+            st = ss_fill_gaps(h, SS_F64, out64.data(), dc, geom->template_h, geom->template_w, n, 0);
+            if (st) return st;
+        } else {
+            for (size_t k = 0; k < np * n; k++) dc[k] = out[k];
         }
     }
     return SS_OK;

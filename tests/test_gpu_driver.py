@@ -58,3 +58,86 @@ def test_driver_equals_oracle(case):
     if case == "failures":
         assert ref["failed"][:-1].any()
     compare(r, ref)
+
+
+@pytest.mark.parametrize("case", ["blur", "blur_u16", "mednorm"])
+def test_driver_with_prefilter(case):
+    """estimate_displacements prefilters each set's frames (steady_scan.m:228)."""
+    args = dict(W.default_args(), max_iter=12)
+    kw = dict(nknots=9, frames_per_set=16)
+    T, fr, geom, _ = W.config0(n_frames=20, size=64, seed=65)
+    if case == "blur_u16":
+        fr = np.asfortranarray(np.rint(fr).astype(np.uint16))
+    pre = dict(pregauss=0.0 if case == "mednorm" else 1.0, mednorm=case == "mednorm")
+    from oracle import prefilter as P
+    Tp = P.prefilter_foralignment(np.asarray(T, np.float64), pre["pregauss"], pre["mednorm"])
+    ref = D.estimate_displacements(fr, Tp, dict(args, flags="D"), threads=8, **kw, **pre)
+    r = ssa.estimate_displacements(fr, ssa.prefilter_foralignment(np.asarray(T, np.float64), **dict(
+        gsd=pre["pregauss"], mednorm=pre["mednorm"])), flags="D", **kw, **pre, **args)
+    compare(r, ref)
+
+
+@pytest.mark.parametrize("case", ["plain", "prefilter_inpaint", "f32_inpaint_crop"])
+def test_reconstruction_loop_equals_oracle(case):
+    """ss_reconstruct (steady_scan.m:124-134) against the oracle's loop, on the
+    same knots: set by set, crop + prefilter + invert_bilinear_sampling (+ fill).
+    Without the prefilter the corrected frames are bit-exact; the prefilter
+    (FFT vs circulant GEMM) and the fill (direct vs conjugate gradients) leave
+    FP64 rounding-level differences that could move a Gauss-Seidel stop, so
+    there the bar is 1e-4 relative (observed 1e-14)."""
+    T, fr, geom, _ = W.config0(n_frames=7, size=48, seed=66)
+    rng = np.random.default_rng(67)
+    nk = 7
+    dx = np.cumsum(rng.normal(0, 0.4, (nk, 7)), 0) + 1.5
+    dy = np.cumsum(rng.normal(0, 0.4, (nk, 7)), 0) - 1.0
+    dx[:, -1] = np.nan                          # the stack's last frame is never estimated
+    kw = dict(nknots=nk, frames_per_set=3)
+    if case == "plain":
+        fr = np.asarray(fr, np.float64)
+        opts = {}
+    elif case == "prefilter_inpaint":
+        fr = np.asarray(fr, np.float64)
+        opts = dict(pregauss=0.8, ibs_inpaint=True)
+    else:
+        fr = np.asarray(fr, np.float32)
+        kw.update(cropping=(2, 1, 0, 0))
+        opts = dict(ibs_inpaint=True)
+    th, tw = T.shape
+    g = ssa.geometry_desc(fr.shape[0], fr.shape[1], T.shape, nknots=nk, cropping=kw.get("cropping"))
+    dc, mk = ssa.reconstruct(fr, g, dx, dy, frames_per_set=3, **opts)
+    rdc, rmk = D.reconstruct(fr, dx, dy, (th, tw), nknots=nk, cropping=kw.get("cropping", (0, 0, 0, 0)),
+                             frames_per_set=3, **opts)
+    np.testing.assert_array_equal(mk, rmk)
+    assert np.isnan(dc[:, :, -1]).all() and not mk[:, :, -1].any()
+    if case == "plain":
+        np.testing.assert_array_equal(dc, rdc)
+    else:
+        both = ~np.isnan(rdc)
+        assert np.array_equal(np.isnan(dc), np.isnan(rdc))
+        err = np.abs(dc[both] - rdc[both]).max() / np.abs(rdc[both]).max()
+        print(f"{case}: max rel diff {err:.2e}", flush=True)
+        assert err <= 1e-4
+
+
+def test_steady_scan_composes_the_driver_and_keeps_the_extra_channel_slots():
+    """steady_scan (steady_scan.m:93-154): prefiltered template, estimation,
+    reconstruction; extra channel k lands in slot k (the reference's loop
+    writes data_corrected(:, :, frames, ch) for ch = 1..n_extra)."""
+    T, fr, geom, _ = W.config0(n_frames=10, size=48, seed=68)
+    fr = np.asarray(fr, np.float64)
+    extra = np.stack([fr * 2.0, fr + 5.0], axis=3)
+    args = dict(W.default_args(), max_iter=8)
+    args.pop("flags", None)
+    kw = dict(nknots=7, frames_per_set=4, pregauss=0.7, corr_thresh=0.5)
+    r = ssa.steady_scan(fr, T, data_extra_channels=extra, **kw, **args)
+    assert r["data_corrected"].shape == (48, 48, 10, 3)
+    Tp = ssa.prefilter_foralignment(np.asarray(T, np.float64), 0.7, False)
+    est = ssa.estimate_displacements(fr, Tp, nknots=7, frames_per_set=4, pregauss=0.7, corr_thresh=0.5, **args)
+    for k in ("dx_knots", "dy_knots"):
+        np.testing.assert_array_equal(r[k], est[k])
+    g = ssa.geometry_desc(48, 48, T.shape, nknots=7)
+    ch1, m1 = ssa.reconstruct(extra[:, :, :, 0], g, est["dx_knots"], est["dy_knots"], frames_per_set=4, pregauss=0.7)
+    ch2, m2 = ssa.reconstruct(extra[:, :, :, 1], g, est["dx_knots"], est["dy_knots"], frames_per_set=4, pregauss=0.7)
+    np.testing.assert_array_equal(r["data_corrected"][:, :, :, 0], ch1)   # the main channel is overwritten
+    np.testing.assert_array_equal(r["data_corrected"][:, :, :, 1], ch2)
+    assert not r["data_corrected"][:, :, :, 2].any() and not r["mask"][:, :, :, 2].any()
