@@ -250,6 +250,26 @@ def ibs_leg(args, torch, ssa, frames, geom, results, G, h):
            "inputs": "uint16 frames and float32/uint8 outputs resident in HBM; knots of the alignment step"}
     print(f"[bench] reconstruction: {n} frames in {el:.3f}s ({n / el:.0f} frames/s), "
           f"Gauss-Seidel iterations mean {it.mean():.1f} max {it.max()}", file=sys.stderr, flush=True)
+    # inpainting of the corrected frames (imgapfill_leastsquares, in place, single precision)
+    from steadyscan_amd import _native as N
+    missing = float((1 - d_mask.float().mean()).item())
+    t = time.perf_counter()
+    rc = N.lib().ss_fill_gaps(h.ptr, N.SS_F32, ctypes.c_void_p(d_out.data_ptr()), ctypes.c_void_p(d_out.data_ptr()),
+                              th, tw, n, 1)
+    torch.cuda.synchronize()
+    el_fill = time.perf_counter() - t
+    out["gap_fill"] = {"frames_per_s": n / el_fill, "seconds": el_fill, "missing_fraction": missing, "status": rc}
+    # prefilter_foralignment of the raw frames (Gaussian sigma 1, class kept), in HBM
+    d_pre = torch.empty_like(d_in)
+    for rep in range(2):
+        t = time.perf_counter()
+        rc = N.lib().ss_prefilter(h.ptr, N.SS_U16, ctypes.c_void_p(d_in.data_ptr()), ctypes.c_void_p(d_pre.data_ptr()),
+                                  frames.shape[0], frames.shape[1], n, 1.0, 0, 1)
+        torch.cuda.synchronize()
+        el_pre = time.perf_counter() - t
+    out["prefilter"] = {"frames_per_s": n / el_pre, "seconds": el_pre, "pregauss": 1.0, "status": rc}
+    print(f"[bench] gap fill: {n / el_fill:.0f} frames/s ({missing * 100:.1f}% missing); "
+          f"prefilter (sigma 1): {n / el_pre:.0f} frames/s", file=sys.stderr, flush=True)
     if not args.no_cpu:
         from oracle import oracle as O
         m = 3
@@ -264,6 +284,15 @@ def ibs_leg(args, torch, ssa, frames, geom, results, G, h):
         out["oracle_bit_exact_first_frames"] = same
         out["cpu_baseline"] = {"value": m / tcpu, "unit": "frames/s", "cores": 1, "kind": "port",
                                "sample": f"oracle (C, one thread) on the first {m} frames, {tcpu:.1f} s"}
+        from oracle import prefilter as P
+        t = time.perf_counter()
+        P.prefilter_foralignment(frames[:, :, :m], 1.0, False)
+        out["prefilter"]["cpu_baseline"] = {"value": m / (time.perf_counter() - t), "unit": "frames/s", "cores": 1,
+                                            "kind": "port", "sample": f"numpy FFT oracle, first {m} frames"}
+        t = time.perf_counter()
+        P.imgapfill_leastsquares(gout)
+        out["gap_fill"]["cpu_baseline"] = {"value": m / (time.perf_counter() - t), "unit": "frames/s", "cores": 1,
+                                           "kind": "port", "sample": f"scipy sparse direct oracle, first {m} frames"}
         print(f"[bench] reconstruction oracle: first {m} frames bit-exact {same}, {m / tcpu:.2f} frames/s on 1 core",
               file=sys.stderr, flush=True)
     return out
