@@ -169,3 +169,32 @@ def test_device_resident_frames_and_outputs_match_the_host_path():
             b.free()
     np.testing.assert_array_equal(mask, host_mask)
     np.testing.assert_array_equal(out, host_out)
+
+
+def test_bidirectional_scan_and_fractional_knot_spacing():
+    # per-pixel geometry tables: odd lines scanned right to left, and 10 knots
+    # over 64 lines (equal spacing in time, not on line boundaries)
+    rng = np.random.default_rng(40)
+    fr = 30 + 60 * rng.random((64, 64, 3))
+    nk = 10
+    dx, dy = smooth_knots(rng, nk, 3, 0.3), smooth_knots(rng, nk, 3, 0.3)
+    nl, lw, n = fr.shape
+    geom = ssa.init_settings(nl, lw, (64, 64), nknots=nk, isbidirectional=True)
+    info = {}
+    out, mask = ssa.invert_bilinear_sampling(fr, geom, dx, dy, info=info)
+    (hc, wc), pk, tf, bx, by = OD._geometry(nl, lw, 64, 64, nk, (0, 0, 0, 0), None, True, 0.3)
+    s = (hc, wc)
+    og = dict(prevknotind=pk.reshape(s, order="F"), tfrac=tf.reshape(s, order="F"),
+              basex=bx.reshape(s, order="F"), basey=by.reshape(s, order="F"))
+    ref, rmask, rit = O.invert_bilinear_sampling(fr, dx, dy, og, (64, 64))
+    np.testing.assert_array_equal(info["iterations"], rit)
+    np.testing.assert_array_equal(mask, rmask)
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_frame_1024():
+    T, fr, geom, mot = W.config1(n_frames=1, size=1024, seed=41)
+    rng = np.random.default_rng(42)
+    nk = geom["nknots"]
+    dx, dy = smooth_knots(rng, nk, 1, 0.3), smooth_knots(rng, nk, 1, 0.3)
+    compare(np.asarray(fr), dx, dy, 1024, 1024, nk, label="1024x1024 uint16")

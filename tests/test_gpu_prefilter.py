@@ -133,3 +133,15 @@ def test_reconstruction_with_inpainting():
     np.testing.assert_array_equal(filled[mask], raw[mask])
     ref = P.imgapfill_leastsquares(raw)
     assert np.all(np.abs(filled - ref) <= np.spacing(np.abs(ref)))
+
+
+def test_fill_gaps_large_region():
+    # half of a 256 x 256 frame missing: a large Dirichlet problem
+    a = gappy((256, 256, 1), 9)
+    a[:, 128:, 0] = np.nan
+    a[100:140, 60:128, 0] = np.nan
+    got = ssa.imgapfill_leastsquares(a)
+    ref = P.imgapfill_leastsquares(a)
+    err = np.abs(got - ref).max() / np.nanmax(np.abs(a))
+    print(f"large region: max rel err {err:.2e}", flush=True)
+    assert not np.isnan(got).any() and err <= 1e-9
