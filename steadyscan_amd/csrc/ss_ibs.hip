@@ -539,12 +539,14 @@ __device__ inline float gs_update(const float *x, const float4 a, const float4 b
 }
 
 // One iteration: the colours A B C D (1-based odd/odd, odd/even, even/odd,
-// even/even row/column) in turn; four pixels per thread and step, all loads
+// even/even row/column) in turn; P pixels per thread and step, all loads
 // before the stores (pixels of one colour do not read each other except
-// through zero coefficients).
+// through zero coefficients).  P = 1 keeps the kernel within 64 VGPRs, so two
+// frames' workgroups share a CU (more waves to hide the gathers' latency;
+// measured faster than P = 4 at one workgroup per CU).
 __device__ void gs_sweep(const Chunk &C, float *x, const float *lk, const float *rm) {
     const int th = C.th, tw = C.tw, np = C.np;
-    constexpr int P = 4;
+    constexpr int P = 1;
     for (int color = 0; color < 4; color++) {
         const int rp = color >> 1, cp = color & 1;
         const int nr = (th - rp + 1) / 2, nc = (tw - cp + 1) / 2;
@@ -592,7 +594,7 @@ __device__ int walk_interval(const Chunk &C, int f, int *wj, int *wn) {
     return *wj == f ? *wn : 0;
 }
 
-__global__ __launch_bounds__(kGsThreads) void k_gs(Chunk C) {
+__global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_gs(Chunk C) {
     const int f = blockIdx.x;
     if (!C.any[f]) return;
     const size_t fo = (size_t)f * C.np;
