@@ -53,6 +53,25 @@ def measured_traffic(kernel, precision):
     return None
 
 
+def measured_limiter(kernel, precision):
+    """Pipeline busy fractions of `kernel` (rocprofv3 --pmc) from the committed
+    counter summary; None when absent."""
+    out = {}
+    try:
+        with open(os.path.join(ROOT, "profiles", "align_512_u16_D_counters.md")) as f:
+            for line in f:
+                for key in ("td_busy", "valu_busy"):
+                    tag = f"{key}[{kernel},{precision}]:"
+                    if line.strip().startswith(tag):
+                        out[key] = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    if not out:
+        return None
+    out["source"] = "profiles/align_512_u16_D_counters.md (TD = texture-data return of the template gathers)"
+    return out
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -569,6 +588,7 @@ def main():
             "traffic": measured_traffic(N.KERNEL_CLASSES[dom], hp),
             "bytes_per_launch": head["bytes_per_launch"],
             "avg_launch_us": head["avg_s"] * 1e6,
+            "limiter": measured_limiter(N.KERNEL_CLASSES[dom], hp),
         },
         "cpu_baseline": cpu,
     }
