@@ -331,7 +331,9 @@ typedef struct {
                                many as fit in work_bytes                          */
     int32_t outputs_on_device; /* out and mask are device pointers                */
     int64_t work_bytes;     /* device work memory the handle may keep for this
-                               call; 0 = 32 GiB                                   */
+                               call; 0 = half of the device memory free or
+                               already held by the handle (about 53 MB per
+                               512x512 frame)                                     */
 } ss_ibs_settings;
 
 void ss_ibs_settings_default(ss_ibs_settings *s);

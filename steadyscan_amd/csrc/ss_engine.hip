@@ -1285,6 +1285,8 @@ void *handle_scratch(ss_handle *h, size_t bytes, ss_status *st) {
     h->scratchBytes = bytes;
     return h->scratch;
 }
+
+size_t handle_scratch_held(const ss_handle *h) { return h->scratchBytes; }
 }  // namespace ss
 
 extern "C" {

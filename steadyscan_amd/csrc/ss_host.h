@@ -18,5 +18,7 @@ hipStream_t handle_stream(const ss_handle *h);
 // Device work memory of at least `bytes` kept by the handle across calls
 // (grown on demand, freed by ss_destroy); nullptr on failure (st set).
 void *handle_scratch(ss_handle *h, size_t bytes, ss_status *st);
+// Bytes of work memory the handle holds now.
+size_t handle_scratch_held(const ss_handle *h);
 
 }  // namespace ss

@@ -106,6 +106,7 @@ struct Chunk {
     int *stop;                // B x kNSched: 1 + the iteration a check interval stopped at, else 0
     int *slot;                // B x kNSched: the snapshot holding that interval's result
     int *diag;                // B x 2: iterations run, iteration the interval became known
+    unsigned long long *prog; // B: progress word (iteration, possible and stopped intervals), 0 before start
     int carry;                // check interval of the chunk's first frame
 };
 
@@ -594,6 +595,45 @@ __device__ int walk_interval(const Chunk &C, int f, int *wj, int *wn) {
     return *wj == f ? *wn : 0;
 }
 
+// Progress word of a frame: bits 0-31 the last finished iteration k, 32-41
+// the intervals still possible for it, 42-51 those of them that stopped.
+__device__ inline unsigned long long pack_progress(int k, unsigned active, unsigned stopped) {
+    return (unsigned long long)(unsigned)k | ((unsigned long long)(active & 0x3ffu) << 32) |
+           ((unsigned long long)(stopped & 0x3ffu) << 42);
+}
+
+// The intervals frame f can still need, from its predecessor's progress:
+// frame f's interval is check_interval(the predecessor's iteration count),
+// which is check_interval(stop) for a possible interval that stopped, and at
+// least check_interval(k + 1) for one still running at iteration k.  Frames
+// skipped for an empty mask pass the interval on.  Returns all intervals
+// when nothing is known yet (or a stop is not visible yet); the set only
+// shrinks as the predecessor progresses.
+__device__ unsigned possible_intervals(const Chunk &C, int f) {
+    const unsigned all = (1u << kNSched) - 1;
+    int j = f - 1;
+    while (j >= 0 && !C.any[j]) j--;
+    if (j < 0) return all;
+    const unsigned long long w = __hip_atomic_load(&C.prog[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned act = (unsigned)(w >> 32) & 0x3ffu, stp = (unsigned)(w >> 42) & 0x3ffu;
+    const int k = (int)(unsigned)(w & 0xffffffffu);
+    if (!act) return all;
+    unsigned out = 0;
+    for (int n = 1; n <= kNSched; n++) {
+        const unsigned bit = 1u << (n - 1);
+        if (!(act & bit)) continue;
+        if (stp & bit) {
+            const int v = __hip_atomic_load(&C.stop[(size_t)j * kNSched + n - 1], __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+            if (v == 0) return all;
+            out |= 1u << (check_interval(v - 1) - 1);
+        } else {
+            for (int m = check_interval(k + 1); m <= kNSched; m++) out |= 1u << (m - 1);
+        }
+    }
+    return out ? out : all;
+}
+
 __global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_gs(Chunk C) {
     const int f = blockIdx.x;
     if (!C.any[f]) return;
@@ -673,8 +713,12 @@ __global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8)))
                     s_knownAt = k;
                     s_known = kn;
                     s_active = 1u << (kn - 1);
+                } else {
+                    s_active &= possible_intervals(C, f);
                 }
             }
+            __hip_atomic_store(&C.prog[f], pack_progress(k, s_active, s_stopped), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
             s_copy = copy;
             s_k = k;
             s_done = (s_active & ~s_stopped) == 0 || k >= C.cap;
@@ -769,11 +813,18 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
         align256(sizeof(int) * 4 * (size_t)np) + align256(sizeof(double) * 9 * (size_t)np) +
         align256(sizeof(double) * (size_t)np) + align256(sizeof(float) * (size_t)np) * 4 +
         align256((size_t)np) * 2 + align256(sizeof(int)) + align256(sizeof(double) * QS_COUNT * (size_t)np) +
-        align256(sizeof(float) * 8 * (size_t)np) + align256(sizeof(float) * kNSlots * (size_t)np) +
-        align256(sizeof(int) * 2 * kNSched) + align256(sizeof(int)) * 3 +
+        align256(sizeof(float) * 8 * (size_t)np) +
+        align256(sizeof(int) * 2 * kNSched) + align256(sizeof(int)) * 3 + align256(sizeof(unsigned long long)) +
         (in->data_on_device ? 0 : align256(esz * raw)) +
         (devOut ? 0 : align256(sizeof(float) * (size_t)np) + align256((size_t)np));
-    const size_t budget = st->work_bytes > 0 ? (size_t)st->work_bytes : ((size_t)32 << 30);
+    size_t budget = (size_t)st->work_bytes;
+    if (budget == 0) {
+        // half of the device memory that is free or already held by the handle
+        size_t freeB = 0, totalB = 0;
+        if (hipSetDevice(ss::handle_device(h)) != hipSuccess || hipMemGetInfo(&freeB, &totalB) != hipSuccess)
+            return handle_fail(h, SS_ERR_DEVICE, "invert_bilinear_sampling: hipMemGetInfo failed");
+        budget = std::max<size_t>((size_t)1 << 30, (freeB + ss::handle_scratch_held(h)) / 2);
+    }
     int B = st->chunk_frames > 0 ? st->chunk_frames
                                  : (int)std::min<size_t>(65535, std::max<size_t>(1, budget / perFrame));
     B = std::min(B, in->n_frames);
@@ -832,9 +883,12 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
         C.lk = (float *)take(sizeof(float) * (size_t)np * B);
         C.rm = (float *)take(sizeof(float) * 8 * (size_t)np * B);
         C.x = (float *)take(sizeof(float) * (size_t)np * B);
-        C.snap = (float *)take(sizeof(float) * kNSlots * (size_t)np * B);
+        // the snapshots reuse the assembly's raw sums, dead once k_interact has run
+        static_assert(sizeof(float) * kNSlots <= sizeof(double) * 9, "snapshots must fit the raw sums");
+        C.snap = (float *)C.qraw;
         C.stop = (int *)take(sizeof(int) * kNSched * (size_t)B);
         C.slot = (int *)take(sizeof(int) * kNSched * (size_t)B);
+        C.prog = (unsigned long long *)take(sizeof(unsigned long long) * (size_t)B);
         C.diag = (int *)take(sizeof(int) * 2 * (size_t)B);
         int *dchosen = (int *)take(sizeof(int) * (size_t)B);
         void *ddata = in->data_on_device ? nullptr : take(esz * raw * B);
@@ -873,6 +927,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
             ICHK(hipMemsetAsync(C.win, 0xFF, sizeof(int) * 4 * (size_t)np * nb, s));
             ICHK(hipMemsetAsync(C.any, 0, sizeof(int) * (size_t)nb, s));
             ICHK(hipMemsetAsync(C.stop, 0, sizeof(int) * kNSched * (size_t)nb, s));
+            ICHK(hipMemsetAsync(C.prog, 0, sizeof(unsigned long long) * (size_t)nb, s));
             C.carry = carry;
             const dim3 gp((npd + 255) / 256, nb), gt((np + 255) / 256, nb);
             hipLaunchKernelGGL(k_points, gp, dim3(256), 0, s, C);

@@ -92,6 +92,22 @@ def test_frame_off_the_template_is_skipped():
     assert it[1] == 0 and it[3] == 0 and not mask[:, :, 1].any() and np.isnan(out[:, :, 3]).all()
 
 
+def test_long_chain_with_skipped_frames_in_one_chunk():
+    # 48 frames run at once: most learn their check interval late or never and
+    # stop once every interval their predecessor still allows has stopped;
+    # skipped frames in the chain pass the interval on
+    n = 48
+    T, fr, geom, mot = W.config0(n_frames=n, size=48, seed=43)
+    rng = np.random.default_rng(44)
+    nk = geom["nknots"]
+    dx, dy = smooth_knots(rng, nk, n), smooth_knots(rng, nk, n)
+    dx[:, 10] = 100.0
+    dx[:, 11] = -100.0
+    dy[:, 30] = np.nan
+    _, _, it = compare(np.asarray(fr, np.float64), dx, dy, 48, 48, nk, regfac=0.3, label="48-frame chain")
+    assert it[10] == 0 and it[11] == 0 and it[30] == 0 and len(set(it.tolist())) > 3
+
+
 def test_rectangular_template_aspect_and_cropping():
     rng = np.random.default_rng(19)
     nl, lw = 40, 56
