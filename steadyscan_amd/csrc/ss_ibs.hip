@@ -99,13 +99,13 @@ struct Chunk {
     uint8_t *full;            // B x np: masked, interior, all eight neighbours masked
     int *any;                 // B
     double *qs;               // B x QS_COUNT x np
-    float *lf, *lk;           // B x np
-    float *rm;                // B x np x 8
+    float *lf, *lk;           // B x np (lk colour-major, in sweep order)
+    float *rm;                // B x np x 8, colour-major in sweep order
     float *x;                 // B x np
     float *snap;              // B x kNSlots x np
     int *stop;                // B x kNSched: 1 + the iteration a check interval stopped at, else 0
     int *slot;                // B x kNSched: the snapshot holding that interval's result
-    int *diag;                // B x 2: iterations run, iteration the interval became known
+    int *diag;                // B x 3: iterations run, iteration the interval became known, energies evaluated
     unsigned long long *prog; // B: progress word (iteration, possible and stopped intervals), 0 before start
     int carry;                // check interval of the chunk's first frame
 };
@@ -369,6 +369,21 @@ __global__ __launch_bounds__(256) void k_assemble(Chunk C, const TD *__restrict_
 // interact: masked stencil, calc_interactions (steady_scan.m:501-523), the
 // right-hand side (steady_scan.m:358-362) and the initial x
 // ---------------------------------------------------------------------------
+// Rm and L are stored colour-major, in the order the sweep visits pixels:
+// the colours A (even row, even column), B (even, odd), C (odd, even),
+// D (odd, odd) in turn, within a colour by column then row.  Rm of a colour
+// with n pixels is two planes of n float4: the N NE E SE coefficients, then
+// the S SW W NW ones, so a wave's loads are contiguous.
+__device__ inline int colour_rows(int th, int rp) { return (th - rp + 1) / 2; }
+__device__ inline int colour_count(int th, int tw, int color) {
+    return colour_rows(th, color >> 1) * ((tw - (color & 1) + 1) / 2);
+}
+__device__ inline int colour_base(int th, int tw, int color) {
+    int b = 0;
+    for (int k = 0; k < color; k++) b += colour_count(th, tw, k);
+    return b;
+}
+
 __global__ __launch_bounds__(256) void k_interact(Chunk C) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -391,9 +406,12 @@ __global__ __launch_bounds__(256) void k_interact(Chunk C) {
     const float Lf = (float)(m ? C.wy[fo + i] : 0.0);
     float L = Lf / R;
     if (isnan(L) || isinf(L)) L = 0.f;
-    C.lk[fo + i] = L;
+    const int color = 2 * (r & 1) + (c & 1);
+    const int nColour = colour_count(th, tw, color), base = colour_base(th, tw, color);
+    const int kk = (r >> 1) + (c >> 1) * colour_rows(th, r & 1);
+    C.lk[fo + base + kk] = L;
     C.lf[fo + i] = Lf;
-    float *rm = C.rm + (fo + i) * 8;
+    float rm[8];
     for (int d = 0; d < NB_COUNT; d++) {
         float v = 0.f;
         if (R != 0.f) {
@@ -406,6 +424,9 @@ __global__ __launch_bounds__(256) void k_interact(Chunk C) {
         }
         rm[d] = v;
     }
+    float4 *rm4 = (float4 *)(C.rm + fo * 8) + 2 * (size_t)base;
+    rm4[kk] = make_float4(rm[0], rm[1], rm[2], rm[3]);
+    rm4[nColour + kk] = make_float4(rm[4], rm[5], rm[6], rm[7]);
     double *qs = C.qs + (size_t)f * QS_COUNT * C.np;
     qs[QS_D * (size_t)C.np + i] = qm[0];
     qs[QS_S * (size_t)C.np + i] = qm[1 + NB_S];
@@ -548,10 +569,14 @@ __device__ inline float gs_update(const float *x, const float4 a, const float4 b
 __device__ void gs_sweep(const Chunk &C, float *x, const float *lk, const float *rm) {
     const int th = C.th, tw = C.tw, np = C.np;
     constexpr int P = 1;
+    int base = 0;
     for (int color = 0; color < 4; color++) {
         const int rp = color >> 1, cp = color & 1;
         const int nr = (th - rp + 1) / 2, nc = (tw - cp + 1) / 2;
         const int n = nr * nc;
+        const float4 *rv = (const float4 *)rm + 2 * (size_t)base;
+        const float *lv = lk + base;
+        base += n;
         for (int k0 = threadIdx.x; k0 < n; k0 += P * kGsThreads) {
             int ii[P], rr[P], cc[P];
             float4 a[P], b[P];
@@ -562,10 +587,9 @@ __device__ void gs_sweep(const Chunk &C, float *x, const float *lk, const float 
                 rr[q] = rp + 2 * (kk % nr);
                 cc[q] = cp + 2 * (kk / nr);
                 ii[q] = rr[q] + cc[q] * th;
-                const float4 *rv = (const float4 *)(rm + (size_t)ii[q] * 8);
-                a[q] = rv[0];
-                b[q] = rv[1];
-                l[q] = lk[ii[q]];
+                a[q] = rv[kk];
+                b[q] = rv[n + kk];
+                l[q] = lv[kk];
             }
             for (int q = 0; q < P; q++) v[q] = gs_update(x, a[q], b[q], l[q], ii[q], rr[q], cc[q], th, tw, np);
             for (int q = 0; q < P; q++)
@@ -649,7 +673,7 @@ __global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8)))
     __shared__ float s_eo[kNSched];
     __shared__ int s_slot[kNSched], s_ref[kNSlots];
     __shared__ unsigned s_active, s_stopped;
-    __shared__ int s_known, s_needE, s_copy, s_done, s_wj, s_wn, s_knownAt, s_k;
+    __shared__ int s_known, s_needE, s_copy, s_done, s_wj, s_wn, s_knownAt, s_k, s_nE;
 
     const float e0 = gs_energy(C, x, mask, full, qs, lf, s_pq, s_pl);
     if (threadIdx.x == 0) {
@@ -659,6 +683,7 @@ __global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8)))
         }
         for (int s = 0; s < kNSlots; s++) s_ref[s] = 0;
         s_stopped = 0;
+        s_nE = 0;
         s_wj = 0;
         s_wn = C.carry;
         s_known = walk_interval(C, f, &s_wj, &s_wn);
@@ -679,6 +704,7 @@ __global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8)))
         if (s_needE) e = gs_energy(C, x, mask, full, qs, lf, s_pq, s_pl);
         if (threadIdx.x == 0) {
             unsigned pass = 0;
+            s_nE += s_needE;
             if (s_needE) {
                 for (int n = 1; n <= kNSched; n++) {
                     const unsigned bit = 1u << (n - 1);
@@ -736,8 +762,9 @@ __global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8)))
     }
     if (threadIdx.x == 0) {
         for (int n = 0; n < kNSched; n++) C.slot[(size_t)f * kNSched + n] = s_slot[n];
-        C.diag[2 * f] = s_k;
-        C.diag[2 * f + 1] = s_knownAt;
+        C.diag[3 * f] = s_k;
+        C.diag[3 * f + 1] = s_knownAt;
+        C.diag[3 * f + 2] = s_nE;
     }
 }
 
@@ -889,7 +916,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
         C.stop = (int *)take(sizeof(int) * kNSched * (size_t)B);
         C.slot = (int *)take(sizeof(int) * kNSched * (size_t)B);
         C.prog = (unsigned long long *)take(sizeof(unsigned long long) * (size_t)B);
-        C.diag = (int *)take(sizeof(int) * 2 * (size_t)B);
+        C.diag = (int *)take(sizeof(int) * 3 * (size_t)B);
         int *dchosen = (int *)take(sizeof(int) * (size_t)B);
         void *ddata = in->data_on_device ? nullptr : take(esz * raw * B);
         float *dout = devOut ? nullptr : (float *)take(sizeof(float) * (size_t)np * B);
@@ -899,12 +926,13 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
             err = "invert_bilinear_sampling: internal sizing error";
             goto done;
         }
-        std::vector<int> stop(kNSched * (size_t)B), slot(kNSched * (size_t)B), any(B), chosen(B), diag(2 * (size_t)B);
+        std::vector<int> stop(kNSched * (size_t)B), slot(kNSched * (size_t)B), any(B), chosen(B), diag(3 * (size_t)B);
         // SS_IBS_STATS=1 prints, per chunk, the Gauss-Seidel iterations the
         // workgroups ran and when they learned their check interval
         const char *statsEnv = std::getenv("SS_IBS_STATS");
         const bool stats = statsEnv && statsEnv[0] == '1';
         std::vector<double> kbuf(2 * (size_t)nk * B);
+        long needE = 0;  // energies the reference's rule evaluates (statistics only)
         int carry = 1;  // n_iter_error starts at 1 for the call (steady_scan.m:326)
         for (int f0 = 0; f0 < in->n_frames; f0 += B) {
             const int nb = std::min(B, in->n_frames - f0);
@@ -945,7 +973,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
             ICHK(hipGetLastError());
             ICHK(hipMemcpyAsync(stop.data(), C.stop, sizeof(int) * kNSched * nb, hipMemcpyDeviceToHost, s));
             ICHK(hipMemcpyAsync(slot.data(), C.slot, sizeof(int) * kNSched * nb, hipMemcpyDeviceToHost, s));
-            if (stats) ICHK(hipMemcpyAsync(diag.data(), C.diag, sizeof(int) * 2 * nb, hipMemcpyDeviceToHost, s));
+            if (stats) ICHK(hipMemcpyAsync(diag.data(), C.diag, sizeof(int) * 3 * nb, hipMemcpyDeviceToHost, s));
             ICHK(hipMemcpyAsync(any.data(), C.any, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
             ICHK(hipStreamSynchronize(s));
             // resolve the check intervals in frame order (steady_scan.m:326-395)
@@ -964,27 +992,31 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
                     goto done;
                 }
                 chosen[f] = slot[(size_t)f * kNSched + n - 1];
+                needE += (v - 1) / n;
                 if (iterations) iterations[f0 + f] = v - 1;
                 carry = check_interval(v - 1);
             }
             if (stats) {
-                long runSum = 0, knownSum = 0, needSum = 0;
+                long runSum = 0, knownSum = 0, needSum = 0, eSum = 0;
                 int runMax = 0, knownMax = -1, unknown = 0, nf = 0;
                 for (int f = 0; f < nb; f++) {
                     if (!any[f]) continue;
                     nf++;
-                    runSum += diag[2 * f];
-                    runMax = std::max(runMax, diag[2 * f]);
-                    if (diag[2 * f + 1] < 0) unknown++;
-                    else { knownSum += diag[2 * f + 1]; knownMax = std::max(knownMax, diag[2 * f + 1]); }
+                    runSum += diag[3 * f];
+                    runMax = std::max(runMax, diag[3 * f]);
+                    eSum += diag[3 * f + 2];
+                    if (diag[3 * f + 1] < 0) unknown++;
+                    else { knownSum += diag[3 * f + 1]; knownMax = std::max(knownMax, diag[3 * f + 1]); }
                 }
                 for (int f0i = 0; f0i < nb; f0i++)
                     if (any[f0i] && iterations) needSum += iterations[f0 + f0i];
                 std::fprintf(stderr,
                              "[ibs] chunk of %d frames: iterations run mean %.1f max %d (needed mean %.1f); "
-                             "interval known at mean %.1f max %d, never in %d\n",
+                             "interval known at mean %.1f max %d, never in %d; energies mean %.1f (needed %.1f)\n",
                              nf, nf ? (double)runSum / nf : 0.0, runMax, nf ? (double)needSum / nf : 0.0,
-                             nf - unknown ? (double)knownSum / (nf - unknown) : 0.0, knownMax, unknown);
+                             nf - unknown ? (double)knownSum / (nf - unknown) : 0.0, knownMax, unknown,
+                             nf ? (double)eSum / nf : 0.0, nf ? (double)needE / nf : 0.0);
+                needE = 0;
             }
             ICHK(hipMemcpyAsync(dchosen, chosen.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
             float *o = devOut ? out + (size_t)np * f0 : dout;
