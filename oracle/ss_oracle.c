@@ -1315,36 +1315,53 @@ int orc_ibs_system(const double *data, int npd_, const double *dx, const double 
  * one that parallel hardware can follow: (Q x)_i sums x_j Q_ij over the
  * neighbours j in ascending linear index (the order of a column of the
  * sparse matrix); the pixel terms x_i (Q x)_i (double) and Le_i x_i (single)
- * are summed in ORC_IBS_LANES interleaved partial sums, partial t taking the
- * pixels t, t + LANES, t + 2 LANES, ... in ascending order, and the partials
- * are then added by a fixed binary tree (t += t + 512, then 256, ... 1). */
+ * are summed in ORC_IBS_LANES partial sums, then added by a fixed binary
+ * tree (t += t + 512, then 256, ... 1).  The partials come in 16 groups of
+ * 64.  Rows are cut into blocks of 64 (nrb of them) and columns into
+ * nseg = max(1, 16 / nrb) equal segments (segment s: columns s tw / nseg up
+ * to (s + 1) tw / nseg); unit u is row block u mod nrb by segment u / nrb.
+ * Group g takes the units g, g + 16, ... in turn, and within a unit partial
+ * 64 g + l takes row 64 rb + l over the segment's columns left to right. */
 #define ORC_IBS_LANES 1024
+#define ORC_IBS_GROUP 64
 float orc_ibs_energy(const double *qm, const double *wy, const float *x, const uint8_t *mask, int th,
                      int tw) {
     static const int ord[9][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {0, 0},
                                   {1, 0},   {-1, 1}, {0, 1},  {1, 1}};
-    const long np = (long)th * tw;
+    const int groups = ORC_IBS_LANES / ORC_IBS_GROUP;
+    const int nrb = (th + ORC_IBS_GROUP - 1) / ORC_IBS_GROUP;
+    const int nseg = nrb >= groups ? 1 : groups / nrb;
     double pq[ORC_IBS_LANES];
     float pl[ORC_IBS_LANES];
     for (int t = 0; t < ORC_IBS_LANES; t++) {
         pq[t] = 0;
         pl[t] = 0;
-        for (long i = t; i < np; i += ORC_IBS_LANES) {
-            if (!mask[i]) continue;
-            long r = i % th, c = i / th;
-            double qx = 0;
-            for (int k = 0; k < 9; k++) {
-                long rr = r + ord[k][0], cc = c + ord[k][1];
-                if (rr < 0 || rr >= th || cc < 0 || cc >= tw) continue;
-                long j = rr + cc * (long)th;
-                if (!mask[j]) continue;
-                int slot = k == 4 ? 0 : 1 + ib_dir(ord[k][0], ord[k][1]);
-                qx += (double)x[j] * qm[i * 9 + slot];
-            }
-            pq[t] += (double)x[i] * qx;
-            pl[t] += (float)wy[i] * x[i];
-        }
     }
+    for (int g = 0; g < groups; g++)
+        for (int u = g; u < nrb * nseg; u += groups) {
+            const int rb = u % nrb, sg = u / nrb;
+            const long c0 = (long)sg * tw / nseg, c1 = (long)(sg + 1) * tw / nseg;
+            for (int l = 0; l < ORC_IBS_GROUP; l++) {
+                const long r = (long)rb * ORC_IBS_GROUP + l;
+                if (r >= th) continue;
+                const int t = g * ORC_IBS_GROUP + l;
+                for (long c = c0; c < c1; c++) {
+                    const long i = r + c * (long)th;
+                    if (!mask[i]) continue;
+                    double qx = 0;
+                    for (int k = 0; k < 9; k++) {
+                        long rr = r + ord[k][0], cc = c + ord[k][1];
+                        if (rr < 0 || rr >= th || cc < 0 || cc >= tw) continue;
+                        long j = rr + cc * (long)th;
+                        if (!mask[j]) continue;
+                        int slot = k == 4 ? 0 : 1 + ib_dir(ord[k][0], ord[k][1]);
+                        qx += (double)x[j] * qm[i * 9 + slot];
+                    }
+                    pq[t] += (double)x[i] * qx;
+                    pl[t] += (float)wy[i] * x[i];
+                }
+            }
+        }
     for (int off = ORC_IBS_LANES / 2; off >= 1; off >>= 1)
         for (int t = 0; t < off; t++) {
             pq[t] += pq[t + off];

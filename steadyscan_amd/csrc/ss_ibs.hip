@@ -101,7 +101,7 @@ struct Chunk {
     double *qs;               // B x QS_COUNT x np
     float *lf, *lk;           // B x np (lk colour-major, in sweep order)
     float *rm;                // B x np x 8, colour-major in sweep order
-    float *x;                 // B x np
+    float *x;                 // B x np, per column even rows then odd rows
     float *snap;              // B x kNSlots x np
     int *stop;                // B x kNSched: 1 + the iteration a check interval stopped at, else 0
     int *slot;                // B x kNSched: the snapshot holding that interval's result
@@ -375,6 +375,28 @@ __global__ __launch_bounds__(256) void k_assemble(Chunk C, const TD *__restrict_
 // with n pixels is two planes of n float4: the N NE E SE coefficients, then
 // the S SW W NW ones, so a wave's loads are contiguous.
 __device__ inline int colour_rows(int th, int rp) { return (th - rp + 1) / 2; }
+// x (and its snapshots) keeps each column's even rows first, then its odd
+// rows (h0 = (th + 1) / 2 of them even), so the pixels of one colour in a
+// column, and each of their neighbours in turn, are contiguous.
+__device__ inline int xrow(int r, int h0) { return (r & 1) * h0 + (r >> 1); }
+
+// Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1): lane l receives lane
+// l - 1's (from_below) or lane l + 1's (from_above) value; the end lane
+// keeps its own.
+__device__ inline int dpp_wave_shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ inline int dpp_wave_shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+__device__ inline float from_below(float v) { return __int_as_float(dpp_wave_shr1(__float_as_int(v))); }
+__device__ inline float from_above(float v) { return __int_as_float(dpp_wave_shl1(__float_as_int(v))); }
+__device__ inline double from_below(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_wave_shr1((int)b), hi = dpp_wave_shr1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double from_above(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_wave_shl1((int)b), hi = dpp_wave_shl1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 __device__ inline int colour_count(int th, int tw, int color) {
     return colour_rows(th, color >> 1) * ((tw - (color & 1) + 1) / 2);
 }
@@ -434,7 +456,7 @@ __global__ __launch_bounds__(256) void k_interact(Chunk C) {
     qs[QS_E * (size_t)C.np + i] = qm[1 + NB_E];
     qs[QS_SE * (size_t)C.np + i] = qm[1 + NB_SE];
     const float iv = C.iv[fo + i];
-    C.x[fo + i] = isnan(iv) ? 0.0f : iv;
+    C.x[fo + c * th + xrow(r, (th + 1) / 2)] = isnan(iv) ? 0.0f : iv;
     bool full = m && r >= 1 && r + 1 < th && c >= 1 && c + 1 < tw;
     for (int d = 0; d < NB_COUNT && full; d++) full = mask[(r + kSlotDr[d]) + (c + kSlotDc[d]) * th] != 0;
     C.full[fo + i] = full ? 1 : 0;
@@ -448,63 +470,130 @@ __global__ __launch_bounds__(256) void k_interact(Chunk C) {
 // (the pixel with the smaller index holds the coupling).
 __device__ inline double qx_generic(const float *x, const uint8_t *mask, const double *qD, const double *qS,
                                     const double *qNE, const double *qE, const double *qSE, int i, int r,
-                                    int c, int th, int tw) {
+                                    int c, int th, int tw, int h0) {
 #pragma clang fp contract(off)
     const bool up = r >= 1, dn = r + 1 < th, lt = c >= 1, rt = c + 1 < tw;
     double qx = 0.0;
     if (lt) {
-        if (up && mask[i - th - 1]) qx += (double)x[i - th - 1] * qSE[i - th - 1];  // NW
-        if (mask[i - th]) qx += (double)x[i - th] * qE[i - th];                     // W
-        if (dn && mask[i - th + 1]) qx += (double)x[i - th + 1] * qNE[i - th + 1];  // SW
+        const int w = (c - 1) * th;
+        if (up && mask[i - th - 1]) qx += (double)x[w + xrow(r - 1, h0)] * qSE[i - th - 1];  // NW
+        if (mask[i - th]) qx += (double)x[w + xrow(r, h0)] * qE[i - th];                     // W
+        if (dn && mask[i - th + 1]) qx += (double)x[w + xrow(r + 1, h0)] * qNE[i - th + 1];  // SW
     }
-    if (up && mask[i - 1]) qx += (double)x[i - 1] * qS[i - 1];                      // N
-    qx += (double)x[i] * qD[i];
-    if (dn && mask[i + 1]) qx += (double)x[i + 1] * qS[i];                          // S
+    const int m = c * th;
+    if (up && mask[i - 1]) qx += (double)x[m + xrow(r - 1, h0)] * qS[i - 1];                 // N
+    qx += (double)x[m + xrow(r, h0)] * qD[i];
+    if (dn && mask[i + 1]) qx += (double)x[m + xrow(r + 1, h0)] * qS[i];                     // S
     if (rt) {
-        if (up && mask[i + th - 1]) qx += (double)x[i + th - 1] * qNE[i];           // NE
-        if (mask[i + th]) qx += (double)x[i + th] * qE[i];                          // E
-        if (dn && mask[i + th + 1]) qx += (double)x[i + th + 1] * qSE[i];           // SE
+        const int e = (c + 1) * th;
+        if (up && mask[i + th - 1]) qx += (double)x[e + xrow(r - 1, h0)] * qNE[i];           // NE
+        if (mask[i + th]) qx += (double)x[e + xrow(r, h0)] * qE[i];                          // E
+        if (dn && mask[i + th + 1]) qx += (double)x[e + xrow(r + 1, h0)] * qSE[i];           // SE
     }
     return qx;
 }
 
 // The stopping energy single(x'Qx - 2 single(Le x)) over the mask in the
-// fixed order: (Q x)_i over the neighbours in ascending index; thread t sums
-// the pixels t, t + 1024, ... in ascending order; the 1024 partial sums are
-// added by a binary tree (t += t + 512, then 256, ... 1).  The result is
-// returned to every thread.  Pixels whose eight neighbours are all masked
-// take the same sum with no tests.
+// fixed order: (Q x)_i over the neighbours in ascending index; 1024 partial
+// sums (thread t), added by a binary tree (t += t + 512, then 256, ... 1).
+// Rows are cut into blocks of 64 and columns into nSeg = max(1, 16 / blocks)
+// segments; unit u (row block u mod blocks, segment u / blocks) goes to wave
+// u mod 16, and lane l of the wave sums row 64 rb + l over the segment's
+// columns left to right.  Walking a row keeps the W column's x and Q from
+// the step before, and the rows above and below come from the neighbouring
+// lanes (a shuffle; lanes 0 and 63 load their own), so a step loads one x
+// and the pixel's own five Q values.  Pixels whose eight neighbours are all
+// masked take the interior formula; other masked pixels test each term.
 __device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, const uint8_t *full,
                            const double *qs, const float *lf, double *sp, float *sl) {
 #pragma clang fp contract(off)
-    const int th = C.th, tw = C.tw, np = C.np;
+    const int th = C.th, tw = C.tw, np = C.np, h0 = (th + 1) / 2;
     const double *qD = qs, *qS = qs + QS_S * (size_t)np, *qNE = qs + QS_NE * (size_t)np,
                  *qE = qs + QS_E * (size_t)np, *qSE = qs + QS_SE * (size_t)np;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int kWaves = kGsThreads / 64;
+    const int nRB = (th + 63) / 64, nSeg = nRB >= kWaves ? 1 : kWaves / nRB;
     double pq = 0.0;
     float pl = 0.f;
-    for (int i = threadIdx.x; i < np; i += kGsThreads) {
-        const uint8_t fl = full[i];
-        double qx;
-        if (fl) {
-            const float xnw = x[i - th - 1], xw = x[i - th], xsw = x[i - th + 1];
-            const float xn = x[i - 1], xc = x[i], xs = x[i + 1];
-            const float xne = x[i + th - 1], xe = x[i + th], xse = x[i + th + 1];
-            qx = 0.0;
-            qx += (double)xnw * qSE[i - th - 1];
-            qx += (double)xw * qE[i - th];
-            qx += (double)xsw * qNE[i - th + 1];
-            qx += (double)xn * qS[i - 1];
-            qx += (double)xc * qD[i];
-            qx += (double)xs * qS[i];
-            qx += (double)xne * qNE[i];
-            qx += (double)xe * qE[i];
-            qx += (double)xse * qSE[i];
-        } else {
-            if (!mask[i]) continue;
-            qx = qx_generic(x, mask, qD, qS, qNE, qE, qSE, i, i % th, i / th, th, tw);
+    for (int u = wave; u < nRB * nSeg; u += kWaves) {
+        const int rb = u % nRB, sg = u / nRB;
+        const int c0 = (int)((long)sg * tw / nSeg), c1 = (int)((long)(sg + 1) * tw / nSeg);
+        const int r = rb * 64 + lane;
+        const bool valid = r < th, hasN = valid && r >= 1, hasS = valid && r + 1 < th;
+        const int rowN = xrow(r - 1, h0), row0 = xrow(r, h0), rowS = xrow(r + 1, h0);
+        // x of the columns c - 1 and c at the rows r - 1, r, r + 1, and the W
+        // column's couplings to this pixel: qSE(r-1, c-1) qE(r, c-1) qNE(r+1, c-1)
+        float xnw = 0.f, xw = 0.f, xsw = 0.f, xn = 0.f, xc = 0.f, xs = 0.f;
+        double qnw = 0.0, qw = 0.0, qsw = 0.0;
+        if (c0 < c1) {
+            if (c0 >= 1) {
+                const int cw = (c0 - 1) * th, iw = r + cw;
+                if (valid) {
+                    xw = x[cw + row0];
+                    qw = qE[iw];
+                }
+                if (hasN) {
+                    xnw = x[cw + rowN];
+                    qnw = qSE[iw - 1];
+                }
+                if (hasS) {
+                    xsw = x[cw + rowS];
+                    qsw = qNE[iw + 1];
+                }
+            }
+            const int cm = c0 * th;
+            if (valid) xc = x[cm + row0];
+            if (hasN) xn = x[cm + rowN];
+            if (hasS) xs = x[cm + rowS];
         }
-        pq += (double)x[i] * qx;
-        pl += lf[i] * x[i];
+        for (int c = c0; c < c1; c++) {
+            const int i = valid ? r + c * th : 0;
+            const bool hasE = c + 1 < tw;
+            const int ce = (c + 1) * th;
+            const float xe = (valid && hasE) ? x[ce + row0] : 0.f;
+            float xne = from_below(xe), xse = from_above(xe);
+            const double dD = qD[i], dS = qS[i], dNE = qNE[i], dE = qE[i], dSE = qSE[i];
+            double dN = from_below(dS);
+            double nqnw = from_below(dSE), nqsw = from_above(dNE);
+            if (lane == 0) {
+                xne = (hasN && hasE) ? x[ce + rowN] : 0.f;
+                dN = hasN ? qS[i - 1] : 0.0;
+                nqnw = hasN ? qSE[i - 1] : 0.0;
+            }
+            if (lane == 63) {
+                xse = (hasS && hasE) ? x[ce + rowS] : 0.f;
+                nqsw = hasS ? qNE[i + 1] : 0.0;
+            }
+            const uint8_t fl = valid ? full[i] : 0;
+            const float l = valid ? lf[i] : 0.f;
+            if (fl) {
+                double qx = 0.0;
+                qx += (double)xnw * qnw;
+                qx += (double)xw * qw;
+                qx += (double)xsw * qsw;
+                qx += (double)xn * dN;
+                qx += (double)xc * dD;
+                qx += (double)xs * dS;
+                qx += (double)xne * dNE;
+                qx += (double)xe * dE;
+                qx += (double)xse * dSE;
+                pq += (double)xc * qx;
+                pl += l * xc;
+            } else if (valid && mask[i]) {
+                const double qx = qx_generic(x, mask, qD, qS, qNE, qE, qSE, i, r, c, th, tw, h0);
+                pq += (double)xc * qx;
+                pl += l * xc;
+            }
+            xnw = xn;
+            xw = xc;
+            xsw = xs;
+            xn = xne;
+            xc = xe;
+            xs = xse;
+            qnw = nqnw;
+            qw = dE;
+            qsw = nqsw;
+        }
     }
     const int t = threadIdx.x;
     sp[t] = pq;
@@ -522,53 +611,24 @@ __device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, 
     return e;
 }
 
-// x_i = L_i - sum over the slots N..NW of x(nbr) * Rm for one pixel at (r, c).
-// The neighbour indices are find_*'s (steady_scan.m:677-723) written out for
-// a pixel at (r, c): a rejected neighbour reads pixel 0 (its Rm is zero),
-// find_SE rejects the last pixel, find_NW wraps from the top row.  Away from
-// the border every neighbour is in the image and accepted except the one
-// find_SE drops; reading the true neighbour there instead of pixel 0 gives
-// the same result, since its coefficient is zero and x is finite.
-__device__ inline float gs_update(const float *x, const float4 a, const float4 b, float lk, int i,
-                                  int r, int c, int th, int tw, int np) {
-#pragma clang fp contract(off)
-    float xn, xne, xe, xse, xs, xsw, xw, xnw;
-    if (r >= 1 && r + 1 < th && c >= 1 && c + 1 < tw) {
-        xnw = x[i - th - 1]; xw = x[i - th]; xsw = x[i - th + 1];
-        xn = x[i - 1]; xs = x[i + 1];
-        xne = x[i + th - 1]; xe = x[i + th]; xse = x[i + th + 1];
-    } else {
-        const bool rTop = r == 0, rBot = r == th - 1, cLast = c == tw - 1;
-        xn = x[rTop ? 0 : i - 1];
-        xne = x[(rTop || cLast) ? 0 : i + th - 1];
-        xe = x[cLast ? 0 : i + th];
-        xse = x[(rBot || i + th + 2 >= np) ? 0 : i + th + 1];
-        xs = x[rBot ? 0 : i + 1];
-        xsw = x[(rBot || i < th - 1) ? 0 : i - th + 1];
-        xw = x[i < th ? 0 : i - th];
-        xnw = x[i <= th ? 0 : i - th - 1];
-    }
-    float s = 0.f;
-    s += xn * a.x;
-    s += xne * a.y;
-    s += xe * a.z;
-    s += xse * a.w;
-    s += xs * b.x;
-    s += xsw * b.y;
-    s += xw * b.z;
-    s += xnw * b.w;
-    return lk - s;
-}
-
 // One iteration: the colours A B C D (1-based odd/odd, odd/even, even/odd,
-// even/even row/column) in turn; P pixels per thread and step, all loads
-// before the stores (pixels of one colour do not read each other except
-// through zero coefficients).  P = 1 keeps the kernel within 64 VGPRs, so two
-// frames' workgroups share a CU (more waves to hide the gathers' latency;
-// measured faster than P = 4 at one workgroup per CU).
+// even/even row/column) in turn.  Pixels of one colour do not read each
+// other, so any order within a colour gives the same result.  Each wave
+// takes 64 consecutive rows of the colour (one per lane) and walks a segment
+// of its columns left to right, so that
+//   - the W side of a pixel is the E side loaded one step before,
+//   - N and NE are the S and SE of the lane above (a shuffle; lane 0 loads
+//     its own),
+// and a step loads 3 neighbours of x instead of 8.  Neighbours outside the
+// image read 0: their coefficient is 0 (a rejected or non-adjacent
+// neighbour) and x is finite, so the product adds nothing, as when the
+// reference's index reads pixel 0 there.  The sum runs in the reference's
+// order N NE E SE S SW W NW.
 __device__ void gs_sweep(const Chunk &C, float *x, const float *lk, const float *rm) {
-    const int th = C.th, tw = C.tw, np = C.np;
-    constexpr int P = 1;
+#pragma clang fp contract(off)
+    const int th = C.th, tw = C.tw, h0 = (th + 1) / 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int kWaves = kGsThreads / 64;
     int base = 0;
     for (int color = 0; color < 4; color++) {
         const int rp = color >> 1, cp = color & 1;
@@ -577,23 +637,51 @@ __device__ void gs_sweep(const Chunk &C, float *x, const float *lk, const float 
         const float4 *rv = (const float4 *)rm + 2 * (size_t)base;
         const float *lv = lk + base;
         base += n;
-        for (int k0 = threadIdx.x; k0 < n; k0 += P * kGsThreads) {
-            int ii[P], rr[P], cc[P];
-            float4 a[P], b[P];
-            float l[P], v[P];
-            for (int q = 0; q < P; q++) {
-                const int k = k0 + q * kGsThreads;
-                const int kk = k < n ? k : k0;
-                rr[q] = rp + 2 * (kk % nr);
-                cc[q] = cp + 2 * (kk / nr);
-                ii[q] = rr[q] + cc[q] * th;
-                a[q] = rv[kk];
-                b[q] = rv[n + kk];
-                l[q] = lv[kk];
+        const int nRB = (nr + 63) / 64;
+        const int nSeg = nRB >= kWaves ? 1 : kWaves / nRB;
+        for (int u = wave; u < nRB * nSeg; u += kWaves) {
+            const int rb = u % nRB, sg = u / nRB;
+            const int ci0 = (int)((long)sg * nc / nSeg), ci1 = (int)((long)(sg + 1) * nc / nSeg);
+            const int ri = rb * 64 + lane;
+            const bool valid = ri < nr;
+            const int r = rp + 2 * ri;
+            const bool hasN = valid && r >= 1, hasS = valid && r + 1 < th;
+            const int rowN = xrow(r - 1, h0), row0 = xrow(r, h0), rowS = xrow(r + 1, h0);
+            float xw = 0.f, xsw = 0.f, xnw = 0.f;
+            if (ci0 < ci1 && cp + 2 * ci0 >= 1) {
+                const int cw = (cp + 2 * ci0 - 1) * th;
+                if (valid) xw = x[cw + row0];
+                if (hasS) xsw = x[cw + rowS];
+                if (hasN) xnw = x[cw + rowN];
             }
-            for (int q = 0; q < P; q++) v[q] = gs_update(x, a[q], b[q], l[q], ii[q], rr[q], cc[q], th, tw, np);
-            for (int q = 0; q < P; q++)
-                if (k0 + q * kGsThreads < n) x[ii[q]] = v[q];
+            for (int ci = ci0; ci < ci1; ci++) {
+                const int c = cp + 2 * ci, cm = c * th, ce = cm + th;
+                const bool hasE = c + 1 < tw;
+                const int kk = valid ? ri + ci * nr : 0;
+                const float4 a = rv[kk], b = rv[n + kk];
+                const float l = lv[kk];
+                const float xs = hasS ? x[cm + rowS] : 0.f;
+                const float xe = (valid && hasE) ? x[ce + row0] : 0.f;
+                const float xse = (hasS && hasE) ? x[ce + rowS] : 0.f;
+                float xn = from_below(xs), xne = from_below(xse);
+                if (lane == 0) {
+                    xn = hasN ? x[cm + rowN] : 0.f;
+                    xne = (hasN && hasE) ? x[ce + rowN] : 0.f;
+                }
+                float sum = 0.f;
+                sum += xn * a.x;
+                sum += xne * a.y;
+                sum += xe * a.z;
+                sum += xse * a.w;
+                sum += xs * b.x;
+                sum += xsw * b.y;
+                sum += xw * b.z;
+                sum += xnw * b.w;
+                if (valid) x[cm + row0] = l - sum;
+                xw = xe;
+                xsw = xse;
+                xnw = xne;
+            }
         }
         __syncthreads();
     }
@@ -782,7 +870,8 @@ __global__ __launch_bounds__(256) void k_output(Chunk C, const int *chosen, floa
         return;
     }
     const bool m = C.mask[o] != 0;
-    const float v = ch < 0 ? 0.0f : C.snap[((size_t)f * kNSlots + ch) * C.np + i];
+    const int th = C.th, c = i / th;
+    const float v = ch < 0 ? 0.0f : C.snap[((size_t)f * kNSlots + ch) * C.np + c * th + xrow(i - c * th, (th + 1) / 2)];
     out[o] = m ? v : __builtin_nanf("");
     maskOut[o] = m ? 1 : 0;
 }
