@@ -1317,20 +1317,28 @@ int orc_ibs_system(const double *data, int npd_, const double *dx, const double 
  * sparse matrix); the pixel terms x_i (Q x)_i (double) and Le_i x_i (single)
  * are summed in ORC_IBS_LANES partial sums, then added by a fixed binary
  * tree (t += t + 512, then 256, ... 1).  The partials come in 16 groups of
- * 64.  Rows are cut into blocks of 64 (nrb of them) and columns into
- * nseg = max(1, 16 / nrb) equal segments (segment s: columns s tw / nseg up
- * to (s + 1) tw / nseg); unit u is row block u mod nrb by segment u / nrb.
+ * 64.  Rows are cut into blocks of 62 (nrb of them) and columns into
+ * nseg = 16 / gcd(nrb, 16) segments (segment s: columns s tw / nseg up to
+ * (s + 1) tw / nseg); unit u is row block u mod nrb by segment u / nrb.
  * Group g takes the units g, g + 16, ... in turn, and within a unit partial
- * 64 g + l takes row 64 rb + l over the segment's columns left to right. */
+ * 64 g + l (l = 1 .. 62) takes row 62 rb + l - 1 over the segment's columns
+ * left to right; partials 64 g and 64 g + 63 stay zero. */
 #define ORC_IBS_LANES 1024
 #define ORC_IBS_GROUP 64
+#define ORC_IBS_ROWS 62
 float orc_ibs_energy(const double *qm, const double *wy, const float *x, const uint8_t *mask, int th,
                      int tw) {
     static const int ord[9][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {0, 0},
                                   {1, 0},   {-1, 1}, {0, 1},  {1, 1}};
     const int groups = ORC_IBS_LANES / ORC_IBS_GROUP;
-    const int nrb = (th + ORC_IBS_GROUP - 1) / ORC_IBS_GROUP;
-    const int nseg = nrb >= groups ? 1 : groups / nrb;
+    const int nrb = (th + ORC_IBS_ROWS - 1) / ORC_IBS_ROWS;
+    int a = nrb, b = groups;
+    while (b) {
+        int t = a % b;
+        a = b;
+        b = t;
+    }
+    const int nseg = groups / a;
     double pq[ORC_IBS_LANES];
     float pl[ORC_IBS_LANES];
     for (int t = 0; t < ORC_IBS_LANES; t++) {
@@ -1341,8 +1349,8 @@ float orc_ibs_energy(const double *qm, const double *wy, const float *x, const u
         for (int u = g; u < nrb * nseg; u += groups) {
             const int rb = u % nrb, sg = u / nrb;
             const long c0 = (long)sg * tw / nseg, c1 = (long)(sg + 1) * tw / nseg;
-            for (int l = 0; l < ORC_IBS_GROUP; l++) {
-                const long r = (long)rb * ORC_IBS_GROUP + l;
+            for (int l = 1; l <= ORC_IBS_ROWS; l++) {
+                const long r = (long)rb * ORC_IBS_ROWS + l - 1;
                 if (r >= th) continue;
                 const int t = g * ORC_IBS_GROUP + l;
                 for (long c = c0; c < c1; c++) {

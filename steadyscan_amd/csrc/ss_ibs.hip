@@ -493,17 +493,32 @@ __device__ inline double qx_generic(const float *x, const uint8_t *mask, const d
     return qx;
 }
 
+// Units of a wave-per-row-block walk: rows in blocks of `rows`, columns in
+// nSeg segments with nRB * nSeg a multiple of the 16 waves (nSeg = 16 /
+// gcd(nRB, 16)), so every wave gets the same number of units.
+__device__ inline int walk_segments(int nRB) {
+    int a = nRB, b = kGsThreads / 64;
+    while (b) {
+        const int t = a % b;
+        a = b;
+        b = t;
+    }
+    return (kGsThreads / 64) / a;
+}
+
 // The stopping energy single(x'Qx - 2 single(Le x)) over the mask in the
 // fixed order: (Q x)_i over the neighbours in ascending index; 1024 partial
 // sums (thread t), added by a binary tree (t += t + 512, then 256, ... 1).
-// Rows are cut into blocks of 64 and columns into nSeg = max(1, 16 / blocks)
-// segments; unit u (row block u mod blocks, segment u / blocks) goes to wave
-// u mod 16, and lane l of the wave sums row 64 rb + l over the segment's
-// columns left to right.  Walking a row keeps the W column's x and Q from
-// the step before, and the rows above and below come from the neighbouring
-// lanes (a shuffle; lanes 0 and 63 load their own), so a step loads one x
-// and the pixel's own five Q values.  Pixels whose eight neighbours are all
-// masked take the interior formula; other masked pixels test each term.
+// Rows are cut into blocks of 62 and columns into nSeg segments (segment s:
+// columns s tw / nSeg up to (s + 1) tw / nSeg); unit u is row block
+// u mod blocks by segment u / blocks and goes to wave u mod 16.  Lane l of
+// the wave holds row 62 rb + l - 1: lanes 1..62 sum their row over the
+// segment's columns left to right, lanes 0 and 63 only load the rows above
+// and below for them.  Walking a row keeps the W column's x and Q from the
+// step before and takes the rows above and below from the neighbouring
+// lanes, so a step loads one x and the pixel's own five Q values.  Pixels
+// whose eight neighbours are all masked take the interior formula; other
+// masked pixels test each term.
 __device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, const uint8_t *full,
                            const double *qs, const float *lf, double *sp, float *sl) {
 #pragma clang fp contract(off)
@@ -511,61 +526,41 @@ __device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, 
     const double *qD = qs, *qS = qs + QS_S * (size_t)np, *qNE = qs + QS_NE * (size_t)np,
                  *qE = qs + QS_E * (size_t)np, *qSE = qs + QS_SE * (size_t)np;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int kWaves = kGsThreads / 64;
-    const int nRB = (th + 63) / 64, nSeg = nRB >= kWaves ? 1 : kWaves / nRB;
+    constexpr int kWaves = kGsThreads / 64, kRows = 62;
+    const int nRB = (th + kRows - 1) / kRows, nSeg = walk_segments(nRB);
     double pq = 0.0;
     float pl = 0.f;
     for (int u = wave; u < nRB * nSeg; u += kWaves) {
         const int rb = u % nRB, sg = u / nRB;
         const int c0 = (int)((long)sg * tw / nSeg), c1 = (int)((long)(sg + 1) * tw / nSeg);
-        const int r = rb * 64 + lane;
-        const bool valid = r < th, hasN = valid && r >= 1, hasS = valid && r + 1 < th;
-        const int rowN = xrow(r - 1, h0), row0 = xrow(r, h0), rowS = xrow(r + 1, h0);
+        const int r = rb * kRows + lane - 1;
+        const bool inRow = r >= 0 && r < th;
+        const bool sums = inRow && lane >= 1 && lane <= kRows;
+        const int row0 = xrow(r, h0);
         // x of the columns c - 1 and c at the rows r - 1, r, r + 1, and the W
         // column's couplings to this pixel: qSE(r-1, c-1) qE(r, c-1) qNE(r+1, c-1)
-        float xnw = 0.f, xw = 0.f, xsw = 0.f, xn = 0.f, xc = 0.f, xs = 0.f;
-        double qnw = 0.0, qw = 0.0, qsw = 0.0;
-        if (c0 < c1) {
+        float xw = 0.f, xc = 0.f;
+        double qw = 0.0, wSE = 0.0, wNE = 0.0;
+        if (c0 < c1 && inRow) {
             if (c0 >= 1) {
-                const int cw = (c0 - 1) * th, iw = r + cw;
-                if (valid) {
-                    xw = x[cw + row0];
-                    qw = qE[iw];
-                }
-                if (hasN) {
-                    xnw = x[cw + rowN];
-                    qnw = qSE[iw - 1];
-                }
-                if (hasS) {
-                    xsw = x[cw + rowS];
-                    qsw = qNE[iw + 1];
-                }
+                const int iw = r + (c0 - 1) * th;
+                xw = x[(c0 - 1) * th + row0];
+                qw = qE[iw];
+                wSE = qSE[iw];
+                wNE = qNE[iw];
             }
-            const int cm = c0 * th;
-            if (valid) xc = x[cm + row0];
-            if (hasN) xn = x[cm + rowN];
-            if (hasS) xs = x[cm + rowS];
+            xc = x[c0 * th + row0];
         }
+        float xnw = from_below(xw), xsw = from_above(xw), xn = from_below(xc), xs = from_above(xc);
+        double qnw = from_below(wSE), qsw = from_above(wNE);
         for (int c = c0; c < c1; c++) {
-            const int i = valid ? r + c * th : 0;
-            const bool hasE = c + 1 < tw;
-            const int ce = (c + 1) * th;
-            const float xe = (valid && hasE) ? x[ce + row0] : 0.f;
-            float xne = from_below(xe), xse = from_above(xe);
+            const int i = inRow ? r + c * th : 0;
+            const float xe = (inRow && c + 1 < tw) ? x[(c + 1) * th + row0] : 0.f;
+            const float xne = from_below(xe), xse = from_above(xe);
             const double dD = qD[i], dS = qS[i], dNE = qNE[i], dE = qE[i], dSE = qSE[i];
-            double dN = from_below(dS);
-            double nqnw = from_below(dSE), nqsw = from_above(dNE);
-            if (lane == 0) {
-                xne = (hasN && hasE) ? x[ce + rowN] : 0.f;
-                dN = hasN ? qS[i - 1] : 0.0;
-                nqnw = hasN ? qSE[i - 1] : 0.0;
-            }
-            if (lane == 63) {
-                xse = (hasS && hasE) ? x[ce + rowS] : 0.f;
-                nqsw = hasS ? qNE[i + 1] : 0.0;
-            }
-            const uint8_t fl = valid ? full[i] : 0;
-            const float l = valid ? lf[i] : 0.f;
+            const double dN = from_below(dS);
+            const uint8_t fl = sums ? full[i] : 0;
+            const float l = lf[i];
             if (fl) {
                 double qx = 0.0;
                 qx += (double)xnw * qnw;
@@ -579,7 +574,7 @@ __device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, 
                 qx += (double)xse * dSE;
                 pq += (double)xc * qx;
                 pl += l * xc;
-            } else if (valid && mask[i]) {
+            } else if (sums && mask[i]) {
                 const double qx = qx_generic(x, mask, qD, qS, qNE, qE, qSE, i, r, c, th, tw, h0);
                 pq += (double)xc * qx;
                 pl += l * xc;
@@ -590,9 +585,9 @@ __device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, 
             xn = xne;
             xc = xe;
             xs = xse;
-            qnw = nqnw;
+            qnw = from_below(dSE);
             qw = dE;
-            qsw = nqsw;
+            qsw = from_above(dNE);
         }
     }
     const int t = threadIdx.x;
@@ -613,12 +608,11 @@ __device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, 
 
 // One iteration: the colours A B C D (1-based odd/odd, odd/even, even/odd,
 // even/even row/column) in turn.  Pixels of one colour do not read each
-// other, so any order within a colour gives the same result.  Each wave
-// takes 64 consecutive rows of the colour (one per lane) and walks a segment
-// of its columns left to right, so that
+// other, so any order within a colour gives the same result.  A wave takes
+// 63 consecutive rows of the colour (lanes 1..63; lane 0 holds the row
+// above them) and walks a segment of its columns left to right, so that
 //   - the W side of a pixel is the E side loaded one step before,
-//   - N and NE are the S and SE of the lane above (a shuffle; lane 0 loads
-//     its own),
+//   - N and NE are the S and SE of the lane above (a lane shift),
 // and a step loads 3 neighbours of x instead of 8.  Neighbours outside the
 // image read 0: their coefficient is 0 (a rejected or non-adjacent
 // neighbour) and x is finite, so the product adds nothing, as when the
@@ -628,7 +622,7 @@ __device__ void gs_sweep(const Chunk &C, float *x, const float *lk, const float 
 #pragma clang fp contract(off)
     const int th = C.th, tw = C.tw, h0 = (th + 1) / 2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int kWaves = kGsThreads / 64;
+    constexpr int kWaves = kGsThreads / 64, kRows = 63;
     int base = 0;
     for (int color = 0; color < 4; color++) {
         const int rp = color >> 1, cp = color & 1;
@@ -637,37 +631,32 @@ __device__ void gs_sweep(const Chunk &C, float *x, const float *lk, const float 
         const float4 *rv = (const float4 *)rm + 2 * (size_t)base;
         const float *lv = lk + base;
         base += n;
-        const int nRB = (nr + 63) / 64;
-        const int nSeg = nRB >= kWaves ? 1 : kWaves / nRB;
+        const int nRB = (nr + kRows - 1) / kRows, nSeg = walk_segments(nRB);
         for (int u = wave; u < nRB * nSeg; u += kWaves) {
             const int rb = u % nRB, sg = u / nRB;
             const int ci0 = (int)((long)sg * nc / nSeg), ci1 = (int)((long)(sg + 1) * nc / nSeg);
-            const int ri = rb * 64 + lane;
-            const bool valid = ri < nr;
+            const int ri = rb * kRows + lane - 1;
             const int r = rp + 2 * ri;
-            const bool hasN = valid && r >= 1, hasS = valid && r + 1 < th;
-            const int rowN = xrow(r - 1, h0), row0 = xrow(r, h0), rowS = xrow(r + 1, h0);
-            float xw = 0.f, xsw = 0.f, xnw = 0.f;
+            const bool upd = lane >= 1 && ri < nr;
+            const bool hasS = (lane == 0 || upd) && r + 1 >= 0 && r + 1 < th;
+            const int row0 = xrow(r, h0), rowS = xrow(r + 1, h0);
+            float xw = 0.f, xsw = 0.f;
             if (ci0 < ci1 && cp + 2 * ci0 >= 1) {
                 const int cw = (cp + 2 * ci0 - 1) * th;
-                if (valid) xw = x[cw + row0];
+                if (upd) xw = x[cw + row0];
                 if (hasS) xsw = x[cw + rowS];
-                if (hasN) xnw = x[cw + rowN];
             }
+            float xnw = from_below(xsw);
             for (int ci = ci0; ci < ci1; ci++) {
                 const int c = cp + 2 * ci, cm = c * th, ce = cm + th;
                 const bool hasE = c + 1 < tw;
-                const int kk = valid ? ri + ci * nr : 0;
+                const int kk = upd ? ri + ci * nr : 0;
                 const float4 a = rv[kk], b = rv[n + kk];
                 const float l = lv[kk];
                 const float xs = hasS ? x[cm + rowS] : 0.f;
-                const float xe = (valid && hasE) ? x[ce + row0] : 0.f;
+                const float xe = (upd && hasE) ? x[ce + row0] : 0.f;
                 const float xse = (hasS && hasE) ? x[ce + rowS] : 0.f;
-                float xn = from_below(xs), xne = from_below(xse);
-                if (lane == 0) {
-                    xn = hasN ? x[cm + rowN] : 0.f;
-                    xne = (hasN && hasE) ? x[ce + rowN] : 0.f;
-                }
+                const float xn = from_below(xs), xne = from_below(xse);
                 float sum = 0.f;
                 sum += xn * a.x;
                 sum += xne * a.y;
@@ -677,7 +666,7 @@ __device__ void gs_sweep(const Chunk &C, float *x, const float *lk, const float 
                 sum += xsw * b.y;
                 sum += xw * b.z;
                 sum += xnw * b.w;
-                if (valid) x[cm + row0] = l - sum;
+                if (upd) x[cm + row0] = l - sum;
                 xw = xe;
                 xsw = xse;
                 xnw = xne;
