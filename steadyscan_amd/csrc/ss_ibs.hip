@@ -192,34 +192,46 @@ __global__ __launch_bounds__(256) void k_points(Chunk C) {
 
 // Exclusive scan of the per-cell counts (one workgroup per frame); the
 // counts are cleared for use as fill cursors.
+// Exclusive prefix sum of a frame's per-cell counts into start (start[np] =
+// total), clearing the counts for k_fill.  Tiles of 4096 counts, 4
+// consecutive ones per thread: a wave scan by shuffles, the 16 wave totals
+// through LDS, and a carry from tile to tile.
 __global__ __launch_bounds__(1024) void k_scan(Chunk C) {
-    __shared__ int part[1024];
-    const int f = blockIdx.x, t = threadIdx.x;
+    __shared__ int wsum[16];
+    const int f = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
     int *cnt = C.cnt + (size_t)f * C.np;
     int *start = C.start + (size_t)f * (C.np + 1);
-    const int per = (C.np + 1023) / 1024;
-    const int lo = t * per < C.np ? t * per : C.np;
-    const int hi = lo + per < C.np ? lo + per : C.np;
-    int s = 0;
-    for (int i = lo; i < hi; i++) s += cnt[i];
-    part[t] = s;
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int k = 0; k < 1024; k++) {
-            const int v = part[k];
-            part[k] = acc;
-            acc += v;
+    const int np = C.np;
+    int carry = 0;
+    for (int base = 0; base < np; base += 4096) {
+        const int i0 = base + 4 * t;
+        int v[4];
+        for (int q = 0; q < 4; q++) v[q] = i0 + q < np ? cnt[i0 + q] : 0;
+        const int own = v[0] + v[1] + v[2] + v[3];
+        int incl = own;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
         }
-        start[C.np] = acc;
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        int before = 0, tile = 0;
+        for (int k = 0; k < 16; k++) {
+            const int ws = wsum[k];
+            if (k < w) before += ws;
+            tile += ws;
+        }
+        int acc = carry + before + incl - own;
+        for (int q = 0; q < 4; q++)
+            if (i0 + q < np) {
+                start[i0 + q] = acc;
+                acc += v[q];
+                cnt[i0 + q] = 0;
+            }
+        carry += tile;
+        __syncthreads();
     }
-    __syncthreads();
-    int acc = part[t];
-    for (int i = lo; i < hi; i++) {
-        start[i] = acc;
-        acc += cnt[i];
-        cnt[i] = 0;
-    }
+    if (t == 0) start[np] = carry;
 }
 
 __global__ __launch_bounds__(256) void k_fill(Chunk C) {
