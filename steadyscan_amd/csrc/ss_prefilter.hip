@@ -24,6 +24,8 @@
 // a fixed order (deterministic).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -210,51 +212,64 @@ struct FillChunk {
 };
 
 // Deterministic dot product over the first n entries: thread t sums indices
-// t, t + 1024, ... in order, then a binary tree over the 1024 partials.
+// t, t + 1024, ... in order; each wave adds its 64 partials by a butterfly,
+// and the 16 wave sums are added in order.
 __device__ double cg_dot(const double *u, const double *v, int n, double *sp) {
     double s = 0.0;
     for (int m = threadIdx.x; m < n; m += kCgThreads) s += u[m] * v[m];
-    sp[threadIdx.x] = s;
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sp[threadIdx.x >> 6] = s;
     __syncthreads();
-    for (int off = kCgThreads / 2; off >= 1; off >>= 1) {
-        if ((int)threadIdx.x < off) sp[threadIdx.x] += sp[threadIdx.x + off];
-        __syncthreads();
-    }
-    const double r = sp[0];
+    double r = 0.0;
+    for (int w = 0; w < kCgThreads / 64; w++) r += sp[w];
     __syncthreads();
     return r;
 }
 
+// The missing (NaN) pixels of a frame in ascending order, and their count:
+// tiles of 4 * kCgThreads pixels, 4 consecutive ones per thread, a wave
+// scan by shuffles and the wave totals through LDS.
 __global__ __launch_bounds__(kCgThreads) void k_fill_list(FillChunk C) {
-    __shared__ int part[kCgThreads];
-    const int f = blockIdx.x, t = threadIdx.x;
+    constexpr int kW = kCgThreads / 64;
+    __shared__ int wsum[kW];
+    const int f = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
     const double *a = C.a + (size_t)f * C.np;
     int *list = C.list + (size_t)f * C.np;
-    const int per = (C.np + kCgThreads - 1) / kCgThreads;
-    const int lo = t * per < C.np ? t * per : C.np;
-    const int hi = lo + per < C.np ? lo + per : C.np;
-    int cnt = 0;
-    for (int i = lo; i < hi; i++) cnt += isnan(a[i]) ? 1 : 0;
-    part[t] = cnt;
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int k = 0; k < kCgThreads; k++) {
-            const int v = part[k];
-            part[k] = acc;
-            acc += v;
+    const int np = C.np;
+    int carry = 0;
+    for (int base = 0; base < np; base += 4 * kCgThreads) {
+        const int i0 = base + 4 * t;
+        bool miss[4];
+        int own = 0;
+        for (int q = 0; q < 4; q++) {
+            miss[q] = i0 + q < np && isnan(a[i0 + q]);
+            own += miss[q] ? 1 : 0;
         }
-        C.nmiss[f] = acc;
+        int incl = own;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        int before = 0, tile = 0;
+        for (int k = 0; k < kW; k++) {
+            const int ws = wsum[k];
+            if (k < w) before += ws;
+            tile += ws;
+        }
+        int pos = carry + before + incl - own;
+        for (int q = 0; q < 4; q++)
+            if (miss[q]) list[pos++] = i0 + q;
+        carry += tile;
+        __syncthreads();
     }
-    __syncthreads();
-    int pos = part[t];
-    for (int i = lo; i < hi; i++)
-        if (isnan(a[i])) list[pos++] = i;
+    if (t == 0) C.nmiss[f] = carry;
 }
 
 __global__ __launch_bounds__(kCgThreads) void k_fill_cg(FillChunk C) {
 #pragma clang fp contract(off)
-    __shared__ double sp[kCgThreads];
+    __shared__ double sp[kCgThreads / 64];
     const int f = blockIdx.x;
     const int th = C.th, tw = C.tw, np = C.np;
     const size_t fo = (size_t)f * np;
@@ -388,7 +403,12 @@ extern "C" ss_status ss_prefilter(ss_handle *h, ss_dtype type, const void *in, v
         make(S2, g2);
     }
     const size_t perFrame = align256(np * 8) * 2 + (on_device ? 0 : align256(np * es));
-    const int B = (int)std::max<size_t>(1, std::min<size_t>((size_t)n_frames, ((size_t)4 << 30) / perFrame));
+    // work memory: half of what is free on the device or already held by the handle
+    PCHK(hipSetDevice(ss::handle_device(h)));
+    size_t freeB = 0, totalB = 0;
+    PCHK(hipMemGetInfo(&freeB, &totalB));
+    const size_t budget = std::max<size_t>((size_t)1 << 30, (freeB + ss::handle_scratch_held(h)) / 2);
+    const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)n_frames, budget / perFrame, (size_t)65535}));
     ss_status rc;
     const size_t total = align256(8 * (size_t)(S1 + S2)) + perFrame * B;
     char *base = (char *)ss::handle_scratch(h, total, &rc);
@@ -448,7 +468,12 @@ extern "C" ss_status ss_fill_gaps(ss_handle *h, ss_dtype type, const void *in, v
     const size_t np = (size_t)H * W, es = esize(type);
     const size_t perFrame = align256(np * 8) * 7 + align256(np * 4) + align256(8) * 2 +
                             (on_device ? 0 : align256(np * es));
-    const int B = (int)std::max<size_t>(1, std::min<size_t>((size_t)n_frames, ((size_t)4 << 30) / perFrame));
+    // work memory: half of what is free on the device or already held by the handle
+    PCHK(hipSetDevice(ss::handle_device(h)));
+    size_t freeB = 0, totalB = 0;
+    PCHK(hipMemGetInfo(&freeB, &totalB));
+    const size_t budget = std::max<size_t>((size_t)1 << 30, (freeB + ss::handle_scratch_held(h)) / 2);
+    const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)n_frames, budget / perFrame, (size_t)65535}));
     ss_status rc;
     char *base = (char *)ss::handle_scratch(h, perFrame * B, &rc);
     if (!base) return rc;
@@ -500,6 +525,18 @@ extern "C" ss_status ss_fill_gaps(ss_handle *h, ss_dtype type, const void *in, v
         PCHK(hipMemcpyAsync(status.data(), C.status, 4 * nb, hipMemcpyDeviceToHost, s));
         if (!on_device) PCHK(hipMemcpyAsync(dst, dio, es * cnt, hipMemcpyDeviceToHost, s));
         PCHK(hipStreamSynchronize(s));
+        if (const char *e = std::getenv("SS_FILL_STATS"); e && e[0] == '1') {
+            long sum = 0;
+            int mx = 0, nf = 0;
+            for (int f = 0; f < nb; f++)
+                if (status[f] > 0) {
+                    sum += status[f];
+                    mx = std::max(mx, status[f]);
+                    nf++;
+                }
+            std::fprintf(stderr, "[fill] chunk of %d frames: %d filled, CG iterations mean %.1f max %d\n", nb, nf,
+                         nf ? (double)sum / nf : 0.0, mx);
+        }
         for (int f = 0; f < nb; f++)
             if (status[f] < 0)
                 return ss::handle_fail(h, SS_ERR_NUMERICAL,
