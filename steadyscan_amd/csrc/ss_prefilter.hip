@@ -211,21 +211,6 @@ struct FillChunk {
     int maxit;
 };
 
-// Deterministic dot product over the first n entries: thread t sums indices
-// t, t + 1024, ... in order; each wave adds its 64 partials by a butterfly,
-// and the 16 wave sums are added in order.
-__device__ double cg_dot(const double *u, const double *v, int n, double *sp) {
-    double s = 0.0;
-    for (int m = threadIdx.x; m < n; m += kCgThreads) s += u[m] * v[m];
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0) sp[threadIdx.x >> 6] = s;
-    __syncthreads();
-    double r = 0.0;
-    for (int w = 0; w < kCgThreads / 64; w++) r += sp[w];
-    __syncthreads();
-    return r;
-}
-
 // The missing (NaN) pixels of a frame in ascending order, and their count:
 // tiles of 4 * kCgThreads pixels, 4 consecutive ones per thread, a wave
 // scan by shuffles and the wave totals through LDS.
@@ -267,6 +252,131 @@ __global__ __launch_bounds__(kCgThreads) void k_fill_list(FillChunk C) {
     if (t == 0) C.nmiss[f] = carry;
 }
 
+// Sum over the workgroup of one value per thread: a butterfly in each wave,
+// then the 16 wave sums in order.
+__device__ double block_sum(double v, double *sp) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sp[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < kCgThreads / 64; w++) r += sp[w];
+    __syncthreads();
+    return r;
+}
+
+// Conjugate gradients for a frame with at most kFillLds missing pixels, in
+// compact form (entry m = the m-th missing pixel): the search direction p
+// and the residual r live in LDS, the solution x in global memory, and
+// thread t keeps, for its entries t, t + 1024, ..., only the degree and the
+// compact indices of the missing W N S E neighbours (14 bits each).  Same
+// algebra and dot-product order as k_fill_cg; a neighbour's term enters Ap
+// only when it is missing (a known neighbour's direction is zero).
+constexpr int kFillK = 8;
+constexpr int kFillLds = kFillK * kCgThreads;
+constexpr unsigned kNoNbr = 0x3fffu;
+
+__global__ __launch_bounds__(kCgThreads) void k_fill_cg_lds(FillChunk C) {
+#pragma clang fp contract(off)
+    __shared__ double sp[kCgThreads / 64];
+    __shared__ double lp[kFillLds], lr[kFillLds];
+    const int f = blockIdx.x, t = threadIdx.x;
+    const int th = C.th, tw = C.tw, np = C.np;
+    const size_t fo = (size_t)f * np;
+    const double *a = C.a + fo;
+    double *out = C.out + fo, *x = C.x + fo;
+    const int *list = C.list + fo;
+    int *map = (int *)(C.pfull + fo);  // compact index of each missing pixel
+    const int n = C.nmiss[f];
+    if (n > kFillLds) return;  // k_fill_cg takes this frame
+    for (int i = t; i < np; i += kCgThreads) out[i] = a[i];
+    if (n == 0 || n == np) {  // nothing to fill, or no known pixel (left as NaN)
+        if (t == 0) C.status[f] = 0;
+        return;
+    }
+    for (int k = 0; k < kFillK; k++) {
+        const int m = t + k * kCgThreads;
+        if (m < n) map[list[m]] = m;
+    }
+    __syncthreads();
+    unsigned nb01[kFillK], nb23[kFillK];  // id0 | id1 << 14 | degree << 28, id2 | id3 << 14
+    double part = 0.0;
+    for (int k = 0; k < kFillK; k++) {
+        const int m = t + k * kCgThreads;
+        nb01[k] = nb23[k] = kNoNbr | (kNoNbr << 14);
+        if (m >= n) continue;
+        const int i = list[m], row = i % th, col = i / th;
+        // right-hand side: the known neighbours, W N S E (ascending index)
+        double s = 0.0;
+        unsigned d = 0, id[4] = {kNoNbr, kNoNbr, kNoNbr, kNoNbr};
+        const bool in[4] = {col > 0, row > 0, row + 1 < th, col + 1 < tw};
+        const int j[4] = {i - th, i - 1, i + 1, i + th};
+        for (int q = 0; q < 4; q++) {
+            if (!in[q]) continue;
+            d++;
+            const double v = a[j[q]];
+            if (isnan(v)) id[q] = (unsigned)map[j[q]];
+            else s += v;
+        }
+        nb01[k] = id[0] | (id[1] << 14) | (d << 28);
+        nb23[k] = id[2] | (id[3] << 14);
+        lr[m] = s;
+        lp[m] = s;
+        x[m] = 0.0;
+        part += s * s;
+    }
+    // (A p)_m from the direction in LDS
+    auto apply = [&](int k, int m) {
+        const unsigned id[4] = {nb01[k] & kNoNbr, (nb01[k] >> 14) & kNoNbr, nb23[k] & kNoNbr,
+                                (nb23[k] >> 14) & kNoNbr};
+        double s = 0.0;
+        for (int q = 0; q < 4; q++)
+            if (id[q] != kNoNbr) s += lp[id[q]];
+        return (double)(nb01[k] >> 28) * lp[m] - s;
+    };
+    double rr_ = block_sum(part, sp);  // also orders the LDS writes before the reads
+    const double tol2 = 1e-26 * rr_;
+    int it = 0;
+    bool ok = rr_ <= tol2;
+    while (!ok && it < C.maxit) {
+        it++;
+        part = 0.0;
+        for (int k = 0; k < kFillK; k++) {
+            const int m = t + k * kCgThreads;
+            if (m >= n) break;
+            part += lp[m] * apply(k, m);
+        }
+        const double pap = block_sum(part, sp);
+        const double alpha = rr_ / pap;
+        part = 0.0;
+        for (int k = 0; k < kFillK; k++) {
+            const int m = t + k * kCgThreads;
+            if (m >= n) break;
+            x[m] += alpha * lp[m];
+            const double rm = lr[m] - alpha * apply(k, m);
+            lr[m] = rm;
+            part += rm * rm;
+        }
+        const double rn = block_sum(part, sp);
+        if (rn <= tol2) {
+            ok = true;
+            break;
+        }
+        const double beta = rn / rr_;
+        rr_ = rn;
+        for (int k = 0; k < kFillK; k++) {
+            const int m = t + k * kCgThreads;
+            if (m >= n) break;
+            lp[m] = lr[m] + beta * lp[m];
+        }
+        __syncthreads();
+    }
+    for (int k = 0; k < kFillK; k++) {
+        const int m = t + k * kCgThreads;
+        if (m < n) out[list[m]] = x[m];
+    }
+    if (t == 0) C.status[f] = ok ? it : -1;
+}
+
 __global__ __launch_bounds__(kCgThreads) void k_fill_cg(FillChunk C) {
 #pragma clang fp contract(off)
     __shared__ double sp[kCgThreads / 64];
@@ -276,8 +386,10 @@ __global__ __launch_bounds__(kCgThreads) void k_fill_cg(FillChunk C) {
     const double *a = C.a + fo;
     double *out = C.out + fo;
     const int *list = C.list + fo;
-    double *x = C.x + fo, *r = C.r + fo, *ap = C.ap + fo, *p = C.pfull + fo, *b = C.b + fo;
+    double *__restrict__ x = C.x + fo, *__restrict__ r = C.r + fo, *__restrict__ ap = C.ap + fo,
+                         *__restrict__ p = C.pfull + fo, *__restrict__ b = C.b + fo;
     const int n = C.nmiss[f];
+    if (n <= kFillLds) return;  // k_fill_cg_lds takes this frame
     for (int i = threadIdx.x; i < np; i += kCgThreads) {
         out[i] = a[i];
         p[i] = 0.0;
@@ -288,6 +400,7 @@ __global__ __launch_bounds__(kCgThreads) void k_fill_cg(FillChunk C) {
         return;
     }
     // right-hand side: the known neighbours, W N S E (ascending index)
+    double part = 0.0;
     for (int m = threadIdx.x; m < n; m += kCgThreads) {
         const int i = list[m], rr = i % th, cc = i / th;
         double s = 0.0;
@@ -299,15 +412,18 @@ __global__ __launch_bounds__(kCgThreads) void k_fill_cg(FillChunk C) {
         x[m] = 0.0;
         r[m] = s;
         p[i] = s;
+        part += s * s;
     }
-    __syncthreads();
-    double rr_ = cg_dot(r, r, n, sp);
-    const double bb = rr_;
-    const double tol2 = 1e-26 * bb;
+    double rr_ = block_sum(part, sp);  // also orders p's writes before the reads
+    const double tol2 = 1e-26 * rr_;
     int it = 0;
     bool ok = rr_ <= tol2;
+    // Each pass is unrolled by 4 so that a thread has several entries' loads
+    // in flight; the partial sums still run over m = t, t + 1024, ... in order.
     while (!ok && it < C.maxit) {
         it++;
+        part = 0.0;
+#pragma unroll 4
         for (int m = threadIdx.x; m < n; m += kCgThreads) {
             const int i = list[m], row = i % th, col = i / th;
             double deg = 0.0, s = 0.0;
@@ -315,30 +431,30 @@ __global__ __launch_bounds__(kCgThreads) void k_fill_cg(FillChunk C) {
             if (row > 0) { deg += 1.0; s += p[i - 1]; }
             if (row + 1 < th) { deg += 1.0; s += p[i + 1]; }
             if (col + 1 < tw) { deg += 1.0; s += p[i + th]; }
-            ap[m] = deg * p[i] - s;
+            const double pi = p[i], api = deg * pi - s;
+            ap[m] = api;
+            b[m] = pi;
+            part += pi * api;
         }
-        __syncthreads();
-        // p restricted to the list, for the dot product
-        for (int m = threadIdx.x; m < n; m += kCgThreads) b[m] = p[list[m]];
-        __syncthreads();
-        const double pap = cg_dot(b, ap, n, sp);
+        const double pap = block_sum(part, sp);
         const double alpha = rr_ / pap;
+        part = 0.0;
+#pragma unroll 4
         for (int m = threadIdx.x; m < n; m += kCgThreads) {
             x[m] += alpha * b[m];
-            r[m] -= alpha * ap[m];
+            const double rm = r[m] - alpha * ap[m];
+            r[m] = rm;
+            part += rm * rm;
         }
-        __syncthreads();
-        const double rn = cg_dot(r, r, n, sp);
+        const double rn = block_sum(part, sp);
         if (rn <= tol2) {
             ok = true;
             break;
         }
         const double beta = rn / rr_;
         rr_ = rn;
-        for (int m = threadIdx.x; m < n; m += kCgThreads) {
-            const int i = list[m];
-            p[i] = r[m] + beta * p[i];
-        }
+#pragma unroll 4
+        for (int m = threadIdx.x; m < n; m += kCgThreads) p[list[m]] = r[m] + beta * b[m];
         __syncthreads();
     }
     for (int m = threadIdx.x; m < n; m += kCgThreads) out[list[m]] = x[m];
@@ -517,6 +633,7 @@ extern "C" ss_status ss_fill_gaps(ss_handle *h, ss_dtype type, const void *in, v
         if (type == SS_F64) hipLaunchKernelGGL(k_to_double<double>, g1d, dim3(256), 0, s, (const double *)work, da, cnt);
         else hipLaunchKernelGGL(k_to_double<float>, g1d, dim3(256), 0, s, (const float *)work, da, cnt);
         hipLaunchKernelGGL(k_fill_list, dim3(nb), dim3(kCgThreads), 0, s, C);
+        hipLaunchKernelGGL(k_fill_cg_lds, dim3(nb), dim3(kCgThreads), 0, s, C);
         hipLaunchKernelGGL(k_fill_cg, dim3(nb), dim3(kCgThreads), 0, s, C);
         char *o = on_device ? dst : dio;
         if (type == SS_F64) hipLaunchKernelGGL(k_from_double<double>, g1d, dim3(256), 0, s, C.out, (double *)o, cnt);
@@ -526,16 +643,24 @@ extern "C" ss_status ss_fill_gaps(ss_handle *h, ss_dtype type, const void *in, v
         if (!on_device) PCHK(hipMemcpyAsync(dst, dio, es * cnt, hipMemcpyDeviceToHost, s));
         PCHK(hipStreamSynchronize(s));
         if (const char *e = std::getenv("SS_FILL_STATS"); e && e[0] == '1') {
-            long sum = 0;
-            int mx = 0, nf = 0;
-            for (int f = 0; f < nb; f++)
+            std::vector<int> nm(nb);
+            PCHK(hipMemcpy(nm.data(), C.nmiss, 4 * (size_t)nb, hipMemcpyDeviceToHost));
+            long sum = 0, msum = 0;
+            int mx = 0, nf = 0, inLds = 0, mmax = 0;
+            for (int f = 0; f < nb; f++) {
+                msum += nm[f];
+                mmax = std::max(mmax, nm[f]);
+                inLds += nm[f] <= kFillLds;
                 if (status[f] > 0) {
                     sum += status[f];
                     mx = std::max(mx, status[f]);
                     nf++;
                 }
-            std::fprintf(stderr, "[fill] chunk of %d frames: %d filled, CG iterations mean %.1f max %d\n", nb, nf,
-                         nf ? (double)sum / nf : 0.0, mx);
+            }
+            std::fprintf(stderr,
+                         "[fill] chunk of %d frames: %d filled, CG iterations mean %.1f max %d; missing pixels "
+                         "mean %.0f max %d, %d frames in LDS\n",
+                         nb, nf, nf ? (double)sum / nf : 0.0, mx, (double)msum / nb, mmax, inLds);
         }
         for (int f = 0; f < nb; f++)
             if (status[f] < 0)
