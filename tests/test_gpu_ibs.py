@@ -117,6 +117,18 @@ def test_rectangular_template_aspect_and_cropping():
     compare(fr, dx, dy, 40, 50, nk, cropping=(2, 4, 0, 0), aspect=2.0, label="40x50 aspect 2 cropped")
 
 
+@pytest.mark.parametrize("nl,lw,th,tw", [(210, 40, 200, 37), (1110, 10, 1100, 8), (70, 70, 63, 65)])
+def test_tall_and_odd_templates(nl, lw, th, tw):
+    # row counts that are not multiples of the GPU walks' row blocks (62 rows
+    # for the energy, 63 colour rows for the sweep), more blocks than waves
+    # (1100 rows), and a template with an odd row count
+    rng = np.random.default_rng(th + tw)
+    fr = 20 + 50 * rng.random((nl, lw, 2))
+    nk = nl // 16 + 1
+    dx, dy = smooth_knots(rng, nk, 2, 0.2), smooth_knots(rng, nk, 2, 0.2)
+    compare(fr, dx, dy, th, tw, nk, regfac=0.05, label=f"{th}x{tw} template")
+
+
 def test_subrec_squeezes_several_pixels_per_cell():
     # a scan mapped onto a quarter-size template region: many data pixels per
     # cell, repeated pushgrid targets (the last value is kept)
