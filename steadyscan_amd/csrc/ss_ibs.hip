@@ -12,7 +12,7 @@
 // 1 for the call's first frame and max(1, min(10, round(sqrt(iter)))) of the
 // previous frame afterwards, which chains the frames.
 //
-// GPU form, for a chunk of frames at once:
+// GPU form, for a chunk of frames at once (by default all frames of the call):
 //   points     one thread per data pixel: displaced position; the cell of
 //              its constraint (counted per cell) and the pushgrid winners
 //              (largest pixel index per target and corner, as MATLAB's
@@ -25,13 +25,14 @@
 //              data pixels, bit for bit, without floating-point atomics;
 //   interact   the masked stencil, calc_interactions' R, L and Rm;
 //   gs         one workgroup per frame runs the Gauss-Seidel loop with the
-//              energy summed in a fixed order (1024 interleaved partial sums
-//              added by a binary tree) and tracks the stopping
-//              rule for all ten possible check intervals at once, keeping a
-//              snapshot per interval.  Each interval's stop is published as
-//              it happens; a frame walks its predecessors' published stops
-//              and, once its own interval follows from them, drops the
-//              other intervals;
+//              energy summed in a fixed order (1024 partial sums, one per
+//              lane of a row walk, added by a binary tree) and tracks the
+//              stopping rule for all ten possible check intervals at once,
+//              keeping a snapshot per interval.  Each interval's stop is
+//              published as it happens, and each frame publishes its
+//              progress every iteration.  A frame drops the intervals its
+//              predecessor's progress rules out, and all but one once its
+//              own interval follows from the predecessors' stops;
 //   resolve    on the host: walks the frames in order, picks each frame's
 //              interval from its predecessor, and
 //   output     writes the chosen snapshot (NaN outside the mask).
@@ -190,8 +191,6 @@ __global__ __launch_bounds__(256) void k_points(Chunk C) {
     }
 }
 
-// Exclusive scan of the per-cell counts (one workgroup per frame); the
-// counts are cleared for use as fill cursors.
 // Exclusive prefix sum of a frame's per-cell counts into start (start[np] =
 // total), clearing the counts for k_fill.  Tiles of 4096 counts, 4
 // consecutive ones per thread: a wave scan by shuffles, the 16 wave totals
