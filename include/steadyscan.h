@@ -362,7 +362,8 @@ ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample_in *in,
  * result keeps the element type, as MATLAB keeps the class: integer frames are
  * rounded half away from zero and saturated.  Median normalisation of integer
  * frames is not supported (SS_ERR_INVALID_ARGUMENT).  in == out is allowed;
- * on_device: both are device pointers. */
+ * on_device: both are device pointers.  The handle keeps the work memory
+ * (frames are processed in chunks within half of the free device memory). */
 ss_status ss_prefilter(ss_handle *h, ss_dtype type, const void *in, void *out, int32_t height,
                        int32_t width, int32_t n_frames, double pregauss, int32_t mednorm,
                        int32_t on_device);
@@ -372,7 +373,9 @@ ss_status ss_prefilter(ss_handle *h, ss_dtype type, const void *in, void *out, i
  * set to the minimiser of the summed squared adjacent differences with the
  * other pixels fixed (a frame without any known pixel is left as it is).
  * type is SS_F32 or SS_F64; the solve runs in FP64 (relative residual 1e-13)
- * and rounds to the type.  in == out is allowed; on_device: device pointers. */
+ * and rounds to the type.  in == out is allowed; on_device: device pointers.
+ * The handle keeps the work memory (about 16 MB per 512 x 512 frame; frames
+ * are processed in chunks within half of the free device memory). */
 ss_status ss_fill_gaps(ss_handle *h, ss_dtype type, const void *in, void *out, int32_t height,
                        int32_t width, int32_t n_frames, int32_t on_device);
 
