@@ -75,7 +75,7 @@ def measured_limiter(kernel, precision):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=1000)
     ap.add_argument("--group", type=int, default=50)
