@@ -206,6 +206,7 @@ struct FillChunk {
     int *nmiss;            // B
     double *x, *r, *ap;    // B x np (first nmiss used)
     double *pfull;         // B x np: search direction on the image grid, 0 at known pixels
+                           // (k_fill_cg); compact index of each missing pixel (k_fill_cg_lds)
     double *b;             // B x np: right-hand side
     int *status;           // B: iterations, or -1 if not converged
     int maxit;
