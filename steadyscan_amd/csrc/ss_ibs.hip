@@ -70,9 +70,12 @@ __constant__ int kCornerDir[4][4] = {{-1, NB_S, NB_E, NB_SE},
 __constant__ int kSlotDr[NB_COUNT] = {-1, -1, 0, 1, 1, 1, 0, -1};
 __constant__ int kSlotDc[NB_COUNT] = {0, 1, 1, 1, 0, -1, -1, -1};
 
-// Symmetric half of the masked stencil kept for the energy, structure of
-// arrays: diagonal, then the couplings to S, NE, E, SE.
-enum { QS_D, QS_S, QS_NE, QS_E, QS_SE, QS_COUNT };
+// The energy's per-pixel record (48 bytes, natural pixel order): the
+// symmetric half of the masked stencil -- diagonal, then the couplings to S,
+// NE, E, SE -- and a tail holding Le (float bits, low word) and the "all
+// eight neighbours masked" flag (bit 32), so that an energy step loads a
+// pixel's record with three 16-byte loads.
+enum { QR_D, QR_S, QR_NE, QR_E, QR_SE, QR_TAIL, QR_COUNT };
 
 struct PixGeo {
     double bx, by, tf;
@@ -97,10 +100,9 @@ struct Chunk {
     double *wy;               // B x np
     float *iv;                // B x np
     uint8_t *mask;            // B x np
-    uint8_t *full;            // B x np: masked, interior, all eight neighbours masked
     int *any;                 // B
-    double *qs;               // B x QS_COUNT x np
-    float *lf, *lk;           // B x np (lk colour-major, in sweep order)
+    double *qrec;             // B x np x QR_COUNT: energy records
+    float *lk;                // B x np, colour-major in sweep order
     float *rm;                // B x np x 8, colour-major in sweep order
     float *x;                 // B x np, per column even rows then odd rows
     float *snap;              // B x kNSlots x np
@@ -443,7 +445,6 @@ __global__ __launch_bounds__(256) void k_interact(Chunk C) {
     const int nColour = colour_count(th, tw, color), base = colour_base(th, tw, color);
     const int kk = (r >> 1) + (c >> 1) * colour_rows(th, r & 1);
     C.lk[fo + base + kk] = L;
-    C.lf[fo + i] = Lf;
     float rm[8];
     for (int d = 0; d < NB_COUNT; d++) {
         float v = 0.f;
@@ -460,46 +461,46 @@ __global__ __launch_bounds__(256) void k_interact(Chunk C) {
     float4 *rm4 = (float4 *)(C.rm + fo * 8) + 2 * (size_t)base;
     rm4[kk] = make_float4(rm[0], rm[1], rm[2], rm[3]);
     rm4[nColour + kk] = make_float4(rm[4], rm[5], rm[6], rm[7]);
-    double *qs = C.qs + (size_t)f * QS_COUNT * C.np;
-    qs[QS_D * (size_t)C.np + i] = qm[0];
-    qs[QS_S * (size_t)C.np + i] = qm[1 + NB_S];
-    qs[QS_NE * (size_t)C.np + i] = qm[1 + NB_NE];
-    qs[QS_E * (size_t)C.np + i] = qm[1 + NB_E];
-    qs[QS_SE * (size_t)C.np + i] = qm[1 + NB_SE];
     const float iv = C.iv[fo + i];
     C.x[fo + c * th + xrow(r, (th + 1) / 2)] = isnan(iv) ? 0.0f : iv;
     bool full = m && r >= 1 && r + 1 < th && c >= 1 && c + 1 < tw;
     for (int d = 0; d < NB_COUNT && full; d++) full = mask[(r + kSlotDr[d]) + (c + kSlotDc[d]) * th] != 0;
-    C.full[fo + i] = full ? 1 : 0;
+    const unsigned long long tail = (unsigned long long)__float_as_uint(Lf) | ((full ? 1ull : 0ull) << 32);
+    double2 *rec = (double2 *)(C.qrec + (fo + i) * QR_COUNT);
+    rec[0] = make_double2(qm[0], qm[1 + NB_S]);
+    rec[1] = make_double2(qm[1 + NB_NE], qm[1 + NB_E]);
+    rec[2] = make_double2(qm[1 + NB_SE], __longlong_as_double((long long)tail));
 }
 
 // ---------------------------------------------------------------------------
 // gs: one workgroup per frame
 // ---------------------------------------------------------------------------
+// Coupling `plane` of pixel j's energy record.
+__device__ inline double qrec_at(const double *qrec, int j, int plane) { return qrec[(size_t)j * QR_COUNT + plane]; }
+
 // (Q x)_i for pixel i at (r, c): x_j Q_ij over the in-bounds, masked
 // neighbours j in ascending index.  Q_ij is read from the symmetric half
 // (the pixel with the smaller index holds the coupling).
-__device__ inline double qx_generic(const float *x, const uint8_t *mask, const double *qD, const double *qS,
-                                    const double *qNE, const double *qE, const double *qSE, int i, int r,
-                                    int c, int th, int tw, int h0) {
+__device__ inline double qx_generic(const float *x, const uint8_t *mask, const double *qrec, int i, int r, int c,
+                                    int th, int tw, int h0) {
 #pragma clang fp contract(off)
     const bool up = r >= 1, dn = r + 1 < th, lt = c >= 1, rt = c + 1 < tw;
     double qx = 0.0;
     if (lt) {
         const int w = (c - 1) * th;
-        if (up && mask[i - th - 1]) qx += (double)x[w + xrow(r - 1, h0)] * qSE[i - th - 1];  // NW
-        if (mask[i - th]) qx += (double)x[w + xrow(r, h0)] * qE[i - th];                     // W
-        if (dn && mask[i - th + 1]) qx += (double)x[w + xrow(r + 1, h0)] * qNE[i - th + 1];  // SW
+        if (up && mask[i - th - 1]) qx += (double)x[w + xrow(r - 1, h0)] * qrec_at(qrec, i - th - 1, QR_SE);  // NW
+        if (mask[i - th]) qx += (double)x[w + xrow(r, h0)] * qrec_at(qrec, i - th, QR_E);                     // W
+        if (dn && mask[i - th + 1]) qx += (double)x[w + xrow(r + 1, h0)] * qrec_at(qrec, i - th + 1, QR_NE);  // SW
     }
     const int m = c * th;
-    if (up && mask[i - 1]) qx += (double)x[m + xrow(r - 1, h0)] * qS[i - 1];                 // N
-    qx += (double)x[m + xrow(r, h0)] * qD[i];
-    if (dn && mask[i + 1]) qx += (double)x[m + xrow(r + 1, h0)] * qS[i];                     // S
+    if (up && mask[i - 1]) qx += (double)x[m + xrow(r - 1, h0)] * qrec_at(qrec, i - 1, QR_S);  // N
+    qx += (double)x[m + xrow(r, h0)] * qrec_at(qrec, i, QR_D);
+    if (dn && mask[i + 1]) qx += (double)x[m + xrow(r + 1, h0)] * qrec_at(qrec, i, QR_S);      // S
     if (rt) {
         const int e = (c + 1) * th;
-        if (up && mask[i + th - 1]) qx += (double)x[e + xrow(r - 1, h0)] * qNE[i];           // NE
-        if (mask[i + th]) qx += (double)x[e + xrow(r, h0)] * qE[i];                          // E
-        if (dn && mask[i + th + 1]) qx += (double)x[e + xrow(r + 1, h0)] * qSE[i];           // SE
+        if (up && mask[i + th - 1]) qx += (double)x[e + xrow(r - 1, h0)] * qrec_at(qrec, i, QR_NE);  // NE
+        if (mask[i + th]) qx += (double)x[e + xrow(r, h0)] * qrec_at(qrec, i, QR_E);                  // E
+        if (dn && mask[i + th + 1]) qx += (double)x[e + xrow(r + 1, h0)] * qrec_at(qrec, i, QR_SE);   // SE
     }
     return qx;
 }
@@ -530,12 +531,10 @@ __device__ inline int walk_segments(int nRB) {
 // lanes, so a step loads one x and the pixel's own five Q values.  Pixels
 // whose eight neighbours are all masked take the interior formula; other
 // masked pixels test each term.
-__device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, const uint8_t *full,
-                           const double *qs, const float *lf, double *sp, float *sl) {
+__device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, const double *qrec, double *sp,
+                           float *sl) {
 #pragma clang fp contract(off)
-    const int th = C.th, tw = C.tw, np = C.np, h0 = (th + 1) / 2;
-    const double *qD = qs, *qS = qs + QS_S * (size_t)np, *qNE = qs + QS_NE * (size_t)np,
-                 *qE = qs + QS_E * (size_t)np, *qSE = qs + QS_SE * (size_t)np;
+    const int th = C.th, tw = C.tw, h0 = (th + 1) / 2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int kWaves = kGsThreads / 64, kRows = 62;
     const int nRB = (th + kRows - 1) / kRows, nSeg = walk_segments(nRB);
@@ -556,9 +555,9 @@ __device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, 
             if (c0 >= 1) {
                 const int iw = r + (c0 - 1) * th;
                 xw = x[(c0 - 1) * th + row0];
-                qw = qE[iw];
-                wSE = qSE[iw];
-                wNE = qNE[iw];
+                qw = qrec_at(qrec, iw, QR_E);
+                wSE = qrec_at(qrec, iw, QR_SE);
+                wNE = qrec_at(qrec, iw, QR_NE);
             }
             xc = x[c0 * th + row0];
         }
@@ -568,10 +567,13 @@ __device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, 
             const int i = inRow ? r + c * th : 0;
             const float xe = (inRow && c + 1 < tw) ? x[(c + 1) * th + row0] : 0.f;
             const float xne = from_below(xe), xse = from_above(xe);
-            const double dD = qD[i], dS = qS[i], dNE = qNE[i], dE = qE[i], dSE = qSE[i];
+            const double2 *rec = (const double2 *)(qrec + (size_t)i * QR_COUNT);
+            const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+            const double dD = r0.x, dS = r0.y, dNE = r1.x, dE = r1.y, dSE = r2.x;
+            const unsigned long long tail = (unsigned long long)__double_as_longlong(r2.y);
             const double dN = from_below(dS);
-            const uint8_t fl = sums ? full[i] : 0;
-            const float l = lf[i];
+            const bool fl = sums && ((tail >> 32) & 1);
+            const float l = __uint_as_float((unsigned)tail);
             if (fl) {
                 double qx = 0.0;
                 qx += (double)xnw * qnw;
@@ -586,7 +588,7 @@ __device__ float gs_energy(const Chunk &C, const float *x, const uint8_t *mask, 
                 pq += (double)xc * qx;
                 pl += l * xc;
             } else if (sums && mask[i]) {
-                const double qx = qx_generic(x, mask, qD, qS, qNE, qE, qSE, i, r, c, th, tw, h0);
+                const double qx = qx_generic(x, mask, qrec, i, r, c, th, tw, h0);
                 pq += (double)xc * qx;
                 pl += l * xc;
             }
@@ -751,9 +753,9 @@ __global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8)))
     if (!C.any[f]) return;
     const size_t fo = (size_t)f * C.np;
     float *x = C.x + fo;
-    const float *lk = C.lk + fo, *lf = C.lf + fo, *rm = C.rm + fo * 8;
-    const uint8_t *mask = C.mask + fo, *full = C.full + fo;
-    const double *qs = C.qs + (size_t)f * QS_COUNT * C.np;
+    const float *lk = C.lk + fo, *rm = C.rm + fo * 8;
+    const uint8_t *mask = C.mask + fo;
+    const double *qrec = C.qrec + fo * QR_COUNT;
     float *snap = C.snap + (size_t)f * kNSlots * C.np;
 
     __shared__ double s_pq[kGsThreads];
@@ -763,7 +765,7 @@ __global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8)))
     __shared__ unsigned s_active, s_stopped;
     __shared__ int s_known, s_needE, s_copy, s_done, s_wj, s_wn, s_knownAt, s_k, s_nE;
 
-    const float e0 = gs_energy(C, x, mask, full, qs, lf, s_pq, s_pl);
+    const float e0 = gs_energy(C, x, mask, qrec, s_pq, s_pl);
     if (threadIdx.x == 0) {
         for (int n = 0; n < kNSched; n++) {
             s_eo[n] = e0;
@@ -789,7 +791,7 @@ __global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8)))
         }
         __syncthreads();
         float e = 0.f;
-        if (s_needE) e = gs_energy(C, x, mask, full, qs, lf, s_pq, s_pl);
+        if (s_needE) e = gs_energy(C, x, mask, qrec, s_pq, s_pl);
         if (threadIdx.x == 0) {
             unsigned pass = 0;
             s_nE += s_needE;
@@ -927,8 +929,8 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
         align256(sizeof(double) * 2 * nk) + align256(sizeof(int) * (size_t)npd) * 2 +
         align256(sizeof(int) * (size_t)np) + align256(sizeof(int) * ((size_t)np + 1)) +
         align256(sizeof(int) * 4 * (size_t)np) + align256(sizeof(double) * 9 * (size_t)np) +
-        align256(sizeof(double) * (size_t)np) + align256(sizeof(float) * (size_t)np) * 4 +
-        align256((size_t)np) * 2 + align256(sizeof(int)) + align256(sizeof(double) * QS_COUNT * (size_t)np) +
+        align256(sizeof(double) * (size_t)np) + align256(sizeof(float) * (size_t)np) * 3 +
+        align256((size_t)np) + align256(sizeof(int)) + align256(sizeof(double) * QR_COUNT * (size_t)np) +
         align256(sizeof(float) * 8 * (size_t)np) +
         align256(sizeof(int) * 2 * kNSched) + align256(sizeof(int)) * 3 + align256(sizeof(unsigned long long)) +
         (in->data_on_device ? 0 : align256(esz * raw)) +
@@ -992,10 +994,8 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
         C.wy = (double *)take(sizeof(double) * (size_t)np * B);
         C.iv = (float *)take(sizeof(float) * (size_t)np * B);
         C.mask = (uint8_t *)take((size_t)np * B);
-        C.full = (uint8_t *)take((size_t)np * B);
         C.any = (int *)take(sizeof(int) * (size_t)B);
-        C.qs = (double *)take(sizeof(double) * QS_COUNT * (size_t)np * B);
-        C.lf = (float *)take(sizeof(float) * (size_t)np * B);
+        C.qrec = (double *)take(sizeof(double) * QR_COUNT * (size_t)np * B);
         C.lk = (float *)take(sizeof(float) * (size_t)np * B);
         C.rm = (float *)take(sizeof(float) * 8 * (size_t)np * B);
         C.x = (float *)take(sizeof(float) * (size_t)np * B);
