@@ -26,10 +26,11 @@
  * The file also restates invert_bilinear_sampling (steady_scan.m:290-404)
  * with its helpers.  Its summation orders are fixed here where MATLAB's
  * sparse() and BLAS leave them open: constraint sums over data pixels in
- * ascending index; the stopping energy in 1024 interleaved partial sums
- * added by a fixed tree; (Q x)_i over neighbours in ascending index.  Those rounding-level orders
- * are parity unpinned against the reference (no MATLAB here); everything
- * else is pinned by the dense-definition tests.
+ * ascending index; the stopping energy in 1024 partial sums, each over a
+ * row segment (the layout is described at orc_ibs_energy), added by a fixed
+ * tree; (Q x)_i over neighbours in ascending index.  Those rounding-level
+ * orders are parity unpinned against the reference (no MATLAB here);
+ * everything else is pinned by the dense-definition tests.
  */
 #include "ss_oracle.h"
 
