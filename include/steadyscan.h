@@ -334,6 +334,14 @@ typedef struct {
                                call; 0 = half of the device memory free or
                                already held by the handle (about 53 MB per
                                512x512 frame)                                     */
+    int32_t chain_frames;   /* 0: the frames form one chain, as one call of the
+                               reference (the energy-check interval starts at 1
+                               and carries from frame to frame).  k > 0: the
+                               interval restarts at 1 on frames 0, k, 2k, ...,
+                               as if every k frames were a separate call (the
+                               driver's sets, steady_scan.m:124-134), so the
+                               sets of a stack run as one launch               */
+    int32_t reserved;       /* must be 0                                          */
 } ss_ibs_settings;
 
 void ss_ibs_settings_default(ss_ibs_settings *s);
@@ -388,7 +396,10 @@ ss_status ss_fill_gaps(ss_handle *h, ss_dtype type, const void *in, void *out, i
  * at each set.  With ibs->inpaint, each set's result is gap-filled
  * (ss_fill_gaps) in single precision for SS_F32 frames and in double
  * otherwise, as the reference fills in the frames' class.  data_corrected
- * (double) and mask are template_h x template_w x n_frames. */
+ * (double) and mask are template_h x template_w x n_frames.  The result is
+ * that of one invert_bilinear_sampling call per set; the sets themselves run
+ * together, up to about 4096 frames per launch (ibs->chain_frames is set to
+ * frames_per_set). */
 ss_status ss_reconstruct(ss_handle *h, const ss_geometry_desc *geometry, const ss_driver_settings *driver,
                          const ss_ibs_settings *ibs, ss_dtype type, const void *frames, int32_t n_frames,
                          const double *dx_knots, const double *dy_knots, double *data_corrected,

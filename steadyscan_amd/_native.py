@@ -113,7 +113,7 @@ class SSIbsSettings(ctypes.Structure):
     _fields_ = [
         ("regfac", c_double), ("pixel_aspect", c_double), ("inpaint", c_int32),
         ("max_iterations", c_int32), ("chunk_frames", c_int32), ("outputs_on_device", c_int32),
-        ("work_bytes", ctypes.c_int64),
+        ("work_bytes", ctypes.c_int64), ("chain_frames", c_int32), ("reserved", c_int32),
     ]
 
 

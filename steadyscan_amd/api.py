@@ -715,7 +715,8 @@ def resample_frames(frames, geometry, dx_knots, dy_knots, handle=None, device=0)
 
 def invert_bilinear_sampling(frames, geometry, dx_knots, dy_knots, ibs_regfac=3e-3,
                              ibs_template_pixel_aspect_ratio=1.0, ibs_inpaint=False, max_iterations=0,
-                             chunk_frames=0, work_bytes=0, device_out=None, handle=None, device=0, info=None):
+                             chunk_frames=0, work_bytes=0, chain_frames=0, device_out=None, handle=None, device=0,
+                             info=None):
     """invert_bilinear_sampling (steady_scan.m:290-404) on the GPU.
 
     frames: nlines x linewidth x n raw frames (cropped here by the geometry's
@@ -726,7 +727,9 @@ def invert_bilinear_sampling(frames, geometry, dx_knots, dy_knots, ibs_regfac=3e
     template_w x n with NaN outside the mask, mask bool); with
     device_out=(out_pointer, mask_pointer) the results are written to device
     memory instead and None is returned.  info, a dict if given, receives the
-    per-frame Gauss-Seidel iteration counts under "iterations"."""
+    per-frame Gauss-Seidel iteration counts under "iterations".  chain_frames
+    = k > 0 restarts the energy-check interval every k frames, as separate
+    calls of k frames would (the driver's sets in one launch)."""
     g = geometry["desc"]
     ri = N.SSResampleIn()
     if isinstance(frames, tuple):
@@ -758,6 +761,7 @@ def invert_bilinear_sampling(frames, geometry, dx_knots, dy_knots, ibs_regfac=3e
     st.max_iterations = int(max_iterations)
     st.chunk_frames = int(chunk_frames)
     st.work_bytes = int(work_bytes)
+    st.chain_frames = int(chain_frames)
     iters = np.zeros(n, dtype=np.int32)
     if device_out is not None:
         st.outputs_on_device = 1

@@ -11,11 +11,24 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <thread>
 #include <vector>
 
 #include "steadyscan.h"
 
 namespace {
+
+// dst[k] = src[k] (float to double) on a few host threads.
+void widen(const float *src, double *dst, size_t n) {
+    const size_t parts = n < ((size_t)1 << 20) ? 1 : 8;
+    std::vector<std::thread> pool;
+    for (size_t p = 1; p < parts; p++)
+        pool.emplace_back([=] {
+            for (size_t k = n * p / parts, e = n * (p + 1) / parts; k < e; k++) dst[k] = src[k];
+        });
+    for (size_t k = 0, e = n / parts; k < e; k++) dst[k] = src[k];
+    for (auto &t : pool) t.join();
+}
 
 // set_start (steady_scan.m:97-99), 0-based.  The merge of a short last set
 // (steady_scan.m:101-106) only rewrites set_size, and estimate_displacements
@@ -225,16 +238,20 @@ extern "C" ss_status ss_reconstruct(ss_handle *h, const ss_geometry_desc *geom, 
     const size_t esz = elem_size(type), rawPix = (size_t)nl * lw;
     const int cl = geom->cropping[0], cb = geom->cropping[2];
     const bool prefilter = ds->pregauss > 0 || ds->mednorm;
+    // The sets restart the energy-check interval (each is one call of
+    // invert_bilinear_sampling in the reference); here whole sets go to one
+    // call as chains of frames_per_set frames, so that a call fills the GPU.
+    // A call takes as many whole sets as fit in about 4096 frames, which
+    // bounds the host staging buffers.
     ss_ibs_settings is = *ibs;
     is.inpaint = 0;
-    std::vector<int> start;
-    set_starts(n_frames, ds->frames_per_set, start);
+    is.chain_frames = ds->frames_per_set;
+    const int perCall = ds->frames_per_set * std::max(1, 4096 / ds->frames_per_set);
     std::vector<char> cropped, raw;
     std::vector<float> out;
     std::vector<double> out64;
-    for (size_t i = 0; i < start.size(); i++) {
-        // ii = set_start(iset):min(set_start(iset + 1) - 1, nframes) (steady_scan.m:131)
-        const int f0 = start[i], f1 = (i + 1 < start.size()) ? start[i + 1] : n_frames, n = f1 - f0;
+    for (int f0 = 0; f0 < n_frames; f0 += perCall) {
+        const int n = std::min(perCall, n_frames - f0);
         const char *src = (const char *)frames + rawPix * esz * f0;
         if (prefilter) {
             // prefilter_foralignment(cropimage(frames)) (steady_scan.m:132): crop,
@@ -270,13 +287,14 @@ extern "C" ss_status ss_reconstruct(ss_handle *h, const ss_geometry_desc *geom, 
             // the frames' class is single: filled in double, rounded to single
             st = ss_fill_gaps(h, SS_F32, out.data(), out.data(), geom->template_h, geom->template_w, n, 0);
             if (st) return st;
-            for (size_t k = 0; k < np * n; k++) dc[k] = out[k];
+            widen(out.data(), dc, np * n);
         } else if (ibs->inpaint) {
-            out64.assign(out.begin(), out.end());
+            out64.resize(np * n);
+            widen(out.data(), out64.data(), np * n);
             st = ss_fill_gaps(h, SS_F64, out64.data(), dc, geom->template_h, geom->template_w, n, 0);
             if (st) return st;
         } else {
-            for (size_t k = 0; k < np * n; k++) dc[k] = out[k];
+            widen(out.data(), dc, np * n);
         }
     }
     return SS_OK;

@@ -40,6 +40,7 @@
 // contraction so that it rounds as the sequential restatement does.
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -111,6 +112,9 @@ struct Chunk {
     int *diag;                // B x 3: iterations run, iteration the interval became known, energies evaluated
     unsigned long long *prog; // B: progress word (iteration, possible and stopped intervals), 0 before start
     int carry;                // check interval of the chunk's first frame
+    int f0;                   // the chunk's first frame within the call
+    int chain;                // frames per chain (the interval restarts at 1 on
+                              // the call's frames 0, chain, 2 chain, ...)
 };
 
 // int32(max(1, min(10, sqrt(single(iter))))) (steady_scan.m:394): sqrt is
@@ -709,6 +713,13 @@ __device__ int walk_interval(const Chunk &C, int f, int *wj, int *wn) {
     return *wj == f ? *wn : 0;
 }
 
+// First frame (chunk-local) of frame f's chain when that chain starts inside
+// the chunk, else -1 (the chain comes from an earlier chunk, with C.carry).
+__device__ inline int chain_start(const Chunk &C, int f) {
+    const int g = C.f0 + f, cs = g - g % C.chain;
+    return cs >= C.f0 ? cs - C.f0 : -1;
+}
+
 // Progress word of a frame: bits 0-31 the last finished iteration k, 32-41
 // the intervals still possible for it, 42-51 those of them that stopped.
 __device__ inline unsigned long long pack_progress(int k, unsigned active, unsigned stopped) {
@@ -725,9 +736,10 @@ __device__ inline unsigned long long pack_progress(int k, unsigned active, unsig
 // shrinks as the predecessor progresses.
 __device__ unsigned possible_intervals(const Chunk &C, int f) {
     const unsigned all = (1u << kNSched) - 1;
+    const int cs = chain_start(C, f), lo = cs > 0 ? cs : 0;
     int j = f - 1;
-    while (j >= 0 && !C.any[j]) j--;
-    if (j < 0) return all;
+    while (j >= lo && !C.any[j]) j--;
+    if (j < lo) return all;
     const unsigned long long w = __hip_atomic_load(&C.prog[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned act = (unsigned)(w >> 32) & 0x3ffu, stp = (unsigned)(w >> 42) & 0x3ffu;
     const int k = (int)(unsigned)(w & 0xffffffffu);
@@ -774,8 +786,9 @@ __global__ __launch_bounds__(kGsThreads) __attribute__((amdgpu_waves_per_eu(8)))
         for (int s = 0; s < kNSlots; s++) s_ref[s] = 0;
         s_stopped = 0;
         s_nE = 0;
-        s_wj = 0;
-        s_wn = C.carry;
+        const int cs = chain_start(C, f);
+        s_wj = cs >= 0 ? cs : 0;
+        s_wn = cs >= 0 ? 1 : C.carry;
         s_known = walk_interval(C, f, &s_wj, &s_wn);
         s_knownAt = s_known ? 0 : -1;
         s_active = s_known ? 1u << (s_known - 1) : (1u << kNSched) - 1;
@@ -900,7 +913,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
     if (st->inpaint)
         return handle_fail(h, SS_ERR_INVALID_ARGUMENT, "invert_bilinear_sampling: inpainting is a separate call (ss_fill_gaps on the result)");
     if (!(st->pixel_aspect > 0) || !std::isfinite(st->regfac) || st->max_iterations < 0 ||
-        st->chunk_frames < 0 || st->work_bytes < 0)
+        st->chunk_frames < 0 || st->work_bytes < 0 || st->chain_frames < 0 || st->reserved != 0)
         return handle_fail(h, SS_ERR_INVALID_ARGUMENT, "invert_bilinear_sampling: bad settings");
     const ss_geometry_desc &g = *in->geometry;
     int32_t hc, wc, fh;
@@ -948,6 +961,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
     B = std::min(B, in->n_frames);
     B = std::min(B, 65535);
     const int cap = st->max_iterations > 0 ? st->max_iterations : kDefaultCap;
+    const int chain = st->chain_frames > 0 ? st->chain_frames : INT_MAX;
 
     hipStream_t s = ss::handle_stream(h);
     if (hipSetDevice(ss::handle_device(h)) != hipSuccess)
@@ -977,6 +991,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
         Chunk C{};
         C.th = th; C.tw = tw; C.np = np; C.npd = npd; C.hc = hc; C.nl = g.nlines;
         C.cl = g.cropping[0]; C.cb = g.cropping[2]; C.nk = nk; C.cap = cap;
+        C.chain = chain;
         C.regfac = st->regfac;
         const double vfac = 1.0 / st->pixel_aspect;
         C.v2 = vfac * vfac;
@@ -1046,6 +1061,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
             ICHK(hipMemsetAsync(C.stop, 0, sizeof(int) * kNSched * (size_t)nb, s));
             ICHK(hipMemsetAsync(C.prog, 0, sizeof(unsigned long long) * (size_t)nb, s));
             C.carry = carry;
+            C.f0 = f0;
             const dim3 gp((npd + 255) / 256, nb), gt((np + 255) / 256, nb);
             hipLaunchKernelGGL(k_points, gp, dim3(256), 0, s, C);
             hipLaunchKernelGGL(k_scan, dim3(nb), dim3(1024), 0, s, C);
@@ -1067,6 +1083,7 @@ extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample
             ICHK(hipStreamSynchronize(s));
             // resolve the check intervals in frame order (steady_scan.m:326-395)
             for (int f = 0; f < nb; f++) {
+                if (((long)f0 + f) % chain == 0) carry = 1;  // a new chain (a call of the reference)
                 if (!any[f]) {
                     chosen[f] = -2;
                     if (iterations) iterations[f0 + f] = 0;

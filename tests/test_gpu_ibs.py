@@ -117,6 +117,33 @@ def test_rectangular_template_aspect_and_cropping():
     compare(fr, dx, dy, 40, 50, nk, cropping=(2, 4, 0, 0), aspect=2.0, label="40x50 aspect 2 cropped")
 
 
+@pytest.mark.parametrize("chain,chunk", [(5, 0), (5, 3), (4, 8), (1, 0)])
+def test_chains_equal_separate_calls(chain, chunk):
+    # chain_frames = k: one call over all frames gives what one call per k
+    # frames gives (each restarts the check interval at 1), also when the
+    # chunks of frames on the device do not line up with the chains
+    n = 17
+    T, fr, geom, mot = W.config0(n_frames=n, size=40, seed=45)
+    rng = np.random.default_rng(46)
+    nk = geom["nknots"]
+    dx, dy = smooth_knots(rng, nk, n), smooth_knots(rng, nk, n)
+    dx[:, 6] = 100.0                       # a skipped frame inside a chain
+    fr = np.asarray(fr, np.float64)
+    info = {}
+    out, mask = ssa.invert_bilinear_sampling(fr, geom, dx, dy, ibs_regfac=0.3, chain_frames=chain,
+                                             chunk_frames=chunk, info=info)
+    its = []
+    for f0 in range(0, n, chain):
+        f1 = min(n, f0 + chain)
+        inf = {}
+        o, m = ssa.invert_bilinear_sampling(fr[:, :, f0:f1], geom, dx[:, f0:f1], dy[:, f0:f1], ibs_regfac=0.3, info=inf)
+        np.testing.assert_array_equal(out[:, :, f0:f1], o)
+        np.testing.assert_array_equal(mask[:, :, f0:f1], m)
+        its += inf["iterations"].tolist()
+    print(f"chain {chain} chunk {chunk}: iterations {info['iterations'].tolist()}", flush=True)
+    np.testing.assert_array_equal(info["iterations"], its)
+
+
 @pytest.mark.parametrize("nl,lw,th,tw", [(210, 40, 200, 37), (1110, 10, 1100, 8), (70, 70, 63, 65)])
 def test_tall_and_odd_templates(nl, lw, th, tw):
     # row counts that are not multiples of the GPU walks' row blocks (62 rows
