@@ -313,22 +313,25 @@ __global__ __launch_bounds__(256) void k_assemble(Chunk C, const TD *__restrict_
     const int cellOf[4] = {i, i - 1, i - th, i - th - 1};
     const bool ok[4] = {r <= th - 2 && c <= tw - 2, r >= 1 && c <= tw - 2, r <= th - 2 && c >= 1,
                         r >= 1 && c >= 1};
+    int head[4];  // each list's next data pixel (INT_MAX once it is exhausted)
     for (int a = 0; a < 4; a++) {
         pos[a] = ok[a] ? start[cellOf[a]] : 0;
         end[a] = ok[a] ? start[cellOf[a] + 1] : 0;
+        head[a] = pos[a] < end[a] ? list[pos[a]] : INT_MAX;
     }
     double q[9];
     for (int k = 0; k < 9; k++) q[k] = 0.0;
     double wy = 0.0;
     while (true) {
-        int a = -1, best = 0x7fffffff;
+        int a = -1, best = INT_MAX;
         for (int b = 0; b < 4; b++)
-            if (pos[b] < end[b] && list[pos[b]] < best) {
-                best = list[pos[b]];
+            if (head[b] < best) {
+                best = head[b];
                 a = b;
             }
         if (a < 0) break;
         pos[a]++;
+        head[a] = pos[a] < end[a] ? list[pos[a]] : INT_MAX;
         const int p = best;
         double x, y;
         position(C.geo[p], kx, ky, &x, &y);
