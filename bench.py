@@ -297,6 +297,17 @@ def ibs_leg(args, torch, ssa, frames, geom, results, G, h):
     out["prefilter"] = {"frames_per_s": n / el_pre, "seconds": el_pre, "pregauss": 1.0, "status": rc}
     print(f"[bench] gap fill: {n / el_fill:.0f} frames/s ({missing * 100:.1f}% missing); "
           f"prefilter (sigma 1): {n / el_pre:.0f} frames/s", file=sys.stderr, flush=True)
+    # the driver's reconstruction loop (ss_reconstruct): host frames in, double
+    # data_corrected out, sets of 500 frames restarting the check interval
+    fr_host = np.asfortranarray(frames[:, :, :n])
+    ssa.reconstruct(fr_host[:, :, :8], geom, kx[:, :8], ky[:, :8], frames_per_set=500, handle=h)
+    t = time.perf_counter()
+    ssa.reconstruct(fr_host, geom, kx, ky, frames_per_set=500, handle=h)
+    el_drv = time.perf_counter() - t
+    out["driver_loop"] = {"frames_per_s": n / el_drv, "seconds": el_drv, "frames_per_set": 500,
+                          "inputs": "uint16 frames in host memory, float64 data_corrected and mask returned to host"}
+    print(f"[bench] reconstruction loop (ss_reconstruct, host I/O, sets of 500): {n / el_drv:.0f} frames/s",
+          file=sys.stderr, flush=True)
     if not args.no_cpu:
         from oracle import oracle as O
         m = 3
