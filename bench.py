@@ -5,9 +5,10 @@ groups of 50 frames, knots every 16 lines (33 knots), default alignment
 settings (steady_scan.m:836-867), synthetic data (seeded generator), with
 frames, template and geometry resident in HBM.  One step aligns all 20
 groups.  The headline times the K steps as one stream of groups through
-the continuous-batching server (ss_server_*, at most two steps' groups
-resident), so a step's last, slowest groups share their ticks with the next
-step's groups; the knots are checked equal to the batched path's.  The
+the continuous-batching server (ss_server_*; every group of the K steps may
+be resident, within the server's HBM budget), so a step's last, slowest
+groups share their ticks with other steps' groups; the knots are checked
+equal to the batched path's.  The
 same steps with one batched submission (ss_align_batch) and a
 synchronisation per step are reported beside it (barrier_per_step).
 
@@ -97,7 +98,8 @@ def parse():
                     help="run only the reconstruction leg (knots from one alignment step)")
     ap.add_argument("--stream-groups", type=int, default=-1,
                     help="time the steps as one stream through the continuous-batching server with at most "
-                         "this many resident groups (-1: two steps' groups; 0: off, one barrier per step)")
+                         "this many resident groups (-1: every group of the timed steps, within the server's "
+                         "HBM budget; 0: off, one barrier per step)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of known size (rocprof counter calibration)")
     return ap.parse_args()
@@ -475,7 +477,8 @@ def main():
     stream = None
     if args.stream_groups:
         runs_ref[precs[0]] = results
-        stream = timed_stream(precs[0], 2 * ngroups if args.stream_groups < 0 else args.stream_groups)
+        stream = timed_stream(precs[0], ngroups * max(args.steps, args.warmup) if args.stream_groups < 0
+                              else args.stream_groups)
 
     # latency of the plain path: one group per submission
     single = []
