@@ -119,6 +119,25 @@ def test_reconstruction_loop_equals_oracle(case):
         assert err <= 1e-4
 
 
+def test_reconstruction_loop_over_several_launches():
+    """More frames than one launch takes (4096 at most, in whole sets): the
+    loop equals one invert_bilinear_sampling call per set, bit for bit."""
+    n, per = 4300, 1000
+    rng = np.random.default_rng(68)
+    fr = 50 + 100 * rng.random((16, 16, n))
+    nk = 3
+    dx = np.cumsum(rng.normal(0, 0.2, (nk, n)), 0)
+    dy = np.cumsum(rng.normal(0, 0.2, (nk, n)), 0)
+    g = ssa.geometry_desc(16, 16, (16, 16), nknots=nk)
+    geom = ssa.init_settings(16, 16, (16, 16), nknots=nk)
+    dc, mk = ssa.reconstruct(fr, g, dx, dy, frames_per_set=per)
+    for f0 in range(0, n, per):
+        f1 = min(n, f0 + per)
+        o, m = ssa.invert_bilinear_sampling(fr[:, :, f0:f1], geom, dx[:, f0:f1], dy[:, f0:f1])
+        np.testing.assert_array_equal(mk[:, :, f0:f1], m)
+        np.testing.assert_array_equal(dc[:, :, f0:f1], o.astype(np.float64))
+
+
 def test_steady_scan_composes_the_driver_and_keeps_the_extra_channel_slots():
     """steady_scan (steady_scan.m:93-154): prefiltered template, estimation,
     reconstruction; extra channel k lands in slot k (the reference's loop
