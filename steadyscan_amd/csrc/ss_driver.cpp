@@ -8,6 +8,7 @@
 // overlap, the prefiltering, the unpacking and the per-frame success tests
 // follow the reference line by line.
 #include <algorithm>
+#include <new>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -18,15 +19,21 @@
 
 namespace {
 
-// dst[k] = src[k] (float to double) on a few host threads.
+// dst[k] = src[k] (float to double) on a few host threads; the parts whose
+// thread could not be started are done on the calling thread.
 void widen(const float *src, double *dst, size_t n) {
     const size_t parts = n < ((size_t)1 << 20) ? 1 : 8;
+    auto part = [=](size_t p) {
+        for (size_t k = n * p / parts, e = n * (p + 1) / parts; k < e; k++) dst[k] = src[k];
+    };
     std::vector<std::thread> pool;
-    for (size_t p = 1; p < parts; p++)
-        pool.emplace_back([=] {
-            for (size_t k = n * p / parts, e = n * (p + 1) / parts; k < e; k++) dst[k] = src[k];
-        });
-    for (size_t k = 0, e = n / parts; k < e; k++) dst[k] = src[k];
+    size_t started = 1;
+    try {
+        for (; started < parts; started++) pool.emplace_back(part, started);
+    } catch (...) {  // no thread available: the rest runs here
+    }
+    part(0);
+    for (size_t p = started; p < parts; p++) part(p);
     for (auto &t : pool) t.join();
 }
 
@@ -75,7 +82,7 @@ extern "C" ss_status ss_estimate_displacements(ss_handle *h, const ss_geometry_d
                                                const ss_driver_settings *ds, const ss_settings *s,
                                                const double *template_image, ss_dtype type, const void *frames,
                                                int32_t n_frames, const double *est_dx, const double *est_dy,
-                                               ss_displacements_out *out) {
+                                               ss_displacements_out *out) try {
     if (!h || !geom || !ds || !s || !template_image || !frames || !out || n_frames < 1)
         return SS_ERR_INVALID_ARGUMENT;
     if (ds->frames_per_set < 1 || ds->frame_groupsize < 1 || !(ds->pregauss >= 0)) return SS_ERR_INVALID_ARGUMENT;
@@ -221,12 +228,16 @@ extern "C" ss_status ss_estimate_displacements(ss_handle *h, const ss_geometry_d
         }
     }
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 extern "C" ss_status ss_reconstruct(ss_handle *h, const ss_geometry_desc *geom, const ss_driver_settings *ds,
                                     const ss_ibs_settings *ibs, ss_dtype type, const void *frames, int32_t n_frames,
                                     const double *dx_knots, const double *dy_knots, double *data_corrected,
-                                    uint8_t *mask) {
+                                    uint8_t *mask) try {
     if (!h || !geom || !ds || !ibs || !frames || !dx_knots || !dy_knots || !data_corrected || !mask ||
         n_frames < 1 || ds->frames_per_set < 1 || !(ds->pregauss >= 0))
         return SS_ERR_INVALID_ARGUMENT;
@@ -298,4 +309,8 @@ extern "C" ss_status ss_reconstruct(ss_handle *h, const ss_geometry_desc *geom, 
         }
     }
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }

@@ -8,6 +8,7 @@
 // a fixed sequence of stage kernels -- and reads the groups' phases once per
 // tick, until every group is done.
 #include <algorithm>
+#include <new>
 #include <chrono>
 #include <cfloat>
 #include <cmath>
@@ -1133,7 +1134,7 @@ ss_status server_admit(ss_server *S) {
 
 extern "C" {
 
-ss_status ss_server_create(ss_handle *h, const ss_server_config *cfg, ss_server **out) {
+ss_status ss_server_create(ss_handle *h, const ss_server_config *cfg, ss_server **out) try {
     if (!h || !out) return SS_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     ss_server *S = new ss_server();
@@ -1143,6 +1144,10 @@ ss_status ss_server_create(ss_handle *h, const ss_server_config *cfg, ss_server 
     S->maxGroups = (cfg && cfg->max_groups > 0) ? cfg->max_groups : 256;
     *out = S;
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 void ss_server_destroy(ss_server *S) {
@@ -1158,7 +1163,7 @@ void ss_server_destroy(ss_server *S) {
 }
 
 ss_status ss_server_submit(ss_server *S, const ss_group_in *in, const ss_settings *s, ss_group_out *out,
-                           int64_t *ticket) {
+                           int64_t *ticket) try {
     if (!S || !in || !s || !out || !ticket) return SS_ERR_INVALID_ARGUMENT;
     ss_handle *h = S->h;
     if (s->device_id != h->device)
@@ -1171,9 +1176,13 @@ ss_status ss_server_submit(ss_server *S, const ss_group_in *in, const ss_setting
     S->queue.push_back(r);
     *ticket = r.ticket;
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
-ss_status ss_server_step(ss_server *S, int32_t *n_resident, int32_t *n_queued) {
+ss_status ss_server_step(ss_server *S, int32_t *n_resident, int32_t *n_queued) try {
     if (!S) return SS_ERR_INVALID_ARGUMENT;
     ss_handle *h = S->h;
     HIPCHK(hipSetDevice(h->device));
@@ -1230,6 +1239,10 @@ ss_status ss_server_step(ss_server *S, int32_t *n_resident, int32_t *n_queued) {
     if (n_resident) *n_resident = (int32_t)S->resident.size();
     if (n_queued) *n_queued = (int32_t)S->queue.size();
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 int32_t ss_server_completed(ss_server *S, int64_t *tickets, ss_status *status, int32_t cap) {
@@ -1291,7 +1304,7 @@ size_t handle_scratch_held(const ss_handle *h) { return h->scratchBytes; }
 
 extern "C" {
 
-ss_status ss_create(int32_t device_id, ss_handle **out) {
+ss_status ss_create(int32_t device_id, ss_handle **out) try {
     if (!out) return SS_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     int n = 0;
@@ -1310,6 +1323,10 @@ ss_status ss_create(int32_t device_id, ss_handle **out) {
     }
     *out = h;
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 void ss_destroy(ss_handle *h) {
@@ -1337,7 +1354,7 @@ void ss_destroy(ss_handle *h) {
 const char *ss_last_error(const ss_handle *h) { return h ? h->err.c_str() : "no handle"; }
 
 ss_status ss_upload_template(ss_handle *h, int32_t th, int32_t tw, const double *t,
-                             const double *xg, const double *yg, int32_t *id) {
+                             const double *xg, const double *yg, int32_t *id) try {
     if (!h || !id) return SS_ERR_INVALID_ARGUMENT;
     if (th < 2 || tw < 2 || !t || !xg || !yg) return fail(h, SS_ERR_INVALID_ARGUMENT, "template: bad size or NULL array");
     HIPCHK(hipSetDevice(h->device));
@@ -1383,10 +1400,14 @@ ss_status ss_upload_template(ss_handle *h, int32_t th, int32_t tw, const double 
     *id = h->next_id++;
     h->templates[*id] = dt;
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 ss_status ss_upload_geometry(ss_handle *h, int32_t fh, int32_t nb, const double *frac,
-                             const double *bx, const double *by, const uint8_t *bm, int32_t *id) {
+                             const double *bx, const double *by, const uint8_t *bm, int32_t *id) try {
     if (!h || !id) return SS_ERR_INVALID_ARGUMENT;
     if (fh < 1 || nb < 1 || !frac || !bx || !by || !bm) return fail(h, SS_ERR_INVALID_ARGUMENT, "geometry: bad size or NULL array");
     HIPCHK(hipSetDevice(h->device));
@@ -1420,6 +1441,10 @@ ss_status ss_upload_geometry(ss_handle *h, int32_t fh, int32_t nb, const double 
     *id = h->next_id++;
     h->geometries[*id] = g;
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 ss_status ss_release(ss_handle *h, int32_t id) {
@@ -1442,12 +1467,16 @@ ss_status ss_release(ss_handle *h, int32_t id) {
 }
 
 ss_status ss_align_group(ss_handle *h, const ss_group_in *in, const ss_settings *s,
-                         ss_group_out *out) {
+                         ss_group_out *out) try {
     return ss_align_batch(h, 1, in, s, out);
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 ss_status ss_align_batch(ss_handle *h, int32_t n, const ss_group_in *in, const ss_settings *s,
-                         ss_group_out *out) {
+                         ss_group_out *out) try {
     if (!h || n < 0 || (n > 0 && (!in || !s || !out))) return SS_ERR_INVALID_ARGUMENT;
     if (n == 0) return SS_OK;
     for (int i = 0; i < n; i++)
@@ -1456,6 +1485,10 @@ ss_status ss_align_batch(ss_handle *h, int32_t n, const ss_group_in *in, const s
     HIPCHK(hipSetDevice(h->device));
     h->err.clear();
     return run_batch(h, n, in, s, out);
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 ss_status ss_profile_enable(ss_handle *h, int32_t on) {

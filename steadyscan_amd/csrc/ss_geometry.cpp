@@ -4,6 +4,7 @@
 // steady_scan.m:166-175, 229-234; image2blockmatrix, steady_scan.m:601-613)
 // and default_alignment_settings (steady_scan.m:836-867).
 #include <cmath>
+#include <new>
 #include <cstring>
 #include <vector>
 
@@ -157,7 +158,7 @@ extern "C" ss_status ss_geometry_size(const ss_geometry_desc *g, int32_t *hc, in
 extern "C" ss_status ss_geometry_init(const ss_geometry_desc *g, int32_t *prevknotind,
                                       double *tfrac, double *basex, double *basey,
                                       double *frac_bm, double *base_x_bm, double *base_y_bm,
-                                      uint8_t *base_mask_bm, int32_t *min_points) {
+                                      uint8_t *base_mask_bm, int32_t *min_points) try {
     int32_t Hc, Wc, fh;
     ss_status rc = ss_geometry_size(g, &Hc, &Wc, &fh);
     if (rc != SS_OK) return rc;
@@ -202,10 +203,14 @@ extern "C" ss_status ss_geometry_init(const ss_geometry_desc *g, int32_t *prevkn
             min_points[k] = (int32_t)((double)per_knot[k] * g->min_blockpoints_ratio);
     }
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 extern "C" ss_status ss_pack_frames(const ss_geometry_desc *g, ss_dtype type, const void *frames,
-                                    int32_t n, void *out) {
+                                    int32_t n, void *out) try {
     int32_t Hc, Wc, fh;
     ss_status rc = ss_geometry_size(g, &Hc, &Wc, &fh);
     if (rc != SS_OK) return rc;
@@ -235,4 +240,8 @@ extern "C" ss_status ss_pack_frames(const ss_geometry_desc *g, ss_dtype type, co
             }
     }
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }

@@ -39,6 +39,7 @@
 // Everything is deterministic; all floating-point code is compiled without
 // contraction so that it rounds as the sequential restatement does.
 #include <algorithm>
+#include <new>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -907,7 +908,7 @@ extern "C" void ss_ibs_settings_default(ss_ibs_settings *s) {
 
 extern "C" ss_status ss_invert_bilinear_sampling(ss_handle *h, const ss_resample_in *in,
                                                  const ss_ibs_settings *st, float *out,
-                                                 uint8_t *mask_out, int32_t *iterations) {
+                                                 uint8_t *mask_out, int32_t *iterations) try {
     using ss::handle_fail;
     if (!h) return SS_ERR_INVALID_ARGUMENT;
     if (!in || !st || !out || !mask_out || !in->geometry || !in->frames || !in->knots_x ||
@@ -1145,4 +1146,8 @@ done:
 #undef ICHK
     if (rc) return handle_fail(h, rc, err);
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }

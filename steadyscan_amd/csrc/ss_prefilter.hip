@@ -23,6 +23,7 @@
 // gradients in FP64 to a relative residual of 1e-13, with every reduction in
 // a fixed order (deterministic).
 #include <algorithm>
+#include <new>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -486,7 +487,7 @@ void dispatch(ss_dtype t, F &&f) {
 
 extern "C" ss_status ss_prefilter(ss_handle *h, ss_dtype type, const void *in, void *out, int32_t height,
                                   int32_t width, int32_t n_frames, double pregauss, int32_t mednorm,
-                                  int32_t on_device) {
+                                  int32_t on_device) try {
     if (!h) return SS_ERR_INVALID_ARGUMENT;
     if (!in || !out || height < 1 || width < 1 || n_frames < 1 || !(pregauss >= 0) || !std::isfinite(pregauss) ||
         type < SS_F64 || type > SS_I16)
@@ -572,10 +573,14 @@ extern "C" ss_status ss_prefilter(ss_handle *h, ss_dtype type, const void *in, v
         PCHK(hipStreamSynchronize(s));
     }
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 extern "C" ss_status ss_fill_gaps(ss_handle *h, ss_dtype type, const void *in, void *out, int32_t height,
-                                  int32_t width, int32_t n_frames, int32_t on_device) {
+                                  int32_t width, int32_t n_frames, int32_t on_device) try {
     if (!h) return SS_ERR_INVALID_ARGUMENT;
     if (!in || !out || height < 1 || width < 1 || n_frames < 1 || (type != SS_F64 && type != SS_F32))
         return ss::handle_fail(h, SS_ERR_INVALID_ARGUMENT, "fill_gaps: bad arguments (float32 or float64 frames)");
@@ -670,4 +675,8 @@ extern "C" ss_status ss_fill_gaps(ss_handle *h, ss_dtype type, const void *in, v
                                            ": conjugate gradients did not converge");
     }
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }

@@ -13,6 +13,7 @@
 //           single precision in the statement order of pushgrid.
 // Positions are computed in FP64 without contraction, as MATLAB does.
 #include <cmath>
+#include <new>
 #include <cstring>
 #include <vector>
 
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(256) void k_splat_gather(const PixGeo *__restrict__
 
 }  // namespace
 
-extern "C" ss_status ss_resample(ss_handle *h, const ss_resample_in *in, float *out) {
+extern "C" ss_status ss_resample(ss_handle *h, const ss_resample_in *in, float *out) try {
     (void)h;
     if (!in || !out || !in->geometry || !in->frames || !in->knots_x || !in->knots_y || in->n_frames < 1)
         return SS_ERR_INVALID_ARGUMENT;
@@ -181,4 +182,8 @@ done:
     if (s) (void)hipStreamDestroy(s);
 #undef RCHK
     return rc;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }

@@ -11,8 +11,10 @@
 // Scalars are 1x1 records; the settings are written field by field, never
 // as a struct image.
 #include <cstdio>
+#include <new>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -63,7 +65,7 @@ struct Writer {
 
 }  // namespace
 
-extern "C" ss_status ss_rom_write(const char *path, const ss_group_in *in, const ss_settings *s) {
+extern "C" ss_status ss_rom_write(const char *path, const ss_group_in *in, const ss_settings *s) try {
     if (!path || !in || !s) return SS_ERR_INVALID_ARGUMENT;
     if (!little_endian()) return SS_ERR_NOT_AVAILABLE;
     if (in->template_id > 0 || in->geometry_id > 0 || in->data_on_device)
@@ -111,19 +113,29 @@ extern "C" ss_status ss_rom_write(const char *path, const ss_group_in *in, const
               std::fwrite(&w.count, 4, 1, f) == 1 && std::fwrite(w.buf.data(), 1, w.buf.size(), f) == w.buf.size();
     ok = (std::fclose(f) == 0) && ok;
     return ok ? SS_OK : SS_ERR_INVALID_ARGUMENT;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
-extern "C" ss_status ss_rom_read(const char *path, ss_rom **out, ss_group_in *in, ss_settings *s) {
+extern "C" ss_status ss_rom_read(const char *path, ss_rom **out, ss_group_in *in, ss_settings *s) try {
     if (!path || !out || !in || !s) return SS_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     if (!little_endian()) return SS_ERR_NOT_AVAILABLE;
     FILE *f = std::fopen(path, "rb");
     if (!f) return SS_ERR_INVALID_ARGUMENT;
+    struct Closer {  // closes the file also when a record allocation throws
+        FILE *&f;
+        ~Closer() {
+            if (f) std::fclose(f);
+        }
+    } closer{f};
     char magic[8];
     uint32_t bom = 0, count = 0;
     bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kMagic, 8) == 0 &&
               std::fread(&bom, 4, 1, f) == 1 && bom == 0x01020304u && std::fread(&count, 4, 1, f) == 1;
-    ss_rom *r = new ss_rom();
+    std::unique_ptr<ss_rom> r(new ss_rom());
     for (uint32_t i = 0; ok && i < count; i++) {
         char tag[5] = {0};
         uint32_t hdr[3];
@@ -139,6 +151,7 @@ extern "C" ss_status ss_rom_read(const char *path, ss_rom **out, ss_group_in *in
         r->cols[tag] = hdr[2];
     }
     std::fclose(f);
+    f = nullptr;
     auto has = [&](const char *t, uint32_t c) { return r->rec.count(t) && r->code[t] == c; };
     static const char *f64s[] = {"MOVT", "HALT", "CINC", "FSTR", "FCTH", "LAMB", "LSRM", "TMPL", "XGRD",
                                  "YGRD", "FRAC", "BASX", "BASY", "ESTX", "ESTY"};
@@ -146,10 +159,7 @@ extern "C" ss_status ss_rom_read(const char *path, ss_rom **out, ss_group_in *in
     for (const char *t : f64s) ok = ok && has(t, E_F64);
     for (const char *t : i32s) ok = ok && has(t, E_I32);
     ok = ok && has("PREC", E_U8) && has("BMSK", E_U8) && r->rec.count("DATA");
-    if (!ok) {
-        delete r;
-        return SS_ERR_INVALID_ARGUMENT;
-    }
+    if (!ok) return SS_ERR_INVALID_ARGUMENT;
     auto F = [&](const char *t) { double v; std::memcpy(&v, r->rec[t].data(), 8); return v; };
     auto I = [&](const char *t) { int32_t v; std::memcpy(&v, r->rec[t].data(), 4); return v; };
     ss_settings_default(s);
@@ -189,8 +199,12 @@ extern "C" ss_status ss_rom_read(const char *path, ss_rom **out, ss_group_in *in
     in->init_x = P("ESTX");
     in->init_y = P("ESTY");
     in->min_points = (const int32_t *)r->rec["MINP"].data();
-    *out = r;
+    *out = r.release();
     return SS_OK;
+} catch (const std::bad_alloc &) {
+    return SS_ERR_OUT_OF_MEMORY;  // host memory; C callers never see an exception
+} catch (...) {
+    return SS_ERR_DEVICE;
 }
 
 extern "C" void ss_rom_free(ss_rom *r) { delete r; }
