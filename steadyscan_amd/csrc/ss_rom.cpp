@@ -131,6 +131,12 @@ extern "C" ss_status ss_rom_read(const char *path, ss_rom **out, ss_group_in *in
             if (f) std::fclose(f);
         }
     } closer{f};
+    // bytes the records may still claim: a header never makes the reader
+    // allocate more than the file holds
+    long fileBytes = -1;
+    if (std::fseek(f, 0, SEEK_END) == 0) fileBytes = std::ftell(f);
+    if (fileBytes < 16 || std::fseek(f, 0, SEEK_SET) != 0) return SS_ERR_INVALID_ARGUMENT;
+    size_t left = (size_t)fileBytes - 16;
     char magic[8];
     uint32_t bom = 0, count = 0;
     bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kMagic, 8) == 0 &&
@@ -141,8 +147,10 @@ extern "C" ss_status ss_rom_read(const char *path, ss_rom **out, ss_group_in *in
         uint32_t hdr[3];
         ok = std::fread(tag, 1, 4, f) == 4 && std::fread(hdr, 4, 3, f) == 3 && hdr[0] >= E_F64 && hdr[0] <= E_I32;
         if (!ok) break;
-        const size_t n = esize(hdr[0]) * (size_t)hdr[1] * hdr[2];
-        if (n > ((size_t)1 << 34)) { ok = false; break; }
+        const uint64_t cells = (uint64_t)hdr[1] * hdr[2];  // both below 2^32: no wrap
+        if (left < 16 || cells > (left - 16) / esize(hdr[0])) { ok = false; break; }
+        const size_t n = esize(hdr[0]) * cells;
+        left -= 16 + n;
         std::vector<char> d(n);
         ok = n == 0 || std::fread(d.data(), 1, n, f) == n;
         r->rec[tag] = std::move(d);
@@ -159,6 +167,23 @@ extern "C" ss_status ss_rom_read(const char *path, ss_rom **out, ss_group_in *in
     for (const char *t : f64s) ok = ok && has(t, E_F64);
     for (const char *t : i32s) ok = ok && has(t, E_I32);
     ok = ok && has("PREC", E_U8) && has("BMSK", E_U8) && r->rec.count("DATA");
+    if (!ok) return SS_ERR_INVALID_ARGUMENT;
+    // the shapes must agree with each other, since the returned call reads
+    // every array at the sizes the template, block and frame counts give
+    auto shape = [&](const char *t, uint64_t rows, uint64_t cols) {
+        return r->rows[t] == rows && r->cols[t] == cols;
+    };
+    const uint64_t th = r->rows["TMPL"], tw = r->cols["TMPL"], fh = r->rows["FRAC"], nb = r->cols["FRAC"];
+    int32_t nF = 0;
+    std::memcpy(&nF, r->rec["NFRM"].data(), r->rec["NFRM"].size() < 4 ? 0 : 4);
+    for (const char *t : f64s) ok = ok && r->rec[t].size() >= 8;
+    for (const char *t : i32s) ok = ok && r->rec[t].size() >= 4;
+    ok = ok && r->rec["PREC"].size() >= 1 && nF > 0 && shape("XGRD", th, tw) && shape("YGRD", th, tw) &&
+         shape("BASX", fh, nb) && shape("BASY", fh, nb) && shape("BMSK", fh, nb) &&
+         shape("DATA", fh, nb * (uint64_t)nF) && shape("ESTX", nb + 1, (uint64_t)nF) &&
+         shape("ESTY", nb + 1, (uint64_t)nF) && shape("MINP", nb + 1, 1);
+    const uint32_t dcode = r->code["DATA"];
+    ok = ok && (dcode == E_F64 || dcode == E_F32 || dcode == E_U16 || dcode == E_I16);
     if (!ok) return SS_ERR_INVALID_ARGUMENT;
     auto F = [&](const char *t) { double v; std::memcpy(&v, r->rec[t].data(), 8); return v; };
     auto I = [&](const char *t) { int32_t v; std::memcpy(&v, r->rec[t].data(), 4); return v; };

@@ -69,3 +69,44 @@ def test_corrupt_file_is_rejected(call, tmp_path):
     p.write_bytes(bytes(p.read_bytes()[:100]))
     with pytest.raises(ssa.SteadyScanError):
         ssa.load_rom(p)
+
+
+def _records(raw):
+    """(tag, header offset, code, rows, cols, payload size) of each record."""
+    out, off = [], 16
+    for _ in range(struct.unpack("<I", raw[12:16])[0]):
+        code, rows, cols = struct.unpack("<III", raw[off + 4:off + 16])
+        size = {1: 8, 2: 4, 3: 2, 4: 2, 5: 1, 6: 4}[code] * rows * cols
+        out.append((raw[off:off + 4].decode(), off, code, rows, cols, size))
+        off += 16 + size
+    return out
+
+
+def test_record_larger_than_the_file_is_rejected_without_allocating(call, tmp_path):
+    p = tmp_path / "call.rom"
+    ssa.save_rom(p, call)
+    raw = bytearray(p.read_bytes())
+    _, off, _, _, _, _ = next(r for r in _records(raw) if r[0] == "DATA")
+    # 2^32-1 x 2^32-1 cells of 2 bytes: the size wraps 64 bits unless checked
+    raw[off + 8:off + 16] = struct.pack("<II", 0xFFFFFFFF, 0xFFFFFFFF)
+    p.write_bytes(bytes(raw))
+    with pytest.raises(ssa.SteadyScanError):
+        ssa.load_rom(p)
+    raw[off + 8:off + 16] = struct.pack("<II", 1 << 20, 1 << 10)  # 2 GiB claimed in a small file
+    p.write_bytes(bytes(raw))
+    with pytest.raises(ssa.SteadyScanError):
+        ssa.load_rom(p)
+
+
+@pytest.mark.parametrize("tag", ["BASX", "ESTY", "MINP", "XGRD"])
+def test_records_whose_shapes_disagree_are_rejected(call, tmp_path, tag):
+    p = tmp_path / "call.rom"
+    ssa.save_rom(p, call)
+    raw = bytearray(p.read_bytes())
+    _, off, code, rows, cols, _ = next(r for r in _records(raw) if r[0] == tag)
+    # the same cell count in another shape keeps the file well formed
+    assert rows > 1 or cols > 1
+    raw[off + 8:off + 16] = struct.pack("<II", rows * cols, 1) if cols > 1 else struct.pack("<II", 1, rows)
+    p.write_bytes(bytes(raw))
+    with pytest.raises(ssa.SteadyScanError):
+        ssa.load_rom(p)
