@@ -8,6 +8,9 @@
 // overlap, the prefiltering, the unpacking and the per-frame success tests
 // follow the reference line by line.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <new>
 #include <cmath>
 #include <cstring>
@@ -16,15 +19,22 @@
 #include <vector>
 
 #include "steadyscan.h"
+#include "ss_pinned.h"
 
 namespace {
 
-// dst[k] = src[k] (float to double) on a few host threads; the parts whose
-// thread could not be started are done on the calling thread.
-void widen(const float *src, double *dst, size_t n) {
-    const size_t parts = n < ((size_t)1 << 20) ? 1 : 8;
+// dst[k] = src[k] (float to double) for k < n and, when mask_src is given,
+// mask_dst[k] = mask_src[k], on a few host threads; the parts whose thread
+// could not be started are done on the calling thread.
+void widen(const float *src, double *dst, size_t n, const uint8_t *mask_src = nullptr,
+           uint8_t *mask_dst = nullptr) {
+    // the destination is often fresh memory: its page faults, more than the
+    // conversion, set the time, and they too spread over the threads
+    const size_t parts = n < ((size_t)1 << 20) ? 1 : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     auto part = [=](size_t p) {
-        for (size_t k = n * p / parts, e = n * (p + 1) / parts; k < e; k++) dst[k] = src[k];
+        const size_t b = n * p / parts, e = n * (p + 1) / parts;
+        for (size_t k = b; k < e; k++) dst[k] = src[k];
+        if (mask_src) std::memcpy(mask_dst + b, mask_src + b, e - b);
     };
     std::vector<std::thread> pool;
     size_t started = 1;
@@ -261,6 +271,32 @@ extern "C" ss_status ss_reconstruct(ss_handle *h, const ss_geometry_desc *geom, 
     std::vector<char> cropped, raw;
     std::vector<float> out;
     std::vector<double> out64;
+    // The single-precision results and the masks come back through
+    // page-locked staging, which the device writes at DMA speed (a copy into
+    // pageable memory goes through the runtime's bounce buffers at a fraction
+    // of it); without page-locked memory they go straight to pageable memory.
+    // SS_DRIVER_STATS=1 prints where the loop's time goes
+    const char *statsEnv = std::getenv("SS_DRIVER_STATS");
+    const bool stats = statsEnv && statsEnv[0] == '1';
+    auto clock = [] { return std::chrono::steady_clock::now(); };
+    auto secs = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double>(b - a).count();
+    };
+    auto t0 = clock();
+    struct Stage {
+        void *p;
+        bool stats;
+        ~Stage() {
+            const auto a = std::chrono::steady_clock::now();
+            ss::pinned_free(p);
+            if (stats)
+                std::fprintf(stderr, "[driver] staging released in %.4f s\n",
+                             std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count());
+        }
+    } stage{ss::pinned_alloc((sizeof(float) + 1) * np * std::min(perCall, n_frames)), stats};
+    if (stats)
+        std::fprintf(stderr, "[driver] %s staging of %zu bytes in %.4f s\n", stage.p ? "page-locked" : "no page-locked",
+                     (sizeof(float) + 1) * np * std::min(perCall, n_frames), secs(t0, clock()));
     for (int f0 = 0; f0 < n_frames; f0 += perCall) {
         const int n = std::min(perCall, n_frames - f0);
         const char *src = (const char *)frames + rawPix * esz * f0;
@@ -290,23 +326,35 @@ extern "C" ss_status ss_reconstruct(ss_handle *h, const ss_geometry_desc *geom, 
         in.frames = src;
         in.knots_x = dx_knots + (size_t)nk * f0;
         in.knots_y = dy_knots + (size_t)nk * f0;
-        out.resize(np * n);
-        st = ss_invert_bilinear_sampling(h, &in, &is, out.data(), mask + np * f0, nullptr);
+        float *o = (float *)stage.p;
+        uint8_t *m = mask + np * f0, *ms = nullptr;  // ms: the staged masks, copied out with the widening
+        if (o) {
+            ms = (uint8_t *)(o + np * std::min(perCall, n_frames));
+        } else {
+            out.resize(np * n);
+            o = out.data();
+        }
+        t0 = clock();
+        st = ss_invert_bilinear_sampling(h, &in, &is, o, ms ? ms : m, nullptr);
         if (st) return st;
+        const auto t1 = clock();
         double *dc = data_corrected + np * f0;
         if (ibs->inpaint && type == SS_F32) {
             // the frames' class is single: filled in double, rounded to single
-            st = ss_fill_gaps(h, SS_F32, out.data(), out.data(), geom->template_h, geom->template_w, n, 0);
+            st = ss_fill_gaps(h, SS_F32, o, o, geom->template_h, geom->template_w, n, 0);
             if (st) return st;
-            widen(out.data(), dc, np * n);
+            widen(o, dc, np * n, ms, m);
         } else if (ibs->inpaint) {
             out64.resize(np * n);
-            widen(out.data(), out64.data(), np * n);
+            widen(o, out64.data(), np * n, ms, m);
             st = ss_fill_gaps(h, SS_F64, out64.data(), dc, geom->template_h, geom->template_w, n, 0);
             if (st) return st;
         } else {
-            widen(out.data(), dc, np * n);
+            widen(o, dc, np * n, ms, m);
         }
+        if (stats)
+            std::fprintf(stderr, "[driver] frames %d..%d: reconstruction %.4f s, gap fill + widening %.4f s\n", f0,
+                         f0 + n, secs(t0, t1), secs(t1, clock()));
     }
     return SS_OK;
 } catch (const std::bad_alloc &) {

@@ -22,6 +22,7 @@
 
 #include "ss_device.h"
 #include "ss_host.h"
+#include "ss_pinned.h"
 #include "steadyscan.h"
 
 namespace {
@@ -1300,6 +1301,19 @@ void *handle_scratch(ss_handle *h, size_t bytes, ss_status *st) {
 }
 
 size_t handle_scratch_held(const ss_handle *h) { return h->scratchBytes; }
+
+void *pinned_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocPortable) != hipSuccess) {
+        (void)hipGetLastError();  // the caller falls back to pageable memory
+        return nullptr;
+    }
+    return p;
+}
+
+void pinned_free(void *p) {
+    if (p) (void)hipHostFree(p);
+}
 }  // namespace ss
 
 extern "C" {
