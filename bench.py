@@ -10,7 +10,10 @@ be resident, within the server's HBM budget), so a step's last, slowest
 groups share their ticks with other steps' groups; the knots are checked
 equal to the batched path's.  The
 same steps with one batched submission (ss_align_batch) and a
-synchronisation per step are reported beside it (barrier_per_step).
+synchronisation per step are reported beside it (barrier_per_step), and
+one such step with the block matrices in host memory, as the mex boundary
+hands them over, uploaded inside the timed step (host_input_per_step, the
+PCIe-inclusive rate).
 
 Precision: 'D' (the reference's default, FP64 pixel arithmetic) is the
 headline; 'S' (FP32 pixel arithmetic, FP64 reductions and solve) is timed
@@ -86,6 +89,8 @@ def parse():
                          "all = D (the headline), S, SB")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-latency", action="store_true", help="skip the single-group latency leg")
+    ap.add_argument("--no-host-input", action="store_true",
+                    help="skip the step with frames in host memory (uploaded inside the timed step)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
     ap.add_argument("--uneven", action="store_true",
@@ -489,6 +494,31 @@ def main():
             single.append(time.perf_counter() - t)
     single_ms = np.array(single) * 1e3
 
+    # the mex boundary's case: block matrices in host memory, uploaded by the
+    # library inside the timed step (PCIe-inclusive; never the headline value)
+    host_in = None
+    if not args.no_host_input:
+        hcalls = []
+        for g in range(ngroups):
+            a = dict(head["calls"][g])
+            a["bmData"] = ssa.image2blockmatrix(frames[:, :, g * G:(g + 1) * G], geom)
+            hcalls.append(a)
+        # warm-up step: the library's pinned staging and arena grow to the host-input size
+        ssa.align_groups(hcalls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        hres = ssa.align_groups(hcalls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        same = all(np.array_equal(a["knots_x"], b["knots_x"]) and np.array_equal(a["knots_y"], b["knots_y"])
+                   for a, b in zip(hres, results))
+        host_in = {"frames_per_s": ngroups * G / el, "ms_per_step": el * 1e3, "knots_equal_to_resident": same,
+                   "mode": "one batched submission with the frames' block matrices in host (pageable) memory"}
+        del hcalls, hres
+        print(f"[bench] precision {precs[0]}, frames in host memory: one step in {el:.3f}s "
+              f"({ngroups * G / el:.0f} frames/s), knots equal to the resident step: {same}",
+              file=sys.stderr, flush=True)
+
     if args.uneven:
         uneven = uneven_leg(args, torch, ssa, W)
         print(f"[bench] uneven: {uneven['frames_per_s']:.0f} frames/s, {uneven['megapixels_per_s']:.0f} Mpx/s",
@@ -570,6 +600,7 @@ def main():
                            "all groups of a step in one batched submission"),
             "barrier_per_step": {"frames_per_s": head["fps"], "ms_per_step": head["elapsed"] / args.steps * 1e3,
                                  "mode": "all groups of a step in one batched submission, synchronised per step"},
+            "host_input_per_step": host_in,
             "precision": hp,
             "frames_per_step": ngroups * G,
             "groups_per_step": ngroups,

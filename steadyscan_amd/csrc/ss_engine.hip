@@ -18,6 +18,7 @@
 #include <iterator>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ss_device.h"
@@ -475,6 +476,7 @@ struct Plan {
     size_t dBytes = 0, hBytes = 0;
     int imgType = 0;
     size_t imgBytes = 0;
+    bool staged = false;  // host frames already in the staging (stage_host_frames)
 };
 
 // Validates one call and lays out its device block and host staging.
@@ -560,6 +562,56 @@ ss_status plan_group(ss_handle *h, Plan &P) {
 // Fills the group's device description and initial control state and
 // enqueues its uploads on the handle's stream (D: the group's device block,
 // HS: its pinned staging, which must stay untouched until the stream syncs).
+// Pixels [b, e) of a group's host frames into its pinned staging HS, as they
+// are uploaded: copied, or converted for f64 frames in precision 'S'.
+void stage_frames(const Plan &P, char *HS, size_t b, size_t e) {
+    char *stage = HS + P.hImg;
+    if (P.in->data_type == SS_F64 && P.imgType == ss::IMG_F32) {
+        const double *src = (const double *)P.in->data;
+        float *dst = (float *)stage;
+        for (size_t i = b; i < e; i++) dst[i] = (float)src[i];
+    } else {
+        const size_t es = img_size(P.imgType);
+        std::memcpy(stage + b * es, (const char *)P.in->data + b * es, (e - b) * es);
+    }
+}
+
+// The host frames of all groups of a batch into their staging (group gi at
+// HG[gi]) on up to 16 host threads, in pieces of about 8 MB: one thread's
+// copy into pinned memory is several times slower than the upload itself.
+// Pieces whose thread cannot be started run on the calling thread.
+void stage_host_frames(std::vector<Plan> &plans, const std::vector<char *> &HG) {
+    struct Piece {
+        int g;
+        size_t b, e;
+    };
+    std::vector<Piece> pieces;
+    for (size_t gi = 0; gi < plans.size(); gi++) {
+        const Plan &P = plans[gi];
+        if (P.in->data_on_device) continue;
+        const size_t npix = (size_t)P.in->frame_h * P.in->n_blocks * P.in->n_frames;
+        const size_t step = std::max<size_t>(1, ((size_t)8 << 20) / img_size(P.imgType));
+        for (size_t b = 0; b < npix; b += step) pieces.push_back({(int)gi, b, std::min(npix, b + step)});
+    }
+    if (pieces.empty()) return;
+    const size_t nt = std::min<size_t>(pieces.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    auto run = [&](size_t t) {
+        for (size_t k = t; k < pieces.size(); k += nt)
+            stage_frames(plans[pieces[k].g], HG[pieces[k].g], pieces[k].b, pieces[k].e);
+    };
+    std::vector<std::thread> pool;
+    size_t started = 1;
+    try {
+        for (; started < nt; started++) pool.emplace_back(run, started);
+    } catch (...) {  // no thread available: the rest runs here
+    }
+    run(0);
+    for (size_t t = started; t < nt; t++) run(t);
+    for (auto &th : pool) th.join();
+    for (auto &P : plans)
+        if (!P.in->data_on_device) P.staged = true;
+}
+
 ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g, ss::GroupCtl &c) {
     auto dptr = [&](size_t o) { return (void *)(D + o); };
     const ss_group_in &in = *P.in;
@@ -676,14 +728,7 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
         g.img = in.data;
     } else {
         void *stage = HS + P.hImg;
-        const size_t npix = (size_t)fh * fw;
-        if (in.data_type == SS_F64 && P.imgType == ss::IMG_F32) {
-            const double *src = (const double *)in.data;
-            float *dst = (float *)stage;
-            for (size_t i = 0; i < npix; i++) dst[i] = (float)src[i];
-        } else {
-            std::memcpy(stage, in.data, P.imgBytes);
-        }
+        if (!P.staged) stage_frames(P, HS, 0, (size_t)fh * fw);
         HIPCHK(hipMemcpyAsync(dptr(P.oImg), stage, P.imgBytes, hipMemcpyHostToDevice, h->stream));
         g.img = dptr(P.oImg);
     }
@@ -919,14 +964,16 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
     char *D = (char *)h->arena + tb;
     char *HS = (char *)h->harena;
     ss::GroupCtl *hc = (ss::GroupCtl *)(HS + tb);
-    char *HG = HS + tb + cb;
+    std::vector<char *> HG(n);
+    HG[0] = HS + tb + cb;
+    for (int gi = 1; gi < n; gi++) HG[gi] = HG[gi - 1] + ((plans[gi - 1].hBytes + 255) & ~(size_t)255);
+    stage_host_frames(plans, HG);
     std::vector<ss::GroupDev> gd(n);
     for (int gi = 0; gi < n; gi++) {
         Plan &P = plans[gi];
-        st = setup_group(h, P, D, HG, gd[gi], hc[gi]);
+        st = setup_group(h, P, D, HG[gi], gd[gi], hc[gi]);
         if (st) return st;
         D += (P.dBytes + 255) & ~(size_t)255;
-        HG += (P.hBytes + 255) & ~(size_t)255;
     }
     Table T;
     T.mem = h->arena;
