@@ -19,6 +19,10 @@
  *    mex/reporter/reporter.cpp:25-27).  ss_last_error() has the message.
  *  - A handle is used by one host thread at a time; different handles are
  *    independent.
+ *  - Large host-side copies inside a call (host frames into the pinned
+ *    staging of ss_align_group / ss_align_batch, the widening of
+ *    ss_reconstruct's results) run on up to 16 short-lived host threads
+ *    that end before the call returns.
  */
 #ifndef STEADYSCAN_H_
 #define STEADYSCAN_H_
