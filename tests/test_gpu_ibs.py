@@ -14,6 +14,7 @@ import steadyscan_amd as ssa
 from oracle import oracle as O
 from oracle import driver as OD
 from steadyscan_amd import workloads as W
+from devbuf import DeviceBuffer
 
 pytestmark = pytest.mark.gpu
 
@@ -172,35 +173,6 @@ def test_workload_size_512():
     nk = geom["nknots"]
     dx, dy = smooth_knots(rng, nk, 2, 0.3), smooth_knots(rng, nk, 2, 0.3)
     compare(np.asarray(fr), dx, dy, 512, 512, nk, label="512x512 uint16")
-
-
-class DeviceBuffer:
-    """Device memory from the HIP runtime the library itself links (a second
-    runtime, such as torch's, may not share the device with it in one process)."""
-    _hip = None
-
-    def __init__(self, nbytes):
-        import ctypes
-        if DeviceBuffer._hip is None:
-            DeviceBuffer._hip = ctypes.CDLL("libamdhip64.so.7")
-        self.ptr = ctypes.c_void_p()
-        assert DeviceBuffer._hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(nbytes)) == 0
-        self.nbytes = nbytes
-
-    def upload(self, a):
-        import ctypes
-        a = np.ascontiguousarray(a)
-        assert a.nbytes == self.nbytes
-        assert DeviceBuffer._hip.hipMemcpy(self.ptr, ctypes.c_void_p(a.ctypes.data), ctypes.c_size_t(a.nbytes), 1) == 0
-
-    def download(self, dtype, count):
-        import ctypes
-        a = np.empty(count, dtype)
-        assert DeviceBuffer._hip.hipMemcpy(ctypes.c_void_p(a.ctypes.data), self.ptr, ctypes.c_size_t(a.nbytes), 2) == 0
-        return a
-
-    def free(self):
-        DeviceBuffer._hip.hipFree(self.ptr)
 
 
 def test_device_resident_frames_and_outputs_match_the_host_path():

@@ -260,3 +260,25 @@ def test_ilu0_solver(flags):
     check(a, ref, out[0], out[1], out[5], out[2], out[4], label="ILU(0)")
     exact = O.align_group_to_template(W.group_args(T, fr, geom, max_iter=15, flags=flags), threads=8)
     assert not np.array_equal(ref["knots_x"], exact["knots_x"])  # a different solver was run
+
+
+def test_host_frames_staged_in_pieces_equal_resident_and_converted(config1_group):
+    # 50 frames of 512x512: a group's host frames are staged in ~8 MB pieces
+    # on several host threads; f64 frames in 'S' are converted piece by piece
+    from devbuf import DeviceBuffer
+    T, fr, geom = config1_group
+    for flags in ("D", "S"):
+        a = W.group_args(T, fr, geom, flags=flags, max_iter=3)
+        assert a["bmData"].dtype == np.uint16 and a["bmData"].nbytes > (16 << 20)
+        d = DeviceBuffer(a["bmData"].nbytes)
+        d.upload(a["bmData"].ravel(order="F"))
+        variants = {"host u16": a, "resident u16": dict(a, bmData=(d.ptr.value, np.uint16)),
+                    "host f64": dict(a, bmData=np.asfortranarray(a["bmData"].astype(np.float64))),
+                    "host f32": dict(a, bmData=np.asfortranarray(a["bmData"].astype(np.float32)))}
+        res = {k: ssa.align_groups([v], with_mask=False)[0] for k, v in variants.items()}
+        base = res["host u16"]
+        for k, r in res.items():
+            print(f"[{flags}] {k}: iterations {r['iterations']}", flush=True)
+            assert np.array_equal(r["knots_x"], base["knots_x"]), (flags, k)
+            assert np.array_equal(r["knots_y"], base["knots_y"]), (flags, k)
+        d.free()
