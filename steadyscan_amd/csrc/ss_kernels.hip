@@ -632,55 +632,19 @@ __device__ void ilu0_solve(const Eq0 *E, IluRow *F, double2 *X, int n) {
     }
 }
 
-__global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
-    const int gi = blockIdx.x;
-    GroupCtl &ctl = B.ctl[gi];
-    if (ctl.phase != PH_ASSEMBLE) return;
-    const GroupDev &g = B.groups[gi];
-    const int tid = threadIdx.x;
-    const int nF = g.nF, nb = g.nb, N = g.N, n = g.fframeW;
-    const int shift = ctl.shift;
-    // filter deactivation after max_filter_iterations (mainiteration.cpp:80-90)
-    __shared__ int s_filter;
-    if (tid == 0) {
-        if (ctl.nFilterIterations != 0 && ctl.iteration > ctl.nFilterIterations) {
-            ctl.solverFilter = 0;
-            ctl.filterID = 0;
-            ctl.nFilterIterations = 0;
-        }
-        s_filter = ctl.solverFilter;
-    }
-    const double *S = g.colsum;
-    // 1. per-frame errval of this shift (d_calcAllGroupErrval)
-    for (int f = tid; f < nF; f += kSolveWG) {
-        double s = 0.0, c = 0.0;
-        for (int bb = 0; bb < nb; bb++) {
-            s += S[(size_t)(f * nb + bb) * CS_N + CS_D2];
-            c += S[(size_t)(f * nb + bb) * CS_N + CS_CNT];
-        }
-        g.errval[shift * nF + f] = s / c;
-    }
-    // 2. knot usage (d_createParameterMask); the last knot is never used
-    for (int k = tid; k < N; k += kSolveWG) {
-        uint8_t u = 0;
-        if (k < n) {
-            const bool pc = S[(size_t)k * CS_N + CS_CNT] >= (double)g.minPts[k % nb];
-            const bool pp = k > 0 && S[(size_t)(k - 1) * CS_N + CS_CNT] >= (double)g.minPts[(k - 1) % nb];
-            u = (pc || pp) ? 1 : 0;
-        }
-        g.used[k] = u;
-    }
-    __threadfence_block();
-    __syncthreads();
-    // 3. level-0 equations (d_fillDiagonals / d_fillJTr entries, lambda extension)
-    Eq0 *E = reinterpret_cast<Eq0 *>(g.ws);
-    const double lambda = g.lambda;
+// Level-0 equations of knots tid, tid + kSolveWG, ... (d_fillDiagonals /
+// d_fillJTr entries, lambda extension).  The pointers do not alias, so the
+// next knot's loads may be issued before this knot's equation is stored.
+__device__ __forceinline__ void level0_equations(const double *__restrict__ S, const uint8_t *__restrict__ used,
+                                                 Eq0 *__restrict__ E, int n, double lambda, int fracUniform,
+                                                 int tid) {
+#pragma unroll 2
     for (int k = tid; k < n; k += kSolveWG) {
         Eq0 e;
         const double *s = S + (size_t)k * CS_N;
-        const bool u = g.used[k];
-        const bool up = k > 0 && g.used[k - 1];
-        const bool un = (k + 1 < n) && g.used[k + 1];
+        const bool u = used[k];
+        const bool up = k > 0 && used[k - 1];
+        const bool un = (k + 1 < n) && used[k + 1];
         if (!u) {
             e = Eq0{1, 0, 1, 0, 0, 0, 0, 0};
         } else {
@@ -688,7 +652,7 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
             double jx = s[CS_JXA], jy = s[CS_JYA];
             if (k > 0) {
                 const double *sp = s - CS_N;
-                const int q = g.fracUniform ? CS_BXX : CS_QXX;
+                const int q = fracUniform ? CS_BXX : CS_QXX;
                 dxx += sp[q];
                 dyy += sp[q + 1];
                 dxy += sp[q + 2];
@@ -711,6 +675,82 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
         }
         E[k] = e;
     }
+}
+
+__global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
+    const int gi = blockIdx.x;
+    GroupCtl &ctl = B.ctl[gi];
+    if (ctl.phase != PH_ASSEMBLE) return;
+    const GroupDev &g = B.groups[gi];
+    const int tid = threadIdx.x;
+    const int nF = g.nF, nb = g.nb, N = g.N, n = g.fframeW;
+    const int shift = ctl.shift;
+    // filter deactivation after max_filter_iterations (mainiteration.cpp:80-90)
+    __shared__ int s_filter;
+    if (tid == 0) {
+        if (ctl.nFilterIterations != 0 && ctl.iteration > ctl.nFilterIterations) {
+            ctl.solverFilter = 0;
+            ctl.filterID = 0;
+            ctl.nFilterIterations = 0;
+        }
+        s_filter = ctl.solverFilter;
+    }
+    const double *S = g.colsum;
+    const int *minPts = g.minPts;
+    uint8_t *used = g.used;
+    // The loops below load a batch of entries before they use any of it:
+    // one thread per frame or knot would otherwise wait out a memory latency
+    // per entry.  The additions keep their order.
+    constexpr int kBatch = 8;
+    // 1. per-frame errval of this shift (d_calcAllGroupErrval)
+    for (int f = tid; f < nF; f += kSolveWG) {
+        const double *col = S + (size_t)f * nb * CS_N;
+        double s = 0.0, c = 0.0;
+        for (int b0 = 0; b0 < nb; b0 += kBatch) {
+            double vs[kBatch], vc[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                const int bb = min(b0 + u, nb - 1);
+                vs[u] = col[(size_t)bb * CS_N + CS_D2];
+                vc[u] = col[(size_t)bb * CS_N + CS_CNT];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                if (b0 + u < nb) {
+                    s += vs[u];
+                    c += vc[u];
+                }
+            }
+        }
+        g.errval[shift * nF + f] = s / c;
+    }
+    // 2. knot usage (d_createParameterMask); the last knot is never used
+    for (int k0 = tid * kBatch; k0 < N; k0 += kSolveWG * kBatch) {
+        double cnt[kBatch + 1];
+        int mp[kBatch + 1];
+#pragma unroll
+        for (int u = 0; u <= kBatch; u++) {  // knots k0-1 .. k0+kBatch-1
+            const int k = min(max(k0 + u - 1, 0), n - 1);
+            cnt[u] = S[(size_t)k * CS_N + CS_CNT];
+            mp[u] = minPts[k % nb];
+        }
+        uint8_t uk[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int k = k0 + u;
+            const bool pc = k < n && cnt[u + 1] >= (double)mp[u + 1];
+            const bool pp = k < n && k > 0 && cnt[u] >= (double)mp[u];
+            uk[u] = (pc || pp) ? 1 : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+            if (k0 + u < N) used[k0 + u] = uk[u];
+    }
+    __threadfence_block();
+    __syncthreads();
+    // 3. level-0 equations (d_fillDiagonals / d_fillJTr entries, lambda extension)
+    Eq0 *E = reinterpret_cast<Eq0 *>(g.ws);
+    level0_equations(S, used, E, n, g.lambda, g.fracUniform, tid);
     __threadfence_block();
     __syncthreads();
     InteriorRow *IR = reinterpret_cast<InteriorRow *>(g.ws + (size_t)n * 8);
@@ -960,17 +1000,32 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
     }
     // 6. d_expandResults (task_expandresults.cu:10-41) and line-search set-up
     const int G = g.groupSize;
-    for (int idx = tid; idx < 2 * N; idx += kSolveWG) {
-        const int j = idx >= N ? idx - N : idx;
-        const bool fixed = ((((idx + 1) - (shift * (G >> 1))) % G) == 0);
-        double dv = g.deltaP[idx];
-        if (!fixed) {
-            if (j < n && g.used[j]) dv = idx >= N ? X[j].y : X[j].x;
-            else dv = 0.0;
+    double *deltaP = g.deltaP, *d0 = g.d0, *p0 = g.p0;
+    const double *p = g.p;
+    for (int i0 = tid * kBatch; i0 < 2 * N; i0 += kSolveWG * kBatch) {
+        double dv[kBatch], pv[kBatch], xv[kBatch];
+        bool keep[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int idx = min(i0 + u, 2 * N - 1);
+            const int j = idx >= N ? idx - N : idx;
+            const bool fixed = ((((idx + 1) - (shift * (G >> 1))) % G) == 0);
+            dv[u] = deltaP[idx];
+            pv[u] = p[idx];
+            const bool uj = j < n && used[j];
+            xv[u] = uj ? (idx >= N ? X[j].y : X[j].x) : 0.0;
+            keep[u] = fixed;
         }
-        g.deltaP[idx] = dv;
-        g.d0[idx] = dv;
-        g.p0[idx] = g.p[idx];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int idx = i0 + u;
+            if (idx < 2 * N) {
+                const double v = keep[u] ? dv[u] : xv[u];
+                deltaP[idx] = v;
+                d0[idx] = v;
+                p0[idx] = pv[u];
+            }
+        }
     }
     for (int f = tid; f < nF; f += kSolveWG) {
         g.lsActive[f] = 1;
