@@ -60,54 +60,61 @@ PD = POINTER(c_double)
 def lib():
     global _lib
     if _lib is None:
-        _lib = ctypes.CDLL(LIB)
-        L = _lib
-        L.orc_align_group.argtypes = [POINTER(OrcIn), POINTER(OrcSettings), POINTER(OrcOut), c_int32]
-        L.orc_align_group.restype = c_int32
-        L.orc_state_create.argtypes = [POINTER(OrcIn), POINTER(OrcSettings), POINTER(c_void_p)]
-        L.orc_state_create.restype = c_int32
-        L.orc_state_destroy.argtypes = [c_void_p]
-        L.orc_state_get.argtypes = [c_void_p, PD, PD, PD]
-        L.orc_state_set_p.argtypes = [c_void_p, PD]
-        L.orc_state_set_delta.argtypes = [c_void_p, PD]
-        L.orc_half_iteration_assemble.argtypes = [c_void_p, c_int32, POINTER(c_uint8), PD, PD, PD]
-        L.orc_half_iteration_assemble.restype = c_int32
-        L.orc_state_pixels.argtypes = [c_void_p, POINTER(c_uint8), PD, PD, PD]
-        L.orc_half_iteration_linesearch.argtypes = [c_void_p, c_int32]
-        L.orc_half_iteration_linesearch.restype = c_int32
-        L.orc_global_correlation.argtypes = [c_void_p]
-        L.orc_global_correlation.restype = c_double
-        L.orc_organize_in.argtypes = [PD, PD, c_int32, c_int32, PD]
-        L.orc_organize_out.argtypes = [PD, c_int32, c_int32, PD, PD]
-        L.orc_solve_coo.argtypes = [c_int32, c_int32, POINTER(c_int32), POINTER(c_int32), PD, PD, PD, c_int32]
-        L.orc_solve_coo.restype = c_int32
-        L.orc_filter_apply.argtypes = [c_int32, c_double, PD, c_int32]
-        L.orc_geometry.argtypes = [POINTER(OrcGeometryIn), POINTER(c_int32), PD, PD, PD,
-                                   POINTER(c_int32), POINTER(c_int32)]
-        L.orc_geometry.restype = c_int32
-        L.orc_image2blockmatrix.argtypes = [PD, POINTER(c_int32), c_int32, c_int32, PD, POINTER(c_uint8)]
-        L.orc_image2blockmatrix.restype = c_int32
-        L.orc_invert_bilinear_sampling.argtypes = [PD, c_int32, c_int32, c_int32, PD, PD, c_int32,
-                                                   POINTER(c_int32), PD, PD, PD, c_int32, c_int32, c_double,
-                                                   c_double, POINTER(ctypes.c_float), POINTER(c_uint8),
-                                                   POINTER(c_int32)]
-        L.orc_invert_bilinear_sampling.restype = c_int32
-        L.orc_ibs_neighbour.argtypes = [ctypes.c_long, c_int32, ctypes.c_long, ctypes.c_long,
-                                        POINTER(ctypes.c_long)]
-        L.orc_ibs_neighbour.restype = c_int32
-        L.orc_resample_frame.argtypes = [PD, c_int32, PD, PD, PD, PD, POINTER(c_int32), PD, c_int32,
-                                         c_int32, POINTER(ctypes.c_float)]
-        PF = POINTER(ctypes.c_float)
-        L.orc_ibs_system.argtypes = [PD, c_int32, PD, PD, POINTER(c_int32), PD, PD, PD, c_int32, c_int32,
-                                     c_double, c_double, PD, PD, PF, POINTER(c_uint8)]
-        L.orc_ibs_system.restype = c_int32
-        L.orc_ibs_energy.argtypes = [PD, PD, PF, POINTER(c_uint8), c_int32, c_int32]
-        L.orc_ibs_energy.restype = ctypes.c_float
-        L.orc_ibs_interactions.argtypes = [PD, PD, POINTER(c_uint8), c_int32, c_int32, PF, PF]
-        L.orc_ibs_sweep.argtypes = [PF, PF, c_int32, c_int32, PF]
-        L.orc_ibs_check_interval.argtypes = [c_int32]
-        L.orc_ibs_check_interval.restype = c_int32
+        _lib = load(LIB)
     return _lib
+
+
+def load(path):
+    """The oracle library at `path` with its prototypes declared (the tests
+    also load copies built with a planted mistake)."""
+    L = ctypes.CDLL(path)
+    L.orc_align_group.argtypes = [POINTER(OrcIn), POINTER(OrcSettings), POINTER(OrcOut), c_int32]
+    L.orc_align_group.restype = c_int32
+    L.orc_state_create.argtypes = [POINTER(OrcIn), POINTER(OrcSettings), POINTER(c_void_p)]
+    L.orc_state_create.restype = c_int32
+    L.orc_state_destroy.argtypes = [c_void_p]
+    L.orc_state_get.argtypes = [c_void_p, PD, PD, PD]
+    L.orc_state_set_p.argtypes = [c_void_p, PD]
+    L.orc_state_set_delta.argtypes = [c_void_p, PD]
+    L.orc_half_iteration_assemble.argtypes = [c_void_p, c_int32, POINTER(c_uint8), PD, PD, PD]
+    L.orc_half_iteration_assemble.restype = c_int32
+    L.orc_state_pixels.argtypes = [c_void_p, POINTER(c_uint8), PD, PD, PD]
+    L.orc_half_iteration_linesearch.argtypes = [c_void_p, c_int32]
+    L.orc_half_iteration_linesearch.restype = c_int32
+    L.orc_global_correlation.argtypes = [c_void_p]
+    L.orc_global_correlation.restype = c_double
+    L.orc_organize_in.argtypes = [PD, PD, c_int32, c_int32, PD]
+    L.orc_organize_out.argtypes = [PD, c_int32, c_int32, PD, PD]
+    L.orc_solve_coo.argtypes = [c_int32, c_int32, POINTER(c_int32), POINTER(c_int32), PD, PD, PD, c_int32]
+    L.orc_solve_coo.restype = c_int32
+    L.orc_filter_apply.argtypes = [c_int32, c_double, PD, c_int32]
+    L.orc_geometry.argtypes = [POINTER(OrcGeometryIn), POINTER(c_int32), PD, PD, PD,
+                               POINTER(c_int32), POINTER(c_int32)]
+    L.orc_geometry.restype = c_int32
+    L.orc_image2blockmatrix.argtypes = [PD, POINTER(c_int32), c_int32, c_int32, PD, POINTER(c_uint8)]
+    L.orc_image2blockmatrix.restype = c_int32
+    L.orc_invert_bilinear_sampling.argtypes = [PD, c_int32, c_int32, c_int32, PD, PD, c_int32,
+                                               POINTER(c_int32), PD, PD, PD, c_int32, c_int32, c_double,
+                                               c_double, POINTER(ctypes.c_float), POINTER(c_uint8),
+                                               POINTER(c_int32)]
+    L.orc_invert_bilinear_sampling.restype = c_int32
+    L.orc_ibs_neighbour.argtypes = [ctypes.c_long, c_int32, ctypes.c_long, ctypes.c_long,
+                                    POINTER(ctypes.c_long)]
+    L.orc_ibs_neighbour.restype = c_int32
+    L.orc_resample_frame.argtypes = [PD, c_int32, PD, PD, PD, PD, POINTER(c_int32), PD, c_int32,
+                                     c_int32, POINTER(ctypes.c_float)]
+    PF = POINTER(ctypes.c_float)
+    L.orc_ibs_system.argtypes = [PD, c_int32, PD, PD, POINTER(c_int32), PD, PD, PD, c_int32, c_int32,
+                                 c_double, c_double, PD, PD, PF, POINTER(c_uint8)]
+    L.orc_ibs_system.restype = c_int32
+    L.orc_ibs_energy.argtypes = [PD, PD, PF, POINTER(c_uint8), c_int32, c_int32]
+    L.orc_ibs_energy.restype = ctypes.c_float
+    L.orc_ibs_interactions.argtypes = [PD, PD, POINTER(c_uint8), c_int32, c_int32, PF, PF]
+    L.orc_ibs_sweep.argtypes = [PF, PF, c_int32, c_int32, PF]
+    L.orc_ibs_check_interval.argtypes = [c_int32]
+    L.orc_ibs_check_interval.restype = c_int32
+    L.orc_frame_correlation.argtypes = [c_void_p, PD]
+    return L
 
 
 def ref_lib():
@@ -196,35 +203,36 @@ def align_group_to_template(a, threads=1, with_mask=True, with_iterations=False)
 class State:
     """Stage-level access to the oracle (one half-iteration at a time)."""
 
-    def __init__(self, a):
+    def __init__(self, a, L=None):
+        self.L = L if L is not None else lib()
         self.inp = Inputs(a)
         self.ptr = c_void_p()
-        rc = lib().orc_state_create(byref(self.inp.i), byref(self.inp.s), byref(self.ptr))
+        rc = self.L.orc_state_create(byref(self.inp.i), byref(self.inp.s), byref(self.ptr))
         if rc:
             raise RuntimeError(f"oracle state status {rc}")
         self.N = self.inp.nb * self.inp.nF + 1
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            lib().orc_state_destroy(self.ptr)
+            self.L.orc_state_destroy(self.ptr)
 
     def get(self):
         p = np.zeros(2 * self.N)
         d = np.zeros(2 * self.N)
         e = np.zeros(2 * self.inp.nF)
-        lib().orc_state_get(self.ptr, _p(p), _p(d), _p(e))
+        self.L.orc_state_get(self.ptr, _p(p), _p(d), _p(e))
         return p, d, e
 
     def set_p(self, p):
         p = np.ascontiguousarray(p, dtype=np.float64)
-        lib().orc_state_set_p(self.ptr, _p(p))
+        self.L.orc_state_set_p(self.ptr, _p(p))
 
     def assemble(self, shift):
         used = np.zeros(self.N, np.uint8)
         jtr = np.zeros(2 * self.N)
         h = np.zeros(6 * self.N)
         dn = np.zeros(2 * self.N)
-        n = lib().orc_half_iteration_assemble(self.ptr, shift, used.ctypes.data_as(POINTER(c_uint8)),
+        n = self.L.orc_half_iteration_assemble(self.ptr, shift, used.ctypes.data_as(POINTER(c_uint8)),
                                               _p(jtr), _p(h), _p(dn))
         return dict(n_used=n, used=used.astype(bool), jtr=jtr, h=h.reshape(self.N, 6), delta=dn)
 
@@ -234,14 +242,19 @@ class State:
         x = np.zeros((fh, nc), order="F")
         y = np.zeros((fh, nc), order="F")
         d = np.zeros((fh, nc), order="F")
-        lib().orc_state_pixels(self.ptr, m.ctypes.data_as(POINTER(c_uint8)), _p(x), _p(y), _p(d))
+        self.L.orc_state_pixels(self.ptr, m.ctypes.data_as(POINTER(c_uint8)), _p(x), _p(y), _p(d))
         return m.astype(bool), x, y, d
 
     def linesearch(self, shift):
-        return bool(lib().orc_half_iteration_linesearch(self.ptr, shift))
+        return bool(self.L.orc_half_iteration_linesearch(self.ptr, shift))
 
     def global_correlation(self):
-        return lib().orc_global_correlation(self.ptr)
+        return self.L.orc_global_correlation(self.ptr)
+
+    def frame_correlation(self):
+        c = np.zeros(self.inp.nF)
+        self.L.orc_frame_correlation(self.ptr, _p(c))
+        return c
 
 
 def invert_bilinear_sampling(frames, dx_knots, dy_knots, geometry, template_shape, regfac=3e-3, aspect=1.0):

@@ -45,6 +45,21 @@
 
 #define IDX2R(i, j, ld) ((size_t)(i) * (size_t)(ld) + (size_t)(j))
 
+/* ORC_PLANT = n > 0 compiles deliberate mistake n into a copy of this file.
+ * Only tests/test_oracle_closed_forms.py builds such copies (into a temporary
+ * directory), to show that its closed-form checks reject each mistake; the
+ * oracle itself is built with ORC_PLANT = 0 and contains none of them.
+ *   1 bidirectional scan flips the even lines instead of the odd ones
+ *   2 integer-spacing knot times taken from the cropped lines
+ *   3 linspace divides by n instead of n - 1
+ *   4 ILU(0) eliminates rows whose columns were not sorted
+ *   5 ILU(0) divides the multiplier by the row's own pivot
+ *   6 frame correlation means taken over all pixels of the frame
+ *   7 frame correlation of the residual instead of the warped template */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 struct orc_state {
     int frameH, frameW, fframeW, nParamPoints, nParams, nFrames;
     int templateH, templateW, groupSize, nGroups;
@@ -228,7 +243,7 @@ static int solve_ilu0(int n, int nnz, const int *row, const int *col, const doub
         cv[pos] = val[k];
     }
     /* sort each row by column (d_sortColumnIndices, task_filldiagonals.cu:12-33) */
-    for (int r = 0; r < n; r++) {
+    for (int r = 0; r < n && ORC_PLANT != 4; r++) {
         for (int a = cnt[r] + 1; a < cnt[r + 1]; a++) {
             int j = a;
             while (j > cnt[r] && ci[j - 1] > ci[j]) {
@@ -251,7 +266,8 @@ static int solve_ilu0(int n, int nnz, const int *row, const int *col, const doub
         for (int a = cnt[i]; a < cnt[i + 1]; a++) {
             int k = ci[a];
             if (k >= i) break;
-            double ukk = diag[k] >= 0 ? cv[diag[k]] : 0.0;
+            const int pk = ORC_PLANT == 5 ? i : k;
+            double ukk = diag[pk] >= 0 ? cv[diag[pk]] : 0.0;
             cv[a] = cv[a] / ukk;
             double lik = cv[a];
             for (int b = cnt[k]; b < cnt[k + 1]; b++) {
@@ -708,7 +724,7 @@ double orc_global_correlation(orc_state *st) {
 /* hd_frameCorrelation (task_correlation.cu:101-183). */
 static void frame_correlation(orc_state *st) {
     size_t numPixel = (size_t)st->frameH * st->fframeW;
-    for (size_t i = 0; i < numPixel; i++)
+    for (size_t i = 0; i < numPixel && ORC_PLANT != 7; i++)
         if (st->mask[i]) st->d[i] = st->image[i] - st->d[i];
     for (int f = 0; f < st->nFrames; f++) {
         double sa = 0, sb = 0;
@@ -724,7 +740,8 @@ static void frame_correlation(orc_state *st) {
                 }
             }
         }
-        double mA = sa / (double)sc, mB = sb / (double)sc;
+        const double cnt = ORC_PLANT == 6 ? (double)st->frameH * st->frameW : (double)sc;
+        double mA = sa / cnt, mB = sb / cnt;
         double assq = 0, bssq = 0, q = 0;
         for (int t = 0; t < st->frameW; t++) {
             int idx = f * st->frameW + t;
@@ -740,8 +757,13 @@ static void frame_correlation(orc_state *st) {
         }
         st->corr[f] = q / sqrt(assq * bssq);
     }
-    for (size_t i = 0; i < numPixel; i++)
+    for (size_t i = 0; i < numPixel && ORC_PLANT != 7; i++)
         if (st->mask[i]) st->d[i] = st->image[i] - st->d[i];
+}
+
+void orc_frame_correlation(orc_state *st, double *corr) {
+    frame_correlation(st);
+    if (corr) memcpy(corr, st->corr, sizeof(double) * (size_t)st->nFrames);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1029,7 +1051,7 @@ static double linspace_at(double d1, double d2, int n, int i) {
     if (n == 1) return d2;
     if (i == 0) return d1;
     if (i == n - 1) return d2;
-    return d1 + ((double)i * (d2 - d1)) / (double)(n - 1);
+    return d1 + ((double)i * (d2 - d1)) / (double)(ORC_PLANT == 3 ? n : n - 1);
 }
 
 int orc_geometry(const orc_geometry_in *g, int32_t *prevknotind, double *tfrac, double *basex,
@@ -1053,7 +1075,7 @@ int orc_geometry(const orc_geometry_in *g, int32_t *prevknotind, double *tfrac, 
     double npixels = (double)H * W;
     double dwell = 1 / npixels;
     /* pixelscantime(i,j), 0-based i, j (steady_scan.m:792-795) */
-#define PST(i, j) (dwell / 2 + (double)((size_t)(i) * W + (size_t)((g->bidirectional && ((i) % 2 == 1)) ? (W - 1 - (j)) : (j))) / npixels)
+#define PST(i, j) (dwell / 2 + (double)((size_t)(i) * W + (size_t)((g->bidirectional && ((i) % 2 == (ORC_PLANT == 1 ? 0 : 1))) ? (W - 1 - (j)) : (j))) / npixels)
     int nk = g->nknots;
     double *tk = (double *)malloc(sizeof(double) * (size_t)nk);
     double lpk = (double)Hc / (nk - 1);
@@ -1061,8 +1083,9 @@ int orc_geometry(const orc_geometry_in *g, int32_t *prevknotind, double *tfrac, 
         int L = (int)lpk;
         for (int k = 0; k < nk - 1; k++) {
             int line = k * L; /* knotlines index the uncropped pixelscantime (steady_scan.m:802-803) */
-            double mn = PST(line, 0);
-            for (int j = 1; j < W; j++) { double v = PST(line, j); if (v < mn) mn = v; }
+            const int j0 = ORC_PLANT == 2 ? cr[0] : 0, j1 = ORC_PLANT == 2 ? W - cr[1] : W;
+            double mn = PST(line, j0);
+            for (int j = j0 + 1; j < j1; j++) { double v = PST(line, j); if (v < mn) mn = v; }
             tk[k] = mn - dwell / 2;
         }
         tk[nk - 1] = 1;

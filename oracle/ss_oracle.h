@@ -104,6 +104,10 @@ void orc_state_pixels(const orc_state *st, uint8_t *mask, double *x, double *y, 
 int orc_half_iteration_linesearch(orc_state *st, int shift);
 /* Global correlation statistic of the current state (task_correlation.cu:63-99). */
 double orc_global_correlation(orc_state *st);
+/* Per-frame Pearson correlation of the current state's warped template and
+ * image over its masked pixels (hd_frameCorrelation, task_correlation.cu:101-183);
+ * corr has n_frames entries. */
+void orc_frame_correlation(orc_state *st, double *corr);
 
 /* ---- small pure functions ---- */
 /* ParameterOrganizer::organize2D1 / organize1D2 (parameterorganizer.cpp:30-99). */
