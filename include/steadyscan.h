@@ -194,6 +194,10 @@ typedef struct ss_server ss_server;
 typedef struct {
     int64_t hbm_budget_bytes;    /* device memory for admitted groups (0: 64 GiB) */
     int32_t max_groups;          /* groups resident at once (0: 256)            */
+    int32_t reserved;            /* must be 0                                   */
+    int64_t staging_limit_bytes; /* page-locked host staging the server may hold
+                                    (0: no limit); groups whose staging would
+                                    exceed it complete with SS_ERR_OUT_OF_MEMORY */
 } ss_server_config;
 
 ss_status ss_server_create(ss_handle *h, const ss_server_config *cfg, ss_server **out);
@@ -206,11 +210,17 @@ void ss_server_destroy(ss_server *s);
 ss_status ss_server_submit(ss_server *s, const ss_group_in *in, const ss_settings *st,
                            ss_group_out *out, int64_t *ticket);
 /* Admits, advances one tick, retires.  Reports the resident and queued
- * group counts after the step. */
+ * group counts after the step.  A queued group that cannot be admitted (a bad
+ * request, one that can never fit the budget, a failed set-up) completes
+ * with its error and gives back its memory; the step itself fails only when
+ * the device does. */
 ss_status ss_server_step(ss_server *s, int32_t *n_resident, int32_t *n_queued);
 /* Moves up to cap finished tickets (with their status) to the caller;
- * returns how many. */
-int32_t ss_server_completed(ss_server *s, int64_t *tickets, ss_status *status, int32_t cap);
+ * returns how many.  messages (may be NULL) receives each ticket's error
+ * message ("" on success), owned by the server and valid until the next
+ * ss_server_completed call. */
+int32_t ss_server_completed(ss_server *s, int64_t *tickets, ss_status *status, const char **messages,
+                            int32_t cap);
 /* Device bytes reserved by the server's pool and held by resident groups. */
 ss_status ss_server_memory(const ss_server *s, int64_t *reserved, int64_t *in_use);
 
@@ -314,12 +324,15 @@ typedef struct {
     int32_t data_on_device;
     const double *knots_x;       /* nknots x n_frames                              */
     const double *knots_y;
+    int32_t out_on_device;       /* ss_resample only: out is a device pointer      */
 } ss_resample_in;
 
 /* Forward bilinear splat of every frame into template space, normalised by
  * the splatted weights: out is template_h x template_w x n_frames float32,
  * NaN where nothing landed.  Repeated targets keep the last contribution,
- * as MATLAB's indexed assignment does. */
+ * as MATLAB's indexed assignment does.  The frames run on the handle's
+ * stream in launches of as many frames as fit in the handle's work memory
+ * (at least 256 MiB); host frames and host output are staged there. */
 ss_status ss_resample(ss_handle *h, const ss_resample_in *in, float *out);
 
 /* ---- corrected frames: invert_bilinear_sampling (steady_scan.m:290-404) ---- */

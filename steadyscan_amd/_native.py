@@ -81,7 +81,8 @@ class SSDisplacementsOut(ctypes.Structure):
 
 
 class SSServerConfig(ctypes.Structure):
-    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_groups", c_int32)]
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_groups", c_int32), ("reserved", c_int32),
+                ("staging_limit_bytes", ctypes.c_int64)]
 
 
 class SSGeometryDesc(ctypes.Structure):
@@ -106,6 +107,7 @@ class SSResampleIn(ctypes.Structure):
         ("data_on_device", c_int32),
         ("knots_x", POINTER(c_double)),
         ("knots_y", POINTER(c_double)),
+        ("out_on_device", c_int32),
     ]
 
 
@@ -180,7 +182,8 @@ API = [
     ("ss_server_submit", c_int32, [c_void_p, POINTER(SSGroupIn), POINTER(SSSettings), POINTER(SSGroupOut),
                                    POINTER(ctypes.c_int64)]),
     ("ss_server_step", c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
-    ("ss_server_completed", c_int32, [c_void_p, POINTER(ctypes.c_int64), POINTER(c_int32), c_int32]),
+    ("ss_server_completed", c_int32, [c_void_p, POINTER(ctypes.c_int64), POINTER(c_int32), POINTER(c_char_p),
+                                      c_int32]),
     ("ss_server_memory", c_int32, [c_void_p, POINTER(ctypes.c_int64), POINTER(ctypes.c_int64)]),
 ]
 

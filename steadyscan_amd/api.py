@@ -409,11 +409,12 @@ class Server:
     call."""
 
     def __init__(self, handle=None, device=0, hbm_budget_bytes=0, max_groups=0, linesearch_candidates=2,
-                 with_mask=False):
+                 with_mask=False, staging_limit_bytes=0):
         self.h = handle if handle is not None else _default_handle(device)
         cfg = N.SSServerConfig()
         cfg.hbm_budget_bytes = int(hbm_budget_bytes)
         cfg.max_groups = int(max_groups)
+        cfg.staging_limit_bytes = int(staging_limit_bytes)
         self.ptr = c_void_p()
         _check(N.lib().ss_server_create(self.h.ptr, ctypes.byref(cfg), ctypes.byref(self.ptr)), self.h)
         self.K = linesearch_candidates
@@ -440,12 +441,13 @@ class Server:
         cap = 64
         tickets = (ctypes.c_int64 * cap)()
         status = (c_int32 * cap)()
+        messages = (ctypes.c_char_p * cap)()
         while True:
-            n = N.lib().ss_server_completed(self.ptr, tickets, status, cap)
+            n = N.lib().ss_server_completed(self.ptr, tickets, status, messages, cap)
             for i in range(n):
                 gi, s, go, keep, r = self._pending.pop(tickets[i])
                 if status[i] != 0:
-                    out[tickets[i]] = SteadyScanError(status[i], N.lib().ss_last_error(self.h.ptr).decode())
+                    out[tickets[i]] = SteadyScanError(status[i], (messages[i] or b"").decode())
                 else:
                     out[tickets[i]] = _collect(r, go)
             if n < cap:
@@ -689,25 +691,42 @@ def image2blockmatrix(frames, geometry):
     return out
 
 
-def resample_frames(frames, geometry, dx_knots, dy_knots, handle=None, device=0):
+def resample_frames(frames, geometry, dx_knots, dy_knots, device_out=None, handle=None, device=0):
     """resample_frame for every frame (steady_scan.m:562-572): forward splat
-    into template space, float32, NaN where no data landed."""
-    fr = np.asarray(frames)
-    if fr.ndim == 2:
-        fr = fr[:, :, None]
-    if fr.dtype not in _DTYPES:
-        fr = fr.astype(np.float64)
-    fr = np.asfortranarray(fr)
+    into template space, float32, NaN where no data landed.
+
+    frames: nlines x linewidth x n raw frames, or a tuple (device_pointer,
+    dtype, n) for frames resident in HBM (same layout).  With device_out (a
+    device pointer to template_h x template_w x n float32) the result is
+    written there and None is returned."""
     g = geometry["desc"]
-    n = fr.shape[2]
+    ri = N.SSResampleIn()
+    if isinstance(frames, tuple):
+        ptr, dtype, n = frames
+        ri.frames = c_void_p(int(ptr))
+        ri.data_type = _DTYPES[np.dtype(dtype)]
+        ri.data_on_device = 1
+    else:
+        fr = np.asarray(frames)
+        if fr.ndim == 2:
+            fr = fr[:, :, None]
+        if fr.dtype not in _DTYPES:
+            fr = fr.astype(np.float64)
+        fr = np.asfortranarray(fr)
+        if fr.shape[:2] != (g.nlines, g.linewidth):
+            raise ValueError(f"frames are {fr.shape[:2]}, the geometry expects {(g.nlines, g.linewidth)}")
+        n = fr.shape[2]
+        ri.frames = c_void_p(fr.ctypes.data)
+        ri.data_type = _DTYPES[fr.dtype]
     kx, ky = _f64(np.reshape(dx_knots, (g.nknots, n), order="F")), _f64(np.reshape(dy_knots, (g.nknots, n), order="F"))
     h = handle if handle is not None else _default_handle(device)
-    ri = N.SSResampleIn()
     ri.geometry = ctypes.pointer(g)
     ri.n_frames = n
-    ri.data_type = _DTYPES[fr.dtype]
-    ri.frames = c_void_p(fr.ctypes.data)
     ri.knots_x, ri.knots_y = _ptr(kx, c_double), _ptr(ky, c_double)
+    if device_out is not None:
+        ri.out_on_device = 1
+        _check(N.lib().ss_resample(h.ptr, byref(ri), ctypes.cast(c_void_p(int(device_out)), POINTER(c_float))), h)
+        return None
     out = np.zeros((g.template_h, g.template_w, n), dtype=np.float32, order="F")
     _check(N.lib().ss_resample(h.ptr, byref(ri), out.ctypes.data_as(POINTER(c_float))), h)
     return out

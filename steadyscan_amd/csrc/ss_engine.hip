@@ -841,13 +841,13 @@ ss_status upload_table(ss_handle *h, Table &T, std::vector<ss::GroupDev> &gd, co
                  oF = A.take<int>(nFrames);
     if (A.off > hstageBytes) return fail(h, SS_ERR_OUT_OF_MEMORY, "group table staging too small");
     if (T.bytes < A.off) {
+        const size_t want = std::max(A.off, T.bytes * 2);
         if (T.mem) {
             HIPCHK(hipStreamSynchronize(h->stream));
             HIPCHK(hipFree(T.mem));
             T.mem = nullptr;
             T.bytes = 0;
         }
-        const size_t want = std::max(A.off, T.bytes * 2);
         if (hipMalloc(&T.mem, want) != hipSuccess) return fail(h, SS_ERR_OUT_OF_MEMORY, "unable to allocate device memory");
         T.bytes = want;
     }
@@ -1091,12 +1091,19 @@ struct ss_server {
         ss::GroupCtl c;
         double admitTime = 0;
     };
+    struct Done {
+        int64_t ticket;
+        ss_status status;
+        std::string message;  // empty on success
+    };
     std::deque<Request> queue;
     std::vector<Slot> resident;
-    std::vector<std::pair<int64_t, ss_status>> completed;
+    std::vector<Done> completed;
+    std::vector<Done> reported;  // the last ss_server_completed's entries (their messages)
     Table table;
     void *stage = nullptr;  // pinned: group staging + table
     size_t stageBytes = 0;
+    size_t stageLimit = 0;  // 0: no limit
     int64_t nextTicket = 1;
     bool dirty = true;      // membership changed since the table was uploaded
     ss::LaunchBatch L{};
@@ -1105,28 +1112,43 @@ struct ss_server {
 
 namespace {
 
-ss_status server_stage(ss_server *S, size_t bytes) {
+// Grows the page-locked staging to at least `bytes`.  The limit applies to
+// the staging of admitted groups (`groups`), not to the group table's.
+ss_status server_stage(ss_server *S, size_t bytes, bool groups) {
     ss_handle *h = S->h;
     if (S->stageBytes >= bytes) return SS_OK;
+    if (groups && S->stageLimit && bytes > S->stageLimit)
+        return fail(h, SS_ERR_OUT_OF_MEMORY, "the group's page-locked staging would exceed the server's limit");
+    size_t want = std::max(bytes, S->stageBytes * 2);
+    if (groups && S->stageLimit) want = std::min(want, S->stageLimit);
     if (S->stage) {
         HIPCHK(hipStreamSynchronize(h->stream));
         HIPCHK(hipHostFree(S->stage));
         S->stage = nullptr;
         S->stageBytes = 0;
     }
-    const size_t want = std::max(bytes, S->stageBytes * 2);
     if (hipHostMalloc(&S->stage, want, hipHostMallocDefault) != hipSuccess)
         return fail(h, SS_ERR_OUT_OF_MEMORY, "unable to allocate pinned host memory");
     S->stageBytes = want;
     return SS_OK;
 }
 
+// Completes a ticket that never ran (a bad request, no memory, or a failed
+// set-up) with its status and message.
+void complete_failed(ss_server *S, int64_t ticket, ss_status st, const std::string &msg) {
+    S->completed.push_back({ticket, st, msg.empty() ? std::string("request failed") : msg});
+}
+
 // Admits queued requests in order while slots and the memory budget allow.
+// A request joins the resident set only once its device block is set up; a
+// request that cannot be set up completes with its error and returns its
+// memory, so the group table never holds a half-admitted group.  The
+// return value is an error only when the device itself failed.
 ss_status server_admit(ss_server *S) {
     ss_handle *h = S->h;
-    std::vector<size_t> newSlots;
+    std::vector<ss_server::Slot> fresh;
     size_t hNeed = 0;
-    while (!S->queue.empty() && (int)S->resident.size() < S->maxGroups) {
+    while (!S->queue.empty() && (int)(S->resident.size() + fresh.size()) < S->maxGroups) {
         ss_server::Slot slot;
         slot.req = S->queue.front();
         slot.P.in = &slot.req.in;
@@ -1134,47 +1156,70 @@ ss_status server_admit(ss_server *S) {
         slot.P.out = slot.req.out;
         ss_status st = plan_group(h, slot.P);
         if (st) {  // a bad request completes with its error; the others go on
-            S->completed.push_back({slot.req.ticket, st});
+            complete_failed(S, slot.req.ticket, st, h->err);
             S->queue.pop_front();
             continue;
         }
+        const size_t hb = (slot.P.hBytes + 255) & ~(size_t)255;
+        if (S->stageLimit && hNeed + hb > S->stageLimit) {
+            if (fresh.empty()) {  // its staging alone exceeds the limit
+                complete_failed(S, slot.req.ticket, SS_ERR_OUT_OF_MEMORY,
+                                "the group's page-locked staging would exceed the server's limit");
+                S->queue.pop_front();
+                continue;
+            }
+            break;  // admitted at a later step
+        }
         slot.mem = S->pool.alloc(slot.P.dBytes);
         if (!slot.mem) {
-            if (S->resident.empty() && newSlots.empty()) {  // can never fit
-                S->completed.push_back({slot.req.ticket, SS_ERR_OUT_OF_MEMORY});
+            if (S->resident.empty() && fresh.empty()) {  // can never fit
+                complete_failed(S, slot.req.ticket, SS_ERR_OUT_OF_MEMORY,
+                                "the group does not fit in the server's HBM budget");
                 S->queue.pop_front();
                 continue;
             }
             break;  // wait for memory to free up
         }
         S->queue.pop_front();
-        hNeed += (slot.P.hBytes + 255) & ~(size_t)255;
-        S->resident.push_back(std::move(slot));
-        newSlots.push_back(S->resident.size() - 1);
+        hNeed += hb;
+        fresh.push_back(std::move(slot));
     }
-    if (newSlots.empty()) return SS_OK;
-    // the table is staged after the groups
-    int nItems = 0, nFrames = 0;
-    for (auto &r : S->resident) {
-        nItems += r.req.in.n_blocks * r.req.in.n_frames;
-        nFrames += r.req.in.n_frames;
+    if (fresh.empty()) return SS_OK;
+    auto drop = [&](ss_server::Slot &slot, ss_status st) {
+        S->pool.release(slot.mem, slot.P.dBytes);
+        slot.mem = nullptr;
+        complete_failed(S, slot.req.ticket, st, h->err);
+    };
+    ss_status st = server_stage(S, hNeed, true);
+    if (st) {
+        for (auto &slot : fresh) drop(slot, st);
+        return SS_OK;
     }
-    const size_t tb = table_bytes((int)S->resident.size(), nItems, nFrames);
-    ss_status st = server_stage(S, hNeed + tb + 256);
-    if (st) return st;
     char *HS = (char *)S->stage;
-    for (size_t k : newSlots) {
-        auto &slot = S->resident[k];
-        // the Plan points at the slot's own request copy
-        slot.P.in = &slot.req.in;
+    std::vector<bool> ok(fresh.size(), false);
+    for (size_t k = 0; k < fresh.size(); k++) {
+        auto &slot = fresh[k];
+        slot.P.in = &slot.req.in;  // the Plan points at the slot's own request copy
         slot.P.s = &slot.req.s;
         st = setup_group(h, slot.P, slot.mem, HS, slot.g, slot.c);
-        if (st) return st;
+        if (st) {
+            drop(slot, st);
+            continue;
+        }
+        ok[k] = true;
         HS += (slot.P.hBytes + 255) & ~(size_t)255;
     }
     // staged uploads must land before the staging is reused for the table
     HIPCHK(hipStreamSynchronize(h->stream));
-    S->dirty = true;
+    for (size_t k = 0; k < fresh.size(); k++) {
+        if (!ok[k]) continue;
+        S->resident.push_back(std::move(fresh[k]));
+        S->dirty = true;
+    }
+    for (auto &slot : S->resident) {  // plans point into the moved requests
+        slot.P.in = &slot.req.in;
+        slot.P.s = &slot.req.s;
+    }
     return SS_OK;
 }
 
@@ -1190,6 +1235,7 @@ ss_status ss_server_create(ss_handle *h, const ss_server_config *cfg, ss_server 
     const int64_t budget = (cfg && cfg->hbm_budget_bytes > 0) ? cfg->hbm_budget_bytes : ((int64_t)64 << 30);
     S->pool.budget = (size_t)budget;
     S->maxGroups = (cfg && cfg->max_groups > 0) ? cfg->max_groups : 256;
+    S->stageLimit = (cfg && cfg->staging_limit_bytes > 0) ? (size_t)cfg->staging_limit_bytes : 0;
     *out = S;
     return SS_OK;
 } catch (const std::bad_alloc &) {
@@ -1252,7 +1298,7 @@ ss_status ss_server_step(ss_server *S, int32_t *n_resident, int32_t *n_queued) t
                 nFrames += g.nF;
             }
             const size_t tb = table_bytes(n, nItems, nFrames);
-            st = server_stage(S, tb);
+            st = server_stage(S, tb, false);
             if (st) return st;
             st = upload_table(h, S->table, gd, S->hc.data(), S->stage, S->stageBytes, S->L);
             if (st) return st;
@@ -1272,7 +1318,7 @@ ss_status ss_server_step(ss_server *S, int32_t *n_resident, int32_t *n_queued) t
                 slot.P.s = &slot.req.s;
                 const ss_status fs = finish_group(h, slot.P, slot.g, slot.c, 0.0);
                 S->pool.release(slot.mem, slot.P.dBytes);
-                S->completed.push_back({slot.req.ticket, fs});
+                S->completed.push_back({slot.req.ticket, fs, fs ? h->err : std::string()});
                 S->dirty = true;
             } else {
                 keep.push_back(std::move(slot));
@@ -1293,14 +1339,17 @@ ss_status ss_server_step(ss_server *S, int32_t *n_resident, int32_t *n_queued) t
     return SS_ERR_DEVICE;
 }
 
-int32_t ss_server_completed(ss_server *S, int64_t *tickets, ss_status *status, int32_t cap) {
+int32_t ss_server_completed(ss_server *S, int64_t *tickets, ss_status *status, const char **messages,
+                            int32_t cap) {
     if (!S || cap <= 0) return 0;
     const int32_t n = std::min<int32_t>(cap, (int32_t)S->completed.size());
-    for (int32_t i = 0; i < n; i++) {
-        if (tickets) tickets[i] = S->completed[i].first;
-        if (status) status[i] = S->completed[i].second;
-    }
+    S->reported.assign(S->completed.begin(), S->completed.begin() + n);
     S->completed.erase(S->completed.begin(), S->completed.begin() + n);
+    for (int32_t i = 0; i < n; i++) {
+        if (tickets) tickets[i] = S->reported[i].ticket;
+        if (status) status[i] = S->reported[i].status;
+        if (messages) messages[i] = S->reported[i].message.c_str();
+    }
     return n;
 }
 

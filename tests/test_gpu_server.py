@@ -84,3 +84,46 @@ def test_server_admits_by_memory_budget():
     reserved, in_use = srv.memory()
     assert reserved <= 6 << 20 and in_use == 0
     srv.close()
+
+
+def test_server_request_that_cannot_be_staged_completes_with_its_error():
+    # A page-locked staging limit that host frames cannot fit in: that group
+    # completes with SS_ERR_OUT_OF_MEMORY and its own message, never joins the
+    # resident set, and leaves the pool as it was; a group with device-resident
+    # frames (a few KB of staging) in the same step runs normally.
+    from devbuf import DeviceBuffer
+    calls = make_calls()[:2]
+    singles = [ssa.align_groups([a], with_mask=False)[0] for a in calls]
+    bm = calls[1]["bmData"]
+    d = DeviceBuffer(bm.nbytes)
+    d.upload(bm.ravel(order="F"))
+    dev_call = dict(calls[1], bmData=(d.ptr.value, bm.dtype))
+    srv = ssa.Server(staging_limit_bytes=64 << 10)
+    assert calls[0]["bmData"].nbytes > 64 << 10
+    t_host = srv.submit(calls[0])
+    t_dev = srv.submit(dev_call)
+    results = {}
+    nr0 = None
+    for _ in range(10000):
+        nr, nq = srv.step()
+        nr0 = nr if nr0 is None else nr0
+        results.update(srv.completed())
+        if len(results) == 2:
+            break
+    assert nr0 == 1  # only the device-frame group was admitted
+    err = results[t_host]
+    assert isinstance(err, ssa.SteadyScanError) and err.status == 2
+    assert "staging" in str(err)
+    same(results[t_dev], singles[1])
+    reserved, in_use = srv.memory()
+    assert in_use == 0
+    # the server goes on: a later request that fits is admitted and runs
+    t2 = srv.submit(dev_call)
+    for _ in range(10000):
+        srv.step()
+        r = srv.completed()
+        if r:
+            same(r[t2], singles[1])
+            break
+    srv.close()
+    d.free()
