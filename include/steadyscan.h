@@ -335,6 +335,21 @@ typedef struct {
  * (at least 256 MiB); host frames and host output are staged there. */
 ss_status ss_resample(ss_handle *h, const ss_resample_in *in, float *out);
 
+/* ss_resample without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Frames and output must be
+ * device memory (data_on_device and out_on_device set); knots stay host
+ * arrays.  Each pixel's geometry is computed inside the kernels, and the knots
+ * and winner tables live in a static device work area of 64 MiB per device,
+ * held from library load until the process ends.  A launch takes as many
+ * frames as fit there: one frame needs 16 bytes per template pixel, so
+ * templates up to about 4.1 million pixels are accepted and larger ones are
+ * rejected with SS_ERR_INVALID_ARGUMENT.  All calls on one device share that
+ * area: calls that overlap on one GPU (from several threads or handles) race
+ * and must be serialised by the caller.  Results equal ss_resample's bit for
+ * bit. */
+ss_status ss_resample_noalloc(ss_handle *h, const ss_resample_in *in, float *out);
+
 /* ---- corrected frames: invert_bilinear_sampling (steady_scan.m:290-404) ---- */
 typedef struct {
     double regfac;          /* settings.ibs_regfac (3e-3)                          */

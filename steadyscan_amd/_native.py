@@ -159,6 +159,7 @@ API = [
                                    POINTER(c_uint8), POINTER(c_int32)]),
     ("ss_pack_frames", c_int32, [POINTER(SSGeometryDesc), c_int32, c_void_p, c_int32, c_void_p]),
     ("ss_resample", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(ctypes.c_float)]),
+    ("ss_resample_noalloc", c_int32, [c_void_p, POINTER(SSResampleIn), POINTER(ctypes.c_float)]),
     ("ss_ibs_settings_default", None, [POINTER(SSIbsSettings)]),
     ("ss_prefilter", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_double, c_int32,
                                c_int32]),

@@ -691,14 +691,16 @@ def image2blockmatrix(frames, geometry):
     return out
 
 
-def resample_frames(frames, geometry, dx_knots, dy_knots, device_out=None, handle=None, device=0):
+def resample_frames(frames, geometry, dx_knots, dy_knots, device_out=None, handle=None, device=0, noalloc=False):
     """resample_frame for every frame (steady_scan.m:562-572): forward splat
     into template space, float32, NaN where no data landed.
 
     frames: nlines x linewidth x n raw frames, or a tuple (device_pointer,
     dtype, n) for frames resident in HBM (same layout).  With device_out (a
     device pointer to template_h x template_w x n float32) the result is
-    written there and None is returned."""
+    written there and None is returned.  noalloc=True calls
+    ss_resample_noalloc, which allocates nothing; it needs device frames and
+    device_out."""
     g = geometry["desc"]
     ri = N.SSResampleIn()
     if isinstance(frames, tuple):
@@ -723,9 +725,12 @@ def resample_frames(frames, geometry, dx_knots, dy_knots, device_out=None, handl
     ri.geometry = ctypes.pointer(g)
     ri.n_frames = n
     ri.knots_x, ri.knots_y = _ptr(kx, c_double), _ptr(ky, c_double)
+    if noalloc and (device_out is None or not ri.data_on_device):
+        raise ValueError("noalloc needs frames and output in device memory")
     if device_out is not None:
         ri.out_on_device = 1
-        _check(N.lib().ss_resample(h.ptr, byref(ri), ctypes.cast(c_void_p(int(device_out)), POINTER(c_float))), h)
+        fn = N.lib().ss_resample_noalloc if noalloc else N.lib().ss_resample
+        _check(fn(h.ptr, byref(ri), ctypes.cast(c_void_p(int(device_out)), POINTER(c_float))), h)
         return None
     out = np.zeros((g.template_h, g.template_w, n), dtype=np.float32, order="F")
     _check(N.lib().ss_resample(h.ptr, byref(ri), out.ctypes.data_as(POINTER(c_float))), h)

@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 INCLUDE = os.path.join(ROOT, "include")
 CSRC = os.path.join(HERE, "csrc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 LIB = os.path.join(HERE, "libsteadyscan_amd.so")
 SYNTH_LIB = os.path.join(HERE, "libss_synth.so")
@@ -65,7 +66,10 @@ def build_hip(verbose=False):
         objs.append(obj)
     for src in CXX_SOURCES:
         obj = os.path.join(objdir, src + ".o")
-        _run([shutil.which("g++") or "g++", "-O2", "-std=c++17", "-fPIC", "-I", INCLUDE, "-c",
+        # host C++ that shares __host__ __device__ formulas with the kernels
+        # (ss_scan.h) sees the HIP headers in host-only form
+        _run([shutil.which("g++") or "g++", "-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__",
+              "-I", os.path.join(ROCM, "include"), "-I", INCLUDE, "-I", CSRC, "-c",
               os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", LIB + ".tmp"], verbose)

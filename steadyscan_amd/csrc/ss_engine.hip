@@ -1421,6 +1421,9 @@ ss_status ss_create(int32_t device_id, ss_handle **out) try {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return SS_ERR_NOT_AVAILABLE;
     if (device_id < 0 || device_id >= n) return SS_ERR_INVALID_ARGUMENT;
     if (hipSetDevice(device_id) != hipSuccess) return SS_ERR_DEVICE;
+    // the allocation-free resampling's static work area on this device (its
+    // entry point reports a failure here)
+    (void)ss::resample_scratch_init(device_id);
     ss_handle *h = new ss_handle();
     h->device = device_id;
     if (const char *g = std::getenv("SS_TICK_GRAPH")) h->useGraph = std::atoi(g) != 0;

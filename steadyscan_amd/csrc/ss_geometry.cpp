@@ -8,107 +8,88 @@
 #include <cstring>
 #include <vector>
 
+#include "ss_scan.h"
 #include "steadyscan.h"
+
+namespace ss {
+
+bool scan_setup(const ss_geometry_desc &g, ScanParams &s, double *kt) {
+    s.H = g.nlines;
+    s.W = g.linewidth;
+    s.cl = g.cropping[0];
+    s.cr = g.cropping[1];
+    s.cb = g.cropping[2];
+    s.ct = g.cropping[3];
+    s.Hc = s.H - s.cb - s.ct;
+    s.Wc = s.W - s.cl - s.cr;
+    s.nknots = g.nknots;
+    s.bidirectional = g.bidirectional != 0;
+    if (s.H < 1 || s.W < 1 || s.Hc < 1 || s.Wc < 1 || s.nknots < 2) return false;
+    if (s.cl < 0 || s.cr < 0 || s.cb < 0 || s.ct < 0) return false;
+    const int nknots = s.nknots;
+    for (int k = 0; k < nknots; k++) kt[k] = 0.0;
+    const double dwell = 1.0 / ((double)s.H * (double)s.W);
+    const double lines_per_knot = (double)s.Hc / (double)(nknots - 1);
+    if (lines_per_knot == std::floor(lines_per_knot)) {
+        // knots at the first pixel of every lines_per_knot-th raw line
+        // (steady_scan.m:801-804)
+        const int L = (int)lines_per_knot;
+        for (int k = 0; k + 1 < nknots; k++) {
+            double mn = scan_pixel_time(s, k * L, 0);
+            for (int j = 1; j < s.W; j++) mn = std::fmin(mn, scan_pixel_time(s, k * L, j));
+            kt[k] = mn - dwell / 2;
+        }
+        kt[nknots - 1] = 1;
+        kt[0] = 0;
+    } else {
+        // equal spacing over the cropped scan (steady_scan.m:805-811)
+        double lo = scan_cropped_time(s, 0, 0), hi = lo;
+        for (int i = 0; i < s.Hc; i++)
+            for (int j = 0; j < s.Wc; j++) {
+                const double t = scan_cropped_time(s, i, j);
+                lo = std::fmin(lo, t);
+                hi = std::fmax(hi, t);
+            }
+        for (int k = 0; k < nknots; k++) kt[k] = matlab_linspace(lo - dwell / 2, hi + dwell / 2, nknots, k);
+        kt[0] = 0;
+        kt[nknots - 1] = 1;
+    }
+    // subrec: given, the frame's own extent when it matches the template
+    // (steady_scan.m:777-778), else the whole template (steady_scan.m:781)
+    if (g.has_subrec) {
+        for (int i = 0; i < 4; i++) s.subrec[i] = g.subrec[i];
+    } else if (s.Hc == g.template_h && s.Wc == g.template_w) {
+        s.subrec[0] = 1 - g.cropping[0];
+        s.subrec[1] = g.template_w + g.cropping[1];
+        s.subrec[2] = 1 - g.cropping[2];
+        s.subrec[3] = g.template_h + g.cropping[3];
+    } else {
+        s.subrec[0] = 1;
+        s.subrec[1] = g.template_w;
+        s.subrec[2] = 1;
+        s.subrec[3] = g.template_h;
+    }
+    return true;
+}
+
+}  // namespace ss
 
 namespace {
 
-// MATLAB linspace(a, b, n): a + (0:n-1).*(b-a)/(n-1) with both ends exact.
-double matlab_linspace(double a, double b, int n, int i) {
-    if (n == 1 || i == n - 1) return b;
-    if (i == 0) return a;
-    return a + ((double)i * (b - a)) / (double)(n - 1);
-}
+using ss::ScanParams;
 
 struct ScanTiming {
-    int H = 0, W = 0, Hc = 0, Wc = 0, nknots = 0;
-    int c_left = 0, c_right = 0, c_bottom = 0, c_top = 0;
-    bool bidirectional = false;
+    ScanParams s{};
     std::vector<double> knot_time;
 
-    // Normalised time of raw pixel (line i, column j), 0-based
-    // (pixelscantime, steady_scan.m:790-795).
-    double pixel_time(int i, int j) const {
-        const double npix = (double)H * (double)W;
-        const int jj = (bidirectional && (i % 2 == 1)) ? (W - 1 - j) : j;
-        return (1.0 / npix) / 2 + (double)((long long)i * W + jj) / npix;
-    }
-    // Time of cropped pixel (i, j).
-    double cropped_time(int i, int j) const { return pixel_time(i + c_bottom, j + c_left); }
-
     bool build(const ss_geometry_desc &g) {
-        H = g.nlines;
-        W = g.linewidth;
-        c_left = g.cropping[0];
-        c_right = g.cropping[1];
-        c_bottom = g.cropping[2];
-        c_top = g.cropping[3];
-        Hc = H - c_bottom - c_top;
-        Wc = W - c_left - c_right;
-        nknots = g.nknots;
-        bidirectional = g.bidirectional != 0;
-        if (H < 1 || W < 1 || Hc < 1 || Wc < 1 || nknots < 2) return false;
-        if (c_left < 0 || c_right < 0 || c_bottom < 0 || c_top < 0) return false;
-        knot_time.assign(nknots, 0.0);
-        const double dwell = 1.0 / ((double)H * (double)W);
-        const double lines_per_knot = (double)Hc / (double)(nknots - 1);
-        if (lines_per_knot == std::floor(lines_per_knot)) {
-            // knots at the first pixel of every lines_per_knot-th raw line
-            // (steady_scan.m:801-804)
-            const int L = (int)lines_per_knot;
-            for (int k = 0; k + 1 < nknots; k++) {
-                double mn = pixel_time(k * L, 0);
-                for (int j = 1; j < W; j++) mn = std::fmin(mn, pixel_time(k * L, j));
-                knot_time[k] = mn - dwell / 2;
-            }
-            knot_time[nknots - 1] = 1;
-            knot_time[0] = 0;
-        } else {
-            // equal spacing over the cropped scan (steady_scan.m:805-811)
-            double lo = cropped_time(0, 0), hi = lo;
-            for (int i = 0; i < Hc; i++)
-                for (int j = 0; j < Wc; j++) {
-                    const double t = cropped_time(i, j);
-                    lo = std::fmin(lo, t);
-                    hi = std::fmax(hi, t);
-                }
-            for (int k = 0; k < nknots; k++)
-                knot_time[k] = matlab_linspace(lo - dwell / 2, hi + dwell / 2, nknots, k);
-            knot_time[0] = 0;
-            knot_time[nknots - 1] = 1;
-        }
-        return true;
+        if (g.nknots < 2) return false;
+        knot_time.assign(g.nknots, 0.0);
+        return ss::scan_setup(g, s, knot_time.data());
     }
-
-    // histc bin of t against the knot times (1-based; 0 outside).
-    int bin(double t) const {
-        const int n = nknots;
-        if (!(t >= knot_time[0]) || t > knot_time[n - 1]) return 0;
-        if (t == knot_time[n - 1]) return n;
-        int lo = 0, hi = n - 1;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (knot_time[mid] <= t) lo = mid;
-            else hi = mid;
-        }
-        return lo + 1;
-    }
+    double cropped_time(int i, int j) const { return ss::scan_cropped_time(s, i, j); }
+    int bin(double t) const { return ss::scan_bin(knot_time.data(), s.nknots, t); }
 };
-
-void subrec_of(const ss_geometry_desc &g, int Hc, int Wc, double sr[4]) {
-    if (g.has_subrec) {
-        for (int i = 0; i < 4; i++) sr[i] = g.subrec[i];
-    } else if (Hc == g.template_h && Wc == g.template_w) {  // steady_scan.m:777-778
-        sr[0] = 1 - g.cropping[0];
-        sr[1] = g.template_w + g.cropping[1];
-        sr[2] = 1 - g.cropping[2];
-        sr[3] = g.template_h + g.cropping[3];
-    } else {  // steady_scan.m:781
-        sr[0] = 1;
-        sr[1] = g.template_w;
-        sr[2] = 1;
-        sr[3] = g.template_h;
-    }
-}
 
 }  // namespace
 
@@ -139,18 +120,18 @@ extern "C" ss_status ss_geometry_size(const ss_geometry_desc *g, int32_t *hc, in
     if (!st.build(*g)) return SS_ERR_INVALID_ARGUMENT;
     // The default basey is built over the cropped line count and cropped
     // again (steady_scan.m:799, 822-824): vertical cropping mis-sizes it.
-    if (st.c_bottom + st.c_top != 0) return SS_ERR_INVALID_ARGUMENT;
-    std::vector<int> count(st.nknots - 1, 0);
-    for (int j = 0; j < st.Wc; j++)
-        for (int i = 0; i < st.Hc; i++) {
+    if (st.s.cb + st.s.ct != 0) return SS_ERR_INVALID_ARGUMENT;
+    std::vector<int> count(st.s.nknots - 1, 0);
+    for (int j = 0; j < st.s.Wc; j++)
+        for (int i = 0; i < st.s.Hc; i++) {
             const int k = st.bin(st.cropped_time(i, j));
-            if (k <= 0 || k >= st.nknots) return SS_ERR_INVALID_ARGUMENT;  // steady_scan.m:814
+            if (k <= 0 || k >= st.s.nknots) return SS_ERR_INVALID_ARGUMENT;  // steady_scan.m:814
             count[k - 1]++;
         }
     int fh = 0;
     for (int c : count) fh = c > fh ? c : fh;
-    if (hc) *hc = st.Hc;
-    if (wc) *wc = st.Wc;
+    if (hc) *hc = st.s.Hc;
+    if (wc) *wc = st.s.Wc;
     if (frame_h) *frame_h = fh;
     return SS_OK;
 }
@@ -164,10 +145,8 @@ extern "C" ss_status ss_geometry_init(const ss_geometry_desc *g, int32_t *prevkn
     if (rc != SS_OK) return rc;
     ScanTiming st;
     st.build(*g);
-    double sr[4];
-    subrec_of(*g, Hc, Wc, sr);
-    const int nb = st.nknots - 1;
-    std::vector<int> fill(nb, 0), per_knot(st.nknots, 0);
+    const int nb = st.s.nknots - 1;
+    std::vector<int> fill(nb, 0), per_knot(st.s.nknots, 0);
     const size_t bm = (size_t)fh * nb;
     if (frac_bm) std::memset(frac_bm, 0, sizeof(double) * bm);
     if (base_x_bm) std::memset(base_x_bm, 0, sizeof(double) * bm);
@@ -177,12 +156,12 @@ extern "C" ss_status ss_geometry_init(const ss_geometry_desc *g, int32_t *prevkn
     for (int j = 0; j < Wc; j++) {
         // meshgrid(linspace(0,1,linewidth)*diff(subrec(1:2)) + subrec(1), ...)
         // cropped by columns (steady_scan.m:820-823)
-        const double bx = matlab_linspace(0, 1, st.W, j + st.c_left) * (sr[1] - sr[0]) + sr[0];
+        const double bx = ss::scan_base_x(st.s, j);
         for (int i = 0; i < Hc; i++) {
-            const double by = matlab_linspace(0, 1, Hc, i) * (sr[3] - sr[2]) + sr[2];
+            const double by = ss::scan_base_y(st.s, i);
             const double t = st.cropped_time(i, j);
             const int k = st.bin(t);
-            const double tf = (t - st.knot_time[k - 1]) / (st.knot_time[k] - st.knot_time[k - 1]);
+            const double tf = ss::scan_tfrac(st.knot_time.data(), k, t);
             const size_t o = (size_t)j * Hc + i;
             if (prevknotind) prevknotind[o] = k;
             if (tfrac) tfrac[o] = tf;
@@ -199,7 +178,7 @@ extern "C" ss_status ss_geometry_init(const ss_geometry_desc *g, int32_t *prevkn
     if (min_points) {
         // histc(prevknotind, 1:nknots) * ratio, truncated toward zero by the
         // gateway's int conversion (steady_scan.m:174-175, mexconverter.cpp:40-59)
-        for (int k = 0; k < st.nknots; k++)
+        for (int k = 0; k < st.s.nknots; k++)
             min_points[k] = (int32_t)((double)per_knot[k] * g->min_blockpoints_ratio);
     }
     return SS_OK;
@@ -217,7 +196,7 @@ extern "C" ss_status ss_pack_frames(const ss_geometry_desc *g, ss_dtype type, co
     if (!frames || !out || n < 1) return SS_ERR_INVALID_ARGUMENT;
     ScanTiming st;
     st.build(*g);
-    const int nb = st.nknots - 1;
+    const int nb = st.s.nknots - 1;
     size_t esz = type == SS_F64 ? 8 : type == SS_F32 ? 4 : 2;
     std::vector<int> bin_of((size_t)Hc * Wc);
     for (int j = 0; j < Wc; j++)

@@ -20,5 +20,9 @@ hipStream_t handle_stream(const ss_handle *h);
 void *handle_scratch(ss_handle *h, size_t bytes, ss_status *st);
 // Bytes of work memory the handle holds now.
 size_t handle_scratch_held(const ss_handle *h);
+// Looks up the device address of ss_resample_noalloc's static work area on
+// `device` (which must be current), once per device; later calls return the
+// first result.
+hipError_t resample_scratch_init(int device);
 
 }  // namespace ss
