@@ -80,8 +80,12 @@ typedef struct {
     int32_t linesearch_candidates;           /* fused candidates in a half-iteration's first
                                                 line-search round: 1, 2 or 4 (0 = 2); later
                                                 rounds use 4.  Results do not depend on it. */
-    int32_t template_bf16;                   /* 1: keep the template (and gradients) as BF16
-                                                pairs, FP32 arithmetic; precision 'S' only.
+    int32_t template_bf16;                   /* 1: mixed precision, precision 'S' only: the
+                                                template's gradients (the Jacobian factors)
+                                                are stored as BF16 pairs, residuals sample the
+                                                FP32 template, sums are FP32 per thread and
+                                                FP64 across threads, and every knot update
+                                                gets one FP64 iterative-refinement step.
                                                 0 (default): the precision's own width.    */
 } ss_settings;
 

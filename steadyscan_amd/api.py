@@ -192,7 +192,9 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     (block_dx_knots, block_dy_knots, frame_corr, block_iter_used, block_mask,
      residrms, deviceInitTime, iterationCorrelations, ..., allParams).
     `info`, if a dict, receives the stop criterion and work counters.
-    `template_bf16` keeps the template as BF16 (precision 'S' only).
+    `template_bf16` selects mixed precision (precision 'S' only): BF16
+    template gradients for the Jacobian, FP32 residuals, one FP64
+    iterative-refinement step of each knot update.
     """
     if nargout < 1 or nargout > 15:
         raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, "invalid number of lhs arguments specified")

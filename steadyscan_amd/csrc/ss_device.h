@@ -83,8 +83,9 @@ struct GroupDev {
     // template: pairs (T[t], T[t+1]) row-major; gradient pairs unless derived
     const float2 *tp, *gxv, *gyh;
     const double2 *tpD, *gxvD, *gyhD;
-    const uint32_t *tpB, *gxvB, *gyhB;  // BF16 pairs (FP32 arithmetic only)
-    int tplBF16;          // read the template from the BF16 pairs
+    const uint32_t *gxvB, *gyhB;  // BF16 gradient pairs (mixed precision, FP32 arithmetic)
+    int gradBF16;         // Jacobian gradients from the BF16 pairs
+    int refine;           // one FP64 iterative-refinement step of the knot update
     int gradFD;           // gradients are forward differences of the template
     // frame data, frameH x fframeW column-contiguous
     const void *img;

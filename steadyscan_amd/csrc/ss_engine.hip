@@ -34,7 +34,7 @@ struct DevTemplate {
     int th = 0, tw = 0, gradFD = 1;
     float2 *tp = nullptr, *gxv = nullptr, *gyh = nullptr;
     double2 *tpD = nullptr, *gxvD = nullptr, *gyhD = nullptr;
-    uint32_t *tpB = nullptr, *gxvB = nullptr, *gyhB = nullptr;
+    uint32_t *gxvB = nullptr, *gyhB = nullptr;
     void *mem = nullptr;  // owning allocation (resident only)
 };
 
@@ -53,7 +53,7 @@ struct DevGeometry {
 struct TemplateHost {
     std::vector<float2> tp, gxv, gyh;
     std::vector<double2> tpD, gxvD, gyhD;
-    std::vector<uint32_t> tpB, gxvB, gyhB;
+    std::vector<uint32_t> gxvB, gyhB;
 };
 
 // Round to nearest even BF16 (through FP32), quiet NaN kept.
@@ -66,19 +66,15 @@ uint32_t to_bf16(double v) {
     return u >> 16;
 }
 
-// Template pairs as packed BF16: low half T[t], high half T[t+1].
-void template_pairs_bf16(int th, int tw, const double *t, const double *xg, const double *yg, bool fd,
-                         std::vector<uint32_t> &tp, std::vector<uint32_t> &gxv, std::vector<uint32_t> &gyh) {
+// Gradient pairs as packed BF16 (the Jacobian factors of mixed precision):
+// gxv low half xgrad[iy][ix], high half xgrad[iy+1][ix]; gyh low half
+// ygrad[iy][ix], high half ygrad[iy][ix+1].  Always built, also when the
+// gradients are the template's forward differences.
+void gradient_pairs_bf16(int th, int tw, const double *xg, const double *yg, std::vector<uint32_t> &gxv,
+                         std::vector<uint32_t> &gyh) {
     const size_t n = (size_t)th * tw;
     auto at = [&](const double *a, int iy, int ix) { return a[(size_t)ix * th + iy]; };
     auto pack = [](double lo, double hi) { return to_bf16(lo) | (to_bf16(hi) << 16); };
-    tp.assign(n, 0u);
-    for (int iy = 0; iy < th; iy++)
-        for (int ix = 0; ix < tw; ix++)
-            tp[(size_t)iy * tw + ix] = pack(at(t, iy, ix), ix + 1 < tw ? at(t, iy, ix + 1) : 0.0);
-    gxv.clear();
-    gyh.clear();
-    if (fd) return;
     gxv.assign(n, 0u);
     gyh.assign(n, 0u);
     for (int iy = 0; iy < th; iy++)
@@ -294,7 +290,7 @@ void convert_template(int th, int tw, const double *t, const double *xg, const d
     gradFD = fd ? 1 : 0;
     if (wantF) template_pairs<float2, float>(th, tw, t, xg, yg, fd, out.tp, out.gxv, out.gyh);
     if (wantD) template_pairs<double2, double>(th, tw, t, xg, yg, fd, out.tpD, out.gxvD, out.gyhD);
-    if (wantB) template_pairs_bf16(th, tw, t, xg, yg, fd, out.tpB, out.gxvB, out.gyhB);
+    if (wantB) gradient_pairs_bf16(th, tw, xg, yg, out.gxvB, out.gyhB);
 }
 
 // Geometry block matrices -> affine description (when the base coordinates
@@ -488,7 +484,7 @@ ss_status plan_group(ss_handle *h, Plan &P) {
     const int fh = in.frame_h, nb = in.n_blocks, nF = in.n_frames, fw = nb * nF, N = fw + 1;
     P.fp64 = P.s->precision == 'D' ? 1 : 0;
     P.bf16 = P.s->template_bf16 ? 1 : 0;
-    const bool wF = !P.fp64 && !P.bf16, wD = P.fp64;
+    const bool wF = !P.fp64, wD = P.fp64;
     if (in.template_id > 0) {
         auto it = h->templates.find(in.template_id);
         if (it == h->templates.end()) return fail(h, SS_ERR_INVALID_ARGUMENT, "unknown template_id");
@@ -499,9 +495,12 @@ ss_status plan_group(ss_handle *h, Plan &P) {
         convert_template(in.template_h, in.template_w, in.template_image, in.xgrad, in.ygrad, wF, wD, P.bf16,
                          P.th, P.tmpl.gradFD);
         const size_t tn = (size_t)in.template_h * in.template_w;
-        const size_t pb = P.fp64 ? sizeof(double2) : P.bf16 ? sizeof(uint32_t) : sizeof(float2);
+        const size_t pb = P.fp64 ? sizeof(double2) : sizeof(float2);
         P.oTp = A.take<char>(tn * pb);
-        if (!P.tmpl.gradFD) {
+        if (P.bf16) {  // BF16 gradient pairs, whether or not they are forward differences
+            P.oGxv = A.take<uint32_t>(tn);
+            P.oGyh = A.take<uint32_t>(tn);
+        } else if (!P.tmpl.gradFD) {
             P.oGxv = A.take<char>(tn * pb);
             P.oGyh = A.take<char>(tn * pb);
         }
@@ -647,7 +646,8 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     auto up = [&](size_t o, const void *src, size_t bytes) {
         return hipMemcpyAsync(dptr(o), src, bytes, hipMemcpyHostToDevice, h->stream);
     };
-    g.tplBF16 = P.bf16;
+    g.gradBF16 = P.bf16;
+    g.refine = P.bf16;  // mixed precision: one FP64 refinement step of each knot update
     if (in.template_id > 0) {
         g.tp = P.tmpl.tp;
         g.gxv = P.tmpl.gxv;
@@ -655,21 +655,17 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
         g.tpD = P.tmpl.tpD;
         g.gxvD = P.tmpl.gxvD;
         g.gyhD = P.tmpl.gyhD;
-        g.tpB = P.tmpl.tpB;
         g.gxvB = P.tmpl.gxvB;
         g.gyhB = P.tmpl.gyhB;
     } else {
         const TemplateHost &T = P.th;
         if (P.bf16) {
-            g.tpB = (const uint32_t *)dptr(P.oTp);
-            HIPCHK(up(P.oTp, T.tpB.data(), T.tpB.size() * sizeof(uint32_t)));
-            if (!P.tmpl.gradFD) {
-                g.gxvB = (const uint32_t *)dptr(P.oGxv);
-                g.gyhB = (const uint32_t *)dptr(P.oGyh);
-                HIPCHK(up(P.oGxv, T.gxvB.data(), T.gxvB.size() * sizeof(uint32_t)));
-                HIPCHK(up(P.oGyh, T.gyhB.data(), T.gyhB.size() * sizeof(uint32_t)));
-            }
-        } else if (P.fp64) {
+            g.gxvB = (const uint32_t *)dptr(P.oGxv);
+            g.gyhB = (const uint32_t *)dptr(P.oGyh);
+            HIPCHK(up(P.oGxv, T.gxvB.data(), T.gxvB.size() * sizeof(uint32_t)));
+            HIPCHK(up(P.oGyh, T.gyhB.data(), T.gyhB.size() * sizeof(uint32_t)));
+        }
+        if (P.fp64) {
             g.tpD = (const double2 *)dptr(P.oTp);
             HIPCHK(up(P.oTp, T.tpD.data(), T.tpD.size() * sizeof(double2)));
             if (!P.tmpl.gradFD) {
@@ -681,7 +677,7 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
         } else {
             g.tp = (const float2 *)dptr(P.oTp);
             HIPCHK(up(P.oTp, T.tp.data(), T.tp.size() * sizeof(float2)));
-            if (!P.tmpl.gradFD) {
+            if (!P.tmpl.gradFD && !P.bf16) {  // mixed precision reads the BF16 pairs instead
                 g.gxv = (const float2 *)dptr(P.oGxv);
                 g.gyh = (const float2 *)dptr(P.oGyh);
                 HIPCHK(up(P.oGxv, T.gxv.data(), T.gxv.size() * sizeof(float2)));
@@ -1480,31 +1476,29 @@ ss_status ss_upload_template(ss_handle *h, int32_t th, int32_t tw, const double 
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t fb = al(tn * sizeof(float2)), db = al(tn * sizeof(double2)), bb = al(tn * sizeof(uint32_t));
     const int planes = dt.gradFD ? 1 : 3;
-    HIPCHK(hipMalloc(&dt.mem, planes * (fb + db + bb)));
+    HIPCHK(hipMalloc(&dt.mem, planes * (fb + db) + 2 * bb));
     char *m = (char *)dt.mem;
-    // [planes x float2][planes x double2][planes x bf16 pairs]; plane 0 the template,
-    // planes 1, 2 the gradient pairs
+    // [planes x float2][planes x double2][2 x bf16 gradient pairs]; plane 0
+    // the template, planes 1, 2 the gradient pairs
     const size_t oF = 0, oD = planes * fb, oB = planes * (fb + db);
     ss_status st = SS_OK;
     dt.tp = (float2 *)(m + oF);
     dt.tpD = (double2 *)(m + oD);
-    dt.tpB = (uint32_t *)(m + oB);
+    dt.gxvB = (uint32_t *)(m + oB);
+    dt.gyhB = (uint32_t *)(m + oB + bb);
     st = upload_owned(h, T.tp.data(), tn * sizeof(float2), m, oF);
     if (!st) st = upload_owned(h, T.tpD.data(), tn * sizeof(double2), m, oD);
-    if (!st) st = upload_owned(h, T.tpB.data(), tn * sizeof(uint32_t), m, oB);
+    if (!st) st = upload_owned(h, T.gxvB.data(), tn * sizeof(uint32_t), m, oB);
+    if (!st) st = upload_owned(h, T.gyhB.data(), tn * sizeof(uint32_t), m, oB + bb);
     if (!st && !dt.gradFD) {
         dt.gxv = (float2 *)(m + oF + fb);
         dt.gyh = (float2 *)(m + oF + 2 * fb);
         dt.gxvD = (double2 *)(m + oD + db);
         dt.gyhD = (double2 *)(m + oD + 2 * db);
-        dt.gxvB = (uint32_t *)(m + oB + bb);
-        dt.gyhB = (uint32_t *)(m + oB + 2 * bb);
         st = upload_owned(h, T.gxv.data(), tn * sizeof(float2), m, oF + fb);
         if (!st) st = upload_owned(h, T.gyh.data(), tn * sizeof(float2), m, oF + 2 * fb);
         if (!st) st = upload_owned(h, T.gxvD.data(), tn * sizeof(double2), m, oD + db);
         if (!st) st = upload_owned(h, T.gyhD.data(), tn * sizeof(double2), m, oD + 2 * db);
-        if (!st) st = upload_owned(h, T.gxvB.data(), tn * sizeof(uint32_t), m, oB + bb);
-        if (!st) st = upload_owned(h, T.gyhB.data(), tn * sizeof(uint32_t), m, oB + 2 * bb);
     }
     if (st) {
         (void)hipFree(dt.mem);

@@ -256,7 +256,8 @@ struct GeoIter {
 
 template <typename R>
 __device__ __forceinline__ bool in_bounds(const GroupDev &g, R x, R y) {
-    return x >= (R)0 && x < (R)g.xmax && y >= (R)0 && y < (R)g.ymax;
+    // non-short-circuit: four compares combined without branches
+    return (x >= (R)0) & (x < (R)g.xmax) & (y >= (R)0) & (y < (R)g.ymax);
 }
 
 // Two BF16 values packed in 32 bits (low half first) as a float pair.
@@ -281,47 +282,35 @@ __device__ __forceinline__ R sample(const GroupDev &g, R x, R y, bool valid, Cel
     cl.fx = xs - (R)ix;
     cl.fy = ys - (R)iy;
     cl.t = iy * g.tw + ix;
-    bool loaded = false;
-    if constexpr (sizeof(R) == 4) {
-        if (g.tplBF16) {  // BF16 pairs: low half T[t], high half T[t+1]
-            cl.top = bf16_pair(g.tpB[cl.t]);
-            cl.bot = bf16_pair(g.tpB[cl.t + g.tw]);
-            loaded = true;
-        }
-    }
-    if (!loaded) {
-        const typename Prec<R>::V2 *tp = Prec<R>::tp(g);
-        cl.top = tp[cl.t];
-        cl.bot = tp[cl.t + g.tw];
-    }
+    const typename Prec<R>::V2 *tp = Prec<R>::tp(g);
+    cl.top = tp[cl.t];
+    cl.bot = tp[cl.t + g.tw];
     const R a = mad(cl.fx, cl.top.y - cl.top.x, cl.top.x);
     const R b = mad(cl.fx, cl.bot.y - cl.bot.x, cl.bot.x);
     return mad(cl.fy, b - a, a);
 }
 
-// Template gradients at the sample of cl (task_updategradients.cu):
-// derived from the cell corners when they are forward differences.
+// Template gradients at the sample of cl (task_updategradients.cu): from
+// the BF16 gradient pairs in mixed precision, else derived from the cell
+// corners when they are forward differences, else from the gradient pairs.
 template <typename R>
 __device__ __forceinline__ void gradients(const GroupDev &g, const Cell<R> &cl, R &wx, R &wy) {
+    if constexpr (sizeof(R) == 4) {
+        if (g.gradBF16) {  // pairs (G[t], G[t + step]) as two BF16 values
+            const float2 gx = bf16_pair(g.gxvB[cl.t]), gy = bf16_pair(g.gyhB[cl.t]);
+            wx = mad(cl.fy, gx.y - gx.x, gx.x);
+            wy = mad(cl.fx, gy.y - gy.x, gy.x);
+            return;
+        }
+    }
     if (g.gradFD) {
         wx = mad(cl.fy, (cl.bot.y - cl.bot.x) - (cl.top.y - cl.top.x), cl.top.y - cl.top.x);
         wy = mad(cl.fx, (cl.bot.y - cl.top.y) - (cl.bot.x - cl.top.x), cl.bot.x - cl.top.x);
         return;
     }
-    bool loaded = false;
-    if constexpr (sizeof(R) == 4) {
-        if (g.tplBF16) {
-            const float2 gx = bf16_pair(g.gxvB[cl.t]), gy = bf16_pair(g.gyhB[cl.t]);
-            wx = mad(cl.fy, gx.y - gx.x, gx.x);
-            wy = mad(cl.fx, gy.y - gy.x, gy.x);
-            loaded = true;
-        }
-    }
-    if (!loaded) {
-        const typename Prec<R>::V2 gx = Prec<R>::gxv(g)[cl.t], gy = Prec<R>::gyh(g)[cl.t];
-        wx = mad(cl.fy, gx.y - gx.x, gx.x);
-        wy = mad(cl.fx, gy.y - gy.x, gy.x);
-    }
+    const typename Prec<R>::V2 gx = Prec<R>::gxv(g)[cl.t], gy = Prec<R>::gyh(g)[cl.t];
+    wx = mad(cl.fy, gx.y - gx.x, gx.x);
+    wy = mad(cl.fx, gy.y - gy.x, gy.x);
 }
 
 // Knots of a column for candidate k of the line search (or the half-
@@ -677,92 +666,11 @@ __device__ __forceinline__ void level0_equations(const double *__restrict__ S, c
     }
 }
 
-__global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
-    const int gi = blockIdx.x;
-    GroupCtl &ctl = B.ctl[gi];
-    if (ctl.phase != PH_ASSEMBLE) return;
-    const GroupDev &g = B.groups[gi];
-    const int tid = threadIdx.x;
-    const int nF = g.nF, nb = g.nb, N = g.N, n = g.fframeW;
-    const int shift = ctl.shift;
-    // filter deactivation after max_filter_iterations (mainiteration.cpp:80-90)
-    __shared__ int s_filter;
-    if (tid == 0) {
-        if (ctl.nFilterIterations != 0 && ctl.iteration > ctl.nFilterIterations) {
-            ctl.solverFilter = 0;
-            ctl.filterID = 0;
-            ctl.nFilterIterations = 0;
-        }
-        s_filter = ctl.solverFilter;
-    }
-    const double *S = g.colsum;
-    const int *minPts = g.minPts;
-    uint8_t *used = g.used;
-    // The loops below load a batch of entries before they use any of it:
-    // one thread per frame or knot would otherwise wait out a memory latency
-    // per entry.  The additions keep their order.
-    constexpr int kBatch = 8;
-    // 1. per-frame errval of this shift (d_calcAllGroupErrval)
-    for (int f = tid; f < nF; f += kSolveWG) {
-        const double *col = S + (size_t)f * nb * CS_N;
-        double s = 0.0, c = 0.0;
-        for (int b0 = 0; b0 < nb; b0 += kBatch) {
-            double vs[kBatch], vc[kBatch];
-#pragma unroll
-            for (int u = 0; u < kBatch; u++) {
-                const int bb = min(b0 + u, nb - 1);
-                vs[u] = col[(size_t)bb * CS_N + CS_D2];
-                vc[u] = col[(size_t)bb * CS_N + CS_CNT];
-            }
-#pragma unroll
-            for (int u = 0; u < kBatch; u++) {
-                if (b0 + u < nb) {
-                    s += vs[u];
-                    c += vc[u];
-                }
-            }
-        }
-        g.errval[shift * nF + f] = s / c;
-    }
-    // 2. knot usage (d_createParameterMask); the last knot is never used
-    for (int k0 = tid * kBatch; k0 < N; k0 += kSolveWG * kBatch) {
-        double cnt[kBatch + 1];
-        int mp[kBatch + 1];
-#pragma unroll
-        for (int u = 0; u <= kBatch; u++) {  // knots k0-1 .. k0+kBatch-1
-            const int k = min(max(k0 + u - 1, 0), n - 1);
-            cnt[u] = S[(size_t)k * CS_N + CS_CNT];
-            mp[u] = minPts[k % nb];
-        }
-        uint8_t uk[kBatch];
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-            const int k = k0 + u;
-            const bool pc = k < n && cnt[u + 1] >= (double)mp[u + 1];
-            const bool pp = k < n && k > 0 && cnt[u] >= (double)mp[u];
-            uk[u] = (pc || pp) ? 1 : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < kBatch; u++)
-            if (k0 + u < N) used[k0 + u] = uk[u];
-    }
-    __threadfence_block();
-    __syncthreads();
-    // 3. level-0 equations (d_fillDiagonals / d_fillJTr entries, lambda extension)
-    Eq0 *E = reinterpret_cast<Eq0 *>(g.ws);
-    level0_equations(S, used, E, n, g.lambda, g.fracUniform, tid);
-    __threadfence_block();
-    __syncthreads();
-    InteriorRow *IR = reinterpret_cast<InteriorRow *>(g.ws + (size_t)n * 8);
-    double2 *X = reinterpret_cast<double2 *>(IR + n);
-    Aff *firstAff = reinterpret_cast<Aff *>(X + n);
-    if (g.solverId == 0) {
-        // 4a. solver id 0: ILU(0) in x-then-y order (one thread, the
-        //     reference's recurrence order)
-        if (tid == 0) ilu0_solve(E, reinterpret_cast<IluRow *>(IR), X, n);
-        __threadfence_block();
-        __syncthreads();
-    } else {
+// Exact FP64 solve of the level-0 block-tridiagonal system E (knots 0..n-1)
+// into X: partitioned block Thomas + cyclic reduction of the interface
+// system.  Called by all threads of the workgroup; IR and firstAff are
+// workspace.
+__device__ void exact_solve(const Eq0 *E, InteriorRow *IR, double2 *X, Aff *firstAff, int n, int tid) {
     // 4. partitioned solve: thread t owns knots [s, e); interior rows s..e-2
     //    are eliminated in terms of the interfaces z_{t-1} = x_{s-1} and
     //    z_t = x_{e-1}; the interface system is solved by cyclic reduction.
@@ -919,6 +827,130 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
     }
     __threadfence_block();
     __syncthreads();
+}
+
+__global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
+    const int gi = blockIdx.x;
+    GroupCtl &ctl = B.ctl[gi];
+    if (ctl.phase != PH_ASSEMBLE) return;
+    const GroupDev &g = B.groups[gi];
+    const int tid = threadIdx.x;
+    const int nF = g.nF, nb = g.nb, N = g.N, n = g.fframeW;
+    const int shift = ctl.shift;
+    // filter deactivation after max_filter_iterations (mainiteration.cpp:80-90)
+    __shared__ int s_filter;
+    if (tid == 0) {
+        if (ctl.nFilterIterations != 0 && ctl.iteration > ctl.nFilterIterations) {
+            ctl.solverFilter = 0;
+            ctl.filterID = 0;
+            ctl.nFilterIterations = 0;
+        }
+        s_filter = ctl.solverFilter;
+    }
+    const double *S = g.colsum;
+    const int *minPts = g.minPts;
+    uint8_t *used = g.used;
+    // The loops below load a batch of entries before they use any of it:
+    // one thread per frame or knot would otherwise wait out a memory latency
+    // per entry.  The additions keep their order.
+    constexpr int kBatch = 8;
+    // 1. per-frame errval of this shift (d_calcAllGroupErrval)
+    for (int f = tid; f < nF; f += kSolveWG) {
+        const double *col = S + (size_t)f * nb * CS_N;
+        double s = 0.0, c = 0.0;
+        for (int b0 = 0; b0 < nb; b0 += kBatch) {
+            double vs[kBatch], vc[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                const int bb = min(b0 + u, nb - 1);
+                vs[u] = col[(size_t)bb * CS_N + CS_D2];
+                vc[u] = col[(size_t)bb * CS_N + CS_CNT];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                if (b0 + u < nb) {
+                    s += vs[u];
+                    c += vc[u];
+                }
+            }
+        }
+        g.errval[shift * nF + f] = s / c;
+    }
+    // 2. knot usage (d_createParameterMask); the last knot is never used
+    for (int k0 = tid * kBatch; k0 < N; k0 += kSolveWG * kBatch) {
+        double cnt[kBatch + 1];
+        int mp[kBatch + 1];
+#pragma unroll
+        for (int u = 0; u <= kBatch; u++) {  // knots k0-1 .. k0+kBatch-1
+            const int k = min(max(k0 + u - 1, 0), n - 1);
+            cnt[u] = S[(size_t)k * CS_N + CS_CNT];
+            mp[u] = minPts[k % nb];
+        }
+        uint8_t uk[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int k = k0 + u;
+            const bool pc = k < n && cnt[u + 1] >= (double)mp[u + 1];
+            const bool pp = k < n && k > 0 && cnt[u] >= (double)mp[u];
+            uk[u] = (pc || pp) ? 1 : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+            if (k0 + u < N) used[k0 + u] = uk[u];
+    }
+    __threadfence_block();
+    __syncthreads();
+    // 3. level-0 equations (d_fillDiagonals / d_fillJTr entries, lambda extension)
+    Eq0 *E = reinterpret_cast<Eq0 *>(g.ws);
+    level0_equations(S, used, E, n, g.lambda, g.fracUniform, tid);
+    __threadfence_block();
+    __syncthreads();
+    InteriorRow *IR = reinterpret_cast<InteriorRow *>(g.ws + (size_t)n * 8);
+    double2 *X = reinterpret_cast<double2 *>(IR + n);
+    Aff *firstAff = reinterpret_cast<Aff *>(X + n);
+    if (g.solverId == 0) {
+        // 4a. solver id 0: ILU(0) in x-then-y order (one thread, the
+        //     reference's recurrence order)
+        if (tid == 0) ilu0_solve(E, reinterpret_cast<IluRow *>(IR), X, n);
+        __threadfence_block();
+        __syncthreads();
+    } else {
+        exact_solve(E, IR, X, firstAff, n, tid);
+        if (g.refine) {
+            // one step of iterative refinement in FP64: r = g - H x, H e = r,
+            // x += e (the matrix entries stay; each knot's right-hand side is
+            // replaced by its residual)
+            double2 *Xs = reinterpret_cast<double2 *>(firstAff + kSolveWG);
+            for (int k = tid; k < n; k += kSolveWG) {
+                const Eq0 q = E[k];
+                const double2 x = X[k];
+                double r0 = q.d0 - (q.b0 * x.x + q.b1 * x.y);
+                double r1 = q.d1 - (q.b1 * x.x + q.b2 * x.y);
+                if (k > 0) {
+                    const Eq0 qp = E[k - 1];
+                    const double2 xp = X[k - 1];
+                    r0 -= qp.c0 * xp.x + qp.c1 * xp.y;
+                    r1 -= qp.c1 * xp.x + qp.c2 * xp.y;
+                }
+                if (k + 1 < n) {
+                    const double2 xn = X[k + 1];
+                    r0 -= q.c0 * xn.x + q.c1 * xn.y;
+                    r1 -= q.c1 * xn.x + q.c2 * xn.y;
+                }
+                Xs[k] = x;
+                E[k].d0 = r0;
+                E[k].d1 = r1;
+            }
+            __threadfence_block();
+            __syncthreads();
+            exact_solve(E, IR, X, firstAff, n, tid);
+            for (int k = tid; k < n; k += kSolveWG) {
+                const double2 e = X[k], x = Xs[k];
+                X[k] = make_double2(x.x + e.x, x.y + e.y);
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
     }
     // 5. step filter over the used knots, x sequence then y sequence
     //    (SmearingFilter::apply / RangeFilter::apply on the compacted halves)
@@ -1602,8 +1634,9 @@ hipError_t launch_stage(const LaunchBatch &L, int stage, hipStream_t s) {
 }
 
 size_t solve_workspace_bytes(int n) {
-    // Eq0 + InteriorRow + solution per knot, first-interior forms per partition
-    return (size_t)n * (sizeof(Eq0) + sizeof(InteriorRow) + sizeof(double2)) + kSolveWG * sizeof(Aff) + 256;
+    // Eq0 + InteriorRow + solution per knot, first-interior forms per
+    // partition, the refinement's saved solution per knot
+    return (size_t)n * (sizeof(Eq0) + sizeof(InteriorRow) + 2 * sizeof(double2)) + kSolveWG * sizeof(Aff) + 256;
 }
 
 }  // namespace ss

@@ -167,6 +167,19 @@ def config4(duration=4.0, seed=400, n512=16, n1024=2, dtype=np.int16):
     return streams, groups
 
 
+def config2_group(size, n_frames, lines_per_knot, seed, dtype=np.uint16):
+    """One group of the batched uneven workload: its own template and motion
+    (as config1) at size x size, knots every lines_per_knot lines."""
+    rng = np.random.default_rng(seed)
+    T = make_template(size, size, 400 * size * size // (512 * 512), (1.5, 8.0), n_background=4,
+                      background_sigma=150.0 * size / 512, target_mean=20.0, seed=seed + 1)
+    amp = (float(rng.uniform(2, 8)), float(rng.uniform(2, 8)))
+    dx, dy = make_motion(n_frames, size, 0.1, 5.6e-4 * 512 / size, amp, jump_prob=0.002, jump_size=5.0,
+                         seed=seed + 2)
+    fr = make_frames(T, dx, dy, size, size, dtype, True, seed=seed + 3)
+    return dict(T=T, frames=fr, size=size, lines_per_knot=lines_per_knot)
+
+
 def config2(n_groups=256, seed=300, dtype=np.uint16):
     """Batched uneven groups: frame counts U{8..96}, half 256x256 and half
     512x512, each group with its own template and motion (as config1) and
@@ -175,15 +188,17 @@ def config2(n_groups=256, seed=300, dtype=np.uint16):
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n_groups):
-        s = seed + 10 * (k + 1)
         size = 256 if k % 2 == 0 else 512
         n = int(rng.integers(8, 97))
         lpk = 8 if rng.random() < 0.5 else 16
-        T = make_template(size, size, 400 * size * size // (512 * 512), (1.5, 8.0), n_background=4,
-                          background_sigma=150.0 * size / 512, target_mean=20.0, seed=s + 1)
-        amp = (float(rng.uniform(2, 8)), float(rng.uniform(2, 8)))
-        dx, dy = make_motion(n, size, 0.1, 5.6e-4 * 512 / size, amp, jump_prob=0.002, jump_size=5.0,
-                             seed=s + 2)
-        fr = make_frames(T, dx, dy, size, size, dtype, True, seed=s + 3)
-        out.append(dict(T=T, frames=fr, size=size, lines_per_knot=lpk))
+        out.append(config2_group(size, n, lpk, seed + 10 * (k + 1), dtype))
     return out
+
+
+def config2_args(groups, **overrides):
+    """The mex arguments of every config2 group (one set of frames each)."""
+    calls = []
+    for g in groups:
+        geom = init_settings(g["size"], g["size"], g["T"].shape, nknots=g["size"] // g["lines_per_knot"] + 1)
+        calls.append(group_args(g["T"], g["frames"], geom, **overrides))
+    return calls

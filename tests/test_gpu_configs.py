@@ -1,0 +1,149 @@
+"""GPU parity on the benchmark configurations beyond the single 512x512 group:
+batched uneven groups (256x256 / 512x512, knots every 8 or 16 lines, odd
+frame counts), the full 256-group batch against single calls, a 1024x1024
+int16 serving group against the FP64 oracle, a serving mix of 512x512 and
+1024x1024 int16 groups against single calls, warm-start knots, and the
+128x128 case at its configured smoothness lambda = 1e-3.
+
+Precision 'D' throughout: knot RMS <= 1e-9 px against the oracle (the same
+bar as tests/test_gpu_parity.py), bit-exact between batched, served and
+single calls."""
+import numpy as np
+import pytest
+
+import steadyscan_amd as ssa
+from oracle import oracle as O
+from steadyscan_amd import workloads as W
+
+from test_gpu_parity import check, knot_rms, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def same(r, ref, label=""):
+    for k in ("knots_x", "knots_y", "frame_corr", "residrms"):
+        assert np.array_equal(r[k], ref[k]), (label, k)
+    assert r["iterations"] == ref["iterations"], label
+    assert r["linesearch_tries"] == ref["linesearch_tries"], label
+
+
+def test_config2_slice_against_oracle():
+    # one group of each shape of the batched uneven workload, odd frame
+    # counts (the red/green group of the last frame, Q15), in one batch
+    spec = [(256, 9, 8), (256, 11, 16), (512, 13, 8), (512, 15, 16)]
+    calls = []
+    for i, (size, n, lpk) in enumerate(spec):
+        g = W.config2_group(size, n, lpk, seed=900 + 10 * i)
+        calls.extend(W.config2_args([g]))
+    res = ssa.align_groups(calls)
+    for (size, n, lpk), a, r in zip(spec, calls, res):
+        ref = O.align_group_to_template(a, threads=16)
+        label = f"config2 {size}x{size} {n} frames, knots every {lpk} lines"
+        print(f"{label}: iterations gpu={r['iterations']} oracle={ref['iterations']}", flush=True)
+        assert r["iterations"] == ref["iterations"], label
+        check(a, ref, r["knots_x"], r["knots_y"], r["residrms"], r["frame_corr"], r["mask"], label=label)
+
+
+@pytest.mark.timeout(900)
+def test_config2_full_batch_equals_single_calls():
+    # the whole 256-group batch (frame counts U{8..96}, about 13k frames) in
+    # one submission; every group equals its own single call bit for bit
+    groups = W.config2()
+    calls = W.config2_args(groups)
+    nfr = sum(a["nsetframes"] for a in calls)
+    print(f"config2: {len(calls)} groups, {nfr} frames", flush=True)
+    res = ssa.align_groups(calls, with_mask=False)
+    for i, a in enumerate(calls):
+        single = ssa.align_groups([a], with_mask=False)[0]
+        same(res[i], single, label=f"group {i}")
+    print("iterations: mean %.1f max %d" % (np.mean([r["iterations"] for r in res]),
+                                            max(r["iterations"] for r in res)), flush=True)
+
+
+@pytest.fixture(scope="module")
+def serving_streams():
+    # 3 streams of 512x512 at 30 Hz and one of 1024x1024 at 15 Hz, int16,
+    # cut into groups of 16-64 frames as the serving workload
+    return W.config4(duration=2.0, n512=3, n1024=1, seed=410)
+
+
+def stream_call(streams, g, **over):
+    st = streams[g["stream"]]
+    size = st["size"]
+    geom = ssa.init_settings(size, size, st["T"].shape, nknots=size // 16 + 1)
+    return W.group_args(st["T"], st["frames"], geom, first=g["first"], count=g["count"], **over)
+
+
+def test_config4_1024_int16_group_against_oracle(serving_streams):
+    streams, groups = serving_streams
+    g = next(g for g in groups if streams[g["stream"]]["size"] == 1024)
+    a = stream_call(streams, g)
+    assert a["bmData"].dtype == np.int16 and a["bmData"].shape[0] == 16 * 1024
+    ref = O.align_group_to_template(a, threads=16)
+    out, info = run_gpu(a)
+    print(f"1024x1024 int16, {a['nsetframes']} frames: iterations gpu={int(out[3])} "
+          f"oracle={ref['iterations']}", flush=True)
+    assert int(out[3]) == ref["iterations"]
+    check(a, ref, out[0], out[1], out[5], out[2], out[4], label="config4 1024")
+
+
+def test_config4_server_mix_equals_single_calls(serving_streams):
+    streams, groups = serving_streams
+    calls = [stream_call(streams, g) for g in groups]
+    sizes = [streams[g["stream"]]["size"] for g in groups]
+    assert 512 in sizes and 1024 in sizes
+    singles = [ssa.align_groups([a], with_mask=False)[0] for a in calls]
+    # arrivals in the workload's order, two per step, at most 4 resident:
+    # groups join while others run and wait for slots
+    srv = ssa.Server(max_groups=4)
+    results, tickets, step, k = {}, {}, 0, 0
+    while len(results) < len(calls):
+        for _ in range(2):
+            if k < len(calls):
+                tickets[srv.submit(calls[k])] = k
+                k += 1
+        srv.step()
+        for t, r in srv.completed().items():
+            assert not isinstance(r, Exception), r
+            results[tickets[t]] = r
+        step += 1
+        assert step < 20000
+    for i, ref in enumerate(singles):
+        same(results[i], ref, label=f"group {i} ({sizes[i]}x{sizes[i]})")
+    srv.close()
+
+
+def test_warm_start_knots_against_oracle():
+    # non-zero initial estimates that jump between frames: frame f's last
+    # knot differs from frame f+1's first, so the shared knots are averages
+    # (organize2D1, parameterorganizer.cpp:30-55)
+    T, fr, geom, (dx, dy) = W.config0(n_frames=12, size=64, seed=61)
+    nk = geom["nknots"]
+    L = 64 // (nk - 1)
+    lines = np.minimum(np.arange(nk) * L, 63)
+    rng = np.random.default_rng(62)
+    jump = np.where(np.arange(12) % 2 == 0, 0.4, -0.3)
+    ex = np.asfortranarray(dx[:, lines].T + jump[None, :] + 0.05 * rng.standard_normal((nk, 12)))
+    ey = np.asfortranarray(dy[:, lines].T - jump[None, :] + 0.05 * rng.standard_normal((nk, 12)))
+    assert np.all(ex[-1, :-1] != ex[0, 1:])
+    a = W.group_args(T, fr, geom, est=(ex, ey), max_iter=15)
+    ref = O.align_group_to_template(a, threads=8)
+    out, _ = run_gpu(a)
+    print(f"warm start: iterations gpu={int(out[3])} oracle={ref['iterations']}", flush=True)
+    assert int(out[3]) == ref["iterations"]
+    check(a, ref, out[0], out[1], out[5], out[2], out[4], label="warm start")
+    cold = O.align_group_to_template(W.group_args(T, fr, geom, max_iter=15), threads=8)
+    assert knot_rms(a, cold["knots_x"], cold["knots_y"], ref) > 1e-6  # the estimates were used
+
+
+def test_config0_at_configured_lambda():
+    # the 128x128 configuration at its smoothness lambda = 1e-3
+    T, fr, geom, _ = W.config0()
+    a = W.group_args(T, fr, geom, max_iter=20, lambda_=1e-3)
+    ref = O.align_group_to_template(a, threads=16)
+    out, _ = run_gpu(a)
+    print(f"config0 lambda 1e-3: iterations gpu={int(out[3])} oracle={ref['iterations']}", flush=True)
+    assert int(out[3]) == ref["iterations"]
+    check(a, ref, out[0], out[1], out[5], out[2], out[4], label="config0 lambda 1e-3")
+    ref0 = O.align_group_to_template(W.group_args(T, fr, geom, max_iter=20), threads=16)
+    assert not np.array_equal(ref0["knots_x"], ref["knots_x"])  # lambda changed the solve

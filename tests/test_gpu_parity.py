@@ -215,24 +215,37 @@ def test_config1_group_single_precision(config1_group):
 
 
 def test_bf16_template(config1_group):
-    # int16 frames and a BF16 template with FP32 arithmetic: the error
-    # against the FP64 oracle after two iterations and at convergence is
-    # stated beside the all-FP32 run (no bar beyond the FP32 path's: BF16
-    # keeps 8 significant bits of the template)
+    # mixed precision on the 512x512 workload group with int16 frames: BF16
+    # template gradients (the Jacobian), FP32 residuals and sums, FP64
+    # solve plus one FP64 refinement step.  Held to 0.01 px after two
+    # iterations, and at convergence to twice the oracle's own spread under
+    # a 1e-6 px perturbation of the initial knots, as the FP32 path is.
     T, fr, geom = config1_group
+
+    def args(it):
+        a = W.group_args(T, fr, geom, flags="S", max_iter=it)
+        a["bmData"] = np.asfortranarray(a["bmData"].astype(np.int16))
+        return a
+
     out = {}
-    for bf in (False, True):
-        for it in (2, 120):
-            a = W.group_args(T, fr[:, :, :20], geom, flags="S", max_iter=it)
-            a["bmData"] = np.asfortranarray(a["bmData"].astype(np.int16))
-            ref = O.align_group_to_template(a, threads=16, with_mask=False)
+    for it in (2, 120):
+        a = args(it)
+        ref = O.align_group_to_template(a, threads=16, with_mask=False)
+        for bf in (False, True):
             o, _ = run_gpu(a, template_bf16=bf)
             out[(bf, it)] = knot_rms(a, o[0], o[1], ref)
-    print("BF16 template: knot RMS vs FP64 oracle (2 it / converged) = "
+        if it == 120:
+            rng = np.random.default_rng(1)
+            ap = dict(a)
+            ap["est_dx_knots"] = a["est_dx_knots"] + 1e-6 * rng.standard_normal(a["est_dx_knots"].shape)
+            ap["est_dy_knots"] = a["est_dy_knots"] + 1e-6 * rng.standard_normal(a["est_dy_knots"].shape)
+            refp = O.align_group_to_template(ap, threads=16, with_mask=False)
+            spread = knot_rms(a, refp["knots_x"], refp["knots_y"], ref)
+    print("mixed precision (BF16 gradients): knot RMS vs FP64 oracle (2 it / converged) = "
           f"{out[(True, 2)]:.3e} / {out[(True, 120)]:.3e} px; all-FP32: "
-          f"{out[(False, 2)]:.3e} / {out[(False, 120)]:.3e} px", flush=True)
-    assert out[(True, 2)] < 0.05
-    assert np.isfinite(out[(True, 120)])
+          f"{out[(False, 2)]:.3e} / {out[(False, 120)]:.3e} px; oracle spread {spread:.3e} px", flush=True)
+    assert out[(True, 2)] <= 0.01
+    assert out[(True, 120)] <= max(0.01, 2.0 * spread)
 
 
 def test_rom_replay_gives_the_same_result(tmp_path):
