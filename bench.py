@@ -4,26 +4,29 @@ Workload (the headline configuration): 1000 frames of 512x512 uint16 in
 groups of 50 frames, knots every 16 lines (33 knots), default alignment
 settings (steady_scan.m:836-867), synthetic data (seeded generator), with
 frames, template and geometry resident in HBM.  One step aligns all 20
-groups.  The headline times the K steps as one stream of groups through
-the continuous-batching server (ss_server_*; every group of the K steps may
-be resident, within the server's HBM budget), so a step's last, slowest
-groups share their ticks with other steps' groups; the knots are checked
-equal to the batched path's.  The
-same steps with one batched submission (ss_align_batch) and a
-synchronisation per step are reported beside it (barrier_per_step), and
-one such step with the block matrices in host memory, as the mex boundary
-hands them over, uploaded inside the timed step (host_input_per_step, the
-PCIe-inclusive rate).
+groups and produces every output of the call, the masks included (written
+in place into device buffers).
+
+The headline streams the K steps through the continuous-batching server
+(ss_server_*): every step has its own device copy of the frames and its own
+output buffers, and step s is submitted once every group of step s-1 has
+been admitted, so at most about two steps are resident and a step's slowest
+groups share their ticks with the next step's groups.  Beside it:
+barrier_per_step (all groups of a step in one batched submission, a
+synchronisation per step), host_input_per_step (the block matrices in host
+memory, uploaded inside the step: the mex boundary's PCIe-inclusive case),
+end_to_end (a batched step followed by ss_resample of its 1000 frames,
+device to device, with the bytes of the measurement plan), the
+single-group path (p50/p99 group latency and its frames/s, and the ratio of
+the headline to it), the batched uneven-groups workload and the serving
+workload.
 
 Precision: 'D' (the reference's default, FP64 pixel arithmetic) is the
-headline; 'S' (FP32 pixel arithmetic, FP64 reductions and solve) is timed
-beside it.
+headline; 'S' (FP32 pixel arithmetic, FP64 reductions and solve) and the
+mixed-precision mode are timed beside it.
 
-Printed JSON line: frames/s (whole job); group latency p50/p99 of
-single-group submissions (the plain path) and the batch latency; knot RMS
-and corrected-frame error of one group against the FP64 oracle; the
-roofline of the dominant kernel (HIP events on the library's stream); the
-oracle timed on the host as the CPU baseline.
+CPU baseline: the FP64 oracle (the C restatement) on the GPU box's host, on
+group 0 with OpenMP and on the first frames of group 0 with one thread.
 """
 import argparse
 import ctypes
@@ -93,8 +96,10 @@ def parse():
                     help="skip the step with frames in host memory (uploaded inside the timed step)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
-    ap.add_argument("--uneven", action="store_true",
-                    help="also time the batched uneven-groups workload (256 groups, 8-96 frames, 256^2/512^2)")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the batched uneven-groups workload (256 groups, 8-96 frames, 256^2/512^2)")
+    ap.add_argument("--no-end-to-end", action="store_true",
+                    help="skip the step followed by the resampling of its frames")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="simulated acquisition time of the serving leg")
     ap.add_argument("--no-ibs", action="store_true",
@@ -102,9 +107,8 @@ def parse():
     ap.add_argument("--ibs-only", action="store_true",
                     help="run only the reconstruction leg (knots from one alignment step)")
     ap.add_argument("--stream-groups", type=int, default=-1,
-                    help="time the steps as one stream through the continuous-batching server with at most "
-                         "this many resident groups (-1: every group of the timed steps, within the server's "
-                         "HBM budget; 0: off, one barrier per step)")
+                    help="time the steps as a stream through the continuous-batching server with at most "
+                         "this many resident groups (-1: two steps' groups; 0: off, one barrier per step)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of known size (rocprof counter calibration)")
     return ap.parse_args()
@@ -343,6 +347,20 @@ def ibs_leg(args, torch, ssa, frames, geom, results, G, h):
     return out
 
 
+def host_cpu():
+    """Host processor of this run: logical CPUs and the model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -367,6 +385,7 @@ def main():
     nb = nk - 1
     G = args.group
     ngroups = args.frames // G
+    fh = geom["basex_bm"].shape[0]
     print(f"[bench] generated {args.frames} frames in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
     h = ssa.Handle(0)
@@ -378,23 +397,32 @@ def main():
     for g in range(ngroups):
         bm = ssa.image2blockmatrix(frames[:, :, g * G:(g + 1) * G], geom)
         dev.append(torch.from_numpy(bm.view(np.int16).ravel(order="F")).to("cuda"))
+    mask_bytes = fh * nb * G
+
+    def new_masks():
+        return [torch.empty(mask_bytes, dtype=torch.uint8, device="cuda") for _ in range(ngroups)]
+
+    masks = new_masks()  # the step's masks, written in place by the kernels
     torch.cuda.synchronize()
     zeros = np.zeros((nk, G))
 
-    def call(g, prec):
+    def call(g, prec, bufs=None):
+        d = (bufs or dev)[g]
         a = dict(flags=prec[0], template_bf16=(prec == "SB"), template_image=tid, xgrad=xg, ygrad=yg, tfrac=gid,
                  basex=geom["basex_bm"], basey=geom["basey_bm"], base_mask=geom["base_mask_bm"],
-                 bmData=(dev[g].data_ptr(), np.uint16), est_dx_knots=zeros, est_dy_knots=zeros,
+                 bmData=(d.data_ptr(), np.uint16), est_dx_knots=zeros, est_dy_knots=zeros,
                  group_size=2 * nb, nsetframes=G,
                  min_points_per_block=geom["min_points_per_block"].astype(np.float64))
         a.update(W.default_args())
         return a
 
+    mask_ptrs = [m.data_ptr() for m in masks]
+
     def timed(prec):
         calls = [call(g, prec) for g in range(ngroups)]
 
         def step():
-            return ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+            return ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=mask_ptrs)
 
         res = None
         for _ in range(args.warmup):
@@ -428,23 +456,30 @@ def main():
         avg_s = kms[dom] / launches[dom] / 1e3 if launches[dom] else float("nan")
         bpl = kbytes[dom] / launches[dom] if launches[dom] else 0.0
         return dict(calls=calls, results=res, elapsed=elapsed, lat=lat, per_kernel=per_kernel, dom=dom,
-                    avg_s=avg_s, bytes_per_launch=bpl,
+                    avg_s=avg_s, bytes_per_launch=bpl, align_frame_bytes=float(sum(kbytes)),
                     achieved=bpl / avg_s / 1e9 if launches[dom] else 0.0,
                     fps=ngroups * G * args.steps / elapsed)
 
     def timed_stream(prec, max_groups):
-        """The same K steps as one stream of groups through ss_server: a
-        step's groups are submitted together and a new group is admitted as
-        soon as a resident one finishes, so the next step's groups fill the
-        ticks where the last groups of a step run alone."""
-        calls = [call(g, prec) for g in range(ngroups)]
+        """The K steps as a stream through ss_server: every step has its own
+        device copy of the frames and its own mask buffers; step s is
+        submitted once every group of step s-1 has been admitted."""
+        nsteps = max(args.steps, args.warmup)
+        copies = [[d.clone() for d in dev] for _ in range(nsteps)]
+        smasks = [new_masks() for _ in range(nsteps)]
+        torch.cuda.synchronize()
         srv = ssa.Server(handle=h, max_groups=max_groups, linesearch_candidates=args.candidates)
 
-        def run(nsteps):
-            tickets = [srv.submit(c) for _ in range(nsteps) for c in calls]
+        def run(k):
+            tickets = []
             done = {}
-            while len(done) < len(tickets):
-                srv.step()
+            nxt, queued = 0, 0
+            while len(done) < k * ngroups:
+                if nxt < k and queued == 0:
+                    for g in range(ngroups):
+                        tickets.append(srv.submit(call(g, prec, copies[nxt]), mask=smasks[nxt][g].data_ptr()))
+                    nxt += 1
+                _, queued = srv.step()
                 done.update(srv.completed())
             bad = [r for r in done.values() if isinstance(r, Exception)]
             if bad:
@@ -461,11 +496,17 @@ def main():
         srv.close()
         same = all(np.array_equal(a["knots_x"], b["knots_x"]) and np.array_equal(a["knots_y"], b["knots_y"])
                    for a, b in zip(res[:ngroups], runs_ref.get(prec, res[:ngroups])))
+        # the batched step's mask of group 0 in this precision
+        ssa.align_groups([call(0, prec)], handle=h, linesearch_candidates=args.candidates,
+                         with_mask=[masks[0].data_ptr()])
+        same_mask = bool(torch.equal(smasks[0][0], masks[0]))
+        del copies, smasks
         print(f"[bench] stream {prec} (<= {max_groups} resident groups): {args.steps} steps in {el:.3f}s "
               f"({ngroups * G * args.steps / el:.0f} frames/s), knots equal to the batched step: {same}",
               file=sys.stderr, flush=True)
         return {"frames_per_s": ngroups * G * args.steps / el, "ms_per_step": el / args.steps * 1e3,
-                "max_resident_groups": max_groups, "knots_equal_to_batched": same}
+                "max_resident_groups": max_groups, "knots_equal_to_batched": same,
+                "mask_equal_to_batched": same_mask}
 
     runs_ref = {}
     precs = ["D", "S", "SB"] if args.precision == "all" else [args.precision]
@@ -482,17 +523,18 @@ def main():
     stream = None
     if args.stream_groups:
         runs_ref[precs[0]] = results
-        stream = timed_stream(precs[0], ngroups * max(args.steps, args.warmup) if args.stream_groups < 0
-                              else args.stream_groups)
+        stream = timed_stream(precs[0], 2 * ngroups if args.stream_groups < 0 else args.stream_groups)
 
-    # latency of the plain path: one group per submission
+    # the single-group path: one group per submission (latency, and its throughput)
     single = []
     if not args.no_latency:
         for g in range(ngroups):
             t = time.perf_counter()
-            ssa.align_groups([head["calls"][g]], handle=h, linesearch_candidates=args.candidates, with_mask=False)
+            ssa.align_groups([head["calls"][g]], handle=h, linesearch_candidates=args.candidates,
+                             with_mask=[mask_ptrs[g]])
             single.append(time.perf_counter() - t)
     single_ms = np.array(single) * 1e3
+    single_fps = ngroups * G / float(np.sum(single)) if single else None
 
     # the mex boundary's case: block matrices in host memory, uploaded by the
     # library inside the timed step (PCIe-inclusive; never the headline value)
@@ -504,10 +546,10 @@ def main():
             a["bmData"] = ssa.image2blockmatrix(frames[:, :, g * G:(g + 1) * G], geom)
             hcalls.append(a)
         # warm-up step: the library's pinned staging and arena grow to the host-input size
-        ssa.align_groups(hcalls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+        ssa.align_groups(hcalls, handle=h, linesearch_candidates=args.candidates, with_mask=mask_ptrs)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        hres = ssa.align_groups(hcalls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+        hres = ssa.align_groups(hcalls, handle=h, linesearch_candidates=args.candidates, with_mask=mask_ptrs)
         torch.cuda.synchronize()
         el = time.perf_counter() - t
         same = all(np.array_equal(a["knots_x"], b["knots_x"]) and np.array_equal(a["knots_y"], b["knots_y"])
@@ -519,7 +561,42 @@ def main():
               f"({ngroups * G / el:.0f} frames/s), knots equal to the resident step: {same}",
               file=sys.stderr, flush=True)
 
-    if args.uneven:
+    # end to end: a batched step, then the resampling (resample_frame, the
+    # corrected frames) of all its frames, device to device; algorithmic
+    # bytes of the measurement plan: the step's frame passes + the
+    # resampling's frame read and float32 write
+    e2e = None
+    if not args.no_end_to_end:
+        th, tw = geom["desc"].template_h, geom["desc"].template_w
+        n = ngroups * G
+        d_raw = torch.from_numpy(np.asfortranarray(frames[:, :, :n]).ravel(order="F").view(np.int16)).to("cuda")
+        d_cor = torch.empty(th * tw * n, dtype=torch.float32, device="cuda")
+
+        def e2e_step():
+            res = ssa.align_groups(head["calls"], handle=h, linesearch_candidates=args.candidates,
+                                   with_mask=mask_ptrs)
+            kx = np.concatenate([r["knots_x"].reshape(G, nk).T for r in res], axis=1)
+            ky = np.concatenate([r["knots_y"].reshape(G, nk).T for r in res], axis=1)
+            ssa.resample_frames((d_raw.data_ptr(), np.uint16, n), geom, kx, ky, device_out=d_cor.data_ptr(),
+                                handle=h)
+
+        e2e_step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            e2e_step()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t) / args.steps
+        s_pix = 2.0
+        b_frame = head["align_frame_bytes"] / n + s_pix * frames.shape[0] * frames.shape[1] + 4.0 * th * tw
+        e2e = {"frames_per_s": n / el, "ms_per_step": el * 1e3, "bytes_per_frame": b_frame,
+               "achieved_GBps": b_frame * n / el / 1e9, "frac": b_frame * n / el / 1e9 / HBM_PEAK_GBS,
+               "mode": "one batched step (masks in HBM) + ss_resample of its frames, device to device"}
+        del d_raw, d_cor
+        print(f"[bench] end to end (align + resample): {n / el:.0f} frames/s, {b_frame / 1e6:.1f} MB/frame",
+              file=sys.stderr, flush=True)
+
+    if not args.no_uneven:
         uneven = uneven_leg(args, torch, ssa, W)
         print(f"[bench] uneven: {uneven['frames_per_s']:.0f} frames/s, {uneven['megapixels_per_s']:.0f} Mpx/s",
               file=sys.stderr, flush=True)
@@ -532,7 +609,7 @@ def main():
               f"{serving['frames_per_s']:.0f} frames/s", file=sys.stderr, flush=True)
 
     # parity of group 0 against the FP64 oracle (same precision letter, which
-    # only sets the mask-bound epsilon there), and the CPU baseline
+    # only sets the mask-bound epsilon there), and the CPU baselines
     parity = {}
     cpu = None
     if not args.no_cpu:
@@ -553,9 +630,11 @@ def main():
             dx = (out["knots_x"] - ref["knots_x"]).reshape(G, nk)[:-1]
             dy = (out["knots_y"] - ref["knots_y"]).reshape(G, nk)[:-1]
             rms = float(np.sqrt(np.mean(np.concatenate([dx.ravel(), dy.ravel()]) ** 2)))
-            # corrected frames from the GPU knots vs the oracle knots (same splat)
+            # corrected frames from the GPU knots vs the oracle knots: the
+            # oracle's own resampling (orc_resample_frame) of the oracle
+            # knots against the GPU resampling of the GPU knots
             cg = ssa.resample_frames(fr, geom, out["knots_x"][:m], out["knots_y"][:m], handle=h)
-            co = ssa.resample_frames(fr, geom, ref["knots_x"][:m], ref["knots_y"][:m], handle=h)
+            co = O.resample_frames(fr, geom, ref["knots_x"][:m], ref["knots_y"][:m])
             both = np.isfinite(cg) & np.isfinite(co)
             diff = np.abs(cg[both] - co[both])
             parity[prec] = dict(knot_rms=rms, corr_max=float(diff.max()) if diff.size else 0.0,
@@ -566,7 +645,18 @@ def main():
             if cpu is None:
                 cpu = {"value": G / tcpu, "unit": "frames/s", "cores": threads, "kind": "port",
                        "sample": f"FP64 oracle (C restatement, OpenMP) on 1 group of {G} frames 512x512 "
-                                 f"(group 0 of the workload), {ref['iterations']} GN iterations, {tcpu:.1f} s"}
+                                 f"(group 0 of the workload), {ref['iterations']} GN iterations, {tcpu:.1f} s",
+                       "host": host_cpu()}
+        # the same oracle on one thread, on the first 20 frames of group 0
+        n1 = 20
+        a1 = W.group_args(T, frames, geom, first=0, count=n1, flags=precs[0][0])
+        t = time.perf_counter()
+        r1 = O.align_group_to_template(a1, threads=1, with_mask=False)
+        t1 = time.perf_counter() - t
+        cpu["single_thread"] = {"value": n1 / t1, "unit": "frames/s", "cores": 1,
+                                "sample": f"FP64 oracle on one thread, the first {n1} frames of group 0 as "
+                                          f"one group, {r1['iterations']} GN iterations, {t1:.1f} s"}
+        print(f"[bench] oracle, one thread: {n1} frames in {t1:.1f} s", file=sys.stderr, flush=True)
 
     iters = [r["iterations"] for r in results]
     lat_ms = np.array(head["lat"]) * 1e3
@@ -579,9 +669,10 @@ def main():
                        "gn_iterations_mean": float(np.mean([x["iterations"] for x in r["results"]])),
                        "kernels_one_step": r["per_kernel"], **{k: v for k, v in parity.get(prec, {}).items()}}
     hpar = parity.get(hp, {})
+    value = stream["frames_per_s"] if stream else head["fps"]
     line = {
         "metric": METRIC,
-        "value": stream["frames_per_s"] if stream else head["fps"],
+        "value": value,
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -594,19 +685,24 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{args.frames} frames 512x512 uint16 in groups of {G}, knots every 16 lines, "
-                        "default settings (max_iter 120, smearing filter, exact solve); "
-                        + ("the steps' groups streamed through the continuous-batching server "
-                           f"(at most {stream['max_resident_groups']} resident)" if stream else
+                        "default settings (max_iter 120, smearing filter, exact solve), every output of the "
+                        "call produced (masks in HBM); "
+                        + ("the steps streamed through the continuous-batching server, each step on its own "
+                           "copy of the frames, submitted once the previous step is admitted "
+                           f"(at most {stream['max_resident_groups']} resident groups)" if stream else
                            "all groups of a step in one batched submission"),
             "barrier_per_step": {"frames_per_s": head["fps"], "ms_per_step": head["elapsed"] / args.steps * 1e3,
                                  "mode": "all groups of a step in one batched submission, synchronised per step"},
             "host_input_per_step": host_in,
+            "end_to_end": e2e,
             "precision": hp,
             "frames_per_step": ngroups * G,
             "groups_per_step": ngroups,
             "p50_group_ms": float(np.percentile(single_ms, 50)) if single else None,
             "p99_group_ms": float(np.percentile(single_ms, 99)) if single else None,
             "group_latency_mode": "single-group submissions, inputs resident",
+            "single_group_frames_per_s": single_fps,
+            "ratio_vs_single_group": (value / single_fps) if single_fps else None,
             "batch_ms_p50": float(np.percentile(lat_ms, 50)),
             "gn_iterations_mean": float(np.mean(iters)),
             "gn_iterations_max": int(np.max(iters)),

@@ -133,6 +133,8 @@ typedef struct {
     int32_t half_iterations;     /* executed half-iterations                        */
     int32_t linesearch_rounds;   /* executed fused line-search rounds               */
     int32_t linesearch_tries;    /* sequential tries those rounds resolved          */
+    int32_t mask_on_device;      /* nonzero: mask is a device pointer; the kernels
+                                    write it in place (no copy to the host)         */
 } ss_group_out;
 
 /* Handle lifecycle.  Replaces CudaHelper<T>::initDevice (cudahelper.cu:8-27)

@@ -282,3 +282,33 @@ def invert_bilinear_sampling(frames, dx_knots, dy_knots, geometry, template_shap
     if rc:
         raise RuntimeError(f"oracle returned status {rc}")
     return out, mask.astype(bool), iters
+
+
+def resample_frames(frames, geometry, dx_knots, dy_knots):
+    """Oracle of resample_frame + pushgrid (steady_scan.m:562-599) for every
+    frame: raw nlines x linewidth x n frames (cropped here by the geometry's
+    cropping), knots nknots x n (or flat, frame after frame).  Returns
+    template_h x template_w x n float32, NaN where no data landed."""
+    g = geometry["desc"]
+    cl, cr, cb, ct = list(g.cropping)
+    th, tw = g.template_h, g.template_w
+    fr = np.asarray(frames)
+    if fr.ndim == 2:
+        fr = fr[:, :, None]
+    n = fr.shape[2]
+    kx = np.asarray(dx_knots, dtype=np.float64).reshape(g.nknots, n, order="F")
+    ky = np.asarray(dy_knots, dtype=np.float64).reshape(g.nknots, n, order="F")
+    pk = np.ascontiguousarray(geometry["prevknotind"].ravel(order="F"), np.int32)
+    bx = np.ascontiguousarray(geometry["basex"].ravel(order="F"))
+    by = np.ascontiguousarray(geometry["basey"].ravel(order="F"))
+    tf = np.ascontiguousarray(geometry["tfrac"].ravel(order="F"))
+    out = np.zeros((th, tw, n), np.float32, order="F")
+    for f in range(n):
+        data = np.asarray(fr[cb:fr.shape[0] - ct, cl:fr.shape[1] - cr, f], np.float64)
+        o = np.zeros(th * tw, np.float32)
+        lib().orc_resample_frame(_p(np.ascontiguousarray(data.ravel(order="F"))), data.size,
+                                 _p(np.ascontiguousarray(kx[:, f])), _p(np.ascontiguousarray(ky[:, f])),
+                                 _p(bx), _p(by), pk.ctypes.data_as(POINTER(c_int32)), _p(tf), th, tw,
+                                 o.ctypes.data_as(POINTER(ctypes.c_float)))
+        out[:, :, f] = o.reshape(th, tw, order="F")
+    return out

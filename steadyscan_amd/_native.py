@@ -66,6 +66,7 @@ class SSGroupOut(ctypes.Structure):
         ("iteration_corr", POINTER(c_double)), ("all_params", POINTER(c_double)),
         ("stop_criterion", c_int32), ("half_iterations", c_int32),
         ("linesearch_rounds", c_int32), ("linesearch_tries", c_int32),
+        ("mask_on_device", c_int32),
     ]
 
 

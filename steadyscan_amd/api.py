@@ -366,10 +366,16 @@ def _prepare_call(a, h, s, gi, go, with_mask, linesearch_candidates, keep):
     gi.init_x, gi.init_y, gi.min_points = _ptr(ex, c_double), _ptr(ey, c_double), _ptr(mp, c_int32)
     r = dict(knots_x=np.zeros((nb + 1) * nF), knots_y=np.zeros((nb + 1) * nF),
              frame_corr=np.zeros(nF), residrms=np.zeros(nF),
-             mask=np.zeros((fh, nb * nF), dtype=np.uint8, order="F") if with_mask else None)
+             mask=None)
+    mask_dev = isinstance(with_mask, int) and not isinstance(with_mask, bool)  # a device pointer
+    if not mask_dev and bool(with_mask):
+        r["mask"] = np.zeros((fh, nb * nF), dtype=np.uint8, order="F")
     go.knots_x, go.knots_y = _ptr(r["knots_x"], c_double), _ptr(r["knots_y"], c_double)
     go.frame_corr, go.residrms = _ptr(r["frame_corr"], c_double), _ptr(r["residrms"], c_double)
-    if with_mask:
+    if mask_dev:
+        go.mask = ctypes.cast(c_void_p(with_mask), POINTER(c_uint8))
+        go.mask_on_device = 1
+    elif r["mask"] is not None:
         go.mask = _ptr(r["mask"], c_uint8)
     return r
 
@@ -387,14 +393,17 @@ def align_groups(calls, handle=None, device=0, linesearch_candidates=2, with_mas
     """Several independent align_group_to_template calls in one submission
     (ss_align_batch).  `calls` is a list of dicts keyed by the mex argument
     names (see workloads.ARG_ORDER); returns a list of dicts with the
-    outputs of each call."""
+    outputs of each call.  with_mask: True (host masks), False (none), or a
+    list of device pointers, one per call (frame_h x n_blocks*n_frames
+    bytes each), that receive the masks in place."""
     h = handle if handle is not None else _default_handle(device)
     n = len(calls)
     gins = (N.SSGroupIn * n)()
     sets = (N.SSSettings * n)()
     gouts = (N.SSGroupOut * n)()
     keep = []
-    results = [_prepare_call(a, h, sets[i], gins[i], gouts[i], with_mask, linesearch_candidates, keep)
+    masks = with_mask if isinstance(with_mask, (list, tuple)) else [with_mask] * n
+    results = [_prepare_call(a, h, sets[i], gins[i], gouts[i], masks[i], linesearch_candidates, keep)
                for i, a in enumerate(calls)]
     _check(N.lib().ss_align_batch(h.ptr, n, gins, sets, gouts), h)
     out = [_collect(r, go) for r, go in zip(results, gouts)]
@@ -423,10 +432,12 @@ class Server:
         self.with_mask = with_mask
         self._pending = {}  # ticket -> (structs, arrays, result)
 
-    def submit(self, call):
+    def submit(self, call, mask=None):
+        """mask: None (the server's with_mask) or a device pointer that
+        receives this group's mask in place."""
         gi, s, go = N.SSGroupIn(), N.SSSettings(), N.SSGroupOut()
         keep = []
-        r = _prepare_call(call, self.h, s, gi, go, self.with_mask, self.K, keep)
+        r = _prepare_call(call, self.h, s, gi, go, self.with_mask if mask is None else int(mask), self.K, keep)
         t = ctypes.c_int64()
         _check(N.lib().ss_server_submit(self.ptr, ctypes.byref(gi), ctypes.byref(s), ctypes.byref(go),
                                         ctypes.byref(t)), self.h)

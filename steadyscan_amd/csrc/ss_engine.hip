@@ -547,7 +547,7 @@ ss_status plan_group(ss_handle *h, Plan &P) {
     P.oNext = A.take<int>(nF);
     P.oMk = A.take<int>(nF);
     P.oDk = A.take<int>(nF);
-    if (P.out->mask) P.oMask = A.take<uint8_t>((size_t)fh * fw);
+    if (P.out->mask && !P.out->mask_on_device) P.oMask = A.take<uint8_t>((size_t)fh * fw);
     P.oRes = A.take<double>(nF);
     P.oFc = A.take<double>(nF);
     if (P.out->all_params) P.oAll = A.take<double>(Pn * P.s->max_iterations);
@@ -749,7 +749,9 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     g.lsNextK = (int *)dptr(P.oNext);
     g.lsMk = (int *)dptr(P.oMk);
     g.lsDk = (int *)dptr(P.oDk);
-    g.maskOut = P.out->mask ? (uint8_t *)dptr(P.oMask) : nullptr;
+    // the mask goes straight to a device-resident output, else to the group's
+    // block and is copied to the host when the group finishes
+    g.maskOut = !P.out->mask ? nullptr : P.out->mask_on_device ? P.out->mask : (uint8_t *)dptr(P.oMask);
     g.resid = (double *)dptr(P.oRes);
     g.fcorr = (double *)dptr(P.oFc);
     g.allParams = P.out->all_params ? (double *)dptr(P.oAll) : nullptr;
@@ -792,7 +794,8 @@ ss_status finish_group(ss_handle *h, const Plan &P, const ss::GroupDev &g, const
     HIPCHK(hipMemcpyAsync(pl.data(), g.pLast, Pn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (o.residrms) HIPCHK(hipMemcpyAsync(o.residrms, g.resid, nF * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (o.frame_corr) HIPCHK(hipMemcpyAsync(o.frame_corr, g.fcorr, nF * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    if (o.mask) HIPCHK(hipMemcpyAsync(o.mask, g.maskOut, (size_t)in.frame_h * fw, hipMemcpyDeviceToHost, h->stream));
+    if (o.mask && !o.mask_on_device)
+        HIPCHK(hipMemcpyAsync(o.mask, g.maskOut, (size_t)in.frame_h * fw, hipMemcpyDeviceToHost, h->stream));
     const int nrec = std::min(c.iteration, P.s->max_iterations);
     if (o.all_params && nrec > 0)
         HIPCHK(hipMemcpyAsync(o.all_params, g.allParams, Pn * nrec * sizeof(double), hipMemcpyDeviceToHost, h->stream));

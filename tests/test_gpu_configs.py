@@ -147,3 +147,34 @@ def test_config0_at_configured_lambda():
     check(a, ref, out[0], out[1], out[5], out[2], out[4], label="config0 lambda 1e-3")
     ref0 = O.align_group_to_template(W.group_args(T, fr, geom, max_iter=20), threads=16)
     assert not np.array_equal(ref0["knots_x"], ref["knots_x"])  # lambda changed the solve
+
+
+def test_device_resident_mask_equals_host_mask():
+    # masks written in place into device buffers (mask_on_device) equal the
+    # masks copied to the host, for a batch and for a server group
+    from devbuf import DeviceBuffer
+    calls = []
+    for i, (n, size) in enumerate([(7, 64), (9, 96)]):
+        T, fr, geom, _ = W.config0(n_frames=n, size=size, seed=80 + i)
+        calls.append(W.group_args(T, fr, geom, max_iter=8))
+    host = ssa.align_groups(calls)
+    bufs = [DeviceBuffer(r["mask"].size) for r in host]
+    for b in bufs:
+        b.upload(np.full(b.nbytes, 7, np.uint8))
+    dev = ssa.align_groups(calls, with_mask=[b.ptr.value for b in bufs])
+    for r, d, b in zip(host, dev, bufs):
+        assert d["mask"] is None
+        assert np.array_equal(np.ravel(r["mask"], order="F").astype(np.uint8), b.download(np.uint8, b.nbytes))
+        assert np.array_equal(r["knots_x"], d["knots_x"])
+    srv = ssa.Server(max_groups=2)
+    bufs[0].upload(np.full(bufs[0].nbytes, 7, np.uint8))
+    t = srv.submit(calls[0], mask=bufs[0].ptr.value)
+    done = {}
+    while t not in done:
+        srv.step()
+        done.update(srv.completed())
+    assert np.array_equal(np.ravel(host[0]["mask"], order="F").astype(np.uint8),
+                          bufs[0].download(np.uint8, bufs[0].nbytes))
+    srv.close()
+    for b in bufs:
+        b.free()
