@@ -126,6 +126,25 @@ def calibrate_copy():
     print(json.dumps({"calibration": "copy", "bytes_read": n, "bytes_written": n, "launches": 5}))
 
 
+def tuning_summary(entries):
+    """The autotuner's choices per group shape: the chosen (samples, waves)
+    variant of each pixel kernel and its trial-tick time against the default
+    variant's."""
+    out = []
+    for e in entries:
+        ks = {}
+        gain = 0.0
+        for name, k in e["kernels"].items():
+            c = k["chosen"]
+            ks[name] = {"chosen": {"samples": k["variants"][c][0], "waves": k["variants"][c][1]},
+                        "trial_us": round(k["trial_us"][c], 1), "default_trial_us": round(k["trial_us"][0], 1)}
+            gain += k["trial_us"][0] - k["trial_us"][c]
+        out.append({"shape": f"{e['precision']}{'+bf16' if e['mixed'] else ''} frame_h {e['frame_h']} "
+                             f"n_blocks {e['n_blocks']} frames {e['frames']}",
+                    "kernels": ks, "trial_tick_us_saved": round(gain, 1)})
+    return out
+
+
 def serve_leg(args, torch, ssa, W):
     """Continuous batching of acquisition streams (16 x 512x512 at 30 Hz +
     2 x 1024x1024 at 15 Hz, int16, groups of 16-64 frames, jittered
@@ -242,6 +261,15 @@ def uneven_leg(args, torch, ssa, W):
     out = {"groups": len(groups), "frames": nfr, "gigapixels": px / 1e9, "seconds": el,
            "frames_per_s": nfr / el, "megapixels_per_s": px / el / 1e6,
            "gn_iterations_mean": float(np.mean([r["iterations"] for r in res]))}
+    # the same batch with the launch variants tuned per group shape (same
+    # handle: templates and geometries are resident per handle)
+    h.tuning_mode(-1)
+    ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+    t = time.perf_counter()
+    ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
+    out["frames_per_s_tuned_variants"] = nfr / (time.perf_counter() - t)
+    out["tuned"] = tuning_summary(h.tuning())
+    h.tuning_mode(0)
     if not args.no_cpu:
         from oracle import oracle as O
         gr = groups[0]

@@ -87,6 +87,11 @@ typedef struct {
                                                 FP64 across threads, and every knot update
                                                 gets one FP64 iterative-refinement step.
                                                 0 (default): the precision's own width.    */
+    int32_t save_frame_correlations;         /* flag 'c' (configuration.cpp:106-108): frame
+                                                correlations after every iteration, on the
+                                                state the global correlation leaves
+                                                (task.cpp:286-306); frame_corr then holds the
+                                                last iteration's (task.cpp:392-394).  0: off */
 } ss_settings;
 
 /* Fills the defaults of default_alignment_settings (steady_scan.m:836-867). */
@@ -118,7 +123,7 @@ typedef struct {
 } ss_group_in;
 
 /* Outputs 1-7 of the mex (mexgateway.cpp:320-354) and selected diagnostics
- * (outputs 8 and 15).  NULL pointers skip an output. */
+ * (outputs 8, 11, 12 and 15).  NULL pointers skip an output. */
 typedef struct {
     double *knots_x;             /* out 1: (n_blocks+1) x n_frames                  */
     double *knots_y;             /* out 2                                           */
@@ -135,6 +140,11 @@ typedef struct {
     int32_t linesearch_tries;    /* sequential tries those rounds resolved          */
     int32_t mask_on_device;      /* nonzero: mask is a device pointer; the kernels
                                     write it in place (no copy to the host)         */
+    double *frame_corr_iter;     /* out 11 ('c'): n_frames x max_iterations, written
+                                    when settings.save_frame_correlations is set    */
+    double *iteration_errvals;   /* out 12 ('e'): 2*max_iterations x n_frames, rows
+                                    2k and 2k+1 the two shifts' errvals after every
+                                    iteration k+1 that continues (task.cpp:362-370) */
 } ss_group_out;
 
 /* Handle lifecycle.  Replaces CudaHelper<T>::initDevice (cudahelper.cu:8-27)
@@ -186,6 +196,39 @@ ss_status ss_profile_enable(ss_handle *h, int32_t on);
 /* Launch counts, summed kernel milliseconds and summed algorithmic frame
  * bytes (frame pixels read x element size) per kernel class. */
 ss_status ss_profile_read(ss_handle *h, int64_t *launches, double *ms, double *frame_bytes);
+
+/* ---- autotuning of the pixel kernels ----
+ * No reference counterpart (the reference fixes THREADS_PER_BLOCK 256,
+ * cudaincludes.h:36).  The assembly, line-search and correlation kernels are
+ * built in three launch variants each: (template samples a thread keeps in
+ * flight, minimum waves per SIMD); variant 0, the default, was measured
+ * fastest on batches of 512x512 groups.  With tuning on, the first time a
+ * handle sees a group shape (precision, mixed precision, frame element type,
+ * frame_h, n_blocks, lines per run, geometry form, gradient form, frame-count
+ * class floor(log2(n_frames))), it runs that group's first tick twice per
+ * variant from the group's initial state (a warm-up pass, then a pass timed
+ * with HIP events) and keeps the fastest variant per kernel for the shape
+ * (ss_align_group, ss_align_batch and server admission alike).  Every
+ * variant computes the same results bit for bit, so tuning changes times
+ * only.  A batch whose groups use different variants launches each kernel
+ * once per variant, which can cost more than the tuning gains. */
+typedef struct {
+    char precision;              /* 'D' or 'S'                                      */
+    int32_t mixed;               /* template_bf16                                   */
+    int32_t data_type;           /* ss_dtype                                        */
+    int32_t frame_h, n_blocks;
+    int32_t frames;              /* frame count of the group the shape was tuned on */
+    int32_t chosen[3];           /* variant per kernel: assembly, line search, correlation */
+    double trial_us[3][3];       /* [kernel][variant]: kernel time in the trial tick */
+    int32_t samples[3][3];       /* [kernel][variant]: the variants' parameters      */
+    int32_t waves[3][3];
+} ss_tuning_entry;
+/* 0 (default), 1 or 2: use that variant for every kernel, no tuning;
+ * -1: tune each new group shape. */
+ss_status ss_tuning_mode(ss_handle *h, int32_t mode);
+/* Copies up to cap of the handle's tuned shapes to out; returns how many
+ * there are. */
+int32_t ss_tuning_entries(const ss_handle *h, ss_tuning_entry *out, int32_t cap);
 
 /* ---- serving: continuous batching of groups ----
  * No reference counterpart: the mex aligns one group per call and resets

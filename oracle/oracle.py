@@ -42,7 +42,8 @@ class OrcOut(ctypes.Structure):
                 ("mask", POINTER(c_uint8)), ("iteration_corr", POINTER(c_double)),
                 ("all_params", POINTER(c_double)), ("iterations", c_int32),
                 ("stop_criterion", c_int32), ("half_iterations", c_int32),
-                ("linesearch_tries", c_int32)]
+                ("linesearch_tries", c_int32), ("frame_corr_iter", POINTER(c_double)),
+                ("iteration_errvals", POINTER(c_double))]
 
 
 class OrcGeometryIn(ctypes.Structure):
@@ -171,8 +172,11 @@ class Inputs:
                              precision=prec.encode())
 
 
-def align_group_to_template(a, threads=1, with_mask=True, with_iterations=False):
-    """Runs the oracle on a dict of the 26 mex arguments; returns a dict."""
+def align_group_to_template(a, threads=1, with_mask=True, with_iterations=False, L=None):
+    """Runs the oracle on a dict of the 26 mex arguments; returns a dict.
+    Flag 'c' adds frame_corr_iter (n_frames x max_iter, and frame_corr
+    becomes the last iteration's values), flag 'e' iteration_errvals
+    (2*max_iter x n_frames)."""
     inp = Inputs(a)
     nb, nF, fh = inp.nb, inp.nF, inp.fh
     kx = np.zeros((nb + 1) * nF)
@@ -191,13 +195,21 @@ def align_group_to_template(a, threads=1, with_mask=True, with_iterations=False)
     if with_iterations:
         allp = np.zeros((2 * (nb * nF + 1), mi), order="F")
         o.all_params = _p(allp)
-    rc = lib().orc_align_group(byref(inp.i), byref(inp.s), byref(o), int(threads))
+    flags = a.get("flags", "D")
+    fci = np.zeros((nF, mi), order="F") if "c" in flags else None
+    if fci is not None:
+        o.frame_corr_iter = _p(fci)
+    ierr = np.zeros((2 * mi, nF), order="F") if "e" in flags else None
+    if ierr is not None:
+        o.iteration_errvals = _p(ierr)
+    rc = (L or lib()).orc_align_group(byref(inp.i), byref(inp.s), byref(o), int(threads))
     if rc:
         raise RuntimeError(f"oracle returned status {rc}")
     return dict(knots_x=kx, knots_y=ky, frame_corr=fc, residrms=rr, mask=mask,
                 iterations=o.iterations, stop_criterion=o.stop_criterion,
                 half_iterations=o.half_iterations, linesearch_tries=o.linesearch_tries,
-                iteration_corr=icorr[:min(o.iterations, mi)], all_params=allp)
+                iteration_corr=icorr[:min(o.iterations, mi)], all_params=allp,
+                frame_corr_iter=fci, iteration_errvals=ierr)
 
 
 class State:

@@ -55,7 +55,9 @@
  *   4 ILU(0) eliminates rows whose columns were not sorted
  *   5 ILU(0) divides the multiplier by the row's own pivot
  *   6 frame correlation means taken over all pixels of the frame
- *   7 frame correlation of the residual instead of the warped template */
+ *   7 frame correlation of the residual instead of the warped template
+ *   8 the global correlation swaps every pixel back
+ *   9 per-iteration errvals store the two shifts in each other's rows */
 #ifndef ORC_PLANT
 #define ORC_PLANT 0
 #endif
@@ -714,7 +716,7 @@ double orc_global_correlation(orc_state *st) {
     }
     double corr = q / sqrt(assq * bssq);
     size_t swapBack = (size_t)((st->fframeW + 255) / 256) * 256;
-    if (swapBack > numPixel) swapBack = numPixel;
+    if (swapBack > numPixel || ORC_PLANT == 8) swapBack = numPixel;
     for (size_t i = 0; i < swapBack; i++)
         if (st->mask[i]) st->d[i] = st->image[i] - st->d[i];
     free(qcImg);
@@ -978,6 +980,13 @@ int orc_align_group(const orc_in *in, const orc_settings *s, orc_out *out, int n
             memcpy(out->all_params + (size_t)st->nParams * (iteration - 1), st->p,
                    sizeof(double) * (size_t)st->nParams);
         corr = orc_global_correlation(st);
+        if (out->frame_corr_iter) {
+            /* on the state the global correlation left (its partial swap-back) */
+            frame_correlation(st);
+            if (iteration <= s->max_iterations)
+                memcpy(out->frame_corr_iter + (size_t)st->nFrames * (iteration - 1), st->corr,
+                       sizeof(double) * (size_t)st->nFrames);
+        }
         if (out->iteration_corr && iteration <= s->max_iterations) out->iteration_corr[iteration - 1] = corr;
         if (corr - correlation < s->correlation_increase_threshold) {
             retv = ORC_STOP_CORRELATION_INCREASE;
@@ -1006,10 +1015,17 @@ int orc_align_group(const orc_in *in, const orc_settings *s, orc_out *out, int n
         }
         correlation = corr;
         memcpy(st->pLast, st->p, sizeof(double) * (size_t)st->nParams);
+        if (out->iteration_errvals && iteration <= s->max_iterations) {
+            const size_t ld = 2 * (size_t)s->max_iterations, s0 = ORC_PLANT == 9 ? 1 : 0;
+            for (int f = 0; f < st->nFrames; f++) {
+                out->iteration_errvals[(size_t)f * ld + 2 * (size_t)(iteration - 1) + s0] = st->errval[f];
+                out->iteration_errvals[(size_t)f * ld + 2 * (size_t)(iteration - 1) + 1 - s0] = st->errval[f + st->nFrames];
+            }
+        }
     }
     if (iteration == 1) memcpy(st->pLast, st->p, sizeof(double) * (size_t)st->nParams);
     calc_group_errvals(st, 0);
-    frame_correlation(st);
+    if (!out->frame_corr_iter) frame_correlation(st);
     out->iterations = iteration;
     out->stop_criterion = retv;
     out->half_iterations = st->halfIterations;

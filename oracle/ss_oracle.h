@@ -76,6 +76,13 @@ typedef struct {
     int32_t stop_criterion;
     int32_t half_iterations;       /* executed half-iterations               */
     int32_t linesearch_tries;      /* sum over frames of executed tries       */
+    /* flag 'c' (configuration.cpp:106-108): n_frames x max_iterations, the
+     * frame correlations after every iteration (task.cpp:286-306); non-NULL
+     * also makes frame_corr the last iteration's values (task.cpp:392-394) */
+    double *frame_corr_iter;
+    /* flag 'e' (configuration.cpp:145-147): 2*max_iterations x n_frames,
+     * both shifts' errvals after every iteration that continues (task.cpp:362-370) */
+    double *iteration_errvals;
 } orc_out;
 
 /* Whole call: MexGateway<double>::execute (mex/mexgateway.cpp:373-445). */

@@ -40,6 +40,7 @@ class SSSettings(ctypes.Structure):
         ("precision", c_char),
         ("linesearch_candidates", c_int32),
         ("template_bf16", c_int32),
+        ("save_frame_correlations", c_int32),
     ]
 
 
@@ -67,6 +68,7 @@ class SSGroupOut(ctypes.Structure):
         ("stop_criterion", c_int32), ("half_iterations", c_int32),
         ("linesearch_rounds", c_int32), ("linesearch_tries", c_int32),
         ("mask_on_device", c_int32),
+        ("frame_corr_iter", POINTER(c_double)), ("iteration_errvals", POINTER(c_double)),
     ]
 
 
@@ -79,6 +81,13 @@ class SSDriverSettings(ctypes.Structure):
 class SSDisplacementsOut(ctypes.Structure):
     _fields_ = [("dx_knots", POINTER(c_double)), ("dy_knots", POINTER(c_double)), ("corr", POINTER(c_double)),
                 ("iter_used", POINTER(c_double)), ("residrms", POINTER(c_double)), ("failed", POINTER(c_uint8))]
+
+
+class SSTuningEntry(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_char), ("mixed", c_int32), ("data_type", c_int32),
+                ("frame_h", c_int32), ("n_blocks", c_int32), ("frames", c_int32),
+                ("chosen", c_int32 * 3), ("trial_us", (c_double * 3) * 3),
+                ("samples", (c_int32 * 3) * 3), ("waves", (c_int32 * 3) * 3)]
 
 
 class SSServerConfig(ctypes.Structure):
@@ -179,6 +188,8 @@ API = [
     ("ss_rom_write", c_int32, [c_char_p, POINTER(SSGroupIn), POINTER(SSSettings)]),
     ("ss_rom_read", c_int32, [c_char_p, POINTER(c_void_p), POINTER(SSGroupIn), POINTER(SSSettings)]),
     ("ss_rom_free", None, [c_void_p]),
+    ("ss_tuning_mode", c_int32, [c_void_p, c_int32]),
+    ("ss_tuning_entries", c_int32, [c_void_p, POINTER(SSTuningEntry), c_int32]),
     ("ss_server_create", c_int32, [c_void_p, POINTER(SSServerConfig), POINTER(c_void_p)]),
     ("ss_server_destroy", None, [c_void_p]),
     ("ss_server_submit", c_int32, [c_void_p, POINTER(SSGroupIn), POINTER(SSSettings), POINTER(SSGroupOut),

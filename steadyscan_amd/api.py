@@ -88,6 +88,28 @@ class Handle:
     def release(self, rid):
         _check(N.lib().ss_release(self.ptr, int(rid)), self)
 
+    def tuning_mode(self, mode):
+        """0 (default), 1, 2: use that launch variant for every pixel
+        kernel; -1: tune each new group shape on its first tick."""
+        _check(N.lib().ss_tuning_mode(self.ptr, int(mode)), self)
+
+    def tuning(self):
+        """The autotuner's choices so far: one dict per group shape."""
+        n = N.lib().ss_tuning_entries(self.ptr, None, 0)
+        arr = (N.SSTuningEntry * max(n, 1))()
+        n = N.lib().ss_tuning_entries(self.ptr, arr, n)
+        kernels = ("assemble", "ls_round", "corr")
+        out = []
+        for e in arr[:n]:
+            d = dict(precision=e.precision.decode(), mixed=bool(e.mixed), data_type=int(e.data_type),
+                     frame_h=e.frame_h, n_blocks=e.n_blocks, frames=e.frames, kernels={})
+            for k, name in enumerate(kernels):
+                d["kernels"][name] = dict(
+                    chosen=int(e.chosen[k]), trial_us=[float(e.trial_us[k][v]) for v in range(3)],
+                    variants=[(int(e.samples[k][v]), int(e.waves[k][v])) for v in range(3)])
+            out.append(d)
+        return out
+
 
 _handles = {}
 
@@ -132,7 +154,7 @@ def load_rom(path):
         dtype = {N.SS_F64: (np.float64, c_double), N.SS_F32: (np.float32, c_float),
                  N.SS_U16: (np.uint16, ctypes.c_uint16), N.SS_I16: (np.int16, ctypes.c_int16)}[gi.data_type]
         call = dict(
-            flags=s.precision.decode(), template_image=arr(gi.template_image, (th, tw), c_double),
+            flags=s.precision.decode() + ("c" if s.save_frame_correlations else ""), template_image=arr(gi.template_image, (th, tw), c_double),
             xgrad=arr(gi.xgrad, (th, tw), c_double), ygrad=arr(gi.ygrad, (th, tw), c_double),
             tfrac=arr(gi.frac, (fh, nb), c_double), basex=arr(gi.base_x, (fh, nb), c_double),
             basey=arr(gi.base_y, (fh, nb), c_double), base_mask=arr(gi.base_mask, (fh, nb), c_uint8).astype(bool),
@@ -190,7 +212,13 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
 
     Returns the first `nargout` of
     (block_dx_knots, block_dy_knots, frame_corr, block_iter_used, block_mask,
-     residrms, deviceInitTime, iterationCorrelations, ..., allParams).
+     residrms, deviceInitTime, iterationCorrelations, iterationTimes,
+     functionTimes, frameCorrelations, iterationErrvals, rawParameters,
+     rgParams, allParams); outputs 9, 10, 13 and 14 are empty.  Flag 'c'
+    fills output 11 (n_frames x max_iter) and makes frame_corr the last
+    iteration's values, as the reference does (task.cpp:392-394); flag 'e'
+    fills output 12 (2*max_iter x n_frames; the reference stores it in
+    output 11's slot, mexgateway.cpp:287, which leaves output 12 unset).
     `info`, if a dict, receives the stop criterion and work counters.
     `template_bf16` selects mixed precision (precision 'S' only): BF16
     template gradients for the Jacobian, FP32 residuals, one FP64
@@ -216,6 +244,7 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     s.precision = precision.encode()
     s.linesearch_candidates = int(linesearch_candidates)
     s.template_bf16 = int(bool(template_bf16))
+    s.save_frame_correlations = int("c" in opts)
 
     gi = N.SSGroupIn()
     keep = []
@@ -286,6 +315,12 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     allp = np.zeros((2 * (nb * nF + 1), int(max_iter)), order="F") if nargout >= 15 else None
     if allp is not None:
         go.all_params = _ptr(allp, c_double)
+    fci = np.zeros((nF, int(max_iter)), order="F") if (nargout >= 11 and "c" in opts) else None
+    if fci is not None:
+        go.frame_corr_iter = _ptr(fci, c_double)
+    ierr = np.zeros((2 * int(max_iter), nF), order="F") if (nargout >= 12 and "e" in opts) else None
+    if ierr is not None:
+        go.iteration_errvals = _ptr(ierr, c_double)
     if "F" in opts:  # save the call's arguments (the 'F' flag, configuration.h:10-28)
         if N.lib().ss_rom_write(ROM_FILE.encode(), byref(gi), byref(s)) != 0:
             raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, f"could not write {ROM_FILE}")
@@ -300,8 +335,10 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     outs = [kx.reshape(-1, 1), ky.reshape(-1, 1), fc.reshape(-1, 1), np.int32(go.iterations),
             None if mask is None else mask.astype(bool), rr.reshape(-1, 1), go.device_init_time,
             icorr if (icorr is not None and "C" in opts) else np.zeros((0, 0))]
-    outs += [np.zeros((0, 0))] * 6
-    outs.append(allp if allp is not None else np.zeros((0, 0)))
+    empty = np.zeros((0, 0))
+    # 9, 10: timings; 11 ('c'), 12 ('e'); 13, 14: raw and red/green parameters
+    outs += [empty, empty, empty if fci is None else fci, empty if ierr is None else ierr, empty, empty]
+    outs.append(allp if allp is not None else empty)
     return tuple(outs[:nargout])
 
 
@@ -309,7 +346,7 @@ def _prepare_call(a, h, s, gi, go, with_mask, linesearch_candidates, keep):
     """Fills settings s, group input gi and output go for one call (a dict
     keyed by the mex argument names); returns the result dict whose arrays
     go points into.  Arrays that must outlive the call are appended to keep."""
-    precision, _ = parse_flags(a.get("flags", "D"))
+    precision, opts = parse_flags(a.get("flags", "D"))
     ds = default_settings()
     ctypes.memmove(ctypes.addressof(s), ctypes.addressof(ds), ctypes.sizeof(ds))
     s.max_iterations = int(a["max_iter"])
@@ -327,6 +364,7 @@ def _prepare_call(a, h, s, gi, go, with_mask, linesearch_candidates, keep):
     s.precision = precision.encode()
     s.linesearch_candidates = int(a.get("linesearch_candidates", linesearch_candidates))
     s.template_bf16 = int(bool(a.get("template_bf16", False)))
+    s.save_frame_correlations = int("c" in opts)
     if isinstance(a["template_image"], int):  # resident template id
         gi.template_id = a["template_image"]
         gi.template_h, gi.template_w = np.shape(a["xgrad"])
@@ -377,6 +415,13 @@ def _prepare_call(a, h, s, gi, go, with_mask, linesearch_candidates, keep):
         go.mask_on_device = 1
     elif r["mask"] is not None:
         go.mask = _ptr(r["mask"], c_uint8)
+    mi = int(a["max_iter"])
+    if "c" in opts:  # per-iteration frame correlations (output 11)
+        r["frame_corr_iter"] = np.zeros((nF, mi), order="F")
+        go.frame_corr_iter = _ptr(r["frame_corr_iter"], c_double)
+    if "e" in opts:  # per-iteration errvals (output 12)
+        r["iteration_errvals"] = np.zeros((2 * mi, nF), order="F")
+        go.iteration_errvals = _ptr(r["iteration_errvals"], c_double)
     return r
 
 

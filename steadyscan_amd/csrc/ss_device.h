@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ss_tuning.h"
+
 namespace ss {
 
 // Per-column partial sums written by the assembly pass (FP64).
@@ -32,6 +34,10 @@ enum CorrSum {
 
 // Per-column sums of the final pass (FP64).
 enum FinalSum { FS_D2 = 0, FS_CNT, FS_SA, FS_SAA, FS_SB, FS_SBB, FS_SAB, FS_N };
+// per-iteration frame correlation sums (flag 'c'), kept in finalsum's slots
+// while the group is in its correlation phase
+enum FrameCorrSum { FC_CNT = 0, FC_SA, FC_SAA, FC_SB, FC_SBB, FC_SAB, FC_N };
+static_assert(FC_N <= FS_N, "frame correlation sums share the final sums' slots");
 
 constexpr int kMaxCand = 4;  // fused line-search candidates per round
 
@@ -61,6 +67,7 @@ struct GroupDev {
     // sizes (mexgateway.cpp:176-198)
     int frameH, nb, nF, fframeW, N, th, tw, groupSize;
     int itemStart;        // first work item (= column) of this group in the batch
+    int var[TK_COUNT];    // launch variant of the assembly, line-search and correlation kernels
     int frameStart;       // first frame item of this group in the batch
     // settings
     double xmax, ymax;    // template_w - 1 - eps, template_h - 1 - eps
@@ -99,6 +106,10 @@ struct GroupDev {
     uint8_t *maskOut;
     double *resid, *fcorr;
     double *allParams, *iterCorr;  // optional diagnostics (or null)
+    // flag 'c': nF x maxIter frame correlations after every iteration, taken
+    // on the state the global correlation leaves (or null); flag 'e':
+    // 2*maxIter x nF errvals of both shifts after every continuing iteration
+    double *fcIter, *iterErr;
 };
 
 // Dynamic control state of one group (read by the host after each tick).
@@ -124,6 +135,8 @@ struct Batch {
 struct LaunchBatch {
     Batch b;
     int hasType[2 * IMG_NTYPES];  // [imgType + IMG_NTYPES * fp64]
+    int hasVar[TK_COUNT][2 * IMG_NTYPES];  // bit v: some group of the type uses variant v
+    int hasFrameCorr;     // some group saves per-iteration frame correlations (flag 'c')
     int itemGrid, frameGrid;
 };
 

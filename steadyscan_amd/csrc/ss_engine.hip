@@ -106,6 +106,19 @@ struct Profile {
 
 constexpr int kTickCap = 200000;
 
+// The group shape the autotuner keys its choices on: what sets the cost of a
+// column (precision, frame element type, mixed precision, gradients read or
+// derived, geometry form, column height, lines per run) and the number of
+// columns (blocks, frame count class).
+struct TuneKey {
+    int fp64, bf16, gradFD, imgType, affine, frameH, runLen, nb, framesClass;
+    bool operator<(const TuneKey &o) const {
+        const int a[] = {fp64, bf16, gradFD, imgType, affine, frameH, runLen, nb, framesClass};
+        const int b[] = {o.fp64, o.bf16, o.gradFD, o.imgType, o.affine, o.frameH, o.runLen, o.nb, o.framesClass};
+        return std::lexicographical_compare(a, a + 9, b, b + 9);
+    }
+};
+
 }  // namespace
 
 struct ss_handle {
@@ -127,6 +140,13 @@ struct ss_handle {
     bool useGraph = true;
     hipGraphExec_t tickExec = nullptr;
     ss::LaunchBatch tickFor{};
+    // autotuning: -1 tunes each new group shape, 0..kVariants-1 forces that variant
+    int tuneMode = 0;  // the default variants; -1 tunes each new group shape
+    std::map<TuneKey, ss_tuning_entry> tuned;
+    void *trialTable = nullptr;  // group table of the trial ticks
+    size_t trialTableBytes = 0;
+    void *trialStage = nullptr;  // its pinned staging
+    size_t trialStageBytes = 0;
 };
 
 namespace {
@@ -467,12 +487,13 @@ struct Plan {
     size_t oTp = 0, oGxv = 0, oGyh = 0, oGeo = 0, oFn = 0, oImg = 0, oMin = 0;
     size_t oP = 0, oPL = 0, oP0 = 0, oD0 = 0, oDP = 0, oErr = 0, oCol = 0, oLs = 0, oCorr = 0,
            oFin = 0, oWs = 0, oUsed = 0, oAct = 0, oNext = 0, oMk = 0, oDk = 0, oMask = 0,
-           oRes = 0, oFc = 0, oAll = 0, oIc = 0;
+           oRes = 0, oFc = 0, oAll = 0, oIc = 0, oFci = 0, oIe = 0;
     size_t hImg = 0, hP = 0;  // host staging offsets
     size_t dBytes = 0, hBytes = 0;
     int imgType = 0;
     size_t imgBytes = 0;
     bool staged = false;  // host frames already in the staging (stage_host_frames)
+    int var[ss::TK_COUNT] = {0, 0, 0};  // launch variants of the pixel kernels
 };
 
 // Validates one call and lays out its device block and host staging.
@@ -552,6 +573,8 @@ ss_status plan_group(ss_handle *h, Plan &P) {
     P.oFc = A.take<double>(nF);
     if (P.out->all_params) P.oAll = A.take<double>(Pn * P.s->max_iterations);
     if (P.out->iteration_corr) P.oIc = A.take<double>(P.s->max_iterations);
+    if (P.s->save_frame_correlations) P.oFci = A.take<double>((size_t)nF * P.s->max_iterations);
+    if (P.out->iteration_errvals) P.oIe = A.take<double>(2 * (size_t)nF * P.s->max_iterations);
     P.hP = H.take<double>(Pn + 2 * (size_t)nF);
     P.dBytes = A.off;
     P.hBytes = H.off;
@@ -626,6 +649,7 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     g.th = P.tmpl.th;
     g.tw = P.tmpl.tw;
     g.groupSize = in.group_size;
+    for (int k = 0; k < ss::TK_COUNT; k++) g.var[k] = P.var[k];
     const double eps = s.precision == 'S' ? (double)FLT_EPSILON : DBL_EPSILON;
     g.fp64 = P.fp64;
     g.xmax = (double)(g.tw - 1) - eps;
@@ -756,6 +780,12 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     g.fcorr = (double *)dptr(P.oFc);
     g.allParams = P.out->all_params ? (double *)dptr(P.oAll) : nullptr;
     g.iterCorr = P.out->iteration_corr ? (double *)dptr(P.oIc) : nullptr;
+    // flags 'c' and 'e': entries of iterations that do not run stay 0
+    const size_t mi = (size_t)s.max_iterations;
+    g.fcIter = s.save_frame_correlations ? (double *)dptr(P.oFci) : nullptr;
+    g.iterErr = P.out->iteration_errvals ? (double *)dptr(P.oIe) : nullptr;
+    if (g.fcIter) HIPCHK(hipMemsetAsync(g.fcIter, 0, (size_t)nF * mi * sizeof(double), h->stream));
+    if (g.iterErr) HIPCHK(hipMemsetAsync(g.iterErr, 0, 2 * (size_t)nF * mi * sizeof(double), h->stream));
     // initial knots (task_allocate.cu:130-134), deltaP = 0, errval = 1e6 (task_allocate.cu:150-184)
     double *hp = (double *)(HS + P.hP);
     std::memset(hp, 0, sizeof(double) * Pn);
@@ -801,6 +831,12 @@ ss_status finish_group(ss_handle *h, const Plan &P, const ss::GroupDev &g, const
         HIPCHK(hipMemcpyAsync(o.all_params, g.allParams, Pn * nrec * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (o.iteration_corr && nrec > 0)
         HIPCHK(hipMemcpyAsync(o.iteration_corr, g.iterCorr, nrec * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    const size_t mi = (size_t)P.s->max_iterations;
+    if (o.frame_corr_iter && g.fcIter)
+        HIPCHK(hipMemcpyAsync(o.frame_corr_iter, g.fcIter, nF * mi * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (o.iteration_errvals)
+        HIPCHK(hipMemcpyAsync(o.iteration_errvals, g.iterErr, 2 * nF * mi * sizeof(double), hipMemcpyDeviceToHost,
+                              h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (o.knots_x && o.knots_y) knots_out(pl.data(), nF, nb, o.knots_x, o.knots_y);
     o.iterations = c.iteration;
@@ -854,7 +890,11 @@ ss_status upload_table(ss_handle *h, Table &T, std::vector<ss::GroupDev> &gd, co
     ss::GroupDev *hg = (ss::GroupDev *)(HS + oG);
     int *hi = (int *)(HS + oI), *hf = (int *)(HS + oF);
     int item = 0, frame = 0;
-    for (int i = 0; i < 2 * ss::IMG_NTYPES; i++) L.hasType[i] = 0;
+    for (int i = 0; i < 2 * ss::IMG_NTYPES; i++) {
+        L.hasType[i] = 0;
+        for (int k = 0; k < ss::TK_COUNT; k++) L.hasVar[k][i] = 0;
+    }
+    L.hasFrameCorr = 0;
     for (int gi = 0; gi < n; gi++) {
         ss::GroupDev &g = gd[gi];
         g.itemStart = item;
@@ -865,6 +905,8 @@ ss_status upload_table(ss_handle *h, Table &T, std::vector<ss::GroupDev> &gd, co
         item += g.fframeW;
         frame += g.nF;
         L.hasType[g.imgType + ss::IMG_NTYPES * g.fp64] = 1;
+        for (int k = 0; k < ss::TK_COUNT; k++) L.hasVar[k][g.imgType + ss::IMG_NTYPES * g.fp64] |= 1 << g.var[k];
+        if (g.fcIter) L.hasFrameCorr = 1;
     }
     std::memcpy(HS + oC, gc, n * sizeof(ss::GroupCtl));
     char *D = (char *)T.mem;
@@ -892,14 +934,16 @@ size_t table_bytes(int nGroups, int nItems, int nFrames) {
 
 // One tick: every group advances through the stage sequence (a kernel skips
 // groups in other phases); the control blocks come back to hc.
+// three line-search rounds per half-iteration: most half-iterations finish
+// within them, so a tick is usually one Gauss-Newton iteration
+const int kTick[] = {ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE, ss::ST_LS, ss::ST_LS_DECIDE,
+                     ss::ST_LS, ss::ST_LS_DECIDE,
+                     ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE, ss::ST_LS, ss::ST_LS_DECIDE,
+                     ss::ST_LS, ss::ST_LS_DECIDE,
+                     ss::ST_CORR, ss::ST_CORR_FINISH, ss::ST_FINAL, ss::ST_FINAL_FINISH};
+
 ss_status run_tick(ss_handle *h, const ss::LaunchBatch &L, ss::GroupCtl *hc) {
-    // three line-search rounds per half-iteration: most half-iterations
-    // finish within them, so a tick is usually one Gauss-Newton iteration
-    static const int tick[] = {ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE, ss::ST_LS, ss::ST_LS_DECIDE,
-                               ss::ST_LS, ss::ST_LS_DECIDE,
-                               ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE, ss::ST_LS, ss::ST_LS_DECIDE,
-                               ss::ST_LS, ss::ST_LS_DECIDE,
-                               ss::ST_CORR, ss::ST_CORR_FINISH, ss::ST_FINAL, ss::ST_FINAL_FINISH};
+    const auto &tick = kTick;
     if (h->prof.on || !h->useGraph) {  // per-stage events need plain launches
         for (int stg : tick) HIPCHK(run_stage(h, L, stg));
     } else {
@@ -937,6 +981,120 @@ ss_status run_tick(ss_handle *h, const ss::LaunchBatch &L, ss::GroupCtl *hc) {
 
 bool finished(const ss::GroupCtl &c) { return c.phase == ss::PH_DONE || c.phase == ss::PH_ERROR; }
 
+// ---------------------------------------------------------------------------
+// autotuning of the pixel kernels' launch variants
+// ---------------------------------------------------------------------------
+TuneKey tune_key(const Plan &P) {
+    int fc = 0;
+    for (int n = P.in->n_frames; n > 1; n >>= 1) fc++;  // floor(log2(frames))
+    return TuneKey{P.fp64, P.bf16, P.tmpl.gradFD, P.imgType, P.geom.affine, P.in->frame_h, P.geom.runLen,
+                   P.in->n_blocks, fc};
+}
+
+// One tick of the batch L with plain launches, HIP events around the
+// assembly, line-search and correlation launches; ms receives each kernel's
+// summed time.
+ss_status trial_tick(ss_handle *h, const ss::LaunchBatch &L, double ms[ss::TK_COUNT]) {
+    struct Ev {
+        int k;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> ev;
+    ss_status st = SS_OK;
+    for (int stg : kTick) {
+        const int k = stg == ss::ST_ASSEMBLE ? ss::TK_ASSEMBLE : stg == ss::ST_LS ? ss::TK_LS
+                      : stg == ss::ST_CORR   ? ss::TK_CORR
+                                             : -1;
+        hipEvent_t a = k >= 0 ? prof_event(h) : nullptr, b = k >= 0 ? prof_event(h) : nullptr;
+        if (a && b && hipEventRecord(a, h->stream) != hipSuccess) st = fail(h, SS_ERR_DEVICE, "trial tick event");
+        if (!st && ss::launch_stage(L, stg, h->stream) != hipSuccess) st = fail(h, SS_ERR_DEVICE, "trial tick launch");
+        if (a && b && !st && hipEventRecord(b, h->stream) != hipSuccess) st = fail(h, SS_ERR_DEVICE, "trial tick event");
+        if (a && b) ev.push_back({k, a, b});
+        if (st) break;
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess && !st) st = fail(h, SS_ERR_DEVICE, "trial tick");
+    for (auto &e : ev) {
+        float t = 0.f;
+        if (!st && hipEventElapsedTime(&t, e.a, e.b) == hipSuccess) ms[e.k] += t;
+        h->prof.pool.push_back(e.a);
+        h->prof.pool.push_back(e.b);
+    }
+    return st;
+}
+
+// Picks the launch variants of group P (device block D, pinned staging HS):
+// the forced variant, the handle's choice for the group's shape, or, for a
+// shape seen for the first time, the fastest variant per kernel measured
+// on the group's own first tick, run twice per variant from the initial
+// state (the real set-up follows and starts again from it).
+ss_status tune_group(ss_handle *h, Plan &P, char *D, char *HS) {
+    if (h->tuneMode >= 0) {
+        for (int k = 0; k < ss::TK_COUNT; k++) P.var[k] = h->tuneMode;
+        return SS_OK;
+    }
+    const TuneKey key = tune_key(P);
+    auto it = h->tuned.find(key);
+    if (it != h->tuned.end()) {
+        for (int k = 0; k < ss::TK_COUNT; k++) P.var[k] = it->second.chosen[k];
+        return SS_OK;
+    }
+    ss_tuning_entry e;
+    std::memset(&e, 0, sizeof(e));
+    e.precision = P.fp64 ? 'D' : 'S';
+    e.mixed = P.bf16;
+    e.data_type = P.in->data_type;
+    e.frame_h = P.in->frame_h;
+    e.n_blocks = P.in->n_blocks;
+    e.frames = P.in->n_frames;
+    for (int k = 0; k < ss::TK_COUNT; k++)
+        for (int v = 0; v < ss::kVariants; v++) {
+            e.samples[k][v] = ss::kTuneTable[k][P.fp64][v].samples;
+            e.waves[k][v] = ss::kTuneTable[k][P.fp64][v].waves;
+        }
+    const int fw = P.in->n_blocks * P.in->n_frames;
+    const size_t tb = table_bytes(1, fw, P.in->n_frames);
+    if (h->trialStageBytes < tb) {
+        if (h->trialStage) HIPCHK(hipHostFree(h->trialStage));
+        h->trialStage = nullptr;
+        h->trialStageBytes = 0;
+        if (hipHostMalloc(&h->trialStage, tb, hipHostMallocDefault) != hipSuccess)
+            return fail(h, SS_ERR_OUT_OF_MEMORY, "unable to allocate pinned host memory");
+        h->trialStageBytes = tb;
+    }
+    // two passes over the variants: the first only loads and warms each
+    // variant's code, the second is timed
+    for (int pass = 0; pass < 2; pass++)
+    for (int v = 0; v < ss::kVariants; v++) {
+        for (int k = 0; k < ss::TK_COUNT; k++) P.var[k] = v;
+        std::vector<ss::GroupDev> gd(1);
+        ss::GroupCtl c;
+        ss_status st = setup_group(h, P, D, HS, gd[0], c);
+        if (st) return st;
+        Table T;
+        T.mem = h->trialTable;
+        T.bytes = h->trialTableBytes;
+        ss::LaunchBatch L{};
+        st = upload_table(h, T, gd, &c, h->trialStage, h->trialStageBytes, L);
+        h->trialTable = T.mem;
+        h->trialTableBytes = T.bytes;
+        if (st) return st;
+        double ms[ss::TK_COUNT] = {0, 0, 0};
+        st = trial_tick(h, L, ms);  // ends with a synchronisation: HS may be written again
+        if (st) return st;
+        if (pass == 1)
+            for (int k = 0; k < ss::TK_COUNT; k++) e.trial_us[k][v] = ms[k] * 1e3;
+    }
+    for (int k = 0; k < ss::TK_COUNT; k++) {
+        int best = 0;
+        for (int v = 1; v < ss::kVariants; v++)
+            if (e.trial_us[k][v] < e.trial_us[k][best]) best = v;
+        e.chosen[k] = best;
+        P.var[k] = best;
+    }
+    h->tuned[key] = e;
+    return SS_OK;
+}
+
 ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settings *ss_,
                     ss_group_out *outs) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -970,6 +1128,8 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
     std::vector<ss::GroupDev> gd(n);
     for (int gi = 0; gi < n; gi++) {
         Plan &P = plans[gi];
+        st = tune_group(h, P, D, HG[gi]);
+        if (st) return st;
         st = setup_group(h, P, D, HG[gi], gd[gi], hc[gi]);
         if (st) return st;
         D += (P.dBytes + 255) & ~(size_t)255;
@@ -1200,7 +1360,8 @@ ss_status server_admit(ss_server *S) {
         auto &slot = fresh[k];
         slot.P.in = &slot.req.in;  // the Plan points at the slot's own request copy
         slot.P.s = &slot.req.s;
-        st = setup_group(h, slot.P, slot.mem, HS, slot.g, slot.c);
+        st = tune_group(h, slot.P, slot.mem, HS);
+        if (!st) st = setup_group(h, slot.P, slot.mem, HS, slot.g, slot.c);
         if (st) {
             drop(slot, st);
             continue;
@@ -1458,12 +1619,31 @@ void ss_destroy(ss_handle *h) {
     if (h->arena) e = hipFree(h->arena);
     if (h->harena) e = hipHostFree(h->harena);
     if (h->scratch) e = hipFree(h->scratch);
+    if (h->trialTable) e = hipFree(h->trialTable);
+    if (h->trialStage) e = hipHostFree(h->trialStage);
     if (h->stream) e = hipStreamDestroy(h->stream);
     (void)e;
     delete h;
 }
 
 const char *ss_last_error(const ss_handle *h) { return h ? h->err.c_str() : "no handle"; }
+
+ss_status ss_tuning_mode(ss_handle *h, int32_t mode) {
+    if (!h) return SS_ERR_INVALID_ARGUMENT;
+    if (mode < -1 || mode >= ss::kVariants) return fail(h, SS_ERR_INVALID_ARGUMENT, "tuning mode must be -1 .. 2");
+    h->tuneMode = mode;
+    return SS_OK;
+}
+
+int32_t ss_tuning_entries(const ss_handle *h, ss_tuning_entry *out, int32_t cap) {
+    if (!h) return 0;
+    int32_t i = 0;
+    for (const auto &kv : h->tuned) {
+        if (out && i < cap) out[i] = kv.second;
+        i++;
+    }
+    return i;
+}
 
 ss_status ss_upload_template(ss_handle *h, int32_t th, int32_t tw, const double *t,
                              const double *xg, const double *yg, int32_t *id) try {

@@ -364,7 +364,7 @@ __device__ __forceinline__ const TI *column_ptr(const GroupDev &g, int c) {
 // (d_applyParameters, d_createDifferenceImage, d_calcBaseErrval,
 //  d_updateGradients, d_fillDiagonals, d_fillJTr in one pass)
 // ---------------------------------------------------------------------------
-template <typename TI, typename R>
+template <typename TI, typename R, int V>
 __device__ void assemble_column(const GroupDev &g, int c) {
     const int b = c % g.nb;
     const ColKnots<R> kn = current_knots<R>(g, c);
@@ -379,7 +379,7 @@ __device__ void assemble_column(const GroupDev &g, int c) {
 #pragma unroll
     for (int i = 0; i < CS_N - 1; i++) h[i] = (R)0;
     int cnt = 0;
-    constexpr int CH = chunk_rows<R>(2, SS_TUNE(ASSEMBLE, R, SAMPLES));
+    constexpr int CH = chunk_rows<R>(2, tune_variant<TK_ASSEMBLE, R, V>().samples);
     for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
         const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
@@ -447,14 +447,14 @@ __device__ void assemble_column(const GroupDev &g, int c) {
     if (threadIdx.x < CS_N) g.colsum[(size_t)c * CS_N + threadIdx.x] = out[threadIdx.x];
 }
 
-template <typename TI, typename R>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(SS_TUNE(ASSEMBLE, R, WAVES)))) void k_assemble(Batch B) {
+template <typename TI, typename R, int V>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_variant<TK_ASSEMBLE, R, V>().waves))) void k_assemble(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         if (B.ctl[gi].phase != PH_ASSEMBLE) continue;
         const GroupDev &g = B.groups[gi];
-        if (!runs_here<TI, R>(g)) continue;
-        assemble_column<TI, R>(g, item - g.itemStart);
+        if (!runs_here<TI, R>(g) || g.var[TK_ASSEMBLE] != V) continue;
+        assemble_column<TI, R, V>(g, item - g.itemStart);
     }
 }
 
@@ -1080,7 +1080,7 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
 // the round's bookkeeping.  Three kernels, so that no device-scope fence is
 // needed: on gfx950 each one writes back and invalidates the XCD's L2.
 // ---------------------------------------------------------------------------
-template <typename TI, typename R, int K>
+template <typename TI, typename R, int V, int K>
 __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
     ColKnots<R> kn[K];
     {
@@ -1110,7 +1110,7 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
         e2[k] = 0.0;
         cnt[k] = 0;
     }
-    constexpr int CH = chunk_rows<R>(K, SS_TUNE(LS, R, SAMPLES));
+    constexpr int CH = chunk_rows<R>(K, tune_variant<TK_LS, R, V>().samples);
     for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
         const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
@@ -1151,21 +1151,21 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
     if (threadIdx.x < 2 * K) g.lssum[(size_t)c * 2 * kMaxCand + threadIdx.x] = out[threadIdx.x];
 }
 
-template <typename TI, typename R>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(SS_TUNE(LS, R, WAVES)))) void k_ls_round(Batch B) {
+template <typename TI, typename R, int V>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_variant<TK_LS, R, V>().waves))) void k_ls_round(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         const GroupCtl &ctl = B.ctl[gi];
         if (ctl.phase != PH_LS) continue;
         const GroupDev &g = B.groups[gi];
-        if (!runs_here<TI, R>(g)) continue;
+        if (!runs_here<TI, R>(g) || g.var[TK_LS] != V) continue;
         const int c = item - g.itemStart, f = c / g.nb, b = c % g.nb;
         if (!g.lsActive[f]) continue;
         const int K = ctl.K, k0 = g.lsNextK[f];
         switch (K) {
-        case 1: ls_column<TI, R, 1>(g, c, b, k0); break;
-        case 2: ls_column<TI, R, 2>(g, c, b, k0); break;
-        default: ls_column<TI, R, 4>(g, c, b, k0); break;
+        case 1: ls_column<TI, R, V, 1>(g, c, b, k0); break;
+        case 2: ls_column<TI, R, V, 2>(g, c, b, k0); break;
+        default: ls_column<TI, R, V, 4>(g, c, b, k0); break;
         }
     }
 }
@@ -1342,7 +1342,7 @@ __global__ __launch_bounds__(64) void k_ls_finish(Batch B) {
 // task_correlation.cu:63-99) of every frame's last-tried state; the
 // statistic of a column covers its leading run of masked rows only.
 // ---------------------------------------------------------------------------
-template <typename TI, typename R>
+template <typename TI, typename R, int V>
 __device__ void corr_column(const GroupDev &g, int c) {
     const int f = c / g.nb, b = c % g.nb;
     const int mk = g.lsMk[f], dk = g.lsDk[f];
@@ -1354,7 +1354,7 @@ __device__ void corr_column(const GroupDev &g, int c) {
     // first row outside the mask of the last-tried state: the moments cover
     // the rows before it
     int firstOut = g.frameH;
-    constexpr int CH = chunk_rows<R>(1, SS_TUNE(CORR, R, SAMPLES));
+    constexpr int CH = chunk_rows<R>(1, tune_variant<TK_CORR, R, V>().samples);
     for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
         const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
@@ -1425,14 +1425,99 @@ __device__ void corr_column(const GroupDev &g, int c) {
     if (threadIdx.x < CR_N) g.corrsum[(size_t)c * CR_N + threadIdx.x] = out[threadIdx.x];
 }
 
-template <typename TI, typename R>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(SS_TUNE(CORR, R, WAVES)))) void k_corr(Batch B) {
+template <typename TI, typename R, int V>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_variant<TK_CORR, R, V>().waves))) void k_corr(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         if (B.ctl[gi].phase != PH_CORR) continue;
         const GroupDev &g = B.groups[gi];
-        if (!runs_here<TI, R>(g)) continue;
-        corr_column<TI, R>(g, item - g.itemStart);
+        if (!runs_here<TI, R>(g) || g.var[TK_CORR] != V) continue;
+        corr_column<TI, R, V>(g, item - g.itemStart);
+    }
+}
+
+// Flag 'c': the frame correlation taken right after the global correlation
+// (task.cpp:286-293).  The global correlation swaps every masked pixel to
+// image - difference but swaps back only the row-major pixels below its
+// grid, fframeW rounded up to 256 (task_correlation.cu:97); the frame
+// correlation then swaps every masked pixel once more
+// (task_correlation.cu:176-178).  Sums over all of the column's rows, in
+// finalsum's slots (free while the group is in this phase).  A kernel of its
+// own, launched only for batches that ask for it, so that the correlation
+// kernel's register allocation does not carry it.
+template <typename TI, typename R>
+__device__ void frame_corr_column(const GroupDev &g, int c) {
+    const int f = c / g.nb, b = c % g.nb;
+    const int mk = g.lsMk[f], dk = g.lsDk[f];
+    const bool same = (mk == dk);
+    const ColKnots<R> km = state_knots<R>(g, c, mk);
+    const ColKnots<R> kd = state_knots<R>(g, c, dk);
+    const TI *img = column_ptr<TI>(g, c);
+    const typename Prec<R>::V4 *geo = Prec<R>::geo(g) ? Prec<R>::geo(g) + (size_t)b * g.frameH : nullptr;
+    constexpr int CH = chunk_rows<R>(1, sizeof(R) == 8 ? kFinalSamplesF64 : kFinalSamplesF32);
+    const long long swapBack = min((long long)((g.fframeW + 255) / 256) * 256, (long long)g.frameH * g.fframeW);
+    double w[FC_N];
+#pragma unroll
+    for (int i = 0; i < FC_N; i++) w[i] = 0.0;
+    for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
+        const int rend = min(run * g.runLen + g.runLen, g.frameH);
+        GeoIter<R> it;
+        it.start(g, b, run * g.runLen);
+        for (int r0 = run * g.runLen; r0 < rend; r0 += CH) {
+            const int n = min(CH, rend - r0);
+            R I[CH];
+            load_chunk<CH>(img + r0, g.vec, n, I);
+#pragma unroll
+            for (int q = 0; q < CH; q++) {
+                R bx, by, f1;
+                bool m;
+                it.at(g, geo, min(r0 + q, g.frameH - 1), bx, by, f1, m);
+                it.next();
+                m = m && q < n;
+                const R x = mad(km.dx, f1, bx + km.ax);
+                const R y = mad(km.dy, f1, by + km.ay);
+                const bool mm = m && in_bounds(g, x, y);
+                R xd = x, yd = y;
+                bool okd = mm;
+                if (!same) {
+                    xd = mad(kd.dx, f1, bx + kd.ax);
+                    yd = mad(kd.dy, f1, by + kd.ay);
+                    okd = m && in_bounds(g, xd, yd);
+                }
+                Cell<R> cl;
+                const R smp = sample(g, xd, yd, okd, cl);
+                const R Ir = q < n ? I[q] : (R)0;
+                const R dbuf = okd ? Ir - smp : (R)0;
+                const bool pre = (long long)(r0 + q) * g.fframeW + c < swapBack;
+                const R d1 = Ir - dbuf;           // the global correlation's swap
+                const R d2 = pre ? Ir - d1 : d1;  // its partial swap-back
+                const R d3 = mm ? Ir - d2 : dbuf; // the frame correlation's swap
+                const double a = mm ? (double)d3 : 0.0, bb = mm ? (double)Ir : 0.0;
+                // an unmasked pixel enters the reference's second pass times 0
+                // (task_correlation.cu:150-156): NaN or Inf there gives NaN
+                const double za = q < n ? (double)d3 * 0.0 : 0.0, zb = (double)Ir * 0.0;
+                w[FC_CNT] += mm ? 1.0 : 0.0;
+                w[FC_SA] += a;
+                w[FC_SAA] += mm ? a * a : za * za;
+                w[FC_SB] += bb;
+                w[FC_SBB] += mm ? bb * bb : zb * zb;
+                w[FC_SAB] += a * bb;
+            }
+        }
+    }
+    __shared__ double outF[FC_N];
+    block_sum<FC_N>(w, outF);
+    if (threadIdx.x < FC_N) g.finalsum[(size_t)c * FS_N + threadIdx.x] = outF[threadIdx.x];
+}
+
+template <typename TI, typename R>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(2))) void k_corr_frames(Batch B) {
+    for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
+        const int gi = B.itemGroup[item];
+        if (B.ctl[gi].phase != PH_CORR) continue;
+        const GroupDev &g = B.groups[gi];
+        if (!g.fcIter || !runs_here<TI, R>(g)) continue;
+        frame_corr_column<TI, R>(g, item - g.itemStart);
     }
 }
 
@@ -1443,6 +1528,8 @@ __global__ __launch_bounds__(kSolveWG) void k_corr_finish(Batch B) {
     GroupCtl &ctl = B.ctl[gi];
     if (ctl.phase != PH_CORR) return;
     const GroupDev &g = B.groups[gi];
+    // read before the barriers below: thread 0 advances it afterwards
+    const int it = ctl.iteration;
     double v[CR_N + 1];
 #pragma unroll
     for (int i = 0; i <= CR_N; i++) v[i] = 0.0;
@@ -1453,10 +1540,21 @@ __global__ __launch_bounds__(kSolveWG) void k_corr_finish(Batch B) {
     __shared__ double tot[CR_N + 1];
     block_sum<CR_N + 1>(v, tot);
     const int P = 2 * g.N;
-    const int it = ctl.iteration;
+    if (g.fcIter && it <= g.maxIter)  // flag 'c' (task.cpp:304-306)
+        for (int f = threadIdx.x; f < g.nF; f += kSolveWG) {
+            double sf[FC_N];
+            for (int i = 0; i < FC_N; i++) sf[i] = 0.0;
+            for (int bb = 0; bb < g.nb; bb++)
+                for (int i = 0; i < FC_N; i++) sf[i] += g.finalsum[(size_t)(f * g.nb + bb) * FS_N + i];
+            const double nn = sf[FC_CNT], mA = sf[FC_SA] / nn, mB = sf[FC_SB] / nn;
+            const double assq = sf[FC_SAA] - nn * mA * mA;
+            const double bssq = sf[FC_SBB] - nn * mB * mB;
+            const double q = sf[FC_SAB] - nn * mA * mB;
+            g.fcIter[(size_t)(it - 1) * g.nF + f] = q / sqrt(assq * bssq);
+        }
     if (g.allParams && it <= g.maxIter)
         for (int i = threadIdx.x; i < P; i += kSolveWG) g.allParams[(size_t)(it - 1) * P + i] = g.p[i];
-    __shared__ int s_copy;
+    __shared__ int s_copy, s_save;
     if (threadIdx.x == 0) {
         const double nm = tot[CR_NMASK];
         const double mA = tot[CR_N] / nm, mB = tot[CR_SUMD] / nm, ns = tot[CR_NS];
@@ -1467,7 +1565,7 @@ __global__ __launch_bounds__(kSolveWG) void k_corr_finish(Batch B) {
         ctl.corr = corr;
         if (g.iterCorr && it <= g.maxIter) g.iterCorr[it - 1] = corr;
         const double correlation = ctl.correlation;
-        int copy = 0;
+        int copy = 0, save = 0;
         if (corr - correlation < g.cit) {
             ctl.retv = 0;  // CorrelationIncreaseThreshold
             ctl.phase = PH_FINAL;
@@ -1496,6 +1594,7 @@ __global__ __launch_bounds__(kSolveWG) void k_corr_finish(Batch B) {
         } else {
             ctl.correlation = corr;
             copy = 1;
+            save = 1;
             ctl.iteration = it + 1;
             ctl.improved = 1;
             ctl.shift = 0;
@@ -1503,10 +1602,17 @@ __global__ __launch_bounds__(kSolveWG) void k_corr_finish(Batch B) {
         }
         if (ctl.phase == PH_FINAL && it == 1) copy = 1;  // task.cpp:381-383
         s_copy = copy;
+        s_save = save;
     }
     __syncthreads();
     if (s_copy)
         for (int i = threadIdx.x; i < P; i += kSolveWG) g.pLast[i] = g.p[i];
+    if (s_save && g.iterErr)  // flag 'e': an iteration that continues (task.cpp:362-370)
+        for (int f = threadIdx.x; f < g.nF; f += kSolveWG) {
+            const size_t o = (size_t)f * 2 * g.maxIter + 2 * (size_t)(it - 1);
+            g.iterErr[o] = g.errval[f];
+            g.iterErr[o + 1] = g.errval[g.nF + f];
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -1523,7 +1629,7 @@ __device__ void final_column(const GroupDev &g, int c) {
     double v[FS_N];
 #pragma unroll
     for (int i = 0; i < FS_N; i++) v[i] = 0.0;
-    constexpr int CH = chunk_rows<R>(1, SS_TUNE(FINAL, R, SAMPLES));
+    constexpr int CH = chunk_rows<R>(1, sizeof(R) == 8 ? kFinalSamplesF64 : kFinalSamplesF32);
     for (int run = threadIdx.x; run * g.runLen < g.frameH; run += kWG) {
         const int rend = min(run * g.runLen + g.runLen, g.frameH);
         GeoIter<R> it;
@@ -1562,7 +1668,7 @@ __device__ void final_column(const GroupDev &g, int c) {
 }
 
 template <typename TI, typename R>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(SS_TUNE(FINAL, R, WAVES)))) void k_final(Batch B) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? kFinalWavesF64 : kFinalWavesF32))) void k_final(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         if (B.ctl[gi].phase != PH_FINAL) continue;
@@ -1588,7 +1694,8 @@ __global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
         const double assq = s[FS_SAA] - nn * mA * mA;
         const double bssq = s[FS_SBB] - nn * mB * mB;
         const double q = s[FS_SAB] - nn * mA * mB;
-        g.fcorr[f] = q / sqrt(assq * bssq);
+        // flag 'c': the last iteration's correlations stand (task.cpp:392-394)
+        g.fcorr[f] = g.fcIter ? g.fcIter[(size_t)(ctl.iteration - 1) * g.nF + f] : q / sqrt(assq * bssq);
     }
     __syncthreads();
     if (threadIdx.x == 0) ctl.phase = PH_DONE;
@@ -1602,6 +1709,26 @@ __global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
     do {                                                                                          \
         if ((L).hasType[img_type<TI>() + IMG_NTYPES * (sizeof(R) == 8 ? 1 : 0)])                  \
             hipLaunchKernelGGL((kernel<TI, R>), grid, dim3(kWG), 0, s, (L).b);                   \
+    } while (0)
+// kernels built in kVariants variants: one launch per variant some group of
+// the type and precision uses
+#define SS_LAUNCH_VAR_ONE(kernel, TK, TI, R, grid, L, s)                                          \
+    do {                                                                                          \
+        const int m_ = (L).hasVar[TK][img_type<TI>() + IMG_NTYPES * (sizeof(R) == 8 ? 1 : 0)];    \
+        if (m_ & 1) hipLaunchKernelGGL((kernel<TI, R, 0>), grid, dim3(kWG), 0, s, (L).b);         \
+        if (m_ & 2) hipLaunchKernelGGL((kernel<TI, R, 1>), grid, dim3(kWG), 0, s, (L).b);         \
+        if (m_ & 4) hipLaunchKernelGGL((kernel<TI, R, 2>), grid, dim3(kWG), 0, s, (L).b);         \
+    } while (0)
+#define SS_LAUNCH_VAR(kernel, TK, grid, L, s)                       \
+    do {                                                            \
+        SS_LAUNCH_VAR_ONE(kernel, TK, float, float, grid, L, s);    \
+        SS_LAUNCH_VAR_ONE(kernel, TK, uint16_t, float, grid, L, s); \
+        SS_LAUNCH_VAR_ONE(kernel, TK, int16_t, float, grid, L, s);  \
+        SS_LAUNCH_VAR_ONE(kernel, TK, double, float, grid, L, s);   \
+        SS_LAUNCH_VAR_ONE(kernel, TK, float, double, grid, L, s);   \
+        SS_LAUNCH_VAR_ONE(kernel, TK, uint16_t, double, grid, L, s);\
+        SS_LAUNCH_VAR_ONE(kernel, TK, int16_t, double, grid, L, s); \
+        SS_LAUNCH_VAR_ONE(kernel, TK, double, double, grid, L, s);  \
     } while (0)
 #define SS_LAUNCH_TYPED(kernel, grid, L, s)                  \
     do {                                                     \
@@ -1618,14 +1745,17 @@ __global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
 hipError_t launch_stage(const LaunchBatch &L, int stage, hipStream_t s) {
     const dim3 items(L.itemGrid), groups(L.b.nGroups);
     switch (stage) {
-    case ST_ASSEMBLE: SS_LAUNCH_TYPED(k_assemble, items, L, s); break;
+    case ST_ASSEMBLE: SS_LAUNCH_VAR(k_assemble, TK_ASSEMBLE, items, L, s); break;
     case ST_SOLVE: hipLaunchKernelGGL(k_solve, groups, dim3(kSolveWG), 0, s, L.b); break;
-    case ST_LS: SS_LAUNCH_TYPED(k_ls_round, items, L, s); break;
+    case ST_LS: SS_LAUNCH_VAR(k_ls_round, TK_LS, items, L, s); break;
     case ST_LS_DECIDE:
         hipLaunchKernelGGL(k_ls_decide, dim3(L.frameGrid), dim3(64), 0, s, L.b);
         hipLaunchKernelGGL(k_ls_finish, groups, dim3(64), 0, s, L.b);
         break;
-    case ST_CORR: SS_LAUNCH_TYPED(k_corr, items, L, s); break;
+    case ST_CORR:
+        SS_LAUNCH_VAR(k_corr, TK_CORR, items, L, s);
+        if (L.hasFrameCorr) SS_LAUNCH_TYPED(k_corr_frames, items, L, s);
+        break;
     case ST_CORR_FINISH: hipLaunchKernelGGL(k_corr_finish, groups, dim3(kSolveWG), 0, s, L.b); break;
     case ST_FINAL: SS_LAUNCH_TYPED(k_final, items, L, s); break;
     case ST_FINAL_FINISH: hipLaunchKernelGGL(k_final_finish, groups, dim3(kSolveWG), 0, s, L.b); break;

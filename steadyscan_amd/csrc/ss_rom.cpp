@@ -91,6 +91,7 @@ extern "C" ss_status ss_rom_write(const char *path, const ss_group_in *in, const
     w.i32("DEVI", s->device_id);
     w.i32("LSCK", s->linesearch_candidates);
     w.i32("TBF6", s->template_bf16);
+    w.i32("SVFC", s->save_frame_correlations);
     w.i32("GRPS", in->group_size);
     w.i32("NFRM", in->n_frames);
     w.rec("TMPL", E_F64, th, tw, in->template_image);
@@ -203,6 +204,7 @@ extern "C" ss_status ss_rom_read(const char *path, ss_rom **out, ss_group_in *in
     s->device_id = I("DEVI");
     s->linesearch_candidates = I("LSCK");
     s->template_bf16 = I("TBF6");
+    if (r->rec.count("SVFC") && r->rec["SVFC"].size() >= 4) s->save_frame_correlations = I("SVFC");
     std::memset(in, 0, sizeof(*in));
     in->frame_h = (int32_t)r->rows["FRAC"];
     in->n_blocks = (int32_t)r->cols["FRAC"];

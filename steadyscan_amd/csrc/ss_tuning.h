@@ -1,61 +1,45 @@
-// ss_tuning.h -- launch tuning of the pixel kernels, per pixel precision.
+// ss_tuning.h -- launch variants of the pixel kernels, per pixel precision.
 //
-// *_WAVES: minimum waves per SIMD that the kernel's register allocation must
-// allow (more waves hide more gather latency; fewer registers limit how many
-// gathers a wave keeps in flight).  *_SAMPLES: template samples per unrolled
-// chunk of a thread (a power of two).  The defaults are the values measured
-// fastest on MI355X for 512x512 frames (bench.py's headline workload); any
-// build may override them with -D.
+// Each of the assembly, line-search and correlation kernels is built in
+// kVariants variants of (samples, waves):
+//   samples: template samples a thread keeps in flight (the unrolled chunk
+//            of rows times the samples per row; a power of two),
+//   waves:   minimum waves per SIMD that the kernel's register allocation
+//            must allow (more waves hide more gather latency; fewer
+//            registers limit how many gathers a wave keeps in flight).
+// Variant 0 is the default: the values measured fastest on MI355X for
+// 512x512 frames (the headline workload).  The engine's autotuner times all
+// variants on a group shape's first tick and keeps the fastest per kernel.
+// A variant only changes how many rows a thread unrolls and how registers
+// are allocated: every thread still accumulates its pixels in the same
+// order, so all variants give the same results bit for bit.
 #pragma once
 
-#ifndef SS_ASSEMBLE_F32_WAVES
-#define SS_ASSEMBLE_F32_WAVES 4
-#endif
-#ifndef SS_ASSEMBLE_F32_SAMPLES
-#define SS_ASSEMBLE_F32_SAMPLES 8
-#endif
-#ifndef SS_ASSEMBLE_F64_WAVES
-#define SS_ASSEMBLE_F64_WAVES 2
-#endif
-#ifndef SS_ASSEMBLE_F64_SAMPLES
-#define SS_ASSEMBLE_F64_SAMPLES 4
-#endif
-#ifndef SS_LS_F32_WAVES
-#define SS_LS_F32_WAVES 4
-#endif
-#ifndef SS_LS_F32_SAMPLES
-#define SS_LS_F32_SAMPLES 8
-#endif
-#ifndef SS_LS_F64_WAVES
-#define SS_LS_F64_WAVES 4
-#endif
-#ifndef SS_LS_F64_SAMPLES
-#define SS_LS_F64_SAMPLES 4
-#endif
-#ifndef SS_CORR_F32_WAVES
-#define SS_CORR_F32_WAVES 4
-#endif
-#ifndef SS_CORR_F32_SAMPLES
-#define SS_CORR_F32_SAMPLES 16
-#endif
-#ifndef SS_CORR_F64_WAVES
-#define SS_CORR_F64_WAVES 2
-#endif
-#ifndef SS_CORR_F64_SAMPLES
-#define SS_CORR_F64_SAMPLES 8
-#endif
-#ifndef SS_FINAL_F32_WAVES
-#define SS_FINAL_F32_WAVES 4
-#endif
-#ifndef SS_FINAL_F32_SAMPLES
-#define SS_FINAL_F32_SAMPLES 8
-#endif
-#ifndef SS_FINAL_F64_WAVES
-#define SS_FINAL_F64_WAVES 4
-#endif
-#ifndef SS_FINAL_F64_SAMPLES
-#define SS_FINAL_F64_SAMPLES 4
-#endif
+namespace ss {
 
-// per-kernel values for pixel type R
-#define SS_TUNE(KERNEL, R, WHAT) (sizeof(R) == 8 ? SS_##KERNEL##_F64_##WHAT : SS_##KERNEL##_F32_##WHAT)
+constexpr int kVariants = 3;
+enum TuneKernel { TK_ASSEMBLE = 0, TK_LS = 1, TK_CORR = 2, TK_COUNT = 3 };
+
+struct TuneVariant {
+    int samples, waves;
+};
+
+// [kernel][fp64][variant]
+constexpr TuneVariant kTuneTable[TK_COUNT][2][kVariants] = {
+    // assembly: FP32, FP64
+    {{{8, 4}, {4, 4}, {8, 3}}, {{4, 2}, {2, 3}, {8, 2}}},
+    // line search
+    {{{8, 4}, {4, 4}, {8, 3}}, {{4, 4}, {2, 4}, {8, 3}}},
+    // correlation (one sample per row; a chunk holds at most 8 rows)
+    {{{8, 4}, {4, 4}, {2, 4}}, {{8, 2}, {4, 3}, {2, 3}}},
+};
+
+template <int KERNEL, typename R, int V>
+__host__ __device__ constexpr TuneVariant tune_variant() {
+    return kTuneTable[KERNEL][sizeof(R) == 8 ? 1 : 0][V];
+}
+
+// the final pass runs once per group: one variant
+constexpr int kFinalSamplesF32 = 8, kFinalWavesF32 = 4, kFinalSamplesF64 = 4, kFinalWavesF64 = 4;
+
+}  // namespace ss

@@ -43,6 +43,7 @@ def test_ctypes_table_covers_header():
                                        ("ss_group_out", N.SSGroupOut), ("ss_geometry_desc", N.SSGeometryDesc),
                                        ("ss_resample_in", N.SSResampleIn),
                                        ("ss_server_config", N.SSServerConfig),
+                                       ("ss_tuning_entry", N.SSTuningEntry),
                                        ("ss_ibs_settings", N.SSIbsSettings),
                                        ("ss_driver_settings", N.SSDriverSettings),
                                        ("ss_displacements_out", N.SSDisplacementsOut)])

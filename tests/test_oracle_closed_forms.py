@@ -12,7 +12,14 @@
   ILU(0) (IKJ order, zero fill outside the matrix's pattern) in numpy.
 - Frame correlation (hd_frameCorrelation, task_correlation.cu:101-183):
   np.corrcoef of the warped template (sampled here by numpy from the state's
-  coordinates) and the image over each frame's masked pixels.
+  coordinates) and the image over each frame's masked pixels; after the
+  global correlation (the per-iteration correlations of flag 'c',
+  task.cpp:286-306), the same statistic with the pixels past the global
+  correlation's partial swap-back (task_correlation.cu:97) holding the
+  residual.
+- Per-iteration errvals (flag 'e', task.cpp:362-370): the masked mean
+  squared residual in numpy at the knots each half-iteration started from,
+  and the rows and save points against a half-iteration replay.
 
 Each check is also run against copies of the oracle built with one planted
 mistake (ORC_PLANT in oracle/ss_oracle.c), and must reject every one of them.
@@ -34,7 +41,8 @@ from steadyscan_amd import workloads as W
 ORACLE_DIR = os.path.dirname(os.path.abspath(O.__file__))
 PLANTS = {1: "bidirectional flips even lines", 2: "knot times from cropped lines",
           3: "linspace divides by n", 4: "ILU(0) on unsorted rows", 5: "ILU(0) wrong pivot",
-          6: "frame correlation means over all pixels", 7: "frame correlation of the residual"}
+          6: "frame correlation means over all pixels", 7: "frame correlation of the residual",
+          8: "global correlation swaps every pixel back", 9: "iteration errvals rows swapped"}
 
 
 @pytest.fixture(scope="module")
@@ -332,3 +340,115 @@ def test_frame_correlation_is_pearson_of_masked_pixels(seed):
 @pytest.mark.parametrize("plant", [6, 7])
 def test_frame_correlation_check_rejects_planted_mistakes(planted, plant):
     assert not all(frame_correlation_ok(planted[plant], s) for s in (61, 62, 63)), PLANTS[plant]
+
+
+# ---------------------------------------------------------------------------
+# per-frame correlations (hd_frameCorrelation) and per-iteration errvals
+# ---------------------------------------------------------------------------
+def pearson(a, b):
+    va, vb = a - a.mean(), b - b.mean()
+    return (va * vb).sum() / np.sqrt((va * va).sum() * (vb * vb).sum())
+
+
+def swap_back_ok(L):
+    # 3 frames of 3 blocks x 192 rows: 9 columns, 1728 pixels, so the global
+    # correlation's swap-back (task_correlation.cu:97, one thread per column
+    # rounded up to 256) restores row-major pixels 0..255 only: rows 0-27
+    # and the first 4 columns of row 28
+    T = W.make_template(24, 24, 6, (1.5, 3.0), offset=5.0, seed=11)
+    dx, dy = W.make_motion(3, 24, 0.05, 0.0, (0.5, 0.5), seed=12)
+    fr = W.make_frames(T, dx, dy, 24, 24, np.float64, True, seed=13)
+    geom = ssa.init_settings(24, 24, T.shape, nknots=4)
+    a = W.group_args(T, fr, geom, max_iter=5)
+    p = np.random.default_rng(11).uniform(-0.6, 0.6, 2 * (3 * 3 + 1))
+    fh, nb, nF = a["basex"].shape[0], a["basex"].shape[1], a["nsetframes"]
+    fw = nb * nF
+    assert fh * fw > 256 and 256 % fw != 0
+    I = np.asarray(a["bmData"], dtype=np.float64)
+    st = O.State(a, L=L)
+    st.set_p(p)
+    st.assemble(0)
+    m, _, _, d = st.pixels()
+    fc = st.frame_correlation()
+    for f in range(nF):
+        cols = slice(f * nb, (f + 1) * nb)
+        sel = m[:, cols]
+        # warped template = image - difference image
+        if not abs(fc[f] - pearson((I - d)[:, cols][sel], I[:, cols][sel])) < 1e-12:
+            return False
+    st.global_correlation()
+    fcq = st.frame_correlation()
+    rm = np.arange(fh)[:, None] * fw + np.arange(fw)[None, :]  # row-major index
+    prefix = rm < min(-(-fw // 256) * 256, fh * fw)
+    v = np.where(prefix, I - d, d)
+    for f in range(nF):
+        cols = slice(f * nb, (f + 1) * nb)
+        sel = m[:, cols]
+        if not abs(fcq[f] - pearson(v[:, cols][sel], I[:, cols][sel])) < 1e-12:
+            return False
+        assert abs(fcq[f] - fc[f]) > 0.05  # the residual, not the template, past the prefix
+    return True
+
+
+def test_frame_correlation_after_global_correlation():
+    assert swap_back_ok(O.lib())
+
+
+def test_swap_back_check_rejects_planted_mistake(planted):
+    assert not swap_back_ok(planted[8]), PLANTS[8]
+
+
+def errval_np(T, a, p, f):
+    fh, nb, nF = a["basex"].shape[0], a["basex"].shape[1], a["nsetframes"]
+    N = nb * nF + 1
+    th, tw = T.shape
+    e, n = 0.0, 0
+    for c in range(f * nb, (f + 1) * nb):
+        b = c % nb
+        f_ = a["tfrac"][:, b]
+        x = a["basex"][:, b] + p[c] * (1 - f_) + p[c + 1] * f_
+        y = a["basey"][:, b] + p[c + N] * (1 - f_) + p[c + N + 1] * f_
+        m = a["base_mask"][:, b].astype(bool) & (x >= 0) & (x < tw - 1) & (y >= 0) & (y < th - 1)
+        r = np.asarray(a["bmData"], dtype=np.float64)[m, c] - bilinear_np(T, x[m], y[m])
+        e += (r * r).sum()
+        n += m.sum()
+    return e / n
+
+
+def iteration_errvals_ok(L):
+    T, fr, geom, _ = W.config0(n_frames=4, size=32, seed=17)
+    a = W.group_args(T, fr, geom, max_iter=8, filter_id=0)
+    a["flags"] = "De"
+    r = O.align_group_to_template(a, with_iterations=True, L=L)
+    it, E, allp = r["iterations"], r["iteration_errvals"], r["all_params"]
+    nF, mi = a["nsetframes"], 8
+    assert E.shape == (2 * mi, nF) and it >= 3
+    # saved after every iteration that continues: all but the last
+    assert np.all(E[2 * (it - 1):, :] == 0) and np.all(E[:2 * (it - 1), :] > 0)
+    # replay: half-iteration by half-iteration on a state
+    st = O.State(a, L=L)
+    for k in range(1, it):
+        for shift in range(2):
+            start = st.get()[0]
+            st.assemble(shift)
+            st.linesearch(shift)
+            ev = st.get()[2]
+            # the front errval of frames > 0: the masked mean squared residual
+            # at the knots the half-iteration started from (frame 0's slot
+            # takes the line search's accepted value, task_linesearch.cu:91)
+            for f in range(1, nF):
+                e = errval_np(T, a, start, f)
+                if not abs(E[2 * (k - 1) + shift, f] - e) < 1e-9 * e:
+                    return False
+        if not (np.array_equal(E[2 * (k - 1), :], ev[:nF]) and np.array_equal(E[2 * (k - 1) + 1, :], ev[nF:])):
+            return False
+        assert np.array_equal(st.get()[0], allp[:, k - 1])
+    return True
+
+
+def test_iteration_errvals():
+    assert iteration_errvals_ok(O.lib())
+
+
+def test_iteration_errvals_check_rejects_planted_mistake(planted):
+    assert not iteration_errvals_ok(planted[9]), PLANTS[9]

@@ -12,6 +12,8 @@ is pinned by what the mathematics or the reference's own code fixes:
     compacted COO layout; ILU(0) equals LU where the pattern has no fill;
   - filters, expand rule, line-search structure: closed forms and invariants;
   - the global correlation statistic: a hand-computed known-answer case;
+  - per-frame correlations and per-iteration errvals: closed forms in
+    tests/test_oracle_closed_forms.py;
   - end to end: zero motion, global translation, linear motion;
   - resampling: identity at zero displacement, MATLAB last-write-wins KAT.
 """

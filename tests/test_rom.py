@@ -41,7 +41,7 @@ def test_format_header_and_records(call, tmp_path):
     raw = p.read_bytes()
     assert raw[:8] == b"SSROM001"
     bom, count = struct.unpack("<II", raw[8:16])
-    assert bom == 0x01020304 and count == 28
+    assert bom == 0x01020304 and count == 29
     tags, off = [], 16
     for _ in range(count):
         tag = raw[off:off + 4].decode()
