@@ -37,7 +37,7 @@ enum FinalSum { FS_D2 = 0, FS_CNT, FS_SA, FS_SAA, FS_SB, FS_SBB, FS_SAB, FS_N };
 // per-iteration frame correlation sums (flag 'c'), kept in finalsum's slots
 // while the group is in its correlation phase
 enum FrameCorrSum { FC_CNT = 0, FC_SA, FC_SAA, FC_SB, FC_SBB, FC_SAB, FC_N };
-static_assert(FC_N <= FS_N, "frame correlation sums share the final sums' slots");
+static_assert((int)FC_N <= (int)FS_N, "frame correlation sums share the final sums' slots");
 
 constexpr int kMaxCand = 4;  // fused line-search candidates per round
 

@@ -140,6 +140,7 @@ struct ss_handle {
     bool useGraph = true;
     hipGraphExec_t tickExec = nullptr;
     ss::LaunchBatch tickFor{};
+    hipEvent_t tickDone[2] = {nullptr, nullptr};  // control blocks of a tick read back
     // autotuning: -1 tunes each new group shape, 0..kVariants-1 forces that variant
     int tuneMode = 0;  // the default variants; -1 tunes each new group shape
     std::map<TuneKey, ss_tuning_entry> tuned;
@@ -942,7 +943,8 @@ const int kTick[] = {ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE,
                      ss::ST_LS, ss::ST_LS_DECIDE,
                      ss::ST_CORR, ss::ST_CORR_FINISH, ss::ST_FINAL, ss::ST_FINAL_FINISH};
 
-ss_status run_tick(ss_handle *h, const ss::LaunchBatch &L, ss::GroupCtl *hc) {
+// Enqueues one tick on the handle's stream.
+ss_status enqueue_tick(ss_handle *h, const ss::LaunchBatch &L) {
     const auto &tick = kTick;
     if (h->prof.on || !h->useGraph) {  // per-stage events need plain launches
         for (int stg : tick) HIPCHK(run_stage(h, L, stg));
@@ -974,6 +976,13 @@ ss_status run_tick(ss_handle *h, const ss::LaunchBatch &L, ss::GroupCtl *hc) {
         }
         HIPCHK(hipGraphLaunch(h->tickExec, h->stream));
     }
+    return SS_OK;
+}
+
+// One tick; the control blocks come back to hc.
+ss_status run_tick(ss_handle *h, const ss::LaunchBatch &L, ss::GroupCtl *hc) {
+    ss_status st = enqueue_tick(h, L);
+    if (st) return st;
     HIPCHK(hipMemcpyAsync(hc, L.b.ctl, L.b.nGroups * sizeof(ss::GroupCtl), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return SS_OK;
@@ -1115,14 +1124,16 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
     }
     const size_t tb = (table_bytes(n, nItems, nFrames) + 255) & ~(size_t)255;
     const size_t cb = (n * sizeof(ss::GroupCtl) + 255) & ~(size_t)255;
-    ss_status st = ensure_arena(h, dTotal + tb, hTotal + tb + cb);
+    ss_status st = ensure_arena(h, dTotal + tb, hTotal + tb + 2 * cb);
     if (st) return st;
-    // [table][group 0][group 1]... in the arena; the same order in staging
+    // [table][group 0][group 1]... in the arena; the same order in staging,
+    // after two read-back buffers of the control blocks
     char *D = (char *)h->arena + tb;
     char *HS = (char *)h->harena;
-    ss::GroupCtl *hc = (ss::GroupCtl *)(HS + tb);
+    ss::GroupCtl *hcb[2] = {(ss::GroupCtl *)(HS + tb), (ss::GroupCtl *)(HS + tb + cb)};
+    ss::GroupCtl *hc = hcb[0];
     std::vector<char *> HG(n);
-    HG[0] = HS + tb + cb;
+    HG[0] = HS + tb + 2 * cb;
     for (int gi = 1; gi < n; gi++) HG[gi] = HG[gi - 1] + ((plans[gi - 1].hBytes + 255) & ~(size_t)255);
     stage_host_frames(plans, HG);
     std::vector<ss::GroupDev> gd(n);
@@ -1142,14 +1153,37 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
     if (st) return st;
     HIPCHK(hipStreamSynchronize(h->stream));
     const double initTime = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    bool done = false;
-    for (int t = 0; t < kTickCap && !done; t++) {
-        st = run_tick(h, L, hc);
-        if (st) return st;
-        done = true;
+    // One tick stays in flight: tick t is enqueued with the read-back of its
+    // control blocks before the host waits for tick t-1's, so the device
+    // does not idle while the host reads them.  Kernels skip finished
+    // groups, so the tick after the last one changes nothing.
+    for (int i = 0; i < 2; i++)
+        if (!h->tickDone[i]) HIPCHK(hipEventCreateWithFlags(&h->tickDone[i], hipEventDisableTiming));
+    const size_t ctlBytes = n * sizeof(ss::GroupCtl);
+    auto all_finished = [&](const ss::GroupCtl *c) {
         for (int gi = 0; gi < n; gi++)
-            if (!finished(hc[gi])) done = false;
+            if (!finished(c[gi])) return false;
+        return true;
+    };
+    bool done = false;
+    int t = 0;
+    for (; t < kTickCap; t++) {
+        st = enqueue_tick(h, L);
+        if (st) return st;
+        HIPCHK(hipMemcpyAsync(hcb[t & 1], L.b.ctl, ctlBytes, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipEventRecord(h->tickDone[t & 1], h->stream));
+        if (t > 0) {
+            HIPCHK(hipEventSynchronize(h->tickDone[(t - 1) & 1]));
+            if (all_finished(hcb[(t - 1) & 1])) {
+                done = true;
+                break;
+            }
+        }
     }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (t == kTickCap) t--;
+    hc = hcb[t & 1];  // the last tick's control blocks
+    if (!done) done = all_finished(hc);
     if (!done) return fail(h, SS_ERR_NUMERICAL, "alignment did not terminate");
     ss_status result = SS_OK;
     std::string firstErr;
@@ -1616,6 +1650,8 @@ void ss_destroy(ss_handle *h) {
     }
     for (auto ev : h->prof.pool) e = hipEventDestroy(ev);
     if (h->tickExec) e = hipGraphExecDestroy(h->tickExec);
+    for (auto ev : h->tickDone)
+        if (ev) e = hipEventDestroy(ev);
     if (h->arena) e = hipFree(h->arena);
     if (h->harena) e = hipHostFree(h->harena);
     if (h->scratch) e = hipFree(h->scratch);

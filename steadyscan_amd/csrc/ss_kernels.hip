@@ -666,73 +666,21 @@ __device__ __forceinline__ void level0_equations(const double *__restrict__ S, c
     }
 }
 
-// Exact FP64 solve of the level-0 block-tridiagonal system E (knots 0..n-1)
-// into X: partitioned block Thomas + cyclic reduction of the interface
-// system.  Called by all threads of the workgroup; IR and firstAff are
-// workspace.
-__device__ void exact_solve(const Eq0 *E, InteriorRow *IR, double2 *X, Aff *firstAff, int n, int tid) {
-    // 4. partitioned solve: thread t owns knots [s, e); interior rows s..e-2
-    //    are eliminated in terms of the interfaces z_{t-1} = x_{s-1} and
-    //    z_t = x_{e-1}; the interface system is solved by cyclic reduction.
-    const int m = (n + kSolveWG - 1) / kSolveWG;
-    const int P = (n + m - 1) / m;
-    const int s0 = tid * m, e0 = min(n, s0 + m);
+// Interface system of the partitioned solve: thread t < P contributes the
+// equation of its partition's last knot z_t in terms of z_{t-1} and z_{t+1}
+// (its last interior row as lastA, the next partition's first interior row
+// as firstAff[t + 1]); cyclic reduction solves it.  Called by all threads;
+// returns z_t and z_{t-1} (0 for t = 0).
+__device__ void solve_interfaces(bool interior, const Aff &first, const Aff &lastA, const Eq0 &q, const M2 &A,
+                                 Aff *firstAff, int P, int tid, double2 &zo, double2 &zp) {
     __shared__ RedEq red[kSolveWG];
     __shared__ int hasInt[kSolveWG];
-    Aff lastA;  // last interior row as a function of (z_prev, z_own)
     if (tid < P) {
-        const bool interior = e0 - s0 >= 2;
         hasInt[tid] = interior ? 1 : 0;
-        if (interior) {
-            // forward elimination
-            M2 BiPrev = zero2();
-            Aff Rprev;
-            for (int i = s0; i <= e0 - 2; i++) {
-                const Eq0 q = E[i];
-                M2 Bh = sym(q.b0, q.b1, q.b2);
-                Aff R{make_double2(q.d0, q.d1), zero2(), zero2()};
-                if (i == s0) {
-                    if (tid > 0) {
-                        const Eq0 qp = E[i - 1];
-                        R.V = neg(sym(qp.c0, qp.c1, qp.c2));
-                    }
-                } else {
-                    const Eq0 qp = E[i - 1];
-                    const M2 Ap = sym(qp.c0, qp.c1, qp.c2);  // A_i = O_{i-1}
-                    const M2 Lm = mul(Ap, BiPrev);
-                    Bh = sub(Bh, mul(Lm, Ap));
-                    R = aff_sub(R, aff_apply(Lm, Rprev));
-                }
-                if (i == e0 - 2) R.W = neg(sym(q.c0, q.c1, q.c2));
-                const M2 Bi = inv(Bh);
-                IR[i] = InteriorRow{Bi, R};
-                BiPrev = Bi;
-                Rprev = R;
-            }
-            // back substitution of the affine forms
-            Aff Xa = aff_apply(BiPrev, Rprev);
-            lastA = Xa;
-            for (int i = e0 - 3; i >= s0; i--) {
-                const InteriorRow ir = IR[i];
-                const Eq0 q = E[i];
-                const Aff t = aff_sub(ir.R, aff_apply(sym(q.c0, q.c1, q.c2), Xa));
-                Xa = aff_apply(ir.Bi, t);
-            }
-            firstAff[tid] = Xa;
-        } else {
-            lastA = Aff{make_double2(0, 0), eye2(), zero2()};       // x_{e-2} := z_prev
-            firstAff[tid] = Aff{make_double2(0, 0), zero2(), eye2()};  // unused
-        }
+        firstAff[tid] = first;
     }
     __syncthreads();
     if (tid < P) {
-        const int j = e0 - 1;
-        const Eq0 q = E[j];
-        M2 A = zero2();
-        if (j > 0) {
-            const Eq0 qp = E[j - 1];
-            A = sym(qp.c0, qp.c1, qp.c2);
-        }
         const M2 C = sym(q.c0, q.c1, q.c2);
         // x_{j+1}: first interior of partition t+1, or z_{t+1}
         Aff F{make_double2(0, 0), zero2(), eye2()};
@@ -805,28 +753,188 @@ __device__ void exact_solve(const Eq0 *E, InteriorRow *IR, double2 *X, Aff *firs
         }
         __syncthreads();
     }
+    zo = make_double2(0, 0);
+    zp = make_double2(0, 0);
+    if (tid < P) {
+        zo = red[tid].x;
+        if (tid > 0) zp = red[tid - 1].x;
+    }
+}
+
+// Exact FP64 solve of the level-0 block-tridiagonal system E (knots 0..n-1)
+// into X: partitioned block Thomas + cyclic reduction of the interface
+// system.  Thread t owns knots [s, e); its interior rows s..e-2 are
+// eliminated in terms of the interfaces z_{t-1} = x_{s-1} and z_t = x_{e-1}.
+// Partitions of at most M knots keep their rows and eliminated rows in
+// registers (one round of loads); longer ones (M = 0) go through IR in
+// memory.  Both do the same arithmetic in the same order.  Called by all
+// threads of the workgroup; IR and firstAff are workspace.
+template <int M>
+__device__ void exact_solve_t(const Eq0 *__restrict__ E, InteriorRow *__restrict__ IR, double2 *__restrict__ X,
+                              Aff *firstAff, int n, int tid) {
+    const int m = (n + kSolveWG - 1) / kSolveWG;
+    const int P = (n + m - 1) / m;
+    const int s0 = tid * m, e0 = min(n, s0 + m);
+    const int len = e0 - s0;
+    constexpr int MR = M > 0 ? M : 1;
+    Eq0 q[MR];
+    InteriorRow ir[MR];
+    Eq0 qm{};  // row s0 - 1
+    Aff lastA, first;
+    Eq0 qj{};  // row e0 - 1
+    M2 A = zero2();
+    const bool interior = len >= 2;
+    if (tid < P) {
+        if constexpr (M > 0) {
+#pragma unroll
+            for (int r = 0; r < M; r++)
+                if (r < len) q[r] = E[s0 + r];
+            if (s0 > 0) qm = E[s0 - 1];
+        }
+        if (interior) {
+            // forward elimination
+            M2 BiPrev = zero2();
+            Aff Rprev;
+            if constexpr (M > 0) {
+#pragma unroll
+                for (int r = 0; r < M - 1; r++) {
+                    if (r > len - 2) continue;
+                    const Eq0 &qq = q[r];
+                    M2 Bh = sym(qq.b0, qq.b1, qq.b2);
+                    Aff R{make_double2(qq.d0, qq.d1), zero2(), zero2()};
+                    if (r == 0) {
+                        if (tid > 0) R.V = neg(sym(qm.c0, qm.c1, qm.c2));
+                    } else {
+                        const Eq0 &qp = q[r - 1];
+                        const M2 Ap = sym(qp.c0, qp.c1, qp.c2);  // A_i = O_{i-1}
+                        const M2 Lm = mul(Ap, BiPrev);
+                        Bh = sub(Bh, mul(Lm, Ap));
+                        R = aff_sub(R, aff_apply(Lm, Rprev));
+                    }
+                    if (r == len - 2) R.W = neg(sym(qq.c0, qq.c1, qq.c2));
+                    const M2 Bi = inv(Bh);
+                    ir[r] = InteriorRow{Bi, R};
+                    BiPrev = Bi;
+                    Rprev = R;
+                }
+            } else {
+                for (int i = s0; i <= e0 - 2; i++) {
+                    const Eq0 qq = E[i];
+                    M2 Bh = sym(qq.b0, qq.b1, qq.b2);
+                    Aff R{make_double2(qq.d0, qq.d1), zero2(), zero2()};
+                    if (i == s0) {
+                        if (tid > 0) {
+                            const Eq0 qp = E[i - 1];
+                            R.V = neg(sym(qp.c0, qp.c1, qp.c2));
+                        }
+                    } else {
+                        const Eq0 qp = E[i - 1];
+                        const M2 Ap = sym(qp.c0, qp.c1, qp.c2);  // A_i = O_{i-1}
+                        const M2 Lm = mul(Ap, BiPrev);
+                        Bh = sub(Bh, mul(Lm, Ap));
+                        R = aff_sub(R, aff_apply(Lm, Rprev));
+                    }
+                    if (i == e0 - 2) R.W = neg(sym(qq.c0, qq.c1, qq.c2));
+                    const M2 Bi = inv(Bh);
+                    IR[i] = InteriorRow{Bi, R};
+                    BiPrev = Bi;
+                    Rprev = R;
+                }
+            }
+            // back substitution of the affine forms
+            Aff Xa = aff_apply(BiPrev, Rprev);
+            lastA = Xa;
+            if constexpr (M > 0) {
+#pragma unroll
+                for (int r = M - 3; r >= 0; r--) {
+                    if (r > len - 3) continue;
+                    const Eq0 &qq = q[r];
+                    const Aff t = aff_sub(ir[r].R, aff_apply(sym(qq.c0, qq.c1, qq.c2), Xa));
+                    Xa = aff_apply(ir[r].Bi, t);
+                }
+            } else {
+                for (int i = e0 - 3; i >= s0; i--) {
+                    const InteriorRow iri = IR[i];
+                    const Eq0 qq = E[i];
+                    const Aff t = aff_sub(iri.R, aff_apply(sym(qq.c0, qq.c1, qq.c2), Xa));
+                    Xa = aff_apply(iri.Bi, t);
+                }
+            }
+            first = Xa;
+        } else {
+            lastA = Aff{make_double2(0, 0), eye2(), zero2()};      // x_{e-2} := z_prev
+            first = Aff{make_double2(0, 0), zero2(), eye2()};      // unused
+        }
+        // the interface row j = e0 - 1 and its coupling A = O_{j-1}
+        const int j = e0 - 1;
+        if constexpr (M > 0) {
+#pragma unroll
+            for (int r = 0; r < M; r++)
+                if (r == len - 1) qj = q[r];
+            if (len >= 2) {
+#pragma unroll
+                for (int r = 0; r < M - 1; r++)
+                    if (r == len - 2) A = sym(q[r].c0, q[r].c1, q[r].c2);
+            } else if (j > 0) {
+                A = sym(qm.c0, qm.c1, qm.c2);
+            }
+        } else {
+            qj = E[j];
+            if (j > 0) {
+                const Eq0 qp = E[j - 1];
+                A = sym(qp.c0, qp.c1, qp.c2);
+            }
+        }
+    }
+    double2 zo, zp;
+    solve_interfaces(interior, first, lastA, qj, A, firstAff, P, tid, zo, zp);
     // interior reconstruction: x_i = Bi_i (R_i(z) - O_i x_{i+1})
     if (tid < P) {
-        const double2 zo = red[tid].x;
-        const double2 zp = tid > 0 ? red[tid - 1].x : make_double2(0, 0);
         X[e0 - 1] = zo;
         double2 xn = zo;
-        for (int i = e0 - 2; i >= s0; i--) {
-            const InteriorRow ir = IR[i];
-            const double2 v = mulv(ir.R.V, zp), w = mulv(ir.R.W, zo);
-            double2 r = make_double2(ir.R.y.x + v.x + w.x, ir.R.y.y + v.y + w.y);
-            if (i < e0 - 2) {
-                const Eq0 q = E[i];
-                const double2 t = mulv(sym(q.c0, q.c1, q.c2), xn);
-                r.x -= t.x;
-                r.y -= t.y;
+        if constexpr (M > 0) {
+#pragma unroll
+            for (int r = M - 2; r >= 0; r--) {
+                if (r > len - 2) continue;
+                const InteriorRow &iri = ir[r];
+                const double2 v = mulv(iri.R.V, zp), w = mulv(iri.R.W, zo);
+                double2 rr = make_double2(iri.R.y.x + v.x + w.x, iri.R.y.y + v.y + w.y);
+                if (r < len - 2) {
+                    const double2 t = mulv(sym(q[r].c0, q[r].c1, q[r].c2), xn);
+                    rr.x -= t.x;
+                    rr.y -= t.y;
+                }
+                xn = mulv(iri.Bi, rr);
+                X[s0 + r] = xn;
             }
-            xn = mulv(ir.Bi, r);
-            X[i] = xn;
+        } else {
+            for (int i = e0 - 2; i >= s0; i--) {
+                const InteriorRow iri = IR[i];
+                const double2 v = mulv(iri.R.V, zp), w = mulv(iri.R.W, zo);
+                double2 rr = make_double2(iri.R.y.x + v.x + w.x, iri.R.y.y + v.y + w.y);
+                if (i < e0 - 2) {
+                    const Eq0 qq = E[i];
+                    const double2 t = mulv(sym(qq.c0, qq.c1, qq.c2), xn);
+                    rr.x -= t.x;
+                    rr.y -= t.y;
+                }
+                xn = mulv(iri.Bi, rr);
+                X[i] = xn;
+            }
         }
     }
     __threadfence_block();
     __syncthreads();
+}
+
+// knots per partition that stay in registers
+constexpr int kSolveRegRows = 8;
+
+__device__ void exact_solve(const Eq0 *E, InteriorRow *IR, double2 *X, Aff *firstAff, int n, int tid) {
+    if ((n + kSolveWG - 1) / kSolveWG <= kSolveRegRows)
+        exact_solve_t<kSolveRegRows>(E, IR, X, firstAff, n, tid);
+    else
+        exact_solve_t<0>(E, IR, X, firstAff, n, tid);
 }
 
 __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
