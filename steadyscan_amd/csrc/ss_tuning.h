@@ -8,7 +8,9 @@
 //            must allow (more waves hide more gather latency; fewer
 //            registers limit how many gathers a wave keeps in flight).
 // Variant 0 is the default: the values measured fastest on MI355X for
-// 512x512 frames (the headline workload).  The engine's autotuner times all
+// 512x512 frames (the headline workload); in FP32 the assembly and line
+// search take 4 samples, which measure as fast as 8 without spilling
+// registers.  The engine's autotuner (off unless asked for) times all
 // variants on a group shape's first tick and keeps the fastest per kernel.
 // A variant only changes how many rows a thread unrolls and how registers
 // are allocated: every thread still accumulates its pixels in the same
@@ -27,9 +29,9 @@ struct TuneVariant {
 // [kernel][fp64][variant]
 constexpr TuneVariant kTuneTable[TK_COUNT][2][kVariants] = {
     // assembly: FP32, FP64
-    {{{8, 4}, {4, 4}, {8, 3}}, {{4, 2}, {2, 3}, {8, 2}}},
+    {{{4, 4}, {8, 4}, {8, 3}}, {{4, 2}, {2, 3}, {8, 2}}},
     // line search
-    {{{8, 4}, {4, 4}, {8, 3}}, {{4, 4}, {2, 4}, {8, 3}}},
+    {{{4, 4}, {8, 4}, {8, 3}}, {{4, 4}, {2, 4}, {8, 3}}},
     // correlation (one sample per row; a chunk holds at most 8 rows)
     {{{8, 4}, {4, 4}, {2, 4}}, {{8, 2}, {4, 3}, {2, 3}}},
 };
@@ -40,6 +42,6 @@ __host__ __device__ constexpr TuneVariant tune_variant() {
 }
 
 // the final pass runs once per group: one variant
-constexpr int kFinalSamplesF32 = 8, kFinalWavesF32 = 4, kFinalSamplesF64 = 4, kFinalWavesF64 = 4;
+constexpr int kFinalSamplesF32 = 4, kFinalWavesF32 = 4, kFinalSamplesF64 = 4, kFinalWavesF64 = 4;
 
 }  // namespace ss
