@@ -675,8 +675,8 @@ def main():
                        "sample": f"FP64 oracle (C restatement, OpenMP) on 1 group of {G} frames 512x512 "
                                  f"(group 0 of the workload), {ref['iterations']} GN iterations, {tcpu:.1f} s",
                        "host": host_cpu()}
-        # the same oracle on one thread, on the first 20 frames of group 0
-        n1 = 20
+        # the same oracle on one thread, on the first 40 frames of group 0
+        n1 = 40
         a1 = W.group_args(T, frames, geom, first=0, count=n1, flags=precs[0][0])
         t = time.perf_counter()
         r1 = O.align_group_to_template(a1, threads=1, with_mask=False)
@@ -685,6 +685,17 @@ def main():
                                 "sample": f"FP64 oracle on one thread, the first {n1} frames of group 0 as "
                                           f"one group, {r1['iterations']} GN iterations, {t1:.1f} s"}
         print(f"[bench] oracle, one thread: {n1} frames in {t1:.1f} s", file=sys.stderr, flush=True)
+        # FP32 pixel arithmetic: the oracle's source built with float pixels
+        # (template, frames, warp, residuals, per-column sums), OpenMP, group 0
+        af = W.group_args(T, frames, geom, first=0, count=G, flags="S")
+        t = time.perf_counter()
+        rf = O.align_group_to_template(af, threads=threads, with_mask=False, L=O.lib_f32())
+        tf = time.perf_counter() - t
+        cpu["fp32"] = {"value": G / tf, "unit": "frames/s", "cores": threads,
+                       "sample": f"the oracle's source with FP32 pixels, OpenMP, on group 0 ({G} frames), "
+                                 f"{rf['iterations']} GN iterations, {tf:.1f} s"}
+        print(f"[bench] FP32-pixel oracle copy, {threads} threads: {G} frames in {tf:.1f} s", file=sys.stderr,
+              flush=True)
 
     iters = [r["iterations"] for r in results]
     lat_ms = np.array(head["lat"]) * 1e3

@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+LIB_F32 = os.path.join(HERE, "liboracle_f32.so")  # FP32 pixels: a timed baseline, not a checker
 REF_LIB = os.path.join(HERE, "_ref", "libref_paramorg.so")
 
 
@@ -63,6 +64,13 @@ def lib():
     if _lib is None:
         _lib = load(LIB)
     return _lib
+
+
+def lib_f32():
+    """The FP32-pixel copy of the oracle (the same source built with
+    ORC_REAL=float): the benchmark's FP32 CPU baseline.  Not pinned and not
+    used to check anything."""
+    return load(LIB_F32)
 
 
 def load(path):

@@ -62,17 +62,27 @@
 #define ORC_PLANT 0
 #endif
 
+/* Pixel arithmetic type: double in the oracle.  A copy built with
+ * ORC_REAL=float (oracle/liboracle_f32.so) is the FP32 CPU baseline that the
+ * benchmark times: template, frames, warp, residuals and their per-column
+ * sums in float, as the reference's precision 'S' (T = float) keeps them.
+ * Only the double build is pinned and used as the checker. */
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+typedef ORC_REAL real;
+
 struct orc_state {
     int frameH, frameW, fframeW, nParamPoints, nParams, nFrames;
     int templateH, templateW, groupSize, nGroups;
     double eps;
     /* row-major copies, as MexConverter's RowMajor targets (mexconverter.cpp:223-285) */
-    double *tmpl, *xg, *yg, *frac, *baseX, *baseY, *image;
+    real *tmpl, *xg, *yg, *frac, *baseX, *baseY, *image;
     uint8_t *baseMask;
     int *minPoints;
     /* DeviceMemory<T> state (interface/memory.h:86-163) */
     double *p, *pLast, *deltaP, *newDeltaP, *jtr;
-    double *x, *y, *d, *wx, *wy;
+    real *x, *y, *d, *wx, *wy;
     uint8_t *mask, *paramsUsed, *blocksPresent;
     int *subOff;
     int nParamsUsed;
@@ -316,9 +326,9 @@ static int apply_column(orc_state *st, int idx, double px, double px1, double py
     for (int row = 0; row < st->frameH; row++) {
         size_t ipxm = IDX2R(row, idxm, st->frameW);
         size_t ipx = IDX2R(row, idx, st->fframeW);
-        double frac = st->frac[ipxm];
-        double x = st->baseX[ipxm] + px * (1 - frac) + px1 * frac;
-        double y = st->baseY[ipxm] + py * (1 - frac) + py1 * frac;
+        real frac = st->frac[ipxm];
+        real x = st->baseX[ipxm] + (real)px * (1 - frac) + (real)px1 * frac;
+        real y = st->baseY[ipxm] + (real)py * (1 - frac) + (real)py1 * frac;
         int m = st->baseMask[ipxm] && x >= 0 && x < templateW - st->eps && y >= 0 &&
                 y < templateH - st->eps;
         pxCount += m;
@@ -338,10 +348,10 @@ static void diff_column(orc_state *st, int idx) {
             int iX = (int)st->x[rid];
             int iY = (int)st->y[rid];
             size_t iXY = IDX2R(iY, iX, W);
-            double xFrac = st->x[rid] - iX;
-            double yFrac = st->y[rid] - iY;
-            const double *T = st->tmpl;
-            double wI = (1 - xFrac) * ((1 - yFrac) * T[iXY] + yFrac * T[iXY + W]) +
+            real xFrac = st->x[rid] - iX;
+            real yFrac = st->y[rid] - iY;
+            const real *T = st->tmpl;
+            real wI = (1 - xFrac) * ((1 - yFrac) * T[iXY] + yFrac * T[iXY + W]) +
                         xFrac * ((1 - yFrac) * T[iXY + 1] + yFrac * T[iXY + W + 1]);
             st->d[rid] = st->image[rid] - wI;
         } else {
@@ -356,7 +366,7 @@ static double frame_errval(orc_state *st, int f) {
     long cnt = 0;
     for (int t = 0; t < st->frameW; t++) {
         int idx = f * st->frameW + t;
-        double ec = 0;
+        real ec = 0;
         int cc = 0;
         for (int row = 0; row < st->frameH; row++) {
             size_t i = IDX2R(row, idx, st->fframeW);
@@ -404,8 +414,8 @@ static void update_gradients(orc_state *st) {
                 int iX = (int)st->x[i];
                 int iY = (int)st->y[i];
                 size_t t = IDX2R(iY, iX, W);
-                double xFrac = st->x[i] - (double)iX;
-                double yFrac = st->y[i] - (double)iY;
+                real xFrac = st->x[i] - (real)iX;
+                real yFrac = st->y[i] - (real)iY;
                 st->wx[i] = (1 - yFrac) * st->xg[t] + yFrac * st->xg[t + W];
                 st->wy[i] = (1 - xFrac) * st->yg[t] + xFrac * st->yg[t + 1];
             } else {
@@ -449,20 +459,20 @@ static void fill_diagonals(orc_state *st) {
         double *dbg = st->hdbg + (size_t)idx * 6;
         for (int k = 0; k < 6; k++) dbg[k] = 0;
         if (!st->paramsUsed[idx]) continue;
-        double offx = 0, offy = 0, offxy = 0, diagx = 0, diagy = 0, diagxy = 0;
+        real offx = 0, offy = 0, offxy = 0, diagx = 0, diagy = 0, diagxy = 0;
         int lastParam = (idx == st->fframeW);
         int firstParam = (idx == 0);
         long idxy = idx - lastParam;
         long idxy_1 = idx - 1;
         int idxm = idx % st->frameW;
         for (int row = 0; row < st->frameH; row++) {
-            double frac = st->frac[IDX2R(row, idxm, st->frameW)];
-            double fracsq = frac * frac;
-            double compsq = (1 - frac) * (1 - frac);
-            double fcprod = frac - fracsq;
-            double wxgradsq = st->wx[idxy] * st->wx[idxy];
-            double wygradsq = st->wy[idxy] * st->wy[idxy];
-            double wxygrad = st->wx[idxy] * st->wy[idxy];
+            real frac = st->frac[IDX2R(row, idxm, st->frameW)];
+            real fracsq = frac * frac;
+            real compsq = (1 - frac) * (1 - frac);
+            real fcprod = frac - fracsq;
+            real wxgradsq = st->wx[idxy] * st->wx[idxy];
+            real wygradsq = st->wy[idxy] * st->wy[idxy];
+            real wxygrad = st->wx[idxy] * st->wy[idxy];
             offx += wxgradsq * fcprod;
             offy += wygradsq * fcprod;
             offxy += wxygrad * fcprod;
@@ -519,13 +529,13 @@ static void fill_jtr(orc_state *st) {
     for (int idx0 = 0; idx0 < N; idx0++) {
         int idx = idx0;
         if (!st->paramsUsed[idx]) continue;
-        double sumx = 0, sumy = 0, sumx2 = 0, sumy2 = 0;
+        real sumx = 0, sumy = 0, sumx2 = 0, sumy2 = 0;
         int idxm = idx % st->frameW;
         if (idx != st->fframeW) {
             for (int row = 0; row < st->frameH; row++) {
                 size_t idxy = IDX2R(row, idx, st->fframeW);
-                double frac = st->frac[IDX2R(row, idxm, st->frameW)];
-                double fdi = st->d[idxy] * (1 - frac);
+                real frac = st->frac[IDX2R(row, idxm, st->frameW)];
+                real fdi = st->d[idxy] * (1 - frac);
                 sumx += fdi * st->wx[idxy];
                 sumy += fdi * st->wy[idxy];
             }
@@ -535,8 +545,8 @@ static void fill_jtr(orc_state *st) {
             else idxm -= 1;
             for (int row = 0; row < st->frameH; row++) {
                 size_t idxy = IDX2R(row, idx - 1, st->fframeW);
-                double frac = st->frac[IDX2R(row, idxm, st->frameW)];
-                double fdi = st->d[idxy] * frac;
+                real frac = st->frac[IDX2R(row, idxm, st->frameW)];
+                real fdi = st->d[idxy] * frac;
                 sumx2 += fdi * st->wx[idxy];
                 sumy2 += fdi * st->wy[idxy];
             }
@@ -816,25 +826,25 @@ int orc_state_create(const orc_in *in, const orc_settings *s, orc_state **out) {
     size_t tpix = (size_t)st->templateH * st->templateW;
     size_t gpix = (size_t)st->frameH * st->frameW;
     size_t npix = (size_t)st->frameH * st->fframeW;
-    st->tmpl = (double *)malloc(sizeof(double) * tpix);
-    st->xg = (double *)malloc(sizeof(double) * tpix);
-    st->yg = (double *)malloc(sizeof(double) * tpix);
-    st->frac = (double *)malloc(sizeof(double) * gpix);
-    st->baseX = (double *)malloc(sizeof(double) * gpix);
-    st->baseY = (double *)malloc(sizeof(double) * gpix);
+    st->tmpl = (real *)malloc(sizeof(real) * tpix);
+    st->xg = (real *)malloc(sizeof(real) * tpix);
+    st->yg = (real *)malloc(sizeof(real) * tpix);
+    st->frac = (real *)malloc(sizeof(real) * gpix);
+    st->baseX = (real *)malloc(sizeof(real) * gpix);
+    st->baseY = (real *)malloc(sizeof(real) * gpix);
     st->baseMask = (uint8_t *)malloc(gpix);
-    st->image = (double *)malloc(sizeof(double) * npix);
+    st->image = (real *)malloc(sizeof(real) * npix);
     st->minPoints = (int *)malloc(sizeof(int) * (size_t)(st->frameW + 1));
     st->p = (double *)xcalloc((size_t)st->nParams, sizeof(double));
     st->pLast = (double *)xcalloc((size_t)st->nParams, sizeof(double));
     st->deltaP = (double *)xcalloc((size_t)st->nParams, sizeof(double));
     st->newDeltaP = (double *)xcalloc((size_t)st->nParams, sizeof(double));
     st->jtr = (double *)xcalloc((size_t)st->nParams, sizeof(double));
-    st->x = (double *)xcalloc(npix, sizeof(double));
-    st->y = (double *)xcalloc(npix, sizeof(double));
-    st->d = (double *)xcalloc(npix, sizeof(double));
-    st->wx = (double *)xcalloc(npix, sizeof(double));
-    st->wy = (double *)xcalloc(npix, sizeof(double));
+    st->x = (real *)xcalloc(npix, sizeof(real));
+    st->y = (real *)xcalloc(npix, sizeof(real));
+    st->d = (real *)xcalloc(npix, sizeof(real));
+    st->wx = (real *)xcalloc(npix, sizeof(real));
+    st->wy = (real *)xcalloc(npix, sizeof(real));
     st->mask = (uint8_t *)xcalloc(npix, 1);
     st->paramsUsed = (uint8_t *)xcalloc((size_t)st->nParamPoints, 1);
     st->blocksPresent = (uint8_t *)xcalloc((size_t)st->nParamPoints, 1);

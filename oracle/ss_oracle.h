@@ -2,7 +2,9 @@
  * ss_oracle.h -- FP64 CPU restatement of SteadyScan's align_group_to_template
  * path.  TEST INFRASTRUCTURE ONLY: this library is the parity checker for the
  * HIP implementation.  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load it.  The product path never calls it.
+ * cpu_baseline leg may load it.  The product path never calls it.  Built a
+ * second time with ORC_REAL=float (liboracle_f32.so), it is the benchmark's
+ * FP32 CPU baseline; only the FP64 build checks anything.
  *
  * Every array crossing this interface is MATLAB column-major, exactly as the
  * reference mex receives it from steady_scan.m (steady_scan.m:239-242):

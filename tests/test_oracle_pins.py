@@ -569,3 +569,18 @@ def test_resample_last_write_wins():
                            {k: v[::-1].copy() for k, v in geo.items()}, 6, 6)
     assert np.allclose(out2[2:4, 1:3], 7.0)
     assert np.isnan(out[0, 0])
+
+
+# ---------------------------------------------------------------------------
+# the FP32-pixel copy of the oracle (the benchmark's FP32 CPU baseline)
+# ---------------------------------------------------------------------------
+def test_fp32_pixel_copy_tracks_the_oracle():
+    # the same source with float pixels: a baseline to time, not a checker;
+    # it must still solve the problem (knots near the FP64 oracle's)
+    T, fr, geom, _ = W.config0(n_frames=8, size=48, seed=7)
+    a = W.group_args(T, fr, geom, max_iter=6)
+    r64 = O.align_group_to_template(a, threads=2)
+    r32 = O.align_group_to_template(dict(a, flags="S"), threads=2, L=O.lib_f32())
+    d = np.concatenate([r32["knots_x"] - r64["knots_x"], r32["knots_y"] - r64["knots_y"]])
+    assert 0 < np.sqrt(np.mean(d ** 2)) < 0.02
+    assert np.max(np.abs(r32["frame_corr"] - r64["frame_corr"])) < 1e-3
