@@ -122,8 +122,10 @@ typedef struct {
     int32_t geometry_id;         /* > 0: use resident geometry (args 4-7 ignored)   */
 } ss_group_in;
 
-/* Outputs 1-7 of the mex (mexgateway.cpp:320-354) and selected diagnostics
- * (outputs 8, 11, 12 and 15).  NULL pointers skip an output. */
+/* Outputs 1-7 of the mex (mexgateway.cpp:320-354) and the diagnostics 8, 9
+ * and 11-15 (output 10, per-function times, has no counterpart in fused
+ * kernels; ss_profile_* gives per-kernel times).  NULL pointers skip an
+ * output. */
 typedef struct {
     double *knots_x;             /* out 1: (n_blocks+1) x n_frames                  */
     double *knots_y;             /* out 2                                           */
@@ -145,6 +147,16 @@ typedef struct {
     double *iteration_errvals;   /* out 12 ('e'): 2*max_iterations x n_frames, rows
                                     2k and 2k+1 the two shifts' errvals after every
                                     iteration k+1 that continues (task.cpp:362-370) */
+    double *iteration_times;     /* out 9 ('t'/'T'): max_iterations, seconds of each
+                                    iteration by the device clock, from the end of the
+                                    previous one (the first from its first assembly);
+                                    in a batch, the ticks' time shared with its groups */
+    double *raw_params;          /* out 13 ('P'): 2*(n_blocks*n_frames+1), the returned
+                                    knots in the device layout, x then y (task.cpp:216-218) */
+    double *rg_params;           /* out 14: 2*(n_blocks*n_frames+1) x 2*max_iterations,
+                                    the knots after each half-iteration's line search,
+                                    column shift + 2*(iteration-1)
+                                    (timedmainiteration.cpp:124)                      */
 } ss_group_out;
 
 /* Handle lifecycle.  Replaces CudaHelper<T>::initDevice (cudahelper.cu:8-27)

@@ -44,7 +44,7 @@ class OrcOut(ctypes.Structure):
                 ("all_params", POINTER(c_double)), ("iterations", c_int32),
                 ("stop_criterion", c_int32), ("half_iterations", c_int32),
                 ("linesearch_tries", c_int32), ("frame_corr_iter", POINTER(c_double)),
-                ("iteration_errvals", POINTER(c_double))]
+                ("iteration_errvals", POINTER(c_double)), ("rg_params", POINTER(c_double))]
 
 
 class OrcGeometryIn(ctypes.Structure):
@@ -199,10 +199,12 @@ def align_group_to_template(a, threads=1, with_mask=True, with_iterations=False,
     mi = int(a["max_iter"])
     icorr = np.zeros(mi)
     o.iteration_corr = _p(icorr)
-    allp = None
+    allp = rg = None
     if with_iterations:
         allp = np.zeros((2 * (nb * nF + 1), mi), order="F")
         o.all_params = _p(allp)
+        rg = np.zeros((2 * (nb * nF + 1), 2 * mi), order="F")
+        o.rg_params = _p(rg)
     flags = a.get("flags", "D")
     fci = np.zeros((nF, mi), order="F") if "c" in flags else None
     if fci is not None:
@@ -216,7 +218,7 @@ def align_group_to_template(a, threads=1, with_mask=True, with_iterations=False,
     return dict(knots_x=kx, knots_y=ky, frame_corr=fc, residrms=rr, mask=mask,
                 iterations=o.iterations, stop_criterion=o.stop_criterion,
                 half_iterations=o.half_iterations, linesearch_tries=o.linesearch_tries,
-                iteration_corr=icorr[:min(o.iterations, mi)], all_params=allp,
+                iteration_corr=icorr[:min(o.iterations, mi)], all_params=allp, rg_params=rg,
                 frame_corr_iter=fci, iteration_errvals=ierr)
 
 

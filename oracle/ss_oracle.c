@@ -949,8 +949,9 @@ int orc_half_iteration_assemble(orc_state *st, int shift, uint8_t *used, double 
 
 int orc_half_iteration_linesearch(orc_state *st, int shift) { return linesearch(st, shift); }
 
-/* MainIteration::execute (mainiteration.cpp:61-119). */
-static int main_iteration(orc_state *st, int iteration) {
+/* MainIteration::execute (mainiteration.cpp:61-119); rg, if not NULL,
+ * receives p after each half-iteration (timedmainiteration.cpp:124). */
+static int main_iteration(orc_state *st, int iteration, double *rg) {
     int improved = 1;
     for (int shift = 0; shift < 2; shift++) {
         half_iteration_front(st, shift);
@@ -963,12 +964,15 @@ static int main_iteration(orc_state *st, int iteration) {
         expand_results(st, shift);
         improved &= linesearch(st, shift);
         st->halfIterations++;
+        if (rg && iteration <= st->s.max_iterations)
+            memcpy(rg + (size_t)st->nParams * (size_t)(shift + 2 * (iteration - 1)), st->p,
+                   sizeof(double) * (size_t)st->nParams);
     }
     if (!improved) {
         if (st->nFilterIterations != 0) {
             st->solverFilter = 0;
             st->nFilterIterations = 0;
-            return main_iteration(st, iteration);
+            return main_iteration(st, iteration, rg);
         }
         return 0;
     }
@@ -985,7 +989,7 @@ int orc_align_group(const orc_in *in, const orc_settings *s, orc_out *out, int n
     int iteration = 0, retv = ORC_STOP_ERROR;
     for (;;) {
         iteration++;
-        int improved = main_iteration(st, iteration);
+        int improved = main_iteration(st, iteration, out->rg_params);
         if (out->all_params && iteration <= s->max_iterations)
             memcpy(out->all_params + (size_t)st->nParams * (iteration - 1), st->p,
                    sizeof(double) * (size_t)st->nParams);

@@ -85,6 +85,10 @@ typedef struct {
     /* flag 'e' (configuration.cpp:145-147): 2*max_iterations x n_frames,
      * both shifts' errvals after every iteration that continues (task.cpp:362-370) */
     double *iteration_errvals;
+    /* output 14: 2*(n_blocks*n_frames+1) x 2*max_iterations, the parameters
+     * after each half-iteration's line search, column shift + 2*(iteration-1)
+     * (timedmainiteration.cpp:124) */
+    double *rg_params;
 } orc_out;
 
 /* Whole call: MexGateway<double>::execute (mex/mexgateway.cpp:373-445). */

@@ -69,6 +69,8 @@ class SSGroupOut(ctypes.Structure):
         ("linesearch_rounds", c_int32), ("linesearch_tries", c_int32),
         ("mask_on_device", c_int32),
         ("frame_corr_iter", POINTER(c_double)), ("iteration_errvals", POINTER(c_double)),
+        ("iteration_times", POINTER(c_double)), ("raw_params", POINTER(c_double)),
+        ("rg_params", POINTER(c_double)),
     ]
 
 

@@ -214,11 +214,15 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     (block_dx_knots, block_dy_knots, frame_corr, block_iter_used, block_mask,
      residrms, deviceInitTime, iterationCorrelations, iterationTimes,
      functionTimes, frameCorrelations, iterationErrvals, rawParameters,
-     rgParams, allParams); outputs 9, 10, 13 and 14 are empty.  Flag 'c'
-    fills output 11 (n_frames x max_iter) and makes frame_corr the last
-    iteration's values, as the reference does (task.cpp:392-394); flag 'e'
-    fills output 12 (2*max_iter x n_frames; the reference stores it in
-    output 11's slot, mexgateway.cpp:287, which leaves output 12 unset).
+     rgParams, allParams).  Flags 't'/'T' fill output 9 (seconds per
+    iteration, device clock); output 10 is empty (no per-function times in
+    fused kernels).  Flag 'c' fills output 11 (n_frames x max_iter) and
+    makes frame_corr the last iteration's values, as the reference does
+    (task.cpp:392-394); flag 'e' fills output 12 (2*max_iter x n_frames;
+    the reference stores it in output 11's slot, mexgateway.cpp:287, which
+    leaves output 12 unset).  Flag 'P' fills output 13 (the raw knots).
+    Output 14 holds the knots after each half-iteration (the reference fills
+    it in its timing mode only and returns uninitialised memory otherwise).
     `info`, if a dict, receives the stop criterion and work counters.
     `template_bf16` selects mixed precision (precision 'S' only): BF16
     template gradients for the Jacobian, FP32 residuals, one FP64
@@ -321,6 +325,15 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     ierr = np.zeros((2 * int(max_iter), nF), order="F") if (nargout >= 12 and "e" in opts) else None
     if ierr is not None:
         go.iteration_errvals = _ptr(ierr, c_double)
+    itimes = np.zeros(int(max_iter)) if (nargout >= 9 and ("t" in opts or "T" in opts)) else None
+    if itimes is not None:
+        go.iteration_times = _ptr(itimes, c_double)
+    raw = np.zeros((2 * (nb * nF + 1), 1)) if (nargout >= 13 and "P" in opts) else None
+    if raw is not None:
+        go.raw_params = _ptr(raw, c_double)
+    rgp = np.zeros((2 * (nb * nF + 1), 2 * int(max_iter)), order="F") if nargout >= 14 else None
+    if rgp is not None:
+        go.rg_params = _ptr(rgp, c_double)
     if "F" in opts:  # save the call's arguments (the 'F' flag, configuration.h:10-28)
         if N.lib().ss_rom_write(ROM_FILE.encode(), byref(gi), byref(s)) != 0:
             raise SteadyScanError(N.SS_ERR_INVALID_ARGUMENT, f"could not write {ROM_FILE}")
@@ -336,8 +349,9 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
             None if mask is None else mask.astype(bool), rr.reshape(-1, 1), go.device_init_time,
             icorr if (icorr is not None and "C" in opts) else np.zeros((0, 0))]
     empty = np.zeros((0, 0))
-    # 9, 10: timings; 11 ('c'), 12 ('e'); 13, 14: raw and red/green parameters
-    outs += [empty, empty, empty if fci is None else fci, empty if ierr is None else ierr, empty, empty]
+    # 9 ('t'/'T'), 10 (per-function times: none), 11 ('c'), 12 ('e'), 13 ('P'), 14
+    outs += [empty if itimes is None else itimes.reshape(-1, 1), empty, empty if fci is None else fci,
+             empty if ierr is None else ierr, empty if raw is None else raw, empty if rgp is None else rgp]
     outs.append(allp if allp is not None else empty)
     return tuple(outs[:nargout])
 
@@ -422,6 +436,12 @@ def _prepare_call(a, h, s, gi, go, with_mask, linesearch_candidates, keep):
     if "e" in opts:  # per-iteration errvals (output 12)
         r["iteration_errvals"] = np.zeros((2 * mi, nF), order="F")
         go.iteration_errvals = _ptr(r["iteration_errvals"], c_double)
+    if "t" in opts or "T" in opts:  # seconds per iteration (output 9)
+        r["iteration_times"] = np.zeros(mi)
+        go.iteration_times = _ptr(r["iteration_times"], c_double)
+    if "P" in opts:  # raw knots (output 13)
+        r["raw_params"] = np.zeros(2 * (nb * nF + 1))
+        go.raw_params = _ptr(r["raw_params"], c_double)
     return r
 
 

@@ -110,6 +110,10 @@ struct GroupDev {
     // on the state the global correlation leaves (or null); flag 'e':
     // 2*maxIter x nF errvals of both shifts after every continuing iteration
     double *fcIter, *iterErr;
+    // output 14: knots after each half-iteration; output 9: seconds per
+    // iteration by the device clock (clockHz ticks per second)
+    double *rgParams, *iterTimes;
+    double clockHz;
 };
 
 // Dynamic control state of one group (read by the host after each tick).
@@ -122,6 +126,7 @@ struct GroupCtl {
     int lsFramePasses;    // frames evaluated over all line-search rounds
     int pad;
     double correlation, corr;
+    unsigned long long tIter;  // device clock at the end of the last iteration (0: not started)
 };
 
 struct Batch {

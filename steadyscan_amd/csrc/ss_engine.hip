@@ -124,6 +124,7 @@ struct TuneKey {
 struct ss_handle {
     int device = 0;
     int numCU = 256;
+    double clockHz = 1e8;  // device wall clock (wall_clock64) ticks per second
     hipStream_t stream = nullptr;
     std::string err;
     std::map<int, DevTemplate> templates;
@@ -488,7 +489,7 @@ struct Plan {
     size_t oTp = 0, oGxv = 0, oGyh = 0, oGeo = 0, oFn = 0, oImg = 0, oMin = 0;
     size_t oP = 0, oPL = 0, oP0 = 0, oD0 = 0, oDP = 0, oErr = 0, oCol = 0, oLs = 0, oCorr = 0,
            oFin = 0, oWs = 0, oUsed = 0, oAct = 0, oNext = 0, oMk = 0, oDk = 0, oMask = 0,
-           oRes = 0, oFc = 0, oAll = 0, oIc = 0, oFci = 0, oIe = 0;
+           oRes = 0, oFc = 0, oAll = 0, oIc = 0, oFci = 0, oIe = 0, oRg = 0, oIt = 0;
     size_t hImg = 0, hP = 0;  // host staging offsets
     size_t dBytes = 0, hBytes = 0;
     int imgType = 0;
@@ -576,6 +577,8 @@ ss_status plan_group(ss_handle *h, Plan &P) {
     if (P.out->iteration_corr) P.oIc = A.take<double>(P.s->max_iterations);
     if (P.s->save_frame_correlations) P.oFci = A.take<double>((size_t)nF * P.s->max_iterations);
     if (P.out->iteration_errvals) P.oIe = A.take<double>(2 * (size_t)nF * P.s->max_iterations);
+    if (P.out->rg_params) P.oRg = A.take<double>(Pn * 2 * P.s->max_iterations);
+    if (P.out->iteration_times) P.oIt = A.take<double>(P.s->max_iterations);
     P.hP = H.take<double>(Pn + 2 * (size_t)nF);
     P.dBytes = A.off;
     P.hBytes = H.off;
@@ -787,6 +790,12 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     g.iterErr = P.out->iteration_errvals ? (double *)dptr(P.oIe) : nullptr;
     if (g.fcIter) HIPCHK(hipMemsetAsync(g.fcIter, 0, (size_t)nF * mi * sizeof(double), h->stream));
     if (g.iterErr) HIPCHK(hipMemsetAsync(g.iterErr, 0, 2 * (size_t)nF * mi * sizeof(double), h->stream));
+    // outputs 14 and 9
+    g.rgParams = P.out->rg_params ? (double *)dptr(P.oRg) : nullptr;
+    g.iterTimes = P.out->iteration_times ? (double *)dptr(P.oIt) : nullptr;
+    g.clockHz = h->clockHz;
+    if (g.rgParams) HIPCHK(hipMemsetAsync(g.rgParams, 0, Pn * 2 * mi * sizeof(double), h->stream));
+    if (g.iterTimes) HIPCHK(hipMemsetAsync(g.iterTimes, 0, mi * sizeof(double), h->stream));
     // initial knots (task_allocate.cu:130-134), deltaP = 0, errval = 1e6 (task_allocate.cu:150-184)
     double *hp = (double *)(HS + P.hP);
     std::memset(hp, 0, sizeof(double) * Pn);
@@ -838,8 +847,13 @@ ss_status finish_group(ss_handle *h, const Plan &P, const ss::GroupDev &g, const
     if (o.iteration_errvals)
         HIPCHK(hipMemcpyAsync(o.iteration_errvals, g.iterErr, 2 * nF * mi * sizeof(double), hipMemcpyDeviceToHost,
                               h->stream));
+    if (o.rg_params)
+        HIPCHK(hipMemcpyAsync(o.rg_params, g.rgParams, Pn * 2 * mi * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (o.iteration_times)
+        HIPCHK(hipMemcpyAsync(o.iteration_times, g.iterTimes, mi * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (o.knots_x && o.knots_y) knots_out(pl.data(), nF, nb, o.knots_x, o.knots_y);
+    if (o.raw_params) std::memcpy(o.raw_params, pl.data(), Pn * sizeof(double));  // output 13
     o.iterations = c.iteration;
     o.stop_criterion = c.retv;
     o.half_iterations = c.halfIterations;
@@ -1624,6 +1638,9 @@ ss_status ss_create(int32_t device_id, ss_handle **out) try {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device_id) == hipSuccess && prop.multiProcessorCount > 0)
         h->numCU = prop.multiProcessorCount;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_id) == hipSuccess && khz > 0)
+        h->clockHz = 1e3 * khz;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         ss_destroy(h);
         return SS_ERR_DEVICE;

@@ -454,6 +454,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_varian
         if (B.ctl[gi].phase != PH_ASSEMBLE) continue;
         const GroupDev &g = B.groups[gi];
         if (!runs_here<TI, R>(g) || g.var[TK_ASSEMBLE] != V) continue;
+        if (g.iterTimes && item == g.itemStart && threadIdx.x == 0 && B.ctl[gi].tIter == 0)
+            B.ctl[gi].tIter = wall_clock64();  // the first iteration starts
         assemble_column<TI, R, V>(g, item - g.itemStart);
     }
 }
@@ -1401,10 +1403,16 @@ __global__ __launch_bounds__(64) void k_ls_finish(Batch B) {
     if (ctl.phase != PH_LS) return;
     const GroupDev &g = B.groups[gi];
     const int nF = g.nF;
+    const int shift0 = ctl.shift, it0 = ctl.iteration;
     int active = 0;
     for (int f = threadIdx.x; f < nF; f += 64) active += g.lsActive[f];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) active += __shfl_xor(active, off, 64);
+    if (active == 0 && g.rgParams && it0 <= g.maxIter) {  // output 14 (timedmainiteration.cpp:124)
+        const int P = 2 * g.N;
+        double *dst = g.rgParams + (size_t)P * (size_t)(shift0 + 2 * (it0 - 1));
+        for (int i = threadIdx.x; i < P; i += 64) dst[i] = g.p[i];
+    }
     if (threadIdx.x != 0) return;
     ctl.rounds++;
     ctl.roundsThisLS++;
@@ -1672,6 +1680,11 @@ __global__ __launch_bounds__(kSolveWG) void k_corr_finish(Batch B) {
         const double corr = q / sqrt(assq * bssq);
         ctl.corr = corr;
         if (g.iterCorr && it <= g.maxIter) g.iterCorr[it - 1] = corr;
+        if (g.iterTimes) {  // output 9: this iteration's time
+            const unsigned long long now = wall_clock64();
+            if (it <= g.maxIter) g.iterTimes[it - 1] = (double)(now - ctl.tIter) / g.clockHz;
+            ctl.tIter = now;
+        }
         const double correlation = ctl.correlation;
         int copy = 0, save = 0;
         if (corr - correlation < g.cit) {

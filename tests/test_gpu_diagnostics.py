@@ -85,3 +85,36 @@ def test_frame_correlations_in_precision_s():
     pairs = out[11][:2 * (it - 1)].reshape(it - 1, 2, -1)
     assert np.all(pairs[0] > 0)
     assert all(np.all(p > 0) or np.all(p == 0) for p in pairs)
+
+
+def test_iteration_times_raw_and_half_iteration_knots():
+    # output 9 ('t'): seconds per iteration by the device clock; output 13
+    # ('P'): the returned knots in the device layout; output 14: the knots
+    # after each half-iteration, against the oracle's
+    import ctypes
+    import time
+    a = case(1, flags="DtP")
+    nF, mi = a["nsetframes"], 12
+    nb = a["basex"].shape[1]
+    P = 2 * (nb * nF + 1)
+    ref = O.align_group_to_template(a, threads=8, with_iterations=True)
+    t = time.perf_counter()
+    out = call(a, nargout=15)
+    wall = time.perf_counter() - t
+    it = int(out[3])
+    assert it == ref["iterations"]
+    times = np.ravel(out[8])
+    print(f"iteration times (ms): {np.round(times[:it] * 1e3, 3).tolist()}, call {wall * 1e3:.1f} ms", flush=True)
+    assert times.shape == (mi,) and np.all(times[:it] > 0) and np.all(times[it:] == 0)
+    assert times.sum() < wall
+    raw = np.ravel(out[12])
+    assert raw.shape == (P,)
+    kx, ky = np.zeros((nb + 1) * nF), np.zeros((nb + 1) * nF)
+    PD = ctypes.POINTER(ctypes.c_double)
+    O.lib().orc_organize_out(raw.ctypes.data_as(PD), nF, nb, kx.ctypes.data_as(PD), ky.ctypes.data_as(PD))
+    assert np.array_equal(kx, np.ravel(out[0])) and np.array_equal(ky, np.ravel(out[1]))
+    rg = out[13]
+    assert rg.shape == (P, 2 * mi)
+    assert np.all(rg[:, 2 * it:] == 0)
+    assert np.max(np.abs(rg - ref["rg_params"])) <= 1e-9
+    assert out[9].size == 0  # no per-function times

@@ -425,6 +425,8 @@ def iteration_errvals_ok(L):
     assert E.shape == (2 * mi, nF) and it >= 3
     # saved after every iteration that continues: all but the last
     assert np.all(E[2 * (it - 1):, :] == 0) and np.all(E[:2 * (it - 1), :] > 0)
+    assert np.all(r["rg_params"][:, 2 * it:] == 0)
+    assert np.array_equal(r["rg_params"][:, 2 * it - 1], allp[:, it - 1])  # the last half-iteration's p
     # replay: half-iteration by half-iteration on a state
     st = O.State(a, L=L)
     for k in range(1, it):
@@ -433,6 +435,8 @@ def iteration_errvals_ok(L):
             st.assemble(shift)
             st.linesearch(shift)
             ev = st.get()[2]
+            # output 14: the parameters after this half-iteration
+            assert np.array_equal(st.get()[0], r["rg_params"][:, shift + 2 * (k - 1)])
             # the front errval of frames > 0: the masked mean squared residual
             # at the knots the half-iteration started from (frame 0's slot
             # takes the line search's accepted value, task_linesearch.cu:91)
