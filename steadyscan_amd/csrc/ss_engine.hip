@@ -1179,19 +1179,24 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
             if (!finished(c[gi])) return false;
         return true;
     };
+    // a failure leaves no read-back in flight into the staging, which the
+    // next call writes from the host
+    auto stop = [&](ss_status e) {
+        (void)hipStreamSynchronize(h->stream);
+        return e;
+    };
     bool done = false;
     int t = 0;
     for (; t < kTickCap; t++) {
         st = enqueue_tick(h, L);
-        if (st) return st;
-        HIPCHK(hipMemcpyAsync(hcb[t & 1], L.b.ctl, ctlBytes, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipEventRecord(h->tickDone[t & 1], h->stream));
-        if (t > 0) {
-            HIPCHK(hipEventSynchronize(h->tickDone[(t - 1) & 1]));
-            if (all_finished(hcb[(t - 1) & 1])) {
-                done = true;
-                break;
-            }
+        if (st) return stop(st);
+        hipError_t e = hipMemcpyAsync(hcb[t & 1], L.b.ctl, ctlBytes, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipEventRecord(h->tickDone[t & 1], h->stream);
+        if (e == hipSuccess && t > 0) e = hipEventSynchronize(h->tickDone[(t - 1) & 1]);
+        if (e != hipSuccess) return stop(fail(h, SS_ERR_DEVICE, std::string("tick read-back: ") + hipGetErrorString(e)));
+        if (t > 0 && all_finished(hcb[(t - 1) & 1])) {
+            done = true;
+            break;
         }
     }
     HIPCHK(hipStreamSynchronize(h->stream));
