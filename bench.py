@@ -25,8 +25,9 @@ Precision: 'D' (the reference's default, FP64 pixel arithmetic) is the
 headline; 'S' (FP32 pixel arithmetic, FP64 reductions and solve) and the
 mixed-precision mode are timed beside it.
 
-CPU baseline: the FP64 oracle (the C restatement) on the GPU box's host, on
-group 0 with OpenMP and on the first frames of group 0 with one thread.
+CPU baseline: the FP64 oracle (the C restatement) on the host cores of the
+machine that holds the GPU, on group 0 with OpenMP and on the first frames of
+group 0 with one thread; and the same source built with FP32 pixels, OpenMP.
 """
 import argparse
 import ctypes
@@ -393,7 +394,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              "(groups are independent; see DESIGN.md for how the path shards)", file=sys.stderr)
+              "(groups are independent: on several GPUs each would align its own groups)", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
