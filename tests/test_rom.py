@@ -110,3 +110,12 @@ def test_records_whose_shapes_disagree_are_rejected(call, tmp_path, tag):
     p.write_bytes(bytes(raw))
     with pytest.raises(ssa.SteadyScanError):
         ssa.load_rom(p)
+
+
+def test_frame_correlation_flag_round_trips(call, tmp_path):
+    # flag 'c' changes output 3 (the last iteration's correlations), so a
+    # replay must keep it
+    call["flags"] = "Sc"
+    p = tmp_path / "call_c.rom"
+    ssa.save_rom(p, call)
+    assert ssa.load_rom(p)["flags"] == "Sc"
