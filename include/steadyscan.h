@@ -122,10 +122,8 @@ typedef struct {
     int32_t geometry_id;         /* > 0: use resident geometry (args 4-7 ignored)   */
 } ss_group_in;
 
-/* Outputs 1-7 of the mex (mexgateway.cpp:320-354) and the diagnostics 8, 9
- * and 11-15 (output 10, per-function times, has no counterpart in fused
- * kernels; ss_profile_* gives per-kernel times).  NULL pointers skip an
- * output. */
+/* Outputs 1-7 of the mex (mexgateway.cpp:320-354) and the diagnostics 8-15.
+ * NULL pointers skip an output. */
 typedef struct {
     double *knots_x;             /* out 1: (n_blocks+1) x n_frames                  */
     double *knots_y;             /* out 2                                           */
@@ -157,6 +155,13 @@ typedef struct {
                                     the knots after each half-iteration's line search,
                                     column shift + 2*(iteration-1)
                                     (timedmainiteration.cpp:124)                      */
+    double *function_times;      /* out 10 ('t'/'T'): 14 slots of seconds, the
+                                    reference's functionTimes order (timedmainiteration
+                                    .cpp:50-126, task.cpp:295, 355) with the fused stages
+                                    in slot 1 (parameters, difference image, gradients,
+                                    normal equations), 7 (solve, filter, expansion), 9
+                                    (line search) and 10 (correlations, final statistics);
+                                    device clock between the group's stage ends        */
 } ss_group_out;
 
 /* Handle lifecycle.  Replaces CudaHelper<T>::initDevice (cudahelper.cu:8-27)

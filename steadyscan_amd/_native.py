@@ -19,6 +19,7 @@ STATUS_IDS = {
     SS_ERR_NOT_AVAILABLE: "GPUMC:CudaError",
 }
 
+FUNC_SLOTS = 14  # output 10: the reference's function-time slots (timedmainiteration.cpp:28, task.cpp:244)
 STOP_NAMES = ["CorrelationIncreaseThreshold", "HaltCorrelationReached", "MovementThreshold",
               "IterationLimit", "Error"]
 
@@ -70,7 +71,7 @@ class SSGroupOut(ctypes.Structure):
         ("mask_on_device", c_int32),
         ("frame_corr_iter", POINTER(c_double)), ("iteration_errvals", POINTER(c_double)),
         ("iteration_times", POINTER(c_double)), ("raw_params", POINTER(c_double)),
-        ("rg_params", POINTER(c_double)),
+        ("rg_params", POINTER(c_double)), ("function_times", POINTER(c_double)),
     ]
 
 

@@ -214,9 +214,10 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     (block_dx_knots, block_dy_knots, frame_corr, block_iter_used, block_mask,
      residrms, deviceInitTime, iterationCorrelations, iterationTimes,
      functionTimes, frameCorrelations, iterationErrvals, rawParameters,
-     rgParams, allParams).  Flags 't'/'T' fill output 9 (seconds per
-    iteration, device clock); output 10 is empty (no per-function times in
-    fused kernels).  Flag 'c' fills output 11 (n_frames x max_iter) and
+     rgParams, allParams).  Flags 't'/'T' fill outputs 9 (seconds per
+    iteration) and 10 (seconds per stage in the reference's 14 function-time
+    slots: 1 assembly, 7 solve, 9 line search, 10 correlations), both from
+    the device clock.  Flag 'c' fills output 11 (n_frames x max_iter) and
     makes frame_corr the last iteration's values, as the reference does
     (task.cpp:392-394); flag 'e' fills output 12 (2*max_iter x n_frames;
     the reference stores it in output 11's slot, mexgateway.cpp:287, which
@@ -328,6 +329,9 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
     itimes = np.zeros(int(max_iter)) if (nargout >= 9 and ("t" in opts or "T" in opts)) else None
     if itimes is not None:
         go.iteration_times = _ptr(itimes, c_double)
+    ftimes = np.zeros(N.FUNC_SLOTS) if (nargout >= 10 and ("t" in opts or "T" in opts)) else None
+    if ftimes is not None:
+        go.function_times = _ptr(ftimes, c_double)
     raw = np.zeros((2 * (nb * nF + 1), 1)) if (nargout >= 13 and "P" in opts) else None
     if raw is not None:
         go.raw_params = _ptr(raw, c_double)
@@ -349,8 +353,9 @@ def align_group_to_template(flags, template_image, xgrad, ygrad, tfrac, basex, b
             None if mask is None else mask.astype(bool), rr.reshape(-1, 1), go.device_init_time,
             icorr if (icorr is not None and "C" in opts) else np.zeros((0, 0))]
     empty = np.zeros((0, 0))
-    # 9 ('t'/'T'), 10 (per-function times: none), 11 ('c'), 12 ('e'), 13 ('P'), 14
-    outs += [empty if itimes is None else itimes.reshape(-1, 1), empty, empty if fci is None else fci,
+    # 9, 10 ('t'/'T'), 11 ('c'), 12 ('e'), 13 ('P'), 14
+    outs += [empty if itimes is None else itimes.reshape(-1, 1), empty if ftimes is None else ftimes.reshape(-1, 1),
+             empty if fci is None else fci,
              empty if ierr is None else ierr, empty if raw is None else raw, empty if rgp is None else rgp]
     outs.append(allp if allp is not None else empty)
     return tuple(outs[:nargout])

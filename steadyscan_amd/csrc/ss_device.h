@@ -34,6 +34,10 @@ enum CorrSum {
 
 // Per-column sums of the final pass (FP64).
 enum FinalSum { FS_D2 = 0, FS_CNT, FS_SA, FS_SAA, FS_SB, FS_SBB, FS_SAB, FS_N };
+// output 10: the reference's 14 function-time slots and the ones the fused
+// stages fill (timedmainiteration.cpp:50-126, task.cpp:295, 355)
+constexpr int kFuncSlots = 14;
+enum FuncSlot { FT_ASSEMBLY = 1, FT_SOLVE = 7, FT_LINESEARCH = 9, FT_CORRELATION = 10 };
 // per-iteration frame correlation sums (flag 'c'), kept in finalsum's slots
 // while the group is in its correlation phase
 enum FrameCorrSum { FC_CNT = 0, FC_SA, FC_SAA, FC_SB, FC_SBB, FC_SAB, FC_N };
@@ -114,6 +118,7 @@ struct GroupDev {
     // iteration by the device clock (clockHz ticks per second)
     double *rgParams, *iterTimes;
     double clockHz;
+    double *funcTimes;  // output 10: kFuncSlots seconds by stage (or null)
 };
 
 // Dynamic control state of one group (read by the host after each tick).
@@ -127,6 +132,7 @@ struct GroupCtl {
     int pad;
     double correlation, corr;
     unsigned long long tIter;  // device clock at the end of the last iteration (0: not started)
+    unsigned long long tStage; // device clock at the end of the group's last stage (output 10)
 };
 
 struct Batch {

@@ -489,7 +489,7 @@ struct Plan {
     size_t oTp = 0, oGxv = 0, oGyh = 0, oGeo = 0, oFn = 0, oImg = 0, oMin = 0;
     size_t oP = 0, oPL = 0, oP0 = 0, oD0 = 0, oDP = 0, oErr = 0, oCol = 0, oLs = 0, oCorr = 0,
            oFin = 0, oWs = 0, oUsed = 0, oAct = 0, oNext = 0, oMk = 0, oDk = 0, oMask = 0,
-           oRes = 0, oFc = 0, oAll = 0, oIc = 0, oFci = 0, oIe = 0, oRg = 0, oIt = 0;
+           oRes = 0, oFc = 0, oAll = 0, oIc = 0, oFci = 0, oIe = 0, oRg = 0, oIt = 0, oFt = 0;
     size_t hImg = 0, hP = 0;  // host staging offsets
     size_t dBytes = 0, hBytes = 0;
     int imgType = 0;
@@ -579,6 +579,7 @@ ss_status plan_group(ss_handle *h, Plan &P) {
     if (P.out->iteration_errvals) P.oIe = A.take<double>(2 * (size_t)nF * P.s->max_iterations);
     if (P.out->rg_params) P.oRg = A.take<double>(Pn * 2 * P.s->max_iterations);
     if (P.out->iteration_times) P.oIt = A.take<double>(P.s->max_iterations);
+    if (P.out->function_times) P.oFt = A.take<double>(ss::kFuncSlots);
     P.hP = H.take<double>(Pn + 2 * (size_t)nF);
     P.dBytes = A.off;
     P.hBytes = H.off;
@@ -796,6 +797,8 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     g.clockHz = h->clockHz;
     if (g.rgParams) HIPCHK(hipMemsetAsync(g.rgParams, 0, Pn * 2 * mi * sizeof(double), h->stream));
     if (g.iterTimes) HIPCHK(hipMemsetAsync(g.iterTimes, 0, mi * sizeof(double), h->stream));
+    g.funcTimes = P.out->function_times ? (double *)dptr(P.oFt) : nullptr;  // output 10
+    if (g.funcTimes) HIPCHK(hipMemsetAsync(g.funcTimes, 0, ss::kFuncSlots * sizeof(double), h->stream));
     // initial knots (task_allocate.cu:130-134), deltaP = 0, errval = 1e6 (task_allocate.cu:150-184)
     double *hp = (double *)(HS + P.hP);
     std::memset(hp, 0, sizeof(double) * Pn);
@@ -851,6 +854,9 @@ ss_status finish_group(ss_handle *h, const Plan &P, const ss::GroupDev &g, const
         HIPCHK(hipMemcpyAsync(o.rg_params, g.rgParams, Pn * 2 * mi * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (o.iteration_times)
         HIPCHK(hipMemcpyAsync(o.iteration_times, g.iterTimes, mi * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (o.function_times)
+        HIPCHK(hipMemcpyAsync(o.function_times, g.funcTimes, ss::kFuncSlots * sizeof(double), hipMemcpyDeviceToHost,
+                              h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (o.knots_x && o.knots_y) knots_out(pl.data(), nF, nb, o.knots_x, o.knots_y);
     if (o.raw_params) std::memcpy(o.raw_params, pl.data(), Pn * sizeof(double));  // output 13

@@ -454,8 +454,14 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_varian
         if (B.ctl[gi].phase != PH_ASSEMBLE) continue;
         const GroupDev &g = B.groups[gi];
         if (!runs_here<TI, R>(g) || g.var[TK_ASSEMBLE] != V) continue;
-        if (g.iterTimes && item == g.itemStart && threadIdx.x == 0 && B.ctl[gi].tIter == 0)
-            B.ctl[gi].tIter = wall_clock64();  // the first iteration starts
+        if ((g.iterTimes || g.funcTimes) && item == g.itemStart && threadIdx.x == 0) {
+            GroupCtl &c = B.ctl[gi];  // the first iteration starts (outputs 9 and 10)
+            if (c.tIter == 0 || c.tStage == 0) {
+                const unsigned long long now = wall_clock64();
+                if (c.tIter == 0) c.tIter = now;
+                if (c.tStage == 0) c.tStage = now;
+            }
+        }
         assemble_column<TI, R, V>(g, item - g.itemStart);
     }
 }
@@ -939,6 +945,15 @@ __device__ void exact_solve(const Eq0 *E, InteriorRow *IR, double2 *X, Aff *firs
         exact_solve_t<0>(E, IR, X, firstAff, n, tid);
 }
 
+// Output 10: the device time since the group's last stage end goes to slot
+// `slot`.  Called by one thread of a per-group kernel.
+__device__ __forceinline__ void stage_time(const GroupDev &g, GroupCtl &ctl, int slot) {
+    if (!g.funcTimes) return;
+    const unsigned long long now = wall_clock64();
+    if (ctl.tStage) g.funcTimes[slot] += (double)(now - ctl.tStage) / g.clockHz;
+    ctl.tStage = now;
+}
+
 __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
     const int gi = blockIdx.x;
     GroupCtl &ctl = B.ctl[gi];
@@ -956,6 +971,7 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
             ctl.nFilterIterations = 0;
         }
         s_filter = ctl.solverFilter;
+        stage_time(g, ctl, FT_ASSEMBLY);
     }
     const double *S = g.colsum;
     const int *minPts = g.minPts;
@@ -1180,6 +1196,7 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
         ctl.roundsThisLS = 0;
         ctl.K = g.K1;
         ctl.phase = PH_LS;
+        stage_time(g, ctl, FT_SOLVE);
     }
 }
 
@@ -1414,6 +1431,7 @@ __global__ __launch_bounds__(64) void k_ls_finish(Batch B) {
         for (int i = threadIdx.x; i < P; i += 64) dst[i] = g.p[i];
     }
     if (threadIdx.x != 0) return;
+    stage_time(g, ctl, FT_LINESEARCH);
     ctl.rounds++;
     ctl.roundsThisLS++;
     ctl.lsFramePasses += ctl.roundFrames;
@@ -1679,6 +1697,7 @@ __global__ __launch_bounds__(kSolveWG) void k_corr_finish(Batch B) {
         const double q = tot[CR_SAB] - mB * tot[CR_SA] - mA * tot[CR_SB] + ns * mA * mB;
         const double corr = q / sqrt(assq * bssq);
         ctl.corr = corr;
+        stage_time(g, ctl, FT_CORRELATION);
         if (g.iterCorr && it <= g.maxIter) g.iterCorr[it - 1] = corr;
         if (g.iterTimes) {  // output 9: this iteration's time
             const unsigned long long now = wall_clock64();
@@ -1819,7 +1838,10 @@ __global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
         g.fcorr[f] = g.fcIter ? g.fcIter[(size_t)(ctl.iteration - 1) * g.nF + f] : q / sqrt(assq * bssq);
     }
     __syncthreads();
-    if (threadIdx.x == 0) ctl.phase = PH_DONE;
+    if (threadIdx.x == 0) {
+        stage_time(g, ctl, FT_CORRELATION);
+        ctl.phase = PH_DONE;
+    }
 }
 
 // ---------------------------------------------------------------------------

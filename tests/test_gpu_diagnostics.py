@@ -88,7 +88,8 @@ def test_frame_correlations_in_precision_s():
 
 
 def test_iteration_times_raw_and_half_iteration_knots():
-    # output 9 ('t'): seconds per iteration by the device clock; output 13
+    # outputs 9 and 10 ('t'): seconds per iteration and per stage by the
+    # device clock; output 13
     # ('P'): the returned knots in the device layout; output 14: the knots
     # after each half-iteration, against the oracle's
     import ctypes
@@ -117,4 +118,9 @@ def test_iteration_times_raw_and_half_iteration_knots():
     assert rg.shape == (P, 2 * mi)
     assert np.all(rg[:, 2 * it:] == 0)
     assert np.max(np.abs(rg - ref["rg_params"])) <= 1e-9
-    assert out[9].size == 0  # no per-function times
+    ft = np.ravel(out[9])  # seconds per stage in the reference's slots
+    print(f"function times (ms): {dict((i, round(v * 1e3, 3)) for i, v in enumerate(ft) if v)}", flush=True)
+    assert ft.shape == (14,) and np.all(ft >= 0)
+    assert all(ft[i] > 0 for i in (1, 7, 9, 10))
+    assert np.all(np.delete(ft, [1, 7, 9, 10]) == 0)
+    assert ft.sum() < wall
