@@ -635,7 +635,7 @@ __device__ void ilu0_solve(const Eq0 *E, IluRow *F, double2 *X, int n) {
 __device__ __forceinline__ void level0_equations(const double *__restrict__ S, const uint8_t *__restrict__ used,
                                                  Eq0 *__restrict__ E, int n, double lambda, int fracUniform,
                                                  int tid) {
-#pragma unroll 2
+#pragma unroll 4
     for (int k = tid; k < n; k += kSolveWG) {
         Eq0 e;
         const double *s = S + (size_t)k * CS_N;
@@ -938,7 +938,9 @@ __device__ void exact_solve_t(const Eq0 *__restrict__ E, InteriorRow *__restrict
 // knots per partition that stay in registers
 constexpr int kSolveRegRows = 8;
 
-__device__ void exact_solve(const Eq0 *E, InteriorRow *IR, double2 *X, Aff *firstAff, int n, int tid) {
+// Inlined into the solve kernel: as a called function it would save and
+// restore the callee-saved registers it uses (over 200 per call).
+__device__ __forceinline__ void exact_solve(const Eq0 *E, InteriorRow *IR, double2 *X, Aff *firstAff, int n, int tid) {
     if ((n + kSolveWG - 1) / kSolveWG <= kSolveRegRows)
         exact_solve_t<kSolveRegRows>(E, IR, X, firstAff, n, tid);
     else
@@ -980,20 +982,22 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
     // one thread per frame or knot would otherwise wait out a memory latency
     // per entry.  The additions keep their order.
     constexpr int kBatch = 8;
-    // 1. per-frame errval of this shift (d_calcAllGroupErrval)
+    // 1. per-frame errval of this shift (d_calcAllGroupErrval): a frame's
+    //    columns in batches of 32 (one round of loads for 512x512 frames)
+    constexpr int kEvBatch = 32;
     for (int f = tid; f < nF; f += kSolveWG) {
         const double *col = S + (size_t)f * nb * CS_N;
         double s = 0.0, c = 0.0;
-        for (int b0 = 0; b0 < nb; b0 += kBatch) {
-            double vs[kBatch], vc[kBatch];
+        for (int b0 = 0; b0 < nb; b0 += kEvBatch) {
+            double vs[kEvBatch], vc[kEvBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; u++) {
+            for (int u = 0; u < kEvBatch; u++) {
                 const int bb = min(b0 + u, nb - 1);
                 vs[u] = col[(size_t)bb * CS_N + CS_D2];
                 vc[u] = col[(size_t)bb * CS_N + CS_CNT];
             }
 #pragma unroll
-            for (int u = 0; u < kBatch; u++) {
+            for (int u = 0; u < kEvBatch; u++) {
                 if (b0 + u < nb) {
                     s += vs[u];
                     c += vc[u];
