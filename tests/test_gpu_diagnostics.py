@@ -124,3 +124,26 @@ def test_iteration_times_raw_and_half_iteration_knots():
     assert all(ft[i] > 0 for i in (1, 7, 9, 10))
     assert np.all(np.delete(ft, [1, 7, 9, 10]) == 0)
     assert ft.sum() < wall
+
+
+def test_diagnostics_through_the_server_equal_batched():
+    # groups admitted while others run (the table is rebuilt around them)
+    # return the same per-iteration outputs as a batched submission
+    calls = [case(1, flags="Dce", seed=41), case(0, flags="Dc", seed=42), case(1, flags="De", seed=43)]
+    ref = ssa.align_groups(calls, with_mask=False)
+    srv = ssa.Server(max_groups=2)
+    tickets, done = [], {}
+    for a in calls:
+        tickets.append(srv.submit(a))
+        srv.step()
+        done.update(srv.completed())
+    while len(done) < len(calls):
+        srv.step()
+        done.update(srv.completed())
+    for t, r in zip(tickets, ref):
+        got = done[t]
+        assert not isinstance(got, Exception), got
+        for k in ("knots_x", "knots_y", "frame_corr", "residrms", "frame_corr_iter", "iteration_errvals"):
+            if k in r:
+                assert np.array_equal(got[k], r[k]), k
+    srv.close()
