@@ -127,7 +127,7 @@ struct GroupCtl {
     int solverFilter, filterID, nFilterIterations;
     int retv, error;
     int halfIterations, rounds, tries, roundsThisLS;
-    int roundFrames, K;   // frames evaluated in the current round, candidates per frame
+    int slotRounds, K;    // rounds run in the tick's current line-search slots, candidates per frame
     int lsFramePasses;    // frames evaluated over all line-search rounds
     int pad;
     double correlation, corr;
@@ -154,7 +154,16 @@ struct LaunchBatch {
 // Stages of one host submission, in order; see launch_stage.
 enum Stage { ST_ASSEMBLE = 0, ST_SOLVE, ST_LS, ST_LS_DECIDE, ST_CORR, ST_CORR_FINISH, ST_FINAL, ST_FINAL_FINISH };
 
-hipError_t launch_stage(const LaunchBatch &L, int stage, hipStream_t s);
+// Line-search rounds per half-iteration in a tick.  The per-group
+// bookkeeping of the rounds (k_ls_finish) runs once, after the last slot.
+constexpr int kLsSlots = 3;
+// A tick entry: the stage, and for ST_LS and ST_LS_DECIDE the round's slot
+// (0 .. kLsSlots - 1) above bit 8.
+constexpr int tick_entry(int stage, int slot = 0) { return stage | slot << 8; }
+constexpr int entry_stage(int e) { return e & 0xff; }
+constexpr int entry_slot(int e) { return e >> 8; }
+
+hipError_t launch_stage(const LaunchBatch &L, int entry, hipStream_t s);
 size_t solve_workspace_bytes(int n);
 
 }  // namespace ss

@@ -196,8 +196,8 @@ void prof_collect(ss_handle *h) {
     h->prof.pending.clear();
 }
 
-int stage_class(int stage) {
-    switch (stage) {
+int stage_class(int entry) {
+    switch (ss::entry_stage(entry)) {
     case ss::ST_ASSEMBLE: return SS_K_ASSEMBLE;
     case ss::ST_SOLVE: return SS_K_SOLVE;
     case ss::ST_LS: return SS_K_LS_ROUND;
@@ -208,16 +208,16 @@ int stage_class(int stage) {
     }
 }
 
-hipError_t run_stage(ss_handle *h, const ss::LaunchBatch &L, int stage) {
-    if (!h->prof.on) return ss::launch_stage(L, stage, h->stream);
+hipError_t run_stage(ss_handle *h, const ss::LaunchBatch &L, int entry) {
+    if (!h->prof.on) return ss::launch_stage(L, entry, h->stream);
     hipEvent_t a = prof_event(h), b = prof_event(h);
-    if (!a || !b) return ss::launch_stage(L, stage, h->stream);
+    if (!a || !b) return ss::launch_stage(L, entry, h->stream);
     hipError_t e = hipEventRecord(a, h->stream);
     if (e != hipSuccess) return e;
-    e = ss::launch_stage(L, stage, h->stream);
+    e = ss::launch_stage(L, entry, h->stream);
     if (e != hipSuccess) return e;
     e = hipEventRecord(b, h->stream);
-    h->prof.pending.push_back({stage_class(stage), a, b});
+    h->prof.pending.push_back({stage_class(entry), a, b});
     return e;
 }
 
@@ -955,13 +955,21 @@ size_t table_bytes(int nGroups, int nItems, int nFrames) {
 
 // One tick: every group advances through the stage sequence (a kernel skips
 // groups in other phases); the control blocks come back to hc.
-// three line-search rounds per half-iteration: most half-iterations finish
-// within them, so a tick is usually one Gauss-Newton iteration
-const int kTick[] = {ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE, ss::ST_LS, ss::ST_LS_DECIDE,
-                     ss::ST_LS, ss::ST_LS_DECIDE,
-                     ss::ST_ASSEMBLE, ss::ST_SOLVE, ss::ST_LS, ss::ST_LS_DECIDE, ss::ST_LS, ss::ST_LS_DECIDE,
-                     ss::ST_LS, ss::ST_LS_DECIDE,
-                     ss::ST_CORR, ss::ST_CORR_FINISH, ss::ST_FINAL, ss::ST_FINAL_FINISH};
+// kLsSlots = 3 line-search rounds per half-iteration: most half-iterations
+// finish within them, so a tick is usually one Gauss-Newton iteration
+using ss::tick_entry;
+constexpr int kTick[] = {
+    tick_entry(ss::ST_ASSEMBLE),    tick_entry(ss::ST_SOLVE),
+    tick_entry(ss::ST_LS, 0),       tick_entry(ss::ST_LS_DECIDE, 0),
+    tick_entry(ss::ST_LS, 1),       tick_entry(ss::ST_LS_DECIDE, 1),
+    tick_entry(ss::ST_LS, 2),       tick_entry(ss::ST_LS_DECIDE, 2),
+    tick_entry(ss::ST_ASSEMBLE),    tick_entry(ss::ST_SOLVE),
+    tick_entry(ss::ST_LS, 0),       tick_entry(ss::ST_LS_DECIDE, 0),
+    tick_entry(ss::ST_LS, 1),       tick_entry(ss::ST_LS_DECIDE, 1),
+    tick_entry(ss::ST_LS, 2),       tick_entry(ss::ST_LS_DECIDE, 2),
+    tick_entry(ss::ST_CORR),        tick_entry(ss::ST_CORR_FINISH),
+    tick_entry(ss::ST_FINAL),       tick_entry(ss::ST_FINAL_FINISH)};
+static_assert(ss::kLsSlots == 3, "kTick holds three line-search slots per half-iteration");
 
 // Enqueues one tick on the handle's stream.
 ss_status enqueue_tick(ss_handle *h, const ss::LaunchBatch &L) {
@@ -1030,13 +1038,14 @@ ss_status trial_tick(ss_handle *h, const ss::LaunchBatch &L, double ms[ss::TK_CO
     };
     std::vector<Ev> ev;
     ss_status st = SS_OK;
-    for (int stg : kTick) {
+    for (int entry : kTick) {
+        const int stg = ss::entry_stage(entry);
         const int k = stg == ss::ST_ASSEMBLE ? ss::TK_ASSEMBLE : stg == ss::ST_LS ? ss::TK_LS
                       : stg == ss::ST_CORR   ? ss::TK_CORR
                                              : -1;
         hipEvent_t a = k >= 0 ? prof_event(h) : nullptr, b = k >= 0 ? prof_event(h) : nullptr;
         if (a && b && hipEventRecord(a, h->stream) != hipSuccess) st = fail(h, SS_ERR_DEVICE, "trial tick event");
-        if (!st && ss::launch_stage(L, stg, h->stream) != hipSuccess) st = fail(h, SS_ERR_DEVICE, "trial tick launch");
+        if (!st && ss::launch_stage(L, entry, h->stream) != hipSuccess) st = fail(h, SS_ERR_DEVICE, "trial tick launch");
         if (a && b && !st && hipEventRecord(b, h->stream) != hipSuccess) st = fail(h, SS_ERR_DEVICE, "trial tick event");
         if (a && b) ev.push_back({k, a, b});
         if (st) break;

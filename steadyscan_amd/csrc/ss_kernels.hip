@@ -1196,7 +1196,7 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
         g.lsDk[f] = -1;
     }
     if (tid == 0) {
-        ctl.roundFrames = nF;
+        ctl.slotRounds = 0;
         ctl.roundsThisLS = 0;
         ctl.K = g.K1;
         ctl.phase = PH_LS;
@@ -1207,10 +1207,21 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
 // ---------------------------------------------------------------------------
 // line search: K fused candidates p0 + r^k * delta of every active frame
 // (the tries of d_linesearch, task_linesearch.cu:35-106, in one pass), then
-// one wave per frame resolves the tries sequentially and one per group does
-// the round's bookkeeping.  Three kernels, so that no device-scope fence is
-// needed: on gfx950 each one writes back and invalidates the XCD's L2.
+// one wave per frame resolves the tries sequentially.  A tick runs kLsSlots
+// such rounds per half-iteration; after the last one, one wave per group
+// does the rounds' bookkeeping.  Separate kernels, so that no device-scope
+// fence is needed: on gfx950 each one writes back and invalidates the XCD's
+// L2.  Round `slot` of a tick's half-iteration slots runs K = ctl.K
+// candidates in slot 0 (K1 after the solve, Kn when the line search goes on
+// from the previous slots) and Kn after it; no round runs past maxRounds.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ bool ls_round_runs(const GroupDev &g, const GroupCtl &ctl, int slot) {
+    return ctl.roundsThisLS + slot < g.maxRounds;
+}
+__device__ __forceinline__ int ls_round_k(const GroupDev &g, const GroupCtl &ctl, int slot) {
+    return slot == 0 ? ctl.K : g.Kn;
+}
+
 template <typename TI, typename R, int V, int K>
 __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
     ColKnots<R> kn[K];
@@ -1283,16 +1294,16 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
 }
 
 template <typename TI, typename R, int V>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_variant<TK_LS, R, V>().waves))) void k_ls_round(Batch B) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_variant<TK_LS, R, V>().waves))) void k_ls_round(Batch B, int slot) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
         const GroupCtl &ctl = B.ctl[gi];
         if (ctl.phase != PH_LS) continue;
         const GroupDev &g = B.groups[gi];
-        if (!runs_here<TI, R>(g) || g.var[TK_LS] != V) continue;
+        if (!runs_here<TI, R>(g) || g.var[TK_LS] != V || !ls_round_runs(g, ctl, slot)) continue;
         const int c = item - g.itemStart, f = c / g.nb, b = c % g.nb;
         if (!g.lsActive[f]) continue;
-        const int K = ctl.K, k0 = g.lsNextK[f];
+        const int K = ls_round_k(g, ctl, slot), k0 = g.lsNextK[f];
         switch (K) {
         case 1: ls_column<TI, R, V, 1>(g, c, b, k0); break;
         case 2: ls_column<TI, R, V, 2>(g, c, b, k0); break;
@@ -1320,8 +1331,8 @@ __device__ __forceinline__ int wave_or(int x) {
 // Sequential resolution of the candidates of frame f (reproduces the
 // reference's try / reduce / movement-threshold order) and commit.  One
 // wave per frame.
-__device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f) {
-    const int nb = g.nb, N = g.N, nF = g.nF, K = ctl.K;
+__device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f, int K, int slot) {
+    const int nb = g.nb, N = g.N, nF = g.nF;
     const int lane = threadIdx.x;
     double se[kMaxCand], sn[kMaxCand];
 #pragma unroll
@@ -1387,6 +1398,8 @@ __device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f) {
         }
         g.lsMk[f] = mk;
         atomicAdd(&ctl.tries, tries);
+        atomicAdd(&ctl.lsFramePasses, 1);
+        atomicMax(&ctl.slotRounds, slot + 1);
         if (fin < 0) g.lsNextK[f] = k0 + K;
         else g.lsActive[f] = 0;
     }
@@ -1403,21 +1416,24 @@ __device__ void ls_decide_frame(const GroupDev &g, GroupCtl &ctl, int f) {
     }
 }
 
-__global__ __launch_bounds__(64) void k_ls_decide(Batch B) {
+__global__ __launch_bounds__(64) void k_ls_decide(Batch B, int slot) {
     for (int fi = blockIdx.x; fi < B.nFrames; fi += gridDim.x) {
         const int gi = B.frameGroup[fi];
         GroupCtl &ctl = B.ctl[gi];
         if (ctl.phase != PH_LS) continue;
         const GroupDev &g = B.groups[gi];
+        if (!ls_round_runs(g, ctl, slot)) continue;
         const int f = fi - g.frameStart;
         if (!g.lsActive[f]) continue;
-        ls_decide_frame(g, ctl, f);
+        ls_decide_frame(g, ctl, f, ls_round_k(g, ctl, slot), slot);
     }
 }
 
-// End of a line-search round of a group: the next round's frames, or the
-// end of the half-iteration (`improved`, task_linesearch.cu:117-125) and of
-// MainIteration::execute (mainiteration.cpp:103-118).
+// After a tick's last line-search slot: the rounds' bookkeeping, and either
+// the next slots' frames or the end of the half-iteration (`improved`,
+// task_linesearch.cu:117-125) and of MainIteration::execute
+// (mainiteration.cpp:103-118).  A half-iteration that ends in an earlier
+// slot leaves the later slots with no active frame.
 __global__ __launch_bounds__(64) void k_ls_finish(Batch B) {
     const int gi = blockIdx.x;
     GroupCtl &ctl = B.ctl[gi];
@@ -1436,11 +1452,10 @@ __global__ __launch_bounds__(64) void k_ls_finish(Batch B) {
     }
     if (threadIdx.x != 0) return;
     stage_time(g, ctl, FT_LINESEARCH);
-    ctl.rounds++;
-    ctl.roundsThisLS++;
-    ctl.lsFramePasses += ctl.roundFrames;
+    ctl.rounds += ctl.slotRounds;
+    ctl.roundsThisLS += ctl.slotRounds;
+    ctl.slotRounds = 0;
     if (active > 0) {
-        ctl.roundFrames = active;
         ctl.K = g.Kn;
         if (ctl.roundsThisLS >= g.maxRounds) {
             ctl.error = 4;  // SS_ERR_NUMERICAL
@@ -1859,23 +1874,23 @@ __global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
     } while (0)
 // kernels built in kVariants variants: one launch per variant some group of
 // the type and precision uses
-#define SS_LAUNCH_VAR_ONE(kernel, TK, TI, R, grid, L, s)                                          \
-    do {                                                                                          \
-        const int m_ = (L).hasVar[TK][img_type<TI>() + IMG_NTYPES * (sizeof(R) == 8 ? 1 : 0)];    \
-        if (m_ & 1) hipLaunchKernelGGL((kernel<TI, R, 0>), grid, dim3(kWG), 0, s, (L).b);         \
-        if (m_ & 2) hipLaunchKernelGGL((kernel<TI, R, 1>), grid, dim3(kWG), 0, s, (L).b);         \
-        if (m_ & 4) hipLaunchKernelGGL((kernel<TI, R, 2>), grid, dim3(kWG), 0, s, (L).b);         \
+#define SS_LAUNCH_VAR_ONE(kernel, TK, TI, R, grid, L, s, ...)                                                 \
+    do {                                                                                                      \
+        const int m_ = (L).hasVar[TK][img_type<TI>() + IMG_NTYPES * (sizeof(R) == 8 ? 1 : 0)];                \
+        if (m_ & 1) hipLaunchKernelGGL((kernel<TI, R, 0>), grid, dim3(kWG), 0, s, (L).b, ##__VA_ARGS__);      \
+        if (m_ & 2) hipLaunchKernelGGL((kernel<TI, R, 1>), grid, dim3(kWG), 0, s, (L).b, ##__VA_ARGS__);      \
+        if (m_ & 4) hipLaunchKernelGGL((kernel<TI, R, 2>), grid, dim3(kWG), 0, s, (L).b, ##__VA_ARGS__);      \
     } while (0)
-#define SS_LAUNCH_VAR(kernel, TK, grid, L, s)                       \
-    do {                                                            \
-        SS_LAUNCH_VAR_ONE(kernel, TK, float, float, grid, L, s);    \
-        SS_LAUNCH_VAR_ONE(kernel, TK, uint16_t, float, grid, L, s); \
-        SS_LAUNCH_VAR_ONE(kernel, TK, int16_t, float, grid, L, s);  \
-        SS_LAUNCH_VAR_ONE(kernel, TK, double, float, grid, L, s);   \
-        SS_LAUNCH_VAR_ONE(kernel, TK, float, double, grid, L, s);   \
-        SS_LAUNCH_VAR_ONE(kernel, TK, uint16_t, double, grid, L, s);\
-        SS_LAUNCH_VAR_ONE(kernel, TK, int16_t, double, grid, L, s); \
-        SS_LAUNCH_VAR_ONE(kernel, TK, double, double, grid, L, s);  \
+#define SS_LAUNCH_VAR(kernel, TK, grid, L, s, ...)                                   \
+    do {                                                                             \
+        SS_LAUNCH_VAR_ONE(kernel, TK, float, float, grid, L, s, ##__VA_ARGS__);      \
+        SS_LAUNCH_VAR_ONE(kernel, TK, uint16_t, float, grid, L, s, ##__VA_ARGS__);   \
+        SS_LAUNCH_VAR_ONE(kernel, TK, int16_t, float, grid, L, s, ##__VA_ARGS__);    \
+        SS_LAUNCH_VAR_ONE(kernel, TK, double, float, grid, L, s, ##__VA_ARGS__);     \
+        SS_LAUNCH_VAR_ONE(kernel, TK, float, double, grid, L, s, ##__VA_ARGS__);     \
+        SS_LAUNCH_VAR_ONE(kernel, TK, uint16_t, double, grid, L, s, ##__VA_ARGS__);  \
+        SS_LAUNCH_VAR_ONE(kernel, TK, int16_t, double, grid, L, s, ##__VA_ARGS__);   \
+        SS_LAUNCH_VAR_ONE(kernel, TK, double, double, grid, L, s, ##__VA_ARGS__);    \
     } while (0)
 #define SS_LAUNCH_TYPED(kernel, grid, L, s)                  \
     do {                                                     \
@@ -1889,15 +1904,16 @@ __global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
         SS_LAUNCH_ONE(kernel, double, double, grid, L, s);   \
     } while (0)
 
-hipError_t launch_stage(const LaunchBatch &L, int stage, hipStream_t s) {
+hipError_t launch_stage(const LaunchBatch &L, int entry, hipStream_t s) {
     const dim3 items(L.itemGrid), groups(L.b.nGroups);
-    switch (stage) {
+    const int slot = entry_slot(entry);
+    switch (entry_stage(entry)) {
     case ST_ASSEMBLE: SS_LAUNCH_VAR(k_assemble, TK_ASSEMBLE, items, L, s); break;
     case ST_SOLVE: hipLaunchKernelGGL(k_solve, groups, dim3(kSolveWG), 0, s, L.b); break;
-    case ST_LS: SS_LAUNCH_VAR(k_ls_round, TK_LS, items, L, s); break;
+    case ST_LS: SS_LAUNCH_VAR(k_ls_round, TK_LS, items, L, s, slot); break;
     case ST_LS_DECIDE:
-        hipLaunchKernelGGL(k_ls_decide, dim3(L.frameGrid), dim3(64), 0, s, L.b);
-        hipLaunchKernelGGL(k_ls_finish, groups, dim3(64), 0, s, L.b);
+        hipLaunchKernelGGL(k_ls_decide, dim3(L.frameGrid), dim3(64), 0, s, L.b, slot);
+        if (slot == kLsSlots - 1) hipLaunchKernelGGL(k_ls_finish, groups, dim3(64), 0, s, L.b);
         break;
     case ST_CORR:
         SS_LAUNCH_VAR(k_corr, TK_CORR, items, L, s);
