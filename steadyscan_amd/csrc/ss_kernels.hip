@@ -629,19 +629,34 @@ __device__ void ilu0_solve(const Eq0 *E, IluRow *F, double2 *X, int n) {
     }
 }
 
+// Knot usage (d_createParameterMask): knot k < n is used when block column
+// k or k - 1 reaches its block's minimum point count; knots from n on never
+// are.
+__device__ __forceinline__ bool knot_used(const double *__restrict__ S, const int *__restrict__ minPts, int k,
+                                          int n, int nb) {
+    if (k < 0 || k >= n) return false;
+    const bool pc = S[(size_t)k * CS_N + CS_CNT] >= (double)minPts[k % nb];
+    const bool pp = k > 0 && S[(size_t)(k - 1) * CS_N + CS_CNT] >= (double)minPts[(k - 1) % nb];
+    return pc || pp;
+}
+
 // Level-0 equations of knots tid, tid + kSolveWG, ... (d_fillDiagonals /
-// d_fillJTr entries, lambda extension).  The pointers do not alias, so the
-// next knot's loads may be issued before this knot's equation is stored.
-__device__ __forceinline__ void level0_equations(const double *__restrict__ S, const uint8_t *__restrict__ used,
-                                                 Eq0 *__restrict__ E, int n, double lambda, int fracUniform,
-                                                 int tid) {
+// d_fillJTr entries, lambda extension), and the usage flags of the same
+// knots into `used`.  Each knot takes its own and its neighbours' usage from
+// the column counts, so that no pass over the flags has to come first.  The
+// pointers do not alias, so the next knot's loads may be issued before this
+// knot's equation is stored.  E may point to LDS.
+__device__ __forceinline__ void level0_equations(const double *__restrict__ S, const int *__restrict__ minPts,
+                                                 uint8_t *__restrict__ used, Eq0 *__restrict__ E, int n, int nb,
+                                                 double lambda, int fracUniform, int tid) {
 #pragma unroll 4
     for (int k = tid; k < n; k += kSolveWG) {
         Eq0 e;
         const double *s = S + (size_t)k * CS_N;
-        const bool u = used[k];
-        const bool up = k > 0 && used[k - 1];
-        const bool un = (k + 1 < n) && used[k + 1];
+        const bool u = knot_used(S, minPts, k, n, nb);
+        const bool up = k > 0 && knot_used(S, minPts, k - 1, n, nb);
+        const bool un = (k + 1 < n) && knot_used(S, minPts, k + 1, n, nb);
+        used[k] = u ? 1 : 0;
         if (!u) {
             e = Eq0{1, 0, 1, 0, 0, 0, 0, 0};
         } else {
@@ -937,6 +952,9 @@ __device__ void exact_solve_t(const Eq0 *__restrict__ E, InteriorRow *__restrict
 
 // knots per partition that stay in registers
 constexpr int kSolveRegRows = 8;
+// knots whose level-0 equations the solve keeps in LDS (112 KiB; with the
+// kernel's other arrays within the 160 KiB of a workgroup)
+constexpr int kSolveLdsRows = 1792;
 
 // Inlined into the solve kernel: as a called function it would save and
 // restore the callee-saved registers it uses (over 200 per call).
@@ -1006,33 +1024,13 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
         }
         g.errval[shift * nF + f] = s / c;
     }
-    // 2. knot usage (d_createParameterMask); the last knot is never used
-    for (int k0 = tid * kBatch; k0 < N; k0 += kSolveWG * kBatch) {
-        double cnt[kBatch + 1];
-        int mp[kBatch + 1];
-#pragma unroll
-        for (int u = 0; u <= kBatch; u++) {  // knots k0-1 .. k0+kBatch-1
-            const int k = min(max(k0 + u - 1, 0), n - 1);
-            cnt[u] = S[(size_t)k * CS_N + CS_CNT];
-            mp[u] = minPts[k % nb];
-        }
-        uint8_t uk[kBatch];
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-            const int k = k0 + u;
-            const bool pc = k < n && cnt[u + 1] >= (double)mp[u + 1];
-            const bool pp = k < n && k > 0 && cnt[u] >= (double)mp[u];
-            uk[u] = (pc || pp) ? 1 : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < kBatch; u++)
-            if (k0 + u < N) used[k0 + u] = uk[u];
-    }
-    __threadfence_block();
-    __syncthreads();
-    // 3. level-0 equations (d_fillDiagonals / d_fillJTr entries, lambda extension)
-    Eq0 *E = reinterpret_cast<Eq0 *>(g.ws);
-    level0_equations(S, used, E, n, g.lambda, g.fracUniform, tid);
+    // 2. knot usage (d_createParameterMask) and level-0 equations
+    //    (d_fillDiagonals / d_fillJTr entries, lambda extension); the knots
+    //    from n on are never used.  The equations stay in LDS when they fit.
+    __shared__ Eq0 sE[kSolveLdsRows];
+    Eq0 *E = n <= kSolveLdsRows ? sE : reinterpret_cast<Eq0 *>(g.ws);
+    level0_equations(S, minPts, used, E, n, nb, g.lambda, g.fracUniform, tid);
+    for (int k = n + tid; k < N; k += kSolveWG) used[k] = 0;
     __threadfence_block();
     __syncthreads();
     InteriorRow *IR = reinterpret_cast<InteriorRow *>(g.ws + (size_t)n * 8);
