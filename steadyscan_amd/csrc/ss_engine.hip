@@ -141,6 +141,7 @@ struct ss_handle {
     bool useGraph = true;
     hipGraphExec_t tickExec = nullptr;
     ss::LaunchBatch tickFor{};
+    bool tickWithFinal = false;
     hipEvent_t tickDone[2] = {nullptr, nullptr};  // control blocks of a tick read back
     // autotuning: -1 tunes each new group shape, 0..kVariants-1 forces that variant
     int tuneMode = 0;  // the default variants; -1 tunes each new group shape
@@ -824,9 +825,11 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     return SS_OK;
 }
 
-// Copies a finished group's outputs to the caller's buffers.
+// Copies a finished group's outputs to the caller's buffers.  finalInAssembly:
+// the group's final pass ran in an assembly launch (a tick without final
+// stages).
 ss_status finish_group(ss_handle *h, const Plan &P, const ss::GroupDev &g, const ss::GroupCtl &c,
-                       double initTime) {
+                       double initTime, bool finalInAssembly) {
     const ss_group_in &in = *P.in;
     ss_group_out &o = *P.out;
     const int nb = in.n_blocks, nF = in.n_frames, fw = nb * nF;
@@ -872,7 +875,7 @@ ss_status finish_group(ss_handle *h, const Plan &P, const ss::GroupDev &g, const
         h->prof.bytes[SS_K_ASSEMBLE] += fb * nF * c.halfIterations;
         h->prof.bytes[SS_K_LS_ROUND] += fb * c.lsFramePasses;
         h->prof.bytes[SS_K_CORR] += fb * nF * c.iteration;
-        h->prof.bytes[SS_K_FINAL] += fb * nF;
+        h->prof.bytes[finalInAssembly ? SS_K_ASSEMBLE : SS_K_FINAL] += fb * nF;
     }
     return SS_OK;
 }
@@ -956,7 +959,11 @@ size_t table_bytes(int nGroups, int nItems, int nFrames) {
 // One tick: every group advances through the stage sequence (a kernel skips
 // groups in other phases); the control blocks come back to hc.
 // kLsSlots = 3 line-search rounds per half-iteration: most half-iterations
-// finish within them, so a tick is usually one Gauss-Newton iteration
+// finish within them, so a tick is usually one Gauss-Newton iteration.
+// kTick ends a group that stops in it with the final stages; kTickNoFinal
+// leaves it in the final phase, and the assembly and solve launches of the
+// next tick run its final pass and end it (run_batch keeps that tick in
+// flight, so the groups end no later).
 using ss::tick_entry;
 constexpr int kTick[] = {
     tick_entry(ss::ST_ASSEMBLE),    tick_entry(ss::ST_SOLVE),
@@ -970,23 +977,26 @@ constexpr int kTick[] = {
     tick_entry(ss::ST_CORR),        tick_entry(ss::ST_CORR_FINISH),
     tick_entry(ss::ST_FINAL),       tick_entry(ss::ST_FINAL_FINISH)};
 static_assert(ss::kLsSlots == 3, "kTick holds three line-search slots per half-iteration");
+constexpr int kTickNoFinalLen = sizeof(kTick) / sizeof(kTick[0]) - 2;
+static_assert(ss::entry_stage(kTick[kTickNoFinalLen]) == ss::ST_FINAL, "the final stages end kTick");
 
-// Enqueues one tick on the handle's stream.
-ss_status enqueue_tick(ss_handle *h, const ss::LaunchBatch &L) {
-    const auto &tick = kTick;
+// Enqueues one tick on the handle's stream: kTick, or without its final
+// stages.
+ss_status enqueue_tick(ss_handle *h, const ss::LaunchBatch &L, bool withFinal) {
+    const int len = withFinal ? (int)(sizeof(kTick) / sizeof(kTick[0])) : kTickNoFinalLen;
     if (h->prof.on || !h->useGraph) {  // per-stage events need plain launches
-        for (int stg : tick) HIPCHK(run_stage(h, L, stg));
+        for (int i = 0; i < len; i++) HIPCHK(run_stage(h, L, kTick[i]));
     } else {
         // the tick holds kernel launches only (no memset nodes)
-        if (!h->tickExec || std::memcmp(&h->tickFor, &L, sizeof(L)) != 0) {
+        if (!h->tickExec || h->tickWithFinal != withFinal || std::memcmp(&h->tickFor, &L, sizeof(L)) != 0) {
             if (h->tickExec) {
                 HIPCHK(hipGraphExecDestroy(h->tickExec));
                 h->tickExec = nullptr;
             }
             hipGraph_t graph = nullptr;
             HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            for (int stg : tick) {
-                const hipError_t e = ss::launch_stage(L, stg, h->stream);
+            for (int i = 0; i < len; i++) {
+                const hipError_t e = ss::launch_stage(L, kTick[i], h->stream);
                 if (e != hipSuccess) {
                     (void)hipStreamEndCapture(h->stream, &graph);
                     if (graph) (void)hipGraphDestroy(graph);
@@ -1001,15 +1011,16 @@ ss_status enqueue_tick(ss_handle *h, const ss::LaunchBatch &L) {
                 return fail(h, SS_ERR_DEVICE, std::string("tick instantiate: ") + hipGetErrorString(e));
             }
             h->tickFor = L;
+            h->tickWithFinal = withFinal;
         }
         HIPCHK(hipGraphLaunch(h->tickExec, h->stream));
     }
     return SS_OK;
 }
 
-// One tick; the control blocks come back to hc.
+// One tick with the final stages; the control blocks come back to hc.
 ss_status run_tick(ss_handle *h, const ss::LaunchBatch &L, ss::GroupCtl *hc) {
-    ss_status st = enqueue_tick(h, L);
+    ss_status st = enqueue_tick(h, L, true);
     if (st) return st;
     HIPCHK(hipMemcpyAsync(hc, L.b.ctl, L.b.nGroups * sizeof(ss::GroupCtl), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -1185,7 +1196,9 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
     // One tick stays in flight: tick t is enqueued with the read-back of its
     // control blocks before the host waits for tick t-1's, so the device
     // does not idle while the host reads them.  Kernels skip finished
-    // groups, so the tick after the last one changes nothing.
+    // groups.  The ticks have no final stages: a group that stops in tick
+    // t-1 ends in the assembly and solve launches of tick t, so the loop
+    // stops once every group has finished or stopped by tick t-1.
     for (int i = 0; i < 2; i++)
         if (!h->tickDone[i]) HIPCHK(hipEventCreateWithFlags(&h->tickDone[i], hipEventDisableTiming));
     const size_t ctlBytes = n * sizeof(ss::GroupCtl);
@@ -1194,35 +1207,35 @@ ss_status run_batch(ss_handle *h, int n, const ss_group_in *ins, const ss_settin
             if (!finished(c[gi])) return false;
         return true;
     };
+    auto all_stopped = [&](const ss::GroupCtl *c) {
+        for (int gi = 0; gi < n; gi++)
+            if (!finished(c[gi]) && c[gi].phase != ss::PH_FINAL) return false;
+        return true;
+    };
     // a failure leaves no read-back in flight into the staging, which the
     // next call writes from the host
     auto stop = [&](ss_status e) {
         (void)hipStreamSynchronize(h->stream);
         return e;
     };
-    bool done = false;
     int t = 0;
     for (; t < kTickCap; t++) {
-        st = enqueue_tick(h, L);
+        st = enqueue_tick(h, L, false);
         if (st) return stop(st);
         hipError_t e = hipMemcpyAsync(hcb[t & 1], L.b.ctl, ctlBytes, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipEventRecord(h->tickDone[t & 1], h->stream);
         if (e == hipSuccess && t > 0) e = hipEventSynchronize(h->tickDone[(t - 1) & 1]);
         if (e != hipSuccess) return stop(fail(h, SS_ERR_DEVICE, std::string("tick read-back: ") + hipGetErrorString(e)));
-        if (t > 0 && all_finished(hcb[(t - 1) & 1])) {
-            done = true;
-            break;
-        }
+        if (t > 0 && all_stopped(hcb[(t - 1) & 1])) break;
     }
     HIPCHK(hipStreamSynchronize(h->stream));
     if (t == kTickCap) t--;
     hc = hcb[t & 1];  // the last tick's control blocks
-    if (!done) done = all_finished(hc);
-    if (!done) return fail(h, SS_ERR_NUMERICAL, "alignment did not terminate");
+    if (!all_finished(hc)) return fail(h, SS_ERR_NUMERICAL, "alignment did not terminate");
     ss_status result = SS_OK;
     std::string firstErr;
     for (int gi = 0; gi < n; gi++) {
-        st = finish_group(h, plans[gi], gd[gi], hc[gi], initTime);
+        st = finish_group(h, plans[gi], gd[gi], hc[gi], initTime, true);
         if (st && result == SS_OK) {
             result = st;
             firstErr = h->err + " (group " + std::to_string(gi) + ")";
@@ -1544,7 +1557,7 @@ ss_status ss_server_step(ss_server *S, int32_t *n_resident, int32_t *n_queued) t
             if (finished(slot.c)) {
                 slot.P.in = &slot.req.in;
                 slot.P.s = &slot.req.s;
-                const ss_status fs = finish_group(h, slot.P, slot.g, slot.c, 0.0);
+                const ss_status fs = finish_group(h, slot.P, slot.g, slot.c, 0.0, false);
                 S->pool.release(slot.mem, slot.P.dBytes);
                 S->completed.push_back({slot.req.ticket, fs, fs ? h->err : std::string()});
                 S->dirty = true;

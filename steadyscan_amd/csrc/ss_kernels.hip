@@ -447,13 +447,23 @@ __device__ void assemble_column(const GroupDev &g, int c) {
     if (threadIdx.x < CS_N) g.colsum[(size_t)c * CS_N + threadIdx.x] = out[threadIdx.x];
 }
 
+template <typename TI, typename R>
+__device__ __forceinline__ void final_column(const GroupDev &g, int c);
+
+// The assembly launch also runs the final pass of the groups that stopped
+// in the previous tick (phase PH_FINAL), for a tick without final stages.
 template <typename TI, typename R, int V>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_variant<TK_ASSEMBLE, R, V>().waves))) void k_assemble(Batch B) {
     for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
         const int gi = B.itemGroup[item];
-        if (B.ctl[gi].phase != PH_ASSEMBLE) continue;
+        const int phase = B.ctl[gi].phase;
+        if (phase != PH_ASSEMBLE && phase != PH_FINAL) continue;
         const GroupDev &g = B.groups[gi];
         if (!runs_here<TI, R>(g) || g.var[TK_ASSEMBLE] != V) continue;
+        if (phase == PH_FINAL) {
+            final_column<TI, R>(g, item - g.itemStart);
+            continue;
+        }
         if ((g.iterTimes || g.funcTimes) && item == g.itemStart && threadIdx.x == 0) {
             GroupCtl &c = B.ctl[gi];  // the first iteration starts (outputs 9 and 10)
             if (c.tIter == 0 || c.tStage == 0) {
@@ -974,9 +984,17 @@ __device__ __forceinline__ void stage_time(const GroupDev &g, GroupCtl &ctl, int
     ctl.tStage = now;
 }
 
+__device__ void final_finish(const GroupDev &g, GroupCtl &ctl);
+
+// The solve launch also ends the groups whose final pass ran in the
+// assembly launch before it (phase PH_FINAL).
 __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
     const int gi = blockIdx.x;
     GroupCtl &ctl = B.ctl[gi];
+    if (ctl.phase == PH_FINAL) {
+        final_finish(B.groups[gi], ctl);
+        return;
+    }
     if (ctl.phase != PH_ASSEMBLE) return;
     const GroupDev &g = B.groups[gi];
     const int tid = threadIdx.x;
@@ -1777,7 +1795,7 @@ __global__ __launch_bounds__(kSolveWG) void k_corr_finish(Batch B) {
 // (hd_calculateGroupErrvals(0) + hd_frameCorrelation, task.cpp:389-405)
 // ---------------------------------------------------------------------------
 template <typename TI, typename R>
-__device__ void final_column(const GroupDev &g, int c) {
+__device__ __forceinline__ void final_column(const GroupDev &g, int c) {
     const int f = c / g.nb, b = c % g.nb;
     const ColKnots<R> kn = state_knots<R>(g, c, g.lsMk[f]);
     const TI *img = column_ptr<TI>(g, c);
@@ -1835,11 +1853,9 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     }
 }
 
-__global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
-    const int gi = blockIdx.x;
-    GroupCtl &ctl = B.ctl[gi];
-    if (ctl.phase != PH_FINAL) return;
-    const GroupDev &g = B.groups[gi];
+// Residual RMS and frame correlations of the final state, and the end of
+// the group.  Called by every thread of a kSolveWG workgroup.
+__device__ void final_finish(const GroupDev &g, GroupCtl &ctl) {
     for (int f = threadIdx.x; f < g.nF; f += kSolveWG) {
         double s[FS_N];
         for (int i = 0; i < FS_N; i++) s[i] = 0.0;
@@ -1859,6 +1875,13 @@ __global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
         stage_time(g, ctl, FT_CORRELATION);
         ctl.phase = PH_DONE;
     }
+}
+
+__global__ __launch_bounds__(kSolveWG) void k_final_finish(Batch B) {
+    const int gi = blockIdx.x;
+    GroupCtl &ctl = B.ctl[gi];
+    if (ctl.phase != PH_FINAL) return;
+    final_finish(B.groups[gi], ctl);
 }
 
 // ---------------------------------------------------------------------------
