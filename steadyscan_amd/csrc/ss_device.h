@@ -149,6 +149,9 @@ struct LaunchBatch {
     int hasVar[TK_COUNT][2 * IMG_NTYPES];  // bit v: some group of the type uses variant v
     int hasFrameCorr;     // some group saves per-iteration frame correlations (flag 'c')
     int itemGrid, frameGrid;
+    // line-search rounds after a tick's first slot: workgroups per column
+    // (2: each takes two of the four candidates), and their grid
+    int lsSplit, lsSplitGrid;
 };
 
 // Stages of one host submission, in order; see launch_stage.

@@ -944,6 +944,12 @@ ss_status upload_table(ss_handle *h, Table &T, std::vector<ss::GroupDev> &gd, co
     L.b.frameGroup = (int *)(D + oF);
     L.itemGrid = std::max(1, std::min(nItems, 8 * h->numCU));
     L.frameGrid = std::max(1, std::min(nFrames, 16 * h->numCU));
+    // A batch of at most 8 columns per CU leaves most of the chip idle in
+    // the rounds after a half-iteration's first (few frames remain): there a
+    // column's two halves of the candidates go to two workgroups, which
+    // halves the chain of gathers a workgroup waits on.
+    L.lsSplit = nItems <= 8 * h->numCU ? 2 : 1;
+    L.lsSplitGrid = L.lsSplit == 2 ? 2 * nItems : L.itemGrid;
     return SS_OK;
 }
 

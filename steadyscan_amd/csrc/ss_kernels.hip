@@ -1238,8 +1238,10 @@ __device__ __forceinline__ int ls_round_k(const GroupDev &g, const GroupCtl &ctl
     return slot == 0 ? ctl.K : g.Kn;
 }
 
+// Candidates k0 .. k0 + K - 1 of column c; their sums go to the column's
+// candidate slots slot0 .. slot0 + K - 1.
 template <typename TI, typename R, int V, int K>
-__device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
+__device__ void ls_column(const GroupDev &g, int c, int b, int k0, int slot0) {
     ColKnots<R> kn[K];
     {
         const int N = g.N;
@@ -1306,12 +1308,17 @@ __device__ void ls_column(const GroupDev &g, int c, int b, int k0) {
     }
     __shared__ double out[2 * kMaxCand];
     block_sum<2 * K>(v, out);
-    if (threadIdx.x < 2 * K) g.lssum[(size_t)c * 2 * kMaxCand + threadIdx.x] = out[threadIdx.x];
+    if (threadIdx.x < 2 * K) g.lssum[(size_t)c * 2 * kMaxCand + 2 * slot0 + threadIdx.x] = out[threadIdx.x];
 }
 
+// split = 2: work item w covers column w % nItems and, of a round of four
+// candidates, the two of half w / nItems (each candidate's sums are the
+// same whichever workgroup computes them); other rounds run in half 0.
 template <typename TI, typename R, int V>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_variant<TK_LS, R, V>().waves))) void k_ls_round(Batch B, int slot) {
-    for (int item = blockIdx.x; item < B.nItems; item += gridDim.x) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_variant<TK_LS, R, V>().waves))) void k_ls_round(Batch B, int slot, int split) {
+    const int nWork = B.nItems * split;
+    for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
+        const int half = w >= B.nItems ? 1 : 0, item = w - half * B.nItems;
         const int gi = B.itemGroup[item];
         const GroupCtl &ctl = B.ctl[gi];
         if (ctl.phase != PH_LS) continue;
@@ -1319,11 +1326,18 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(tune_varian
         if (!runs_here<TI, R>(g) || g.var[TK_LS] != V || !ls_round_runs(g, ctl, slot)) continue;
         const int c = item - g.itemStart, f = c / g.nb, b = c % g.nb;
         if (!g.lsActive[f]) continue;
-        const int K = ls_round_k(g, ctl, slot), k0 = g.lsNextK[f];
+        int K = ls_round_k(g, ctl, slot), k0 = g.lsNextK[f], slot0 = 0;
+        if (split == 2 && K == 4) {
+            K = 2;
+            k0 += 2 * half;
+            slot0 = 2 * half;
+        } else if (half) {
+            continue;
+        }
         switch (K) {
-        case 1: ls_column<TI, R, V, 1>(g, c, b, k0); break;
-        case 2: ls_column<TI, R, V, 2>(g, c, b, k0); break;
-        default: ls_column<TI, R, V, 4>(g, c, b, k0); break;
+        case 1: ls_column<TI, R, V, 1>(g, c, b, k0, slot0); break;
+        case 2: ls_column<TI, R, V, 2>(g, c, b, k0, slot0); break;
+        default: ls_column<TI, R, V, 4>(g, c, b, k0, slot0); break;
         }
     }
 }
@@ -1931,7 +1945,10 @@ hipError_t launch_stage(const LaunchBatch &L, int entry, hipStream_t s) {
     switch (entry_stage(entry)) {
     case ST_ASSEMBLE: SS_LAUNCH_VAR(k_assemble, TK_ASSEMBLE, items, L, s); break;
     case ST_SOLVE: hipLaunchKernelGGL(k_solve, groups, dim3(kSolveWG), 0, s, L.b); break;
-    case ST_LS: SS_LAUNCH_VAR(k_ls_round, TK_LS, items, L, s, slot); break;
+    case ST_LS:
+        if (slot == 0) SS_LAUNCH_VAR(k_ls_round, TK_LS, items, L, s, slot, 1);
+        else SS_LAUNCH_VAR(k_ls_round, TK_LS, dim3(L.lsSplitGrid), L, s, slot, L.lsSplit);
+        break;
     case ST_LS_DECIDE:
         hipLaunchKernelGGL(k_ls_decide, dim3(L.frameGrid), dim3(64), 0, s, L.b, slot);
         if (slot == kLsSlots - 1) hipLaunchKernelGGL(k_ls_finish, groups, dim3(64), 0, s, L.b);
