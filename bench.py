@@ -270,7 +270,7 @@ def uneven_leg(args, torch, ssa, W):
     ssa.align_groups(calls, handle=h, linesearch_candidates=args.candidates, with_mask=False)
     out["frames_per_s_tuned_variants"] = nfr / (time.perf_counter() - t)
     out["tuned"] = tuning_summary(h.tuning())
-    h.tuning_mode(0)
+    h.tuning_mode(-2)  # the default: variants by batch size
     if not args.no_cpu:
         from oracle import oracle as O
         gr = groups[0]

@@ -218,8 +218,11 @@ ss_status ss_profile_read(ss_handle *h, int64_t *launches, double *ms, double *f
  * No reference counterpart (the reference fixes THREADS_PER_BLOCK 256,
  * cudaincludes.h:36).  The assembly, line-search and correlation kernels are
  * built in three launch variants each: (template samples a thread keeps in
- * flight, minimum waves per SIMD); variant 0, the default, was measured
- * fastest on batches of 512x512 groups.  With tuning on, the first time a
+ * flight, minimum waves per SIMD).  By default each batch picks them by its
+ * size: variant 0, measured fastest on batches of 512x512 groups, for a
+ * batch of more than 8 block columns per CU; for a smaller one (a single
+ * group, say) the variants that allow more waves per SIMD, measured fastest
+ * there.  With tuning on, the first time a
  * handle sees a group shape (precision, mixed precision, frame element type,
  * frame_h, n_blocks, lines per run, geometry form, gradient form, frame-count
  * class floor(log2(n_frames))), it runs that group's first tick twice per
@@ -240,8 +243,9 @@ typedef struct {
     int32_t samples[3][3];       /* [kernel][variant]: the variants' parameters      */
     int32_t waves[3][3];
 } ss_tuning_entry;
-/* 0 (default), 1 or 2: use that variant for every kernel, no tuning;
- * -1: tune each new group shape. */
+/* -2 (default): variants by the size of each batch; 0, 1 or 2: use that
+ * variant for every kernel, no tuning; -1: tune each new group shape.  A
+ * group keeps the mode in force when it was submitted. */
 ss_status ss_tuning_mode(ss_handle *h, int32_t mode);
 /* Copies up to cap of the handle's tuned shapes to out; returns how many
  * there are. */

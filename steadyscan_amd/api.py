@@ -89,8 +89,9 @@ class Handle:
         _check(N.lib().ss_release(self.ptr, int(rid)), self)
 
     def tuning_mode(self, mode):
-        """0 (default), 1, 2: use that launch variant for every pixel
-        kernel; -1: tune each new group shape on its first tick."""
+        """-2 (default): launch variants by the size of each batch; 0, 1,
+        2: use that launch variant for every pixel kernel; -1: tune each new
+        group shape on its first tick."""
         _check(N.lib().ss_tuning_mode(self.ptr, int(mode)), self)
 
     def tuning(self):

@@ -72,6 +72,7 @@ struct GroupDev {
     int frameH, nb, nF, fframeW, N, th, tw, groupSize;
     int itemStart;        // first work item (= column) of this group in the batch
     int var[TK_COUNT];    // launch variant of the assembly, line-search and correlation kernels
+    int varBySize;        // var follows the size of the batch the group runs in
     int frameStart;       // first frame item of this group in the batch
     // settings
     double xmax, ymax;    // template_w - 1 - eps, template_h - 1 - eps

@@ -105,6 +105,8 @@ struct Profile {
 };
 
 constexpr int kTickCap = 200000;
+// ss_tuning_mode: the default, launch variants by the size of each batch
+constexpr int kTuneBySize = -2;
 
 // The group shape the autotuner keys its choices on: what sets the cost of a
 // column (precision, frame element type, mixed precision, gradients read or
@@ -144,7 +146,7 @@ struct ss_handle {
     bool tickWithFinal = false;
     hipEvent_t tickDone[2] = {nullptr, nullptr};  // control blocks of a tick read back
     // autotuning: -1 tunes each new group shape, 0..kVariants-1 forces that variant
-    int tuneMode = 0;  // the default variants; -1 tunes each new group shape
+    int tuneMode = kTuneBySize;  // variants by batch size; -1 tunes each new group shape
     std::map<TuneKey, ss_tuning_entry> tuned;
     void *trialTable = nullptr;  // group table of the trial ticks
     size_t trialTableBytes = 0;
@@ -497,6 +499,7 @@ struct Plan {
     size_t imgBytes = 0;
     bool staged = false;  // host frames already in the staging (stage_host_frames)
     int var[ss::TK_COUNT] = {0, 0, 0};  // launch variants of the pixel kernels
+    bool varBySize = false;             // chosen per batch by its size instead
 };
 
 // Validates one call and lays out its device block and host staging.
@@ -656,6 +659,7 @@ ss_status setup_group(ss_handle *h, Plan &P, char *D, char *HS, ss::GroupDev &g,
     g.tw = P.tmpl.tw;
     g.groupSize = in.group_size;
     for (int k = 0; k < ss::TK_COUNT; k++) g.var[k] = P.var[k];
+    g.varBySize = P.varBySize ? 1 : 0;
     const double eps = s.precision == 'S' ? (double)FLT_EPSILON : DBL_EPSILON;
     g.fp64 = P.fp64;
     g.xmax = (double)(g.tw - 1) - eps;
@@ -919,8 +923,11 @@ ss_status upload_table(ss_handle *h, Table &T, std::vector<ss::GroupDev> &gd, co
         for (int k = 0; k < ss::TK_COUNT; k++) L.hasVar[k][i] = 0;
     }
     L.hasFrameCorr = 0;
+    const bool small = nItems <= 8 * h->numCU;  // see ss_tuning.h
     for (int gi = 0; gi < n; gi++) {
         ss::GroupDev &g = gd[gi];
+        if (g.varBySize)
+            for (int k = 0; k < ss::TK_COUNT; k++) g.var[k] = small ? ss::kSmallBatchVariant[k][g.fp64 ? 1 : 0] : 0;
         g.itemStart = item;
         g.frameStart = frame;
         hg[gi] = g;
@@ -948,7 +955,7 @@ ss_status upload_table(ss_handle *h, Table &T, std::vector<ss::GroupDev> &gd, co
     // the rounds after a half-iteration's first (few frames remain): there a
     // column's two halves of the candidates go to two workgroups, which
     // halves the chain of gathers a workgroup waits on.
-    L.lsSplit = nItems <= 8 * h->numCU ? 2 : 1;
+    L.lsSplit = small ? 2 : 1;
     L.lsSplitGrid = L.lsSplit == 2 ? 2 * nItems : L.itemGrid;
     return SS_OK;
 }
@@ -1083,8 +1090,9 @@ ss_status trial_tick(ss_handle *h, const ss::LaunchBatch &L, double ms[ss::TK_CO
 // on the group's own first tick, run twice per variant from the initial
 // state (the real set-up follows and starts again from it).
 ss_status tune_group(ss_handle *h, Plan &P, char *D, char *HS) {
-    if (h->tuneMode >= 0) {
-        for (int k = 0; k < ss::TK_COUNT; k++) P.var[k] = h->tuneMode;
+    P.varBySize = h->tuneMode == kTuneBySize;
+    if (h->tuneMode >= 0 || P.varBySize) {
+        for (int k = 0; k < ss::TK_COUNT; k++) P.var[k] = P.varBySize ? 0 : h->tuneMode;
         return SS_OK;
     }
     const TuneKey key = tune_key(P);
@@ -1722,7 +1730,8 @@ const char *ss_last_error(const ss_handle *h) { return h ? h->err.c_str() : "no 
 
 ss_status ss_tuning_mode(ss_handle *h, int32_t mode) {
     if (!h) return SS_ERR_INVALID_ARGUMENT;
-    if (mode < -1 || mode >= ss::kVariants) return fail(h, SS_ERR_INVALID_ARGUMENT, "tuning mode must be -1 .. 2");
+    if (mode < kTuneBySize || mode >= ss::kVariants)
+        return fail(h, SS_ERR_INVALID_ARGUMENT, "tuning mode must be -2 .. 2");
     h->tuneMode = mode;
     return SS_OK;
 }

@@ -7,11 +7,16 @@
 //   waves:   minimum waves per SIMD that the kernel's register allocation
 //            must allow (more waves hide more gather latency; fewer
 //            registers limit how many gathers a wave keeps in flight).
-// Variant 0 is the default: the values measured fastest on MI355X for
-// 512x512 frames (the headline workload); in FP32 the assembly and line
-// search take 4 samples, which measure as fast as 8 without spilling
-// registers.  The engine's autotuner (off unless asked for) times all
-// variants on a group shape's first tick and keeps the fastest per kernel.
+// Variant 0 is the default for batches that fill the chip: the values
+// measured fastest on MI355X for batches of 512x512 groups (the headline
+// workload); in FP32 the assembly and line search take 4 samples, which
+// measure as fast as 8 without spilling registers.  A batch of at most 8
+// block columns per CU (one 512x512 group of 50 frames has 1600) runs too
+// few workgroups to fill the chip, and there kSmallBatchVariant picks the
+// variants that allow more waves per SIMD (FP64 assembly and correlation at
+// 4 waves: one 512x512 group's first tick 142 and 72 us against 154 and 75).
+// The engine's autotuner (off unless asked for) times all variants on a
+// group shape's first tick and keeps the fastest per kernel.
 // A variant only changes how many rows a thread unrolls and how registers
 // are allocated: every thread still accumulates its pixels in the same
 // order, so all variants give the same results bit for bit.
@@ -29,12 +34,15 @@ struct TuneVariant {
 // [kernel][fp64][variant]
 constexpr TuneVariant kTuneTable[TK_COUNT][2][kVariants] = {
     // assembly: FP32, FP64
-    {{{4, 4}, {8, 4}, {8, 3}}, {{4, 2}, {2, 3}, {8, 2}}},
+    {{{4, 4}, {8, 4}, {8, 3}}, {{4, 2}, {2, 3}, {2, 4}}},
     // line search
     {{{4, 4}, {8, 4}, {8, 3}}, {{4, 4}, {2, 4}, {8, 3}}},
     // correlation (one sample per row; a chunk holds at most 8 rows)
     {{{8, 4}, {4, 4}, {2, 4}}, {{8, 2}, {4, 3}, {2, 3}}},
 };
+
+// [kernel][fp64]: the variant of batches of at most 8 block columns per CU
+constexpr int kSmallBatchVariant[TK_COUNT][2] = {{0, 2}, {0, 0}, {0, 2}};
 
 template <int KERNEL, typename R, int V>
 __host__ __device__ constexpr TuneVariant tune_variant() {

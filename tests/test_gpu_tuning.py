@@ -1,7 +1,8 @@
 """The autotuner of the pixel kernels' launch variants: every variant gives
 the same results bit for bit, and with tuning on a new group shape is tuned
 once per handle on its first tick (the choice is the fastest variant per
-kernel of the trial).  Tuning is off by default."""
+kernel of the trial).  Tuning is off by default: the default picks the
+variants by the size of each batch, with the same results."""
 import numpy as np
 import pytest
 
@@ -40,6 +41,12 @@ def test_every_variant_gives_the_same_results(calls):
     for v in (1, 2):
         for r, ref in zip(results[v], results[0]):
             same(r, ref)
+    # the default: a small batch (the small-batch variants and two
+    # workgroups per column in the four-candidate line-search rounds)
+    with ssa.Handle(0) as h:
+        for r, ref in zip(ssa.align_groups(calls, handle=h), results[0]):
+            same(r, ref)
+        assert h.tuning() == []
 
 
 def test_new_shapes_are_tuned_once(calls):
