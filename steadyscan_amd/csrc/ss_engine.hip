@@ -973,10 +973,10 @@ size_t table_bytes(int nGroups, int nItems, int nFrames) {
 // groups in other phases); the control blocks come back to hc.
 // kLsSlots = 3 line-search rounds per half-iteration: most half-iterations
 // finish within them, so a tick is usually one Gauss-Newton iteration.
-// kTick ends a group that stops in it with the final stages; kTickNoFinal
-// leaves it in the final phase, and the assembly and solve launches of the
-// next tick run its final pass and end it (run_batch keeps that tick in
-// flight, so the groups end no later).
+// kTick ends a group that stops in it with the final stages; its first
+// kTickNoFinalLen entries leave the group in the final phase, and the
+// assembly and solve launches of the next tick run its final pass and end
+// it (run_batch keeps that tick in flight, so the groups end no later).
 using ss::tick_entry;
 constexpr int kTick[] = {
     tick_entry(ss::ST_ASSEMBLE),    tick_entry(ss::ST_SOLVE),

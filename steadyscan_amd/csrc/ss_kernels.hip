@@ -5,8 +5,12 @@
 // MainIteration<T>::execute) as a small state machine in device memory
 // (GroupCtl), so one host submission ("tick") advances every group by one
 // stage without host round trips:
-//   k_assemble -> k_solve -> k_ls_round ... -> k_corr -> k_corr_finish -> k_final ...
-// Each kernel skips the groups that are not in its phase.
+//   k_assemble -> k_solve -> (k_ls_round -> k_ls_decide) x3 -> k_ls_finish
+//   -> (the same for the second half-iteration) -> k_corr -> k_corr_finish
+//   [-> k_final -> k_final_finish]
+// Each kernel skips the groups that are not in its phase.  Without the
+// final stages, the next tick's k_assemble and k_solve run the final pass
+// and end the groups that stopped.
 //
 // Pixel work: one workgroup (4 wave64) per line-block column (frame f,
 // block b = column c = f * nb + b of the block matrix).  The column is cut
