@@ -221,8 +221,9 @@ ss_status ss_profile_read(ss_handle *h, int64_t *launches, double *ms, double *f
  * flight, minimum waves per SIMD).  By default each batch picks them by its
  * size: variant 0, measured fastest on batches of 512x512 groups, for a
  * batch of more than 8 block columns per CU; for a smaller one (a single
- * group, say) the variants that allow more waves per SIMD, measured fastest
- * there.  With tuning on, the first time a
+ * group, say) the variants measured fastest there (more waves per SIMD for
+ * the FP64 assembly and correlation, more samples in flight for the FP32
+ * line search).  With tuning on, the first time a
  * handle sees a group shape (precision, mixed precision, frame element type,
  * frame_h, n_blocks, lines per run, geometry form, gradient form, frame-count
  * class floor(log2(n_frames))), it runs that group's first tick twice per

@@ -13,8 +13,9 @@
 // measure as fast as 8 without spilling registers.  A batch of at most 8
 // block columns per CU (one 512x512 group of 50 frames has 1600) runs too
 // few workgroups to fill the chip, and there kSmallBatchVariant picks the
-// variants that allow more waves per SIMD (FP64 assembly and correlation at
-// 4 waves: one 512x512 group's first tick 142 and 72 us against 154 and 75).
+// variants measured fastest there: FP64 assembly and correlation at 4 waves
+// per SIMD (one 512x512 group's first tick 142 and 72 us against 154 and 75)
+// and the FP32 line search with 8 samples in flight (131 against 141 us).
 // The engine's autotuner (off unless asked for) times all variants on a
 // group shape's first tick and keeps the fastest per kernel.
 // A variant only changes how many rows a thread unrolls and how registers
@@ -42,7 +43,7 @@ constexpr TuneVariant kTuneTable[TK_COUNT][2][kVariants] = {
 };
 
 // [kernel][fp64]: the variant of batches of at most 8 block columns per CU
-constexpr int kSmallBatchVariant[TK_COUNT][2] = {{0, 2}, {0, 0}, {0, 2}};
+constexpr int kSmallBatchVariant[TK_COUNT][2] = {{0, 2}, {1, 0}, {0, 2}};
 
 template <int KERNEL, typename R, int V>
 __host__ __device__ constexpr TuneVariant tune_variant() {
