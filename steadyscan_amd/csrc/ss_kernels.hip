@@ -1058,6 +1058,11 @@ __global__ __launch_bounds__(kSolveWG) void k_solve(Batch B) {
     InteriorRow *IR = reinterpret_cast<InteriorRow *>(g.ws + (size_t)n * 8);
     double2 *X = reinterpret_cast<double2 *>(IR + n);
     Aff *firstAff = reinterpret_cast<Aff *>(X + n);
+    // The exact solve without refinement, its partitions in registers, has
+    // read every equation before its first barrier and writes the solution
+    // after the last: the solution then takes the equations' LDS.
+    if (E == sE && g.solverId != 0 && !g.refine && (n + kSolveWG - 1) / kSolveWG <= kSolveRegRows)
+        X = reinterpret_cast<double2 *>(sE);
     if (g.solverId == 0) {
         // 4a. solver id 0: ILU(0) in x-then-y order (one thread, the
         //     reference's recurrence order)
