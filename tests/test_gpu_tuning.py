@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import steadyscan_amd as ssa
+from steadyscan_amd import _native as N
 from steadyscan_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -90,3 +91,16 @@ def test_server_admission_tunes_and_matches(calls):
                 assert np.array_equal(done[ti][k], r[k])
         assert len(h.tuning()) == 2
         srv.close()
+
+
+def test_tuning_mode_range():
+    # -2 (the default: variants by batch size), -1 (tune) and the forced
+    # variants 0..2 are accepted; anything else is an invalid argument and
+    # leaves the mode unchanged
+    with ssa.Handle(0) as h:
+        for mode in (-2, -1, 0, 1, 2):
+            h.tuning_mode(mode)
+        for bad in (-3, 3):
+            with pytest.raises(ssa.SteadyScanError) as e:
+                h.tuning_mode(bad)
+            assert e.value.status == N.SS_ERR_INVALID_ARGUMENT
